@@ -1,0 +1,578 @@
+// adrt_capi.hip -- C-ABI of the batched adding-doubling solver (host side).
+//
+// Validation mirrors adrt::ADConfig::validate (reference
+// src/adding_doubling.hpp:123-174); the quadrature, Legendre and Planck
+// helpers restate src/quadrature.cpp:11-76 and src/planck.cpp:35-177 in host
+// C++ of this library (they are also the host half of the product: the C++
+// drop-in adrt::solve uses them for thermal inputs).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "../../include/adrt_hip.h"
+#include "adrt_fwd.hpp"
+
+namespace {
+
+constexpr double kPi = 3.14159265358979323846;
+
+#define ADRT_HIP_TRY(expr)                    \
+  do {                                        \
+    if ((expr) != hipSuccess) return ADRT_ERR_HIP; \
+  } while (0)
+
+int padded_order(int n) {
+  int p = 2;
+  while (p < n) p *= 2;
+  return p;
+}
+
+// Gauss-Legendre nodes/weights on [0,1], ascending (src/quadrature.cpp:11-50).
+void gauss_legendre01(int n, double* nodes, double* weights) {
+  for (int i = 0; i < (n + 1) / 2; ++i) {
+    double z = std::cos(kPi * (i + 0.75) / (n + 0.5));
+    double pp = 0.0;
+    for (int it = 0; it < 100; ++it) {
+      double p0 = 1.0, p1 = z;
+      for (int k = 2; k <= n; ++k) {
+        const double pk = ((2 * k - 1) * z * p1 - (k - 1) * p0) / k;
+        p0 = p1;
+        p1 = pk;
+      }
+      pp = n * (z * p1 - p0) / (z * z - 1.0);
+      const double dz = -p1 / pp;
+      z += dz;
+      if (std::fabs(dz) < 1e-15) break;
+    }
+    const double w = 2.0 / ((1.0 - z * z) * pp * pp);
+    nodes[i] = (1.0 - z) / 2.0;
+    nodes[n - 1 - i] = (1.0 + z) / 2.0;
+    weights[i] = w / 2.0;
+    weights[n - 1 - i] = w / 2.0;
+  }
+}
+
+// P_l(x) by the three-term recurrence (src/quadrature.cpp:53-76).
+void legendre_column(int L, double x, double* out) {
+  if (L > 0) out[0] = 1.0;
+  if (L > 1) out[1] = x;
+  for (int l = 2; l < L; ++l) out[l] = ((2 * l - 1) * x * out[l - 1] - (l - 1) * out[l - 2]) / l;
+}
+
+// ---- band Planck function (src/planck.cpp:35-131) --------------------------
+__host__ __device__ constexpr double kC2 = 1.438786, kSigma = 5.67032E-8, kVcut = 1.5;
+__host__ __device__ constexpr double kA1 = 1.0 / 3.0, kA2 = -1.0 / 8.0, kA3 = 1.0 / 60.0, kA4 = -1.0 / 5040.0,
+                 kA5 = 1.0 / 272160.0, kA6 = -1.0 / 13305600.0;
+
+__host__ __device__ inline double plkernel(double v) { return v * v * v / (exp(v) - 1.0); }
+
+__host__ __device__ int planck_band(double lo, double hi, double T, double* out) {
+  if (T < 0.0 || hi < lo || lo < 0.0) return ADRT_ERR_PLANCK_ARGS;
+  if (T < 1.e-4) { *out = 0.0; return ADRT_OK; }
+  if (hi == lo) {
+    const double arg = exp(-kC2 * hi / T);
+    *out = 1.1911e-8 * hi * hi * hi * arg / (1.0 - arg);
+    return ADRT_OK;
+  }
+  const double vmax = 709.782712893384;  // log(DBL_MAX)
+  const double sigdpi = kSigma / kPi;
+  const double conc = 15.0 / pow(kPi, 4.0);
+  const double v[2] = {kC2 * lo / T, kC2 * hi / T};
+  if (v[0] > DBL_EPSILON && v[1] < vmax && (hi - lo) / hi < 1.e-2) {
+    // narrow band: Simpson's rule refined until 1e-6 relative change
+    const double hh = v[1] - v[0];
+    double oldval = 0.0;
+    const double val0 = plkernel(v[0]) + plkernel(v[1]);
+    for (int n = 1; n <= 10; ++n) {
+      const double del = hh / (2.0 * n);
+      double val = val0;
+      for (int k = 1; k <= 2 * n - 1; ++k) val += 2.0 * (1 + k % 2) * plkernel(v[0] + k * del);
+      val *= del * kA1;
+      if (fabs((val - oldval) / val) <= 1.e-6) {
+        *out = sigdpi * pow(T, 4.0) * conc * val;
+        return ADRT_OK;
+      }
+      oldval = val;
+    }
+    *out = sigdpi * pow(T, 4.0) * conc * oldval;
+    return ADRT_OK;
+  }
+  // series in v (small v) or in exp(-m v) (large v)
+  double p[2] = {0.0, 0.0}, d[2] = {0.0, 0.0};
+  int smallv = 0;
+  const double vcp[7] = {10.25, 5.7, 3.9, 2.9, 2.3, 1.9, 0.0};
+  for (int i = 0; i < 2; ++i) {
+    if (v[i] < kVcut) {
+      ++smallv;
+      const double vsq = v[i] * v[i];
+      p[i] = conc * vsq * v[i] *
+             (kA1 + v[i] * (kA2 + v[i] * (kA3 + vsq * (kA4 + vsq * (kA5 + vsq * kA6)))));
+    } else {
+      int mmax = 1;
+      while (v[i] < vcp[mmax - 1]) ++mmax;
+      const double ex = exp(-v[i]);
+      double exm = 1.0;
+      d[i] = 0.0;
+      for (int m = 1; m <= mmax; ++m) {
+        const double mv = m * v[i];
+        exm *= ex;
+        d[i] += exm * (6.0 + mv * (6.0 + mv * (3.0 + mv))) / (m * m * m * m);
+      }
+      d[i] *= conc;
+    }
+  }
+  double ans;
+  if (smallv == 2) ans = p[1] - p[0];
+  else if (smallv == 1) ans = 1.0 - p[0] - d[1];
+  else ans = d[0] - d[1];
+  ans *= sigdpi * pow(T, 4.0);
+  *out = (ans == 0.0) ? 0.0 : ans;
+  return ADRT_OK;
+}
+
+// dB/dT (src/planck.cpp:135-177)
+__host__ __device__ int planck_band_deriv(double lo, double hi, double T, double* out) {
+  if (T < 0.0 || hi < lo || lo < 0.0) return ADRT_ERR_PLANCK_ARGS;
+  if (T < 1.e-4) { *out = 0.0; return ADRT_OK; }
+  auto fk = [](double v) -> double {
+    if (v < 1.e-6) return v * v;
+    const double em = exp(-v);
+    return v * v * v * em / (1.0 - em);
+  };
+  if (hi == lo) {
+    const double x = kC2 * hi / T;
+    const double em = exp(-x);
+    const double b = 1.1911e-8 * hi * hi * hi * em / (1.0 - em);
+    *out = b * (x / T) / (1.0 - em);
+    return ADRT_OK;
+  }
+  const double sigdpi = kSigma / kPi;
+  const double conc = 15.0 / pow(kPi, 4.0);
+  const double v0 = kC2 * lo / T, v1 = kC2 * hi / T;
+  double B;
+  planck_band(lo, hi, T, &B);
+  const double boundary = sigdpi * conc * T * T * T * (v1 * fk(v1) - v0 * fk(v0));
+  *out = 4.0 * B / T - boundary;
+  return ADRT_OK;
+}
+
+int check_config(const adrt_batch_config* c) {
+  if (!c) return ADRT_ERR_NULL;
+  if (c->num_wavenumbers < 0) return ADRT_ERR_NUM_WAVENUMBERS;
+  if (c->num_layers <= 0) return ADRT_ERR_NUM_LAYERS;
+  if (c->num_quadrature < 2) return ADRT_ERR_NUM_QUADRATURE;
+  if (c->num_quadrature > ADRT_MAX_QUADRATURE) return ADRT_ERR_QUADRATURE_MAX;
+  if (c->num_moments < 1) return ADRT_ERR_MOMENTS;
+  if (!(c->surface_albedo >= 0.0 && c->surface_albedo <= 1.0)) return ADRT_ERR_SURFACE_ALBEDO;
+  if (c->solar_flux > 0.0 && (c->solar_mu <= 0.0 || c->solar_mu > 1.0)) return ADRT_ERR_SOLAR_MU;
+  return ADRT_OK;
+}
+
+// Planck values (or dB/dT) at every level of every column:
+// out[w][l] = B(wn_lo[w], wn_hi[w], T[w][l] or T[l]).
+__global__ void planck_levels_kernel(const double* lo, const double* hi, const double* T,
+                                     int t_shared, int W, int nlev, int deriv, double* out,
+                                     int* flag) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (long)W * nlev) return;
+  const int w = (int)(i / nlev), l = (int)(i % nlev);
+  const double t = t_shared ? T[l] : T[i];
+  double b = 0.0;
+  const int rc = deriv ? planck_band_deriv(lo[w], hi[w], t, &b) : planck_band(lo[w], hi[w], t, &b);
+  if (rc) atomicOr(flag, 1);
+  out[i] = b;
+}
+
+__global__ void validate_kernel(const double* tau, const double* ssa, long n, int* flag) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (tau[i] < 0.0) atomicOr(flag, 1);
+  const double w = ssa[i];
+  if (w < 0.0 || w > 1.0) atomicOr(flag, 2);
+}
+
+}  // namespace
+
+struct adrt_workspace {
+  int L = 0, n_real = 0, Np = 0, M = 0, Lmax = 0;
+  int chunk = 0;
+  size_t bytes = 0;
+  int device = 0;
+  char* base = nullptr;
+  double *mu = nullptr, *wt = nullptr, *Pl = nullptr, *Pl0 = nullptr;
+  double *tau_s = nullptr, *omega_s = nullptr, *tau_cum = nullptr, *fdm = nullptr;
+  double *lay = nullptr, *rb = nullptr;
+  int *mode = nullptr, *lflag = nullptr, *flag = nullptr;
+  double xfac = 0.0;
+  double tables_mu0 = -1.0;
+  bool tables_ready = false;
+  std::vector<double> h_mu, h_wt;
+  // opt-in kernel timing: event pairs recorded on the caller's stream around
+  // every launch (kind 0 prep, 1 layer/doubling, 2 adding)
+  bool profiling = false;
+  std::vector<hipEvent_t> ev_pool;
+  std::vector<int> ev_kind;  // one per recorded pair
+  size_t ev_used = 0;        // pairs in use
+};
+
+namespace {
+
+size_t per_wavenumber_bytes(int L, int Np) {
+  const size_t d = sizeof(double);
+  size_t b = 0;
+  b += 4 * (size_t)L * d + (size_t)(L + 1) * d;      // tau_s, omega_s, fdm, spare, tau_cum
+  b += 2 * (size_t)L * sizeof(int);                   // mode, lflag
+  b += (size_t)L * (2 * Np * Np + 2 * Np) * d;        // layer store
+  b += (size_t)L * (2 * Np * Np + Np) * d;            // base composites
+  return b;
+}
+
+size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+int upload_tables(adrt_workspace* ws, double mu0, hipStream_t stream) {
+  if (ws->tables_ready && ws->tables_mu0 == mu0) return ADRT_OK;
+  const int Np = ws->Np, n = ws->n_real, Lmax = ws->Lmax;
+  std::vector<double> h((size_t)(2 * Np + Lmax * Np + Lmax), 0.0);
+  double* mu = h.data();
+  double* wt = mu + Np;
+  double* Pl = wt + Np;
+  double* Pl0 = Pl + (size_t)Lmax * Np;
+  for (int i = 0; i < Np; ++i) {
+    mu[i] = (i < n) ? ws->h_mu[i] : 1.0;  // pad nodes: mu = 1, w = 0, P_l = 0
+    wt[i] = (i < n) ? ws->h_wt[i] : 0.0;
+  }
+  std::vector<double> col((size_t)Lmax);
+  for (int i = 0; i < n; ++i) {
+    legendre_column(Lmax, mu[i], col.data());
+    for (int l = 0; l < Lmax; ++l) Pl[(size_t)l * Np + i] = col[l];
+  }
+  legendre_column(Lmax, mu0, Pl0);  // P_l(mu0), src/phase_matrix.hpp:145-151
+  ADRT_HIP_TRY(hipMemcpyAsync(ws->mu, h.data(), h.size() * sizeof(double),
+                              hipMemcpyHostToDevice, stream));
+  ADRT_HIP_TRY(hipStreamSynchronize(stream));
+  ws->tables_ready = true;
+  ws->tables_mu0 = mu0;
+  return ADRT_OK;
+}
+
+// Records the start event of a timed launch; returns the pair index or -1.
+int prof_begin(adrt_workspace* ws, int kind, hipStream_t stream) {
+  if (!ws->profiling) return -1;
+  if (ws->ev_used * 2 + 2 > ws->ev_pool.size()) {
+    for (int i = 0; i < 2; ++i) {
+      hipEvent_t e;
+      if (hipEventCreate(&e) != hipSuccess) return -1;
+      ws->ev_pool.push_back(e);
+    }
+    ws->ev_kind.push_back(kind);
+  }
+  const size_t k = ws->ev_used++;
+  ws->ev_kind[k] = kind;
+  hipEventRecord(ws->ev_pool[2 * k], stream);
+  return (int)k;
+}
+
+void prof_end(adrt_workspace* ws, int k, hipStream_t stream) {
+  if (k >= 0) hipEventRecord(ws->ev_pool[2 * k + 1], stream);
+}
+
+template <int N>
+int launch_chunk(const adrt::dev::FwdArgs& a, adrt_workspace* ws, hipStream_t stream) {
+  using namespace adrt::dev;
+  constexpr int GPB = Geo<N>::GPB;  // lane groups per 256-thread block
+  int e = prof_begin(ws, 0, stream);
+  prep_kernel<<<(a.W + 255) / 256, 256, 0, stream>>>(a);
+  prof_end(ws, e, stream);
+  const long tasks = (long)a.W * a.L;
+  e = prof_begin(ws, 1, stream);
+  layer_kernel<N><<<(unsigned)((tasks + GPB - 1) / GPB), 256, 0, stream>>>(a);
+  prof_end(ws, e, stream);
+  e = prof_begin(ws, 2, stream);
+  adding_kernel<N><<<(unsigned)((a.W + GPB - 1) / GPB), 256, 0, stream>>>(a);
+  prof_end(ws, e, stream);
+  return hipGetLastError() == hipSuccess ? ADRT_OK : ADRT_ERR_HIP;
+}
+
+}  // namespace
+
+extern "C" {
+
+int adrt_workspace_create(const adrt_batch_config* cfg, size_t max_bytes, adrt_workspace** out) {
+  if (!out) return ADRT_ERR_NULL;
+  *out = nullptr;
+  int rc = check_config(cfg);
+  if (rc) return rc;
+  adrt_workspace* ws = new (std::nothrow) adrt_workspace();
+  if (!ws) return ADRT_ERR_WORKSPACE;
+  ws->L = cfg->num_layers;
+  ws->n_real = cfg->num_quadrature;
+  ws->Np = padded_order(cfg->num_quadrature);
+  ws->M = cfg->num_moments;
+  ws->Lmax = std::max(cfg->num_moments, 2 * cfg->num_quadrature);
+  ws->h_mu.assign(ws->n_real, 0.0);
+  ws->h_wt.assign(ws->n_real, 0.0);
+  gauss_legendre01(ws->n_real, ws->h_mu.data(), ws->h_wt.data());
+  double s = 0.0;
+  for (int i = 0; i < ws->n_real; ++i) s += ws->h_mu[i] * ws->h_wt[i];
+  ws->xfac = 0.5 / s;  // src/solver.cpp:378-383
+  if (max_bytes == 0) max_bytes = (size_t)4 << 30;
+  const size_t per_w = per_wavenumber_bytes(ws->L, ws->Np);
+  const size_t fixed = align256((size_t)(2 * ws->Np + ws->Lmax * ws->Np + ws->Lmax) * 8) + 256;
+  long chunk = max_bytes > fixed ? (long)((max_bytes - fixed) / (per_w + 2048)) : 1;
+  const int W = std::max(cfg->num_wavenumbers, 1);
+  chunk = std::max(1L, std::min(chunk, (long)W));
+  ws->chunk = (int)chunk;
+  const size_t L = ws->L, Np = ws->Np, C = chunk;
+  size_t off = 0;
+  auto take = [&](size_t b) {
+    size_t o = off;
+    off += align256(b);
+    return o;
+  };
+  const size_t o_tab = take((size_t)(2 * Np + ws->Lmax * Np + ws->Lmax) * 8);
+  const size_t o_tau_s = take(C * L * 8), o_om = take(C * L * 8), o_fdm = take(C * L * 8);
+  const size_t o_cum = take(C * (L + 1) * 8);
+  const size_t o_mode = take(C * L * 4), o_lflag = take(C * L * 4), o_flag = take(4);
+  const size_t o_lay = take(C * L * (2 * Np * Np + 2 * Np) * 8);
+  const size_t o_rb = take(C * L * (2 * Np * Np + Np) * 8);
+  if (hipGetDevice(&ws->device) != hipSuccess || hipMalloc((void**)&ws->base, off) != hipSuccess) {
+    delete ws;
+    return ADRT_ERR_HIP;
+  }
+  ws->bytes = off;
+  ws->mu = (double*)(ws->base + o_tab);
+  ws->wt = ws->mu + Np;
+  ws->Pl = ws->wt + Np;
+  ws->Pl0 = ws->Pl + (size_t)ws->Lmax * Np;
+  ws->tau_s = (double*)(ws->base + o_tau_s);
+  ws->omega_s = (double*)(ws->base + o_om);
+  ws->fdm = (double*)(ws->base + o_fdm);
+  ws->tau_cum = (double*)(ws->base + o_cum);
+  ws->mode = (int*)(ws->base + o_mode);
+  ws->lflag = (int*)(ws->base + o_lflag);
+  ws->flag = (int*)(ws->base + o_flag);
+  ws->lay = (double*)(ws->base + o_lay);
+  ws->rb = (double*)(ws->base + o_rb);
+  *out = ws;
+  return ADRT_OK;
+}
+
+int adrt_workspace_destroy(adrt_workspace* ws) {
+  if (!ws) return ADRT_OK;
+  int rc = ADRT_OK;
+  for (hipEvent_t e : ws->ev_pool) hipEventDestroy(e);
+  if (ws->base && hipFree(ws->base) != hipSuccess) rc = ADRT_ERR_HIP;
+  delete ws;
+  return rc;
+}
+
+int adrt_workspace_info(const adrt_workspace* ws, size_t* bytes, int* chunk) {
+  if (!ws) return ADRT_ERR_NULL;
+  if (bytes) *bytes = ws->bytes;
+  if (chunk) *chunk = ws->chunk;
+  return ADRT_OK;
+}
+
+int adrt_solve_batch(const adrt_batch_config* cfg, const adrt_device_inputs* in,
+                     const adrt_device_outputs* out, adrt_precision precision,
+                     adrt_workspace* ws, hipStream_t stream) {
+  int rc = check_config(cfg);
+  if (rc) return rc;
+  if (!in || !out) return ADRT_ERR_NULL;
+  if (cfg->num_wavenumbers == 0) return ADRT_OK;  // empty batch: nothing to read or write
+  if (!in->delta_tau) return ADRT_ERR_DELTA_TAU;
+  if (!in->single_scat_albedo) return ADRT_ERR_SSA;
+  if (!in->phase_moments) return ADRT_ERR_MOMENTS;
+  if (cfg->use_diffusion_lower_bc && !in->planck_levels) return ADRT_ERR_DIFFUSION_BC;
+  if (!out->flux_up || !out->flux_down || !out->mean_intensity || !out->flux_direct)
+    return ADRT_ERR_OUTPUTS;
+  if (precision != ADRT_PRECISION_F64) return ADRT_ERR_PRECISION;
+  if (!ws) return ADRT_ERR_WORKSPACE;
+  if (ws->L != cfg->num_layers || ws->n_real != cfg->num_quadrature || ws->M != cfg->num_moments)
+    return ADRT_ERR_WORKSPACE;
+  if (cfg->num_wavenumbers == 0) return ADRT_OK;
+  const bool has_solar = cfg->solar_flux > 0.0 && cfg->solar_mu > 0.0;
+  rc = upload_tables(ws, has_solar ? cfg->solar_mu : 1.0, stream);
+  if (rc) return rc;
+
+  adrt::dev::FwdArgs a{};
+  a.L = cfg->num_layers;
+  a.M = cfg->num_moments;
+  a.n_real = cfg->num_quadrature;
+  a.shared_moments = in->phase_moments_shared ? 1 : 0;
+  a.delta_m = cfg->use_delta_m ? 1 : 0;
+  a.diffusion = cfg->use_diffusion_lower_bc ? 1 : 0;
+  a.rev = cfg->index_from_bottom ? 1 : 0;
+  a.has_solar = has_solar ? 1 : 0;
+  a.albedo = cfg->surface_albedo;
+  a.surf_em = cfg->surface_emission;
+  a.top_em = cfg->top_emission;
+  a.F0 = cfg->solar_flux;
+  a.mu0 = cfg->solar_mu;
+  a.scaling = cfg->spectrum_scaling;
+  a.xfac = ws->xfac;
+  a.tau = in->delta_tau;
+  a.ssa = in->single_scat_albedo;
+  a.mom = in->phase_moments;
+  a.B = in->planck_levels;
+  a.surf_w = in->surface_emission;
+  a.top_w = in->top_emission;
+  a.fu = out->flux_up;
+  a.fd = out->flux_down;
+  a.mi = out->mean_intensity;
+  a.fdir = out->flux_direct;
+  a.fdiv = out->flux_divergence;
+  a.mu = ws->mu;
+  a.wt = ws->wt;
+  a.Pl = ws->Pl;
+  a.Pl0 = ws->Pl0;
+  a.tau_s = ws->tau_s;
+  a.omega_s = ws->omega_s;
+  a.tau_cum = ws->tau_cum;
+  a.fdm = ws->fdm;
+  a.mode = ws->mode;
+  a.lay = ws->lay;
+  a.lflag = ws->lflag;
+  a.rb = ws->rb;
+  for (int w0 = 0; w0 < cfg->num_wavenumbers; w0 += ws->chunk) {
+    a.w0 = w0;
+    a.W = std::min(ws->chunk, cfg->num_wavenumbers - w0);
+    switch (ws->Np) {
+#ifndef ADRT_ONLY_N
+      case 2: rc = launch_chunk<2>(a, ws, stream); break;
+      case 4: rc = launch_chunk<4>(a, ws, stream); break;
+      case 8: rc = launch_chunk<8>(a, ws, stream); break;
+      case 16: rc = launch_chunk<16>(a, ws, stream); break;
+      case 32: rc = launch_chunk<32>(a, ws, stream); break;
+#else
+      case ADRT_ONLY_N: rc = launch_chunk<ADRT_ONLY_N>(a, ws, stream); break;
+#endif
+      default: rc = ADRT_ERR_QUADRATURE_MAX;
+    }
+    if (rc) return rc;
+  }
+  return ADRT_OK;
+}
+
+int adrt_validate_inputs(const adrt_batch_config* cfg, const adrt_device_inputs* in,
+                         adrt_workspace* ws, hipStream_t stream) {
+  int rc = check_config(cfg);
+  if (rc) return rc;
+  if (!in || !in->delta_tau || !in->single_scat_albedo) return ADRT_ERR_NULL;
+  if (!ws) return ADRT_ERR_WORKSPACE;
+  const long n = (long)cfg->num_wavenumbers * cfg->num_layers;
+  if (n == 0) return ADRT_OK;
+  ADRT_HIP_TRY(hipMemsetAsync(ws->flag, 0, sizeof(int), stream));
+  validate_kernel<<<(unsigned)((n + 255) / 256), 256, 0, stream>>>(in->delta_tau,
+                                                                   in->single_scat_albedo, n,
+                                                                   ws->flag);
+  int h = 0;
+  ADRT_HIP_TRY(hipMemcpyAsync(&h, ws->flag, sizeof(int), hipMemcpyDeviceToHost, stream));
+  ADRT_HIP_TRY(hipStreamSynchronize(stream));
+  if (h & 1) return ADRT_ERR_NEGATIVE_TAU;
+  if (h & 2) return ADRT_ERR_SSA_RANGE;
+  return ADRT_OK;
+}
+
+int adrt_planck_levels(const double* wn_lo, const double* wn_hi, const double* temperature,
+                       int temperature_shared, int num_wavenumbers, int num_levels,
+                       int derivative, double* out, adrt_workspace* ws, hipStream_t stream) {
+  if (!wn_lo || !wn_hi || !temperature || !out || !ws) return ADRT_ERR_NULL;
+  if (num_wavenumbers < 0 || num_levels < 0) return ADRT_ERR_NUM_WAVENUMBERS;
+  const long n = (long)num_wavenumbers * num_levels;
+  if (n == 0) return ADRT_OK;
+  ADRT_HIP_TRY(hipMemsetAsync(ws->flag, 0, sizeof(int), stream));
+  planck_levels_kernel<<<(unsigned)((n + 255) / 256), 256, 0, stream>>>(
+      wn_lo, wn_hi, temperature, temperature_shared, num_wavenumbers, num_levels, derivative, out,
+      ws->flag);
+  int h = 0;
+  ADRT_HIP_TRY(hipMemcpyAsync(&h, ws->flag, sizeof(int), hipMemcpyDeviceToHost, stream));
+  ADRT_HIP_TRY(hipStreamSynchronize(stream));
+  return h ? ADRT_ERR_PLANCK_ARGS : ADRT_OK;
+}
+
+int adrt_profile(adrt_workspace* ws, int enable) {
+  if (!ws) return ADRT_ERR_WORKSPACE;
+  ws->profiling = enable != 0;
+  ws->ev_used = 0;
+  return ADRT_OK;
+}
+
+int adrt_profile_read(adrt_workspace* ws, double* ms, int* launches) {
+  if (!ws || !ms) return ADRT_ERR_NULL;
+  for (int k = 0; k < 3; ++k) {
+    ms[k] = 0.0;
+    if (launches) launches[k] = 0;
+  }
+  for (size_t k = 0; k < ws->ev_used; ++k) {
+    ADRT_HIP_TRY(hipEventSynchronize(ws->ev_pool[2 * k + 1]));
+    float t = 0.f;
+    ADRT_HIP_TRY(hipEventElapsedTime(&t, ws->ev_pool[2 * k], ws->ev_pool[2 * k + 1]));
+    const int kind = ws->ev_kind[k];
+    ms[kind] += t;
+    if (launches) launches[kind] += 1;
+  }
+  ws->ev_used = 0;
+  return ADRT_OK;
+}
+
+int adrt_gauss_legendre(int n, double* nodes, double* weights) {
+  if (n < 1 || !nodes || !weights) return ADRT_ERR_NULL;
+  gauss_legendre01(n, nodes, weights);
+  return ADRT_OK;
+}
+
+int adrt_planck(double lo, double hi, double T, double* out) {
+  if (!out) return ADRT_ERR_NULL;
+  return planck_band(lo, hi, T, out);
+}
+
+int adrt_planck_deriv(double lo, double hi, double T, double* out) {
+  if (!out) return ADRT_ERR_NULL;
+  return planck_band_deriv(lo, hi, T, out);
+}
+
+const char* adrt_status_string(int s) {
+  switch (s) {
+    case ADRT_OK: return "ok";
+    case ADRT_ERR_NUM_LAYERS: return "ADConfig: num_layers must be > 0";
+    case ADRT_ERR_NUM_QUADRATURE: return "ADConfig: num_quadrature must be >= 2";
+    case ADRT_ERR_DELTA_TAU: return "ADConfig: delta_tau size mismatch";
+    case ADRT_ERR_SSA: return "ADConfig: single_scat_albedo size mismatch";
+    case ADRT_ERR_MOMENTS: return "ADConfig: phase_function_moments size mismatch";
+    case ADRT_ERR_NEGATIVE_TAU: return "ADConfig: delta_tau < 0";
+    case ADRT_ERR_SSA_RANGE: return "ADConfig: single_scat_albedo out of [0,1]";
+    case ADRT_ERR_SURFACE_ALBEDO: return "ADConfig: surface_albedo out of [0,1]";
+    case ADRT_ERR_SOLAR_MU: return "ADConfig: solar_mu must be in (0,1] when solar_flux > 0";
+    case ADRT_ERR_TEMPERATURE_SIZE: return "ADConfig: temperature size must be num_layers+1";
+    case ADRT_ERR_TEMPERATURE: return "ADConfig: temperature < 0";
+    case ADRT_ERR_WAVENUMBER_RANGE: return "ADConfig: invalid wavenumber range";
+    case ADRT_ERR_SURFACE_TEMPERATURE:
+      return "ADConfig: surface_temperature requires use_thermal_emission (use surface_emission "
+             "for raw thermal sources)";
+    case ADRT_ERR_TOP_TEMPERATURE:
+      return "ADConfig: top_temperature requires use_thermal_emission (use top_emission for raw "
+             "thermal sources)";
+    case ADRT_ERR_DIFFUSION_BC:
+      return "ADConfig: use_diffusion_lower_bc requires use_thermal_emission or planck_levels";
+    case ADRT_ERR_PLANCK_ARGS: return "planckFunction: invalid arguments";
+    case ADRT_ERR_QUADRATURE_MAX: return "num_quadrature exceeds 32";
+    case ADRT_ERR_NUM_WAVENUMBERS: return "num_wavenumbers must be >= 0";
+    case ADRT_ERR_OUTPUTS: return "required output buffer is NULL";
+    case ADRT_ERR_WORKSPACE: return "workspace missing or built for another shape";
+    case ADRT_ERR_HIP: return "HIP runtime error";
+    case ADRT_ERR_PRECISION: return "unknown or unsupported precision";
+    case ADRT_ERR_NULL: return "NULL argument";
+    default: return "unknown status";
+  }
+}
+
+const char* adrt_version(void) { return "adrt-mi355x 0.1"; }
+
+}  // extern "C"

@@ -1,0 +1,172 @@
+"""Benchmark of the batched adding-doubling solve on one MI355X.
+
+One step = one solve of the full column set of the headline configuration
+(65536 wavenumbers x 100 layers, N = 16, solar mu0 = 0.3 + thermal 300 -> 180 K,
+albedo 0.15, 32 shared Rayleigh/HG moments per layer) with every input already
+resident in HBM.  Prints one JSON line (see the keys below).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--wavenumbers W]
+
+roofline: the dominant kernel is the layer kernel (phase setup + doubling);
+its algorithmic work is (2 + 2/3 + 2 + 2 + 2 + 2) N^3 flops per doubling step
+summed over the scattering layers of the batch (counted from the inputs), its
+time comes from HIP events recorded by the library around each of its launches
+on the solve stream, and the peak is MI355X FP64 (78.6 TFLOP/s).
+cpu_baseline: the FP64 CPU oracle (a restatement of the reference solver) on a
+bounded sample of the same columns, on this host's cores.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "column-wavenumbers/s fwd+adjoint, mixed vs plain FP64 at ≤1e-6 rel flux err"
+FP64_PEAK_TFLOPS = 78.6
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=3)
+    p.add_argument("--warmup", type=int, default=1)
+    p.add_argument("--wavenumbers", type=int, default=65536)
+    p.add_argument("--config", type=int, default=5)
+    p.add_argument("--cpu-seconds", type=float, default=12.0)
+    p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--workspace-gb", type=float, default=8.0)
+    return p.parse_args()
+
+
+def cpu_baseline(cols, planck_host, seconds):
+    """FP64 oracle on a bounded sample of the same columns (forward only)."""
+    import numpy as np
+
+    import oracle
+
+    threads = min(os.cpu_count() or 1, 16)
+    B, Bs, Bt = planck_host
+
+    def run(n):
+        sl = slice(0, n)
+        mom = cols.phase_moments if cols.phase_moments.ndim == 2 else cols.phase_moments[sl]
+        t0 = time.perf_counter()
+        oracle.solve_batch(cols.delta_tau[sl], cols.single_scat_albedo[sl], mom, cols.N,
+                           planck_levels=None if B is None else B[sl],
+                           surface_emission=0.0 if Bs is None else Bs[sl],
+                           top_emission=0.0 if Bt is None else Bt[sl],
+                           surface_albedo=cols.surface_albedo, solar_flux=cols.solar_flux,
+                           solar_mu=cols.solar_mu, use_delta_m=cols.use_delta_m,
+                           use_diffusion_lower_bc=cols.use_diffusion_lower_bc,
+                           threads=threads)
+        return time.perf_counter() - t0
+
+    n = min(cols.W, 4 * threads)
+    dt = run(n)
+    n2 = int(min(cols.W, max(n, n * seconds / max(dt, 1e-3))))
+    dt2 = run(n2)
+    return dict(value=n2 / dt2, unit="column-wavenumbers/s", cores=threads, kind="port",
+                sample=f"first {n2} of the {cols.W} columns, FP64 forward only "
+                       f"(the reference has no adjoint), {dt2:.1f} s on {threads} threads")
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py runs on one GPU only (got --gpus {a.gpus}); columns are independent, "
+              "so more GPUs would each solve their own share of the wavenumbers.",
+              file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    from addingdoublingrt_amd import _lib, flops, solve_batch, workloads
+    from addingdoublingrt_amd.solver import get_workspace
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    cols = workloads.make(a.config, W=a.wavenumbers)
+    d = workloads.device_inputs(cols, "cuda")
+    cfg = workloads.batch_config(cols)
+    W, L = cols.W, cols.L
+    ws_bytes = int(a.workspace_gb * (1 << 30))
+    out = {k: torch.empty((W, L + 1), dtype=torch.float64, device="cuda")
+           for k in ("flux_up", "flux_down", "mean_intensity", "flux_direct", "flux_divergence")}
+
+    def step():
+        solve_batch(d["delta_tau"], d["single_scat_albedo"], d["phase_moments"], cfg,
+                    planck_levels=d["planck_levels"], surface_emission=d["surface_emission"],
+                    top_emission=d["top_emission"], out=out, max_workspace_bytes=ws_bytes)
+
+    for _ in range(a.warmup):
+        step()
+    torch.cuda.synchronize()
+    ws = get_workspace(W, L, cols.N, cols.phase_moments.shape[-1], ws_bytes)
+    L_ = _lib.lib()
+    _lib.check(L_.adrt_profile(ws.handle, 1))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        step()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    ms = (C.c_double * 3)()
+    nl = (C.c_int * 3)()
+    _lib.check(L_.adrt_profile_read(ws.handle, ms, nl))
+    _lib.check(L_.adrt_profile(ws.handle, 0))
+
+    ms_per_step = elapsed * 1e3 / a.steps
+    Bs_host = d["surface_emission"].cpu().numpy() if d["surface_emission"] is not None else None
+    work = flops.forward_flops(cols.delta_tau, cols.single_scat_albedo, cols.phase_moments,
+                               cols.N, cols.use_delta_m, cols.surface_albedo, Bs_host,
+                               cols.use_diffusion_lower_bc)
+    layer_ms_per_launch = ms[1] / max(nl[1], 1)
+    flops_per_launch = work["doubling"] / (nl[1] / a.steps)
+    achieved = flops_per_launch / (layer_ms_per_launch * 1e-3) / 1e12
+    res = {
+        "metric": METRIC,
+        "value": W / (ms_per_step * 1e-3),
+        "unit": "column-wavenumbers/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (numpy PCG64 seed 20261203+k; Planck levels from the device kernel)",
+        "config": {
+            "workload": f"config {a.config}: {cols.name}, {W} wavenumbers x {L} layers, "
+                        f"N={cols.N}",
+            "form": "plain FP64 forward (adjoint and mixed forms not built yet)",
+            "num_wavenumbers": W, "num_layers": L, "num_quadrature": cols.N,
+            "algorithmic_gflop_per_step": work["total"] / 1e9,
+            "doubling_steps": work["doubling_steps"], "general_adds": work["general_adds"],
+            "kernel_ms_per_step": {"prep": ms[0] / a.steps, "layer_doubling": ms[1] / a.steps,
+                                   "adding_interfaces": ms[2] / a.steps},
+        },
+        "roofline": {
+            "bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+            "kernel": "layer_kernel (phase setup + doubling), FP64",
+            "launches_per_step": nl[1] / a.steps, "avg_launch_ms": layer_ms_per_launch,
+        },
+    }
+    if not a.no_cpu:
+        ph = None
+        if cols.thermal:
+            ph = (d["planck_levels"].cpu().numpy(), Bs_host, d["top_emission"].cpu().numpy())
+        else:
+            ph = (None, None, None)
+        res["cpu_baseline"] = cpu_baseline(cols, ph, a.cpu_seconds)
+    print(json.dumps(res))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,156 @@
+/*
+ * adrt_hip.h -- C-ABI of the MI355X-native batched adding-doubling solver.
+ *
+ * One call solves many independent columns (one per wavenumber) on one GPU.
+ * All arrays are caller-owned device buffers in wavenumber-major layout,
+ * array[w * stride + layer_or_level], TOA first unless index_from_bottom.
+ * Every entry point returns an int status (0 = ADRT_OK) and never throws.
+ * Work is enqueued on the given stream; nothing synchronises unless stated.
+ *
+ * Reference interfaces replaced (paths relative to the reference repository):
+ *   adrt_solve_batch        <- adrt::cuda::solveBatch        src_cuda/cuda_solver.cuh:77-80
+ *                              (FP64, every interface, not only TOA) and the
+ *                              per-column adrt::solve        src/adding_doubling.hpp:258-269
+ *   adrt_batch_config       <- adrt::cuda::BatchConfig       src_cuda/cuda_solver.cuh:18-40
+ *   adrt_device_inputs/outputs <- adrt::cuda::DeviceData     src_cuda/cuda_solver.cuh:45-68
+ *   adrt_workspace_*        <- adrt::cuda::SolverWorkspaceGPU src_cuda/cuda_solver.cuh:140-156
+ *                              and adrt::SolverWorkspace     src/workspace.hpp:23-48
+ *   adrt_gauss_legendre     <- adrt::gaussLegendre           src/adding_doubling.hpp:277-280
+ *   adrt_planck / _deriv    <- adrt::planckFunction(Deriv)   src/planck.hpp, src/planck.cpp:35-177
+ *   status codes 1..17      <- the std::invalid_argument cases of
+ *                              adrt::ADConfig::validate     src/adding_doubling.hpp:123-174
+ */
+#ifndef ADRT_HIP_H
+#define ADRT_HIP_H
+
+#include <stddef.h>
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define ADRT_OK 0
+#define ADRT_ERR_NUM_LAYERS 1          /* num_layers must be > 0 */
+#define ADRT_ERR_NUM_QUADRATURE 2      /* num_quadrature must be >= 2 */
+#define ADRT_ERR_DELTA_TAU 3           /* delta_tau missing */
+#define ADRT_ERR_SSA 4                 /* single_scat_albedo missing */
+#define ADRT_ERR_MOMENTS 5             /* phase moments missing / num_moments < 1 */
+#define ADRT_ERR_NEGATIVE_TAU 6        /* delta_tau[l] < 0 (checked by adrt_validate_inputs) */
+#define ADRT_ERR_SSA_RANGE 7           /* single_scat_albedo out of [0,1] (idem) */
+#define ADRT_ERR_SURFACE_ALBEDO 8      /* surface_albedo out of [0,1] */
+#define ADRT_ERR_SOLAR_MU 9            /* solar_mu must be in (0,1] when solar_flux > 0 */
+#define ADRT_ERR_TEMPERATURE_SIZE 10
+#define ADRT_ERR_TEMPERATURE 11
+#define ADRT_ERR_WAVENUMBER_RANGE 12
+#define ADRT_ERR_SURFACE_TEMPERATURE 13
+#define ADRT_ERR_TOP_TEMPERATURE 14
+#define ADRT_ERR_DIFFUSION_BC 15       /* diffusion lower BC needs Planck levels */
+#define ADRT_ERR_PLANCK_ARGS 16
+#define ADRT_ERR_QUADRATURE_MAX 17     /* num_quadrature > ADRT_MAX_QUADRATURE */
+#define ADRT_ERR_NUM_WAVENUMBERS 18    /* num_wavenumbers must be >= 0 */
+#define ADRT_ERR_OUTPUTS 19            /* a required output pointer is NULL */
+#define ADRT_ERR_WORKSPACE 20          /* workspace missing or made for another shape */
+#define ADRT_ERR_HIP 21                /* a HIP runtime call failed */
+#define ADRT_ERR_PRECISION 22          /* unknown precision value */
+#define ADRT_ERR_NULL 23               /* NULL argument */
+
+#define ADRT_MAX_QUADRATURE 32
+
+typedef enum adrt_precision {
+  ADRT_PRECISION_F64 = 0,   /* plain form: FP64 throughout */
+  ADRT_PRECISION_MIXED = 1  /* FP32 doubling/adding, FP64 init, refined solves, FP64 fluxes */
+} adrt_precision;
+
+/* Scalars shared by every column (names as BatchConfig). */
+typedef struct adrt_batch_config {
+  int num_wavenumbers;
+  int num_layers;
+  int num_quadrature;          /* 2..32; 2,4,8,16,32 native, others padded */
+  int num_moments;             /* moments per layer (M) */
+  int use_delta_m;             /* delta-M when M > 2N (ADConfig::use_delta_m) */
+  int use_diffusion_lower_bc;  /* diffusion lower boundary (needs planck_levels) */
+  int index_from_bottom;       /* inputs/outputs BOA first */
+  double surface_albedo;       /* Lambertian albedo in [0,1] */
+  double surface_emission;     /* used when inputs.surface_emission == NULL */
+  double top_emission;         /* used when inputs.top_emission == NULL */
+  double solar_flux;           /* collimated flux at TOA (0 = no beam) */
+  double solar_mu;             /* cosine of the solar zenith angle, (0,1] */
+  double spectrum_scaling;     /* factor on flux_up (1.0 = none) */
+} adrt_batch_config;
+
+/* Caller-owned device inputs (FP64). */
+typedef struct adrt_device_inputs {
+  const double* delta_tau;           /* [W][L] */
+  const double* single_scat_albedo;  /* [W][L] */
+  const double* phase_moments;       /* [L][M] if shared, else [W][L][M] */
+  int phase_moments_shared;
+  const double* planck_levels;       /* [W][L+1] Planck at levels, or NULL (no thermal) */
+  const double* surface_emission;    /* [W] or NULL */
+  const double* top_emission;        /* [W] or NULL */
+} adrt_device_inputs;
+
+/* Caller-owned device outputs (FP64), [W][L+1] at every interface. */
+typedef struct adrt_device_outputs {
+  double* flux_up;
+  double* flux_down;
+  double* mean_intensity;   /* includes the direct-beam term F_dir/(4 pi mu0) */
+  double* flux_direct;
+  double* flux_divergence;  /* may be NULL */
+} adrt_device_outputs;
+
+typedef struct adrt_workspace adrt_workspace;
+
+/* Create a workspace for columns of this shape (num_layers, num_quadrature,
+ * num_moments).  max_bytes bounds its device memory (0 = 4 GiB); the batch is
+ * processed in chunks of wavenumbers that fit.  Allocates device memory. */
+int adrt_workspace_create(const adrt_batch_config* cfg, size_t max_bytes,
+                          adrt_workspace** ws);
+int adrt_workspace_destroy(adrt_workspace* ws);
+/* Device bytes held and wavenumbers per chunk. */
+int adrt_workspace_info(const adrt_workspace* ws, size_t* bytes, int* chunk_wavenumbers);
+
+/* Forward solve of num_wavenumbers columns.  Asynchronous on stream. */
+int adrt_solve_batch(const adrt_batch_config* cfg, const adrt_device_inputs* in,
+                     const adrt_device_outputs* out, adrt_precision precision,
+                     adrt_workspace* ws, hipStream_t stream);
+
+/* Checks delta_tau >= 0 and single_scat_albedo in [0,1] on the device and
+ * returns ADRT_ERR_NEGATIVE_TAU / ADRT_ERR_SSA_RANGE (synchronises the stream). */
+int adrt_validate_inputs(const adrt_batch_config* cfg, const adrt_device_inputs* in,
+                         adrt_workspace* ws, hipStream_t stream);
+
+/* Band Planck function (or dB/dT when derivative != 0) at every level of every
+ * column, on the device: out[w][l] = B(wn_lo[w], wn_hi[w], T), T = temperature[l]
+ * if temperature_shared else temperature[w][l].  Synchronises the stream;
+ * returns ADRT_ERR_PLANCK_ARGS if any (band, T) is invalid.
+ * Replaces the per-level planckFunction calls of the column solver
+ * (src/solver.cpp:385-419) and the device Planck of src_cuda/cuda_planck.cuh:33-122. */
+int adrt_planck_levels(const double* wn_lo, const double* wn_hi, const double* temperature,
+                       int temperature_shared, int num_wavenumbers, int num_levels,
+                       int derivative, double* out, adrt_workspace* ws, hipStream_t stream);
+
+/* Opt-in kernel timing (no counterpart in the reference, whose benchmark times
+ * whole calls, tests/benchmark_cuda.cu:209-222).  While enabled, every launch
+ * of adrt_solve_batch on this workspace is bracketed by HIP events on the
+ * caller's stream.  adrt_profile_read synchronises those events and returns
+ * the summed milliseconds and launch counts per kernel kind since the last
+ * read: [0] prep, [1] layer setup + doubling, [2] adding + interfaces.
+ * Enabling allocates HIP events (host-side objects) on first use. */
+int adrt_profile(adrt_workspace* ws, int enable);
+int adrt_profile_read(adrt_workspace* ws, double* ms, int* launches);
+
+/* Host utilities. */
+int adrt_gauss_legendre(int n, double* nodes, double* weights);
+int adrt_planck(double wnumlo, double wnumhi, double temp, double* out);
+int adrt_planck_deriv(double wnumlo, double wnumhi, double temp, double* out);
+const char* adrt_status_string(int status);
+/* Library version string. */
+const char* adrt_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ADRT_HIP_H */

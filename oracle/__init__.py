@@ -20,7 +20,9 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "liboracle_adrt.so")
+_LIB_LD_PATH = os.path.join(_HERE, "liboracle_adrt_ld.so")
 _lib = None
+_lib_ld = None
 
 _dp = C.POINTER(C.c_double)
 
@@ -77,8 +79,18 @@ def build() -> str:
     return _LIB_PATH
 
 
-def lib():
-    global _lib
+def lib(extended: bool = False):
+    """The FP64 oracle, or (extended=True) the same code in x87 long double."""
+    global _lib, _lib_ld
+    if extended:
+        if _lib_ld is None:
+            if not os.path.exists(_LIB_LD_PATH):
+                build()
+            L = C.CDLL(_LIB_LD_PATH)
+            L.orc_solve.argtypes = [C.POINTER(_Config), C.POINTER(_Output), C.c_char_p, C.c_int]
+            L.orc_solve_batch.argtypes = [C.POINTER(_Batch), C.c_int, C.c_char_p, C.c_int]
+            _lib_ld = L
+        return _lib_ld
     if _lib is None:
         if not os.path.exists(_LIB_PATH):
             build()
@@ -191,7 +203,7 @@ def solve(num_layers, num_quadrature=8, delta_tau=None, single_scat_albedo=None,
           index_from_bottom=False, compute_temperature_jacobian=False,
           surface_albedo=0.0, surface_emission=0.0, surface_temperature=-1.0,
           top_emission=0.0, top_temperature=-1.0, solar_flux=0.0, solar_mu=1.0,
-          wavenumber_low=0.0, wavenumber_high=0.0):
+          wavenumber_low=0.0, wavenumber_high=0.0, extended=False):
     """One column, mirroring ``adrt::solve`` (src/solver.cpp:348-722).
 
     Returns a dict of numpy arrays with the ``RTOutput`` field names.
@@ -243,7 +255,7 @@ def solve(num_layers, num_quadrature=8, delta_tau=None, single_scat_albedo=None,
         out.jac_mean_intensity = _ptr(jac["mean_intensity"])
         out.jac_flux_divergence = _ptr(jac["flux_divergence"])
     err = C.create_string_buffer(256)
-    rc = lib().orc_solve(C.byref(cfg), C.byref(out), err, 256)
+    rc = lib(extended).orc_solve(C.byref(cfg), C.byref(out), err, 256)
     if rc:
         raise OracleError(rc, err.value.decode())
     if compute_temperature_jacobian:
@@ -258,7 +270,7 @@ def solve_batch(delta_tau, single_scat_albedo, phase_moments, num_quadrature,
                 planck_levels=None, surface_emission=0.0, top_emission=0.0,
                 surface_albedo=0.0, solar_flux=0.0, solar_mu=1.0, use_delta_m=False,
                 use_diffusion_lower_bc=False, index_from_bottom=False,
-                spectrum_scaling=1.0, threads=None):
+                spectrum_scaling=1.0, threads=None, extended=False):
     """Batched columns in wavenumber-major layout, ``[W][L]`` / ``[W][L+1]``.
 
     ``phase_moments`` is ``[L][M]`` (shared) or ``[W][L][M]``;
@@ -292,7 +304,7 @@ def solve_batch(delta_tau, single_scat_albedo, phase_moments, num_quadrature,
     if threads is None:
         threads = min(os.cpu_count() or 1, 16)
     err = C.create_string_buffer(256)
-    rc = lib().orc_solve_batch(C.byref(b), int(threads), err, 256)
+    rc = lib(extended).orc_solve_batch(C.byref(b), int(threads), err, 256)
     if rc:
         raise OracleError(rc, err.value.decode())
     return out

@@ -21,14 +21,27 @@
  */
 #include "adrt_oracle.h"
 
+/* Internal arithmetic type: double for the oracle proper; long double (x87,
+ * 64-bit mantissa) for the extended-precision build that measures how far an
+ * FP64 solve sits from the exact solution of the same discrete problem. */
+#ifdef ORC_LONG_DOUBLE
+typedef long double real_t;
+#define ORC_MAXR LDBL_MAX
+#define ORC_EPSR DBL_EPSILON
+#else
+typedef double real_t;
+#define ORC_MAXR DBL_MAX
+#define ORC_EPSR DBL_EPSILON
+#endif
+
 #include <float.h>
-#include <math.h>
+#include <tgmath.h>
 #include <pthread.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
-static const double PI = 3.14159265358979323846; /* src/constants.hpp:8 */
+static const real_t PI = 3.14159265358979323846264338327950288L; /* src/constants.hpp:8 */
 
 /* ------------------------------------------------------------------------ */
 /* error handling                                                            */
@@ -46,23 +59,23 @@ static int set_err(char* err, int errlen, int code, const char* msg) {
 /* quadrature (src/quadrature.cpp:11-76)                                     */
 /* ------------------------------------------------------------------------ */
 
-void orc_gauss_legendre(int n, double* nodes, double* weights) {
+static void gl_r(int n, real_t* nodes, real_t* weights) {
   for (int i = 0; i < (n + 1) / 2; ++i) {
-    double z = cos(PI * (i + 0.75) / (n + 0.5));
-    double pp = 0.0;
+    real_t z = cos(PI * (i + 0.75) / (n + 0.5));
+    real_t pp = 0.0;
     for (int iter = 0; iter < 100; ++iter) {
-      double p0 = 1.0, p1 = z;
+      real_t p0 = 1.0, p1 = z;
       for (int k = 2; k <= n; ++k) {
-        double pk = ((2 * k - 1) * z * p1 - (k - 1) * p0) / k;
+        real_t pk = ((2 * k - 1) * z * p1 - (k - 1) * p0) / k;
         p0 = p1;
         p1 = pk;
       }
       pp = n * (z * p1 - p0) / (z * z - 1.0);
-      double dz = -p1 / pp;
+      real_t dz = -p1 / pp;
       z += dz;
       if (fabs(dz) < 1e-15) break;
     }
-    double w = 2.0 / ((1.0 - z * z) * pp * pp);
+    real_t w = 2.0 / ((1.0 - z * z) * pp * pp);
     nodes[i] = (1.0 - z) / 2.0;
     nodes[n - 1 - i] = (1.0 + z) / 2.0;
     weights[i] = w / 2.0;
@@ -70,7 +83,7 @@ void orc_gauss_legendre(int n, double* nodes, double* weights) {
   }
 }
 
-void orc_legendre_table(int L, const double* x, int nx, double* Pl) {
+static void legtab_r(int L, const real_t* x, int nx, real_t* Pl) {
   for (int i = 0; i < nx; ++i) Pl[i] = 1.0;
   if (L > 1)
     for (int i = 0; i < nx; ++i) Pl[nx + i] = x[i];
@@ -84,13 +97,13 @@ void orc_legendre_table(int L, const double* x, int nx, double* Pl) {
 /* phase matrices (src/phase_matrix.hpp:61-111, 130-181)                     */
 /* ------------------------------------------------------------------------ */
 
-static void phase_from_table(const double* chi, int L, const double* Pl, int n,
-                             const double* wt, double* Ppp, double* Ppm) {
+static void phase_from_table(const real_t* chi, int L, const real_t* Pl, int n,
+                             const real_t* wt, real_t* Ppp, real_t* Ppm) {
   for (int i = 0; i < n; ++i)
     for (int j = 0; j < n; ++j) {
-      double sum_pp = 0.0, sum_pm = 0.0, sign = 1.0;
+      real_t sum_pp = 0.0, sum_pm = 0.0, sign = 1.0;
       for (int l = 0; l < L; ++l) {
-        double term = (2 * l + 1) * chi[l] * Pl[l * n + i] * Pl[l * n + j];
+        real_t term = (2 * l + 1) * chi[l] * Pl[l * n + i] * Pl[l * n + j];
         sum_pp += term;
         sum_pm += sign * term;
         sign = -sign;
@@ -99,10 +112,10 @@ static void phase_from_table(const double* chi, int L, const double* Pl, int n,
       Ppm[i * n + j] = sum_pm / (2.0 * PI);
     }
   for (int j = 0; j < n; ++j) {
-    double sum = 0.0;
+    real_t sum = 0.0;
     for (int i = 0; i < n; ++i) sum += (Ppp[i * n + j] + Ppm[i * n + j]) * wt[i];
     if (sum > 0.0) {
-      double correction = 1.0 / (2.0 * PI * sum);
+      real_t correction = 1.0 / (2.0 * PI * sum);
       for (int i = 0; i < n; ++i) {
         Ppp[i * n + j] *= correction;
         Ppm[i * n + j] *= correction;
@@ -111,17 +124,17 @@ static void phase_from_table(const double* chi, int L, const double* Pl, int n,
   }
 }
 
-static void solar_from_table(const double* chi, int L, const double* Pl, int n,
-                             const double* wt, double mu0, double* pp, double* pm) {
-  double* Plmu0 = (double*)malloc(sizeof(double) * (size_t)(L > 1 ? L : 2));
+static void solar_from_table(const real_t* chi, int L, const real_t* Pl, int n,
+                             const real_t* wt, real_t mu0, real_t* pp, real_t* pm) {
+  real_t* Plmu0 = (real_t*)malloc(sizeof(real_t) * (size_t)(L > 1 ? L : 2));
   Plmu0[0] = 1.0;
   if (L > 1) Plmu0[1] = mu0;
   for (int l = 2; l < L; ++l)
     Plmu0[l] = ((2 * l - 1) * mu0 * Plmu0[l - 1] - (l - 1) * Plmu0[l - 2]) / l;
   for (int i = 0; i < n; ++i) {
-    double sum_p = 0.0, sum_m = 0.0, sign = 1.0;
+    real_t sum_p = 0.0, sum_m = 0.0, sign = 1.0;
     for (int l = 0; l < L; ++l) {
-      double term = (2 * l + 1) * chi[l] * Pl[l * n + i] * Plmu0[l];
+      real_t term = (2 * l + 1) * chi[l] * Pl[l * n + i] * Plmu0[l];
       sum_p += term;
       sum_m += sign * term;
       sign = -sign;
@@ -129,10 +142,10 @@ static void solar_from_table(const double* chi, int L, const double* Pl, int n,
     pp[i] = sum_p / (2.0 * PI);
     pm[i] = sum_m / (2.0 * PI);
   }
-  double sum = 0.0;
+  real_t sum = 0.0;
   for (int i = 0; i < n; ++i) sum += (pp[i] + pm[i]) * wt[i];
   if (sum > 0.0) {
-    double correction = 1.0 / (2.0 * PI * sum);
+    real_t correction = 1.0 / (2.0 * PI * sum);
     for (int i = 0; i < n; ++i) {
       pp[i] *= correction;
       pm[i] *= correction;
@@ -141,19 +154,19 @@ static void solar_from_table(const double* chi, int L, const double* Pl, int n,
   free(Plmu0);
 }
 
-void orc_phase_matrices(const double* chi, int L, const double* mu, const double* wt,
-                        int n, double* Ppp, double* Ppm) {
-  double* Pl = (double*)malloc(sizeof(double) * (size_t)(L * n + 1));
-  orc_legendre_table(L, mu, n, Pl);
+static void phase_r(const real_t* chi, int L, const real_t* mu, const real_t* wt,
+                        int n, real_t* Ppp, real_t* Ppm) {
+  real_t* Pl = (real_t*)malloc(sizeof(real_t) * (size_t)(L * n + 1));
+  legtab_r(L, mu, n, Pl);
   phase_from_table(chi, L, Pl, n, wt, Ppp, Ppm);
   free(Pl);
 }
 
-void orc_solar_phase_vectors(const double* chi, int L, const double* mu,
-                             const double* wt, int n, double mu0, double* p_plus,
-                             double* p_minus) {
-  double* Pl = (double*)malloc(sizeof(double) * (size_t)(L * n + 1));
-  orc_legendre_table(L, mu, n, Pl);
+static void solar_r(const real_t* chi, int L, const real_t* mu,
+                             const real_t* wt, int n, real_t mu0, real_t* p_plus,
+                             real_t* p_minus) {
+  real_t* Pl = (real_t*)malloc(sizeof(real_t) * (size_t)(L * n + 1));
+  legtab_r(L, mu, n, Pl);
   solar_from_table(chi, L, Pl, n, wt, mu0, p_plus, p_minus);
   free(Pl);
 }
@@ -162,32 +175,32 @@ void orc_solar_phase_vectors(const double* chi, int L, const double* mu,
 /* Planck (src/planck.cpp:13-177)                                            */
 /* ------------------------------------------------------------------------ */
 
-static const double P_C2 = 1.438786, P_SIGMA = 5.67032E-8, P_VCUT = 1.5;
-static const double P_A1 = 1.0 / 3.0, P_A2 = -1.0 / 8.0, P_A3 = 1.0 / 60.0,
+static const real_t P_C2 = 1.438786, P_SIGMA = 5.67032E-8, P_VCUT = 1.5;
+static const real_t P_A1 = 1.0 / 3.0, P_A2 = -1.0 / 8.0, P_A3 = 1.0 / 60.0,
                     P_A4 = -1.0 / 5040.0, P_A5 = 1.0 / 272160.0,
                     P_A6 = -1.0 / 13305600.0;
 
-static double planck_kernel(double v) { return v * v * v / (exp(v) - 1.0); }
+static real_t planck_kernel(real_t v) { return v * v * v / (exp(v) - 1.0); }
 
-int orc_planck(double wnumlo, double wnumhi, double temp, double* out) {
+static int planck_r(real_t wnumlo, real_t wnumhi, real_t temp, real_t* out) {
   if (temp < 0.0 || wnumhi < wnumlo || wnumlo < 0.0) return 16;
   if (temp < 1.e-4) { *out = 0.0; return 0; }
   if (wnumhi == wnumlo) {
-    double arg = exp(-P_C2 * wnumhi / temp);
+    real_t arg = exp(-P_C2 * wnumhi / temp);
     *out = 1.1911e-8 * wnumhi * wnumhi * wnumhi * arg / (1.0 - arg);
     return 0;
   }
-  double vmax = log(DBL_MAX);
-  double sigdpi = P_SIGMA / PI;
-  double conc = 15.0 / pow(PI, 4.0);
-  double v[2] = {P_C2 * wnumlo / temp, P_C2 * wnumhi / temp};
-  if (v[0] > DBL_EPSILON && v[1] < vmax && (wnumhi - wnumlo) / wnumhi < 1.e-2) {
-    double hh = v[1] - v[0];
-    double oldval = 0.0;
-    double val0 = planck_kernel(v[0]) + planck_kernel(v[1]);
+  real_t vmax = log((real_t)DBL_MAX);
+  real_t sigdpi = P_SIGMA / PI;
+  real_t conc = 15.0 / pow(PI, 4.0);
+  real_t v[2] = {P_C2 * wnumlo / temp, P_C2 * wnumhi / temp};
+  if (v[0] > ORC_EPSR && v[1] < vmax && (wnumhi - wnumlo) / wnumhi < 1.e-2) {
+    real_t hh = v[1] - v[0];
+    real_t oldval = 0.0;
+    real_t val0 = planck_kernel(v[0]) + planck_kernel(v[1]);
     for (int n = 1; n <= 10; ++n) {
-      double del = hh / (2.0 * n);
-      double val = val0;
+      real_t del = hh / (2.0 * n);
+      real_t val = val0;
       for (int k = 1; k <= 2 * n - 1; ++k)
         val += 2.0 * (1 + k % 2) * planck_kernel(v[0] + k * del);
       val *= del * P_A1;
@@ -200,30 +213,30 @@ int orc_planck(double wnumlo, double wnumhi, double temp, double* out) {
     *out = sigdpi * pow(temp, 4.0) * conc * oldval;
     return 0;
   }
-  double p[2] = {0.0, 0.0}, d[2] = {0.0, 0.0};
+  real_t p[2] = {0.0, 0.0}, d[2] = {0.0, 0.0};
   int smallv = 0;
-  static const double vcp[7] = {10.25, 5.7, 3.9, 2.9, 2.3, 1.9, 0.0};
+  static const real_t vcp[7] = {10.25, 5.7, 3.9, 2.9, 2.3, 1.9, 0.0};
   for (int i = 0; i < 2; ++i) {
     if (v[i] < P_VCUT) {
       smallv++;
-      double vsq = v[i] * v[i];
+      real_t vsq = v[i] * v[i];
       p[i] = conc * vsq * v[i] *
              (P_A1 + v[i] * (P_A2 + v[i] * (P_A3 + vsq * (P_A4 + vsq * (P_A5 + vsq * P_A6)))));
     } else {
       int mmax = 1;
       while (v[i] < vcp[mmax - 1]) mmax++;
-      double ex = exp(-v[i]);
-      double exm = 1.0;
+      real_t ex = exp(-v[i]);
+      real_t exm = 1.0;
       d[i] = 0.0;
       for (int m = 1; m <= mmax; ++m) {
-        double mv = m * v[i];
+        real_t mv = m * v[i];
         exm *= ex;
         d[i] += exm * (6.0 + mv * (6.0 + mv * (3.0 + mv))) / (m * m * m * m);
       }
       d[i] *= conc;
     }
   }
-  double ans;
+  real_t ans;
   if (smallv == 2) ans = p[1] - p[0];
   else if (smallv == 1) ans = 1.0 - p[0] - d[1];
   else ans = d[0] - d[1];
@@ -232,28 +245,28 @@ int orc_planck(double wnumlo, double wnumhi, double temp, double* out) {
   return 0;
 }
 
-static double planck_fkernel(double v) {
+static real_t planck_fkernel(real_t v) {
   if (v < 1.e-6) return v * v;
-  double em = exp(-v);
+  real_t em = exp(-v);
   return v * v * v * em / (1.0 - em);
 }
 
-int orc_planck_deriv(double wnumlo, double wnumhi, double temp, double* out) {
+static int planckd_r(real_t wnumlo, real_t wnumhi, real_t temp, real_t* out) {
   if (temp < 0.0 || wnumhi < wnumlo || wnumlo < 0.0) return 16;
   if (temp < 1.e-4) { *out = 0.0; return 0; }
   if (wnumhi == wnumlo) {
-    double x = P_C2 * wnumhi / temp;
-    double em = exp(-x);
-    double b = 1.1911e-8 * wnumhi * wnumhi * wnumhi * em / (1.0 - em);
+    real_t x = P_C2 * wnumhi / temp;
+    real_t em = exp(-x);
+    real_t b = 1.1911e-8 * wnumhi * wnumhi * wnumhi * em / (1.0 - em);
     *out = b * (x / temp) / (1.0 - em);
     return 0;
   }
-  double sigdpi = P_SIGMA / PI;
-  double conc = 15.0 / pow(PI, 4.0);
-  double v0 = P_C2 * wnumlo / temp, v1 = P_C2 * wnumhi / temp;
-  double B;
-  orc_planck(wnumlo, wnumhi, temp, &B);
-  double boundary = sigdpi * conc * temp * temp * temp *
+  real_t sigdpi = P_SIGMA / PI;
+  real_t conc = 15.0 / pow(PI, 4.0);
+  real_t v0 = P_C2 * wnumlo / temp, v1 = P_C2 * wnumhi / temp;
+  real_t B;
+  planck_r(wnumlo, wnumhi, temp, &B);
+  real_t boundary = sigdpi * conc * temp * temp * temp *
                     (v1 * planck_fkernel(v1) - v0 * planck_fkernel(v0));
   *out = 4.0 * B / temp - boundary;
   return 0;
@@ -264,18 +277,18 @@ int orc_planck_deriv(double wnumlo, double wnumhi, double temp, double* out) {
 /* ------------------------------------------------------------------------ */
 
 /* C = A*B (n x n, row-major); C must not alias A or B. */
-static void mm(int n, const double* A, const double* B, double* C) {
+static void mm(int n, const real_t* A, const real_t* B, real_t* C) {
   for (int i = 0; i < n; ++i)
     for (int j = 0; j < n; ++j) {
-      double s = 0.0;
+      real_t s = 0.0;
       for (int k = 0; k < n; ++k) s += A[i * n + k] * B[k * n + j];
       C[i * n + j] = s;
     }
 }
 
-static void mv(int n, const double* A, const double* x, double* y) {
+static void mv(int n, const real_t* A, const real_t* x, real_t* y) {
   for (int i = 0; i < n; ++i) {
-    double s = 0.0;
+    real_t s = 0.0;
     for (int k = 0; k < n; ++k) s += A[i * n + k] * x[k];
     y[i] = s;
   }
@@ -283,59 +296,59 @@ static void mv(int n, const double* A, const double* x, double* y) {
 
 /* In-place LU with partial (row) pivoting: pivot = first largest |a_ik|,
  * multipliers formed by division, as Eigen's unblocked PartialPivLU. */
-static void lu_factor(int n, double* A, int* perm) {
+static void lu_factor(int n, real_t* A, int* perm) {
   for (int i = 0; i < n; ++i) perm[i] = i;
   for (int k = 0; k < n; ++k) {
     int p = k;
-    double big = fabs(A[k * n + k]);
+    real_t big = fabs(A[k * n + k]);
     for (int i = k + 1; i < n; ++i) {
-      double v = fabs(A[i * n + k]);
+      real_t v = fabs(A[i * n + k]);
       if (v > big) { big = v; p = i; }
     }
     if (big != 0.0) {
       if (p != k) {
         for (int j = 0; j < n; ++j) {
-          double t = A[k * n + j]; A[k * n + j] = A[p * n + j]; A[p * n + j] = t;
+          real_t t = A[k * n + j]; A[k * n + j] = A[p * n + j]; A[p * n + j] = t;
         }
         int t = perm[k]; perm[k] = perm[p]; perm[p] = t;
       }
-      double piv = A[k * n + k];
+      real_t piv = A[k * n + k];
       for (int i = k + 1; i < n; ++i) A[i * n + k] /= piv;
     }
     for (int i = k + 1; i < n; ++i) {
-      double lik = A[i * n + k];
+      real_t lik = A[i * n + k];
       for (int j = k + 1; j < n; ++j) A[i * n + j] -= lik * A[k * n + j];
     }
   }
 }
 
 /* Solve (P^T L U) x = b given lu_factor output; x may alias nothing. */
-static void lu_solve(int n, const double* LU, const int* perm, const double* b, double* x) {
+static void lu_solve(int n, const real_t* LU, const int* perm, const real_t* b, real_t* x) {
   for (int i = 0; i < n; ++i) x[i] = b[perm[i]];
   for (int i = 0; i < n; ++i) {
-    double s = x[i];
+    real_t s = x[i];
     for (int k = 0; k < i; ++k) s -= LU[i * n + k] * x[k];
     x[i] = s;
   }
   for (int i = n - 1; i >= 0; --i) {
-    double s = x[i];
+    real_t s = x[i];
     for (int k = i + 1; k < n; ++k) s -= LU[i * n + k] * x[k];
     x[i] = s / LU[i * n + i];
   }
 }
 
 /* Solve A x = b (src/matrix.hpp:82-84). */
-static void solve_vec(int n, const double* A, const double* b, double* x, double* work,
+static void solve_vec(int n, const real_t* A, const real_t* b, real_t* x, real_t* work,
                       int* perm) {
-  memcpy(work, A, sizeof(double) * (size_t)(n * n));
+  memcpy(work, A, sizeof(real_t) * (size_t)(n * n));
   lu_factor(n, work, perm);
   lu_solve(n, work, perm, b, x);
 }
 
 /* X = B A^-1 via LU(A^T), solving A^T x_r = b_r for each row r of B
  * (src/matrix.hpp:120-124).  X must not alias A or B. */
-static void right_solve(int n, const double* A, const double* B, double* X, double* work,
-                        int* perm, double* tmp) {
+static void right_solve(int n, const real_t* A, const real_t* B, real_t* X, real_t* work,
+                        int* perm, real_t* tmp) {
   for (int i = 0; i < n; ++i)
     for (int j = 0; j < n; ++j) work[i * n + j] = A[j * n + i];
   lu_factor(n, work, perm);
@@ -348,14 +361,14 @@ static void right_solve(int n, const double* A, const double* B, double* X, doub
 /* ------------------------------------------------------------------------ */
 
 typedef struct {
-  double *Rab, *Rba, *Tab, *Tba;
-  double *su, *sd, *sus, *sds, *jp, *jq;
+  real_t *Rab, *Rba, *Tab, *Tba;
+  real_t *su, *sd, *sus, *sds, *jp, *jq;
   int scat;
 } layer_t;
 
 static size_t layer_doubles(int n) { return (size_t)(4 * n * n + 6 * n); }
 
-static void layer_bind(layer_t* L, double* mem, int n) {
+static void layer_bind(layer_t* L, real_t* mem, int n) {
   L->Rab = mem;
   L->Rba = mem + n * n;
   L->Tab = mem + 2 * n * n;
@@ -371,7 +384,7 @@ static void layer_bind(layer_t* L, double* mem, int n) {
 
 /* default LayerMatrices: R = 0, T = I, sources 0, not scattering */
 static void layer_default(layer_t* L, int n) {
-  memset(L->Rab, 0, sizeof(double) * layer_doubles(n));
+  memset(L->Rab, 0, sizeof(real_t) * layer_doubles(n));
   for (int i = 0; i < n; ++i) {
     L->Tab[i * n + i] = 1.0;
     L->Tba[i * n + i] = 1.0;
@@ -383,13 +396,13 @@ static void layer_default(layer_t* L, int n) {
 /* doubling (src/doubling.hpp:20-230)                                        */
 /* ------------------------------------------------------------------------ */
 
-static int ipow0(double omega) {           /* src/doubling.hpp:20-26 */
+static int ipow0(real_t omega) {           /* src/doubling.hpp:20-26 */
   if (omega < 0.01) return 4;
   if (omega < 0.1) return 10;
   return 16;
 }
 
-int orc_doubling_count(double tau, double omega) { /* src/doubling.hpp:81,107-109 */
+static int nn_r(real_t tau, real_t omega) { /* src/doubling.hpp:81,107-109 */
   if (omega < 0.0) omega = 0.0;
   if (omega > 1.0) omega = 1.0;
   int nn = (int)(log(tau) / log(2.0)) + ipow0(omega);
@@ -397,26 +410,26 @@ int orc_doubling_count(double tau, double omega) { /* src/doubling.hpp:81,107-10
   return nn;
 }
 
-static void doubling_layer(int n, double tau, double omega, double B_top, double B_bottom,
-                           const double* Ppp, const double* Ppm, const double* mu,
-                           const double* wt, double solar_flux, double solar_mu,
-                           double tau_cumulative, const double* pps, const double* pms,
+static void doubling_layer(int n, real_t tau, real_t omega, real_t B_top, real_t B_bottom,
+                           const real_t* Ppp, const real_t* Ppm, const real_t* mu,
+                           const real_t* wt, real_t solar_flux, real_t solar_mu,
+                           real_t tau_cumulative, const real_t* pps, const real_t* pms,
                            layer_t* out) {
   layer_default(out, n);
-  double B_bar = (B_bottom + B_top) / 2.0;
-  double B_d = (tau > 0.0) ? (B_bottom - B_top) / tau : 0.0;
+  real_t B_bar = (B_bottom + B_top) / 2.0;
+  real_t B_d = (tau > 0.0) ? (B_bottom - B_top) / tau : 0.0;
   if (tau <= 0.0) return;
 
   if (omega <= 0.0) { /* pure absorption, src/doubling.hpp:56-77 */
-    memset(out->Tab, 0, sizeof(double) * (size_t)(n * n));
-    memset(out->Tba, 0, sizeof(double) * (size_t)(n * n));
+    memset(out->Tab, 0, sizeof(real_t) * (size_t)(n * n));
+    memset(out->Tba, 0, sizeof(real_t) * (size_t)(n * n));
     for (int i = 0; i < n; ++i) {
-      double tex = -tau / mu[i];
-      double trans = (tex > -200.0) ? exp(tex) : 0.0;
+      real_t tex = -tau / mu[i];
+      real_t trans = (tex > -200.0) ? exp(tex) : 0.0;
       out->Tab[i * n + i] = trans;
       out->Tba[i * n + i] = trans;
-      double one_minus_t = 1.0 - trans;
-      double slope_term = mu[i] * one_minus_t - 0.5 * tau * (1.0 + trans);
+      real_t one_minus_t = 1.0 - trans;
+      real_t slope_term = mu[i] * one_minus_t - 0.5 * tau * (1.0 + trans);
       out->su[i] = B_bar * one_minus_t + B_d * slope_term;
       out->sd[i] = B_bar * one_minus_t - B_d * slope_term;
       out->jp[i] = 0.5 * one_minus_t - slope_term / tau;
@@ -428,36 +441,36 @@ static void doubling_layer(int n, double tau, double omega, double B_top, double
   out->scat = 1;
   if (omega < 0.0) omega = 0.0;
   if (omega > 1.0) omega = 1.0;
-  double con = 2.0 * omega * PI;
+  real_t con = 2.0 * omega * PI;
 
   size_t nn2 = (size_t)(n * n);
-  double* buf = (double*)malloc(sizeof(double) * (nn2 * 10 + (size_t)n * 16));
-  double *Gpp = buf, *Gpm = buf + nn2, *R = buf + 2 * nn2, *T = buf + 3 * nn2;
-  double *Rsq = buf + 4 * nn2, *TG = buf + 5 * nn2, *TGR = buf + 6 * nn2;
-  double *Rn = buf + 7 * nn2, *Tn = buf + 8 * nn2, *work = buf + 9 * nn2;
-  double* v = buf + 10 * nn2;
-  double *y = v, *z = v + n, *sus = v + 2 * n, *sds = v + 3 * n, *zpgy = v + 4 * n;
-  double *a1 = v + 5 * n, *a2 = v + 6 * n, *a3 = v + 7 * n, *a4 = v + 8 * n;
-  double *rup = v + 9 * n, *rdn = v + 10 * n, *ynew = v + 11 * n, *znew = v + 12 * n;
-  double *susn = v + 13 * n, *sdsn = v + 14 * n;
+  real_t* buf = (real_t*)malloc(sizeof(real_t) * (nn2 * 10 + (size_t)n * 16));
+  real_t *Gpp = buf, *Gpm = buf + nn2, *R = buf + 2 * nn2, *T = buf + 3 * nn2;
+  real_t *Rsq = buf + 4 * nn2, *TG = buf + 5 * nn2, *TGR = buf + 6 * nn2;
+  real_t *Rn = buf + 7 * nn2, *Tn = buf + 8 * nn2, *work = buf + 9 * nn2;
+  real_t* v = buf + 10 * nn2;
+  real_t *y = v, *z = v + n, *sus = v + 2 * n, *sds = v + 3 * n, *zpgy = v + 4 * n;
+  real_t *a1 = v + 5 * n, *a2 = v + 6 * n, *a3 = v + 7 * n, *a4 = v + 8 * n;
+  real_t *rup = v + 9 * n, *rdn = v + 10 * n, *ynew = v + 11 * n, *znew = v + 12 * n;
+  real_t *susn = v + 13 * n, *sdsn = v + 14 * n;
   int perm[ORC_MAX_N];
 
   for (int i = 0; i < n; ++i)
     for (int j = 0; j < n; ++j) {
-      double pppc = Ppp[i * n + j] * wt[j];
-      double temp = ((i == j) ? 1.0 : 0.0) + (-con) * pppc;
+      real_t pppc = Ppp[i * n + j] * wt[j];
+      real_t temp = ((i == j) ? 1.0 : 0.0) + (-con) * pppc;
       Gpp[i * n + j] = temp / mu[i];
-      double ppmc = Ppm[i * n + j] * wt[j];
+      real_t ppmc = Ppm[i * n + j] * wt[j];
       Gpm[i * n + j] = con * ppmc / mu[i];
     }
 
   int nn = (int)(log(tau) / log(2.0)) + ipow0(omega);
   if (nn < 1) nn = 1;
-  double xfac = 1.0 / pow(2.0, nn);
-  double tau0 = tau * xfac;
+  real_t xfac = 1.0 / pow(2.0, nn);
+  real_t tau0 = tau * xfac;
 
   int has_solar = (solar_flux > 0.0 && solar_mu > 0.0 && pps != NULL && pms != NULL);
-  double F_top = has_solar ? solar_flux * exp(-tau_cumulative / solar_mu) : 0.0;
+  real_t F_top = has_solar ? solar_flux * exp(-tau_cumulative / solar_mu) : 0.0;
 
   for (int i = 0; i < n; ++i)
     for (int j = 0; j < n; ++j) {
@@ -472,12 +485,12 @@ static void doubling_layer(int n, double tau, double omega, double B_top, double
   }
   if (has_solar)
     for (int i = 0; i < n; ++i) {
-      double base = omega * tau0 / mu[i] * F_top;
+      real_t base = omega * tau0 / mu[i] * F_top;
       sus[i] = base * pms[i];
       sds[i] = base * pps[i];
     }
-  double g = 0.5 * tau0;
-  double gamma = has_solar ? exp(-tau0 / solar_mu) : 0.0;
+  real_t g = 0.5 * tau0;
+  real_t gamma = has_solar ? exp(-tau0 / solar_mu) : 0.0;
 
   for (int k = 0; k < nn; ++k) {
     mm(n, R, R, Rsq);
@@ -512,20 +525,20 @@ static void doubling_layer(int n, double tau, double omega, double B_top, double
         sdsn[i] = a4[i] + gamma * sds[i];
       }
       gamma = gamma * gamma;
-      memcpy(sus, susn, sizeof(double) * (size_t)n);
-      memcpy(sds, sdsn, sizeof(double) * (size_t)n);
+      memcpy(sus, susn, sizeof(real_t) * (size_t)n);
+      memcpy(sds, sdsn, sizeof(real_t) * (size_t)n);
     }
-    memcpy(R, Rn, sizeof(double) * nn2);
-    memcpy(T, Tn, sizeof(double) * nn2);
-    memcpy(y, ynew, sizeof(double) * (size_t)n);
-    memcpy(z, znew, sizeof(double) * (size_t)n);
+    memcpy(R, Rn, sizeof(real_t) * nn2);
+    memcpy(T, Tn, sizeof(real_t) * nn2);
+    memcpy(y, ynew, sizeof(real_t) * (size_t)n);
+    memcpy(z, znew, sizeof(real_t) * (size_t)n);
     g = 2.0 * g;
   }
 
-  memcpy(out->Rab, R, sizeof(double) * nn2);
-  memcpy(out->Rba, R, sizeof(double) * nn2);
-  memcpy(out->Tab, T, sizeof(double) * nn2);
-  memcpy(out->Tba, T, sizeof(double) * nn2);
+  memcpy(out->Rab, R, sizeof(real_t) * nn2);
+  memcpy(out->Rba, R, sizeof(real_t) * nn2);
+  memcpy(out->Tab, T, sizeof(real_t) * nn2);
+  memcpy(out->Tba, T, sizeof(real_t) * nn2);
   for (int i = 0; i < n; ++i) {
     out->su[i] = y[i] * B_bar + z[i] * B_d;
     out->sd[i] = y[i] * B_bar - z[i] * B_d;
@@ -537,26 +550,26 @@ static void doubling_layer(int n, double tau, double omega, double B_top, double
   free(buf);
 }
 
-int orc_doubling(int n, double tau, double omega, double B_top, double B_bottom,
-                 const double* Ppp, const double* Ppm, const double* mu, const double* wt,
-                 double solar_flux, double solar_mu, double tau_cumulative,
-                 const double* p_plus_solar, const double* p_minus_solar, double* R,
-                 double* T, double* s_up, double* s_down, double* s_up_solar,
-                 double* s_down_solar, double* j_p, double* j_q) {
-  double* mem = (double*)malloc(sizeof(double) * layer_doubles(n));
+static int doubling_r(int n, real_t tau, real_t omega, real_t B_top, real_t B_bottom,
+                 const real_t* Ppp, const real_t* Ppm, const real_t* mu, const real_t* wt,
+                 real_t solar_flux, real_t solar_mu, real_t tau_cumulative,
+                 const real_t* p_plus_solar, const real_t* p_minus_solar, real_t* R,
+                 real_t* T, real_t* s_up, real_t* s_down, real_t* s_up_solar,
+                 real_t* s_down_solar, real_t* j_p, real_t* j_q) {
+  real_t* mem = (real_t*)malloc(sizeof(real_t) * layer_doubles(n));
   layer_t L;
   layer_bind(&L, mem, n);
   doubling_layer(n, tau, omega, B_top, B_bottom, Ppp, Ppm, mu, wt, solar_flux, solar_mu,
                  tau_cumulative, p_plus_solar, p_minus_solar, &L);
   size_t nn2 = (size_t)(n * n);
-  memcpy(R, L.Rab, sizeof(double) * nn2);
-  memcpy(T, L.Tab, sizeof(double) * nn2);
-  memcpy(s_up, L.su, sizeof(double) * (size_t)n);
-  memcpy(s_down, L.sd, sizeof(double) * (size_t)n);
-  memcpy(s_up_solar, L.sus, sizeof(double) * (size_t)n);
-  memcpy(s_down_solar, L.sds, sizeof(double) * (size_t)n);
-  memcpy(j_p, L.jp, sizeof(double) * (size_t)n);
-  memcpy(j_q, L.jq, sizeof(double) * (size_t)n);
+  memcpy(R, L.Rab, sizeof(real_t) * nn2);
+  memcpy(T, L.Tab, sizeof(real_t) * nn2);
+  memcpy(s_up, L.su, sizeof(real_t) * (size_t)n);
+  memcpy(s_down, L.sd, sizeof(real_t) * (size_t)n);
+  memcpy(s_up_solar, L.sus, sizeof(real_t) * (size_t)n);
+  memcpy(s_down_solar, L.sds, sizeof(real_t) * (size_t)n);
+  memcpy(j_p, L.jp, sizeof(real_t) * (size_t)n);
+  memcpy(j_q, L.jq, sizeof(real_t) * (size_t)n);
   int s = L.scat;
   free(mem);
   return s;
@@ -569,18 +582,18 @@ int orc_doubling(int n, double tau, double omega, double B_top, double B_bottom,
 /* LU factors captured for the temperature Jacobian (src/adding.hpp:19-25). */
 typedef struct {
   int general;
-  double* lu1; /* of A1 = I - bot.R_ab top.R_ba */
-  double* lu2; /* of A2 = I - top.R_ba bot.R_ab */
+  real_t* lu1; /* of A1 = I - bot.R_ab top.R_ba */
+  real_t* lu2; /* of A2 = I - top.R_ba bot.R_ab */
   int p1[ORC_MAX_N], p2[ORC_MAX_N];
 } addops_t;
 
 typedef struct {
-  double *t1, *t2, *t3, *t4, *t5, *t6, *work, *v1, *v2, *v3;
+  real_t *t1, *t2, *t3, *t4, *t5, *t6, *work, *v1, *v2, *v3;
 } scratch_t;
 
-static void scratch_alloc(scratch_t* s, int n, double** block) {
+static void scratch_alloc(scratch_t* s, int n, real_t** block) {
   size_t nn2 = (size_t)(n * n);
-  *block = (double*)malloc(sizeof(double) * (nn2 * 7 + (size_t)n * 3));
+  *block = (real_t*)malloc(sizeof(real_t) * (nn2 * 7 + (size_t)n * 3));
   s->t1 = *block;
   s->t2 = s->t1 + nn2;
   s->t3 = s->t2 + nn2;
@@ -595,14 +608,14 @@ static void scratch_alloc(scratch_t* s, int n, double** block) {
 
 /* src/adding.hpp:31-62 */
 static void add_sources(int n, layer_t* ans, const layer_t* top, const layer_t* bot,
-                        const double* TbaD1, const double* TbcD2, scratch_t* s) {
+                        const real_t* TbaD1, const real_t* TbcD2, scratch_t* s) {
   for (int pass = 0; pass < 2; ++pass) {
-    const double* s1u = pass ? top->sus : top->su;
-    const double* s1d = pass ? top->sds : top->sd;
-    const double* s2u = pass ? bot->sus : bot->su;
-    const double* s2d = pass ? bot->sds : bot->sd;
-    double* au = pass ? ans->sus : ans->su;
-    double* ad = pass ? ans->sds : ans->sd;
+    const real_t* s1u = pass ? top->sus : top->su;
+    const real_t* s1d = pass ? top->sds : top->sd;
+    const real_t* s2u = pass ? bot->sus : bot->su;
+    const real_t* s2d = pass ? bot->sds : bot->sd;
+    real_t* au = pass ? ans->sus : ans->su;
+    real_t* ad = pass ? ans->sds : ans->sd;
     mv(n, bot->Rab, s1d, s->v1);
     for (int i = 0; i < n; ++i) s->v2[i] = s2u[i] + s->v1[i];
     mv(n, TbaD1, s->v2, s->v3);
@@ -619,7 +632,7 @@ static void add_general(int n, const layer_t* top, const layer_t* bot, layer_t* 
                         addops_t* ops, scratch_t* s) {
   size_t nn2 = (size_t)(n * n);
   int perm[ORC_MAX_N];
-  double *A1 = s->t1, *A2 = s->t2, *TbaD1 = s->t3, *TbcD2 = s->t4;
+  real_t *A1 = s->t1, *A2 = s->t2, *TbaD1 = s->t3, *TbcD2 = s->t4;
   mm(n, bot->Rab, top->Rba, A1);
   mm(n, top->Rba, bot->Rab, A2);
   for (size_t e = 0; e < nn2; ++e) { A1[e] = -A1[e]; A2[e] = -A2[e]; }
@@ -628,14 +641,14 @@ static void add_general(int n, const layer_t* top, const layer_t* bot, layer_t* 
     /* T_ba_D1 = top.T_ba * inv(A1) with the LU of A1 kept for the Jacobian
      * (src/adding.hpp:318-325). */
     ops->general = 1;
-    memcpy(ops->lu1, A1, sizeof(double) * nn2);
-    memcpy(ops->lu2, A2, sizeof(double) * nn2);
+    memcpy(ops->lu1, A1, sizeof(real_t) * nn2);
+    memcpy(ops->lu2, A2, sizeof(real_t) * nn2);
     lu_factor(n, ops->lu1, ops->p1);
     lu_factor(n, ops->lu2, ops->p2);
-    double* inv = s->t5;
-    double* e = s->v1;
+    real_t* inv = s->t5;
+    real_t* e = s->v1;
     for (int pass = 0; pass < 2; ++pass) {
-      const double* LU = pass ? ops->lu2 : ops->lu1;
+      const real_t* LU = pass ? ops->lu2 : ops->lu1;
       const int* P = pass ? ops->p2 : ops->p1;
       for (int c = 0; c < n; ++c) {
         for (int i = 0; i < n; ++i) e[i] = (i == c) ? 1.0 : 0.0;
@@ -664,7 +677,7 @@ static void add_general(int n, const layer_t* top, const layer_t* bot, layer_t* 
 static void add_nonscat_top(int n, const layer_t* top, const layer_t* bot, layer_t* ans,
                             addops_t* ops, scratch_t* s) {
   if (ops) ops->general = 0;
-  double* t = s->v3;
+  real_t* t = s->v3;
   for (int i = 0; i < n; ++i) t[i] = top->Tab[i * n + i];
   for (int i = 0; i < n; ++i)
     for (int j = 0; j < n; ++j) {
@@ -673,14 +686,14 @@ static void add_nonscat_top(int n, const layer_t* top, const layer_t* bot, layer
       ans->Tab[i * n + j] = bot->Tab[i * n + j] * t[j];
       ans->Tba[i * n + j] = t[i] * bot->Tba[i * n + j];
     }
-  double* tmp = s->t6; /* vector scratch */
+  real_t* tmp = s->t6; /* vector scratch */
   for (int pass = 0; pass < 2; ++pass) {
-    const double* s1u = pass ? top->sus : top->su;
-    const double* s1d = pass ? top->sds : top->sd;
-    const double* s2u = pass ? bot->sus : bot->su;
-    const double* s2d = pass ? bot->sds : bot->sd;
-    double* au = pass ? ans->sus : ans->su;
-    double* ad = pass ? ans->sds : ans->sd;
+    const real_t* s1u = pass ? top->sus : top->su;
+    const real_t* s1d = pass ? top->sds : top->sd;
+    const real_t* s2u = pass ? bot->sus : bot->su;
+    const real_t* s2d = pass ? bot->sds : bot->sd;
+    real_t* au = pass ? ans->sus : ans->su;
+    real_t* ad = pass ? ans->sds : ans->sd;
     mv(n, bot->Rab, s1d, s->v1);
     for (int i = 0; i < n; ++i) au[i] = s1u[i] + t[i] * (s2u[i] + s->v1[i]);
     mv(n, bot->Tab, s1d, tmp);
@@ -693,7 +706,7 @@ static void add_nonscat_top(int n, const layer_t* top, const layer_t* bot, layer
 static void add_nonscat_bot(int n, const layer_t* top, const layer_t* bot, layer_t* ans,
                             addops_t* ops, scratch_t* s) {
   if (ops) ops->general = 0;
-  double* t = s->v3;
+  real_t* t = s->v3;
   for (int i = 0; i < n; ++i) t[i] = bot->Tab[i * n + i];
   for (int i = 0; i < n; ++i)
     for (int j = 0; j < n; ++j) {
@@ -702,14 +715,14 @@ static void add_nonscat_bot(int n, const layer_t* top, const layer_t* bot, layer
       ans->Tab[i * n + j] = t[i] * top->Tab[i * n + j];
       ans->Tba[i * n + j] = top->Tba[i * n + j] * t[j];
     }
-  double* tmp = s->t6;
+  real_t* tmp = s->t6;
   for (int pass = 0; pass < 2; ++pass) {
-    const double* s1u = pass ? top->sus : top->su;
-    const double* s1d = pass ? top->sds : top->sd;
-    const double* s2u = pass ? bot->sus : bot->su;
-    const double* s2d = pass ? bot->sds : bot->sd;
-    double* au = pass ? ans->sus : ans->su;
-    double* ad = pass ? ans->sds : ans->sd;
+    const real_t* s1u = pass ? top->sus : top->su;
+    const real_t* s1d = pass ? top->sds : top->sd;
+    const real_t* s2u = pass ? bot->sus : bot->su;
+    const real_t* s2d = pass ? bot->sds : bot->sd;
+    real_t* au = pass ? ans->sus : ans->su;
+    real_t* ad = pass ? ans->sds : ans->sd;
     mv(n, top->Tba, s2u, tmp);
     for (int i = 0; i < n; ++i) au[i] = s1u[i] + tmp[i];
     mv(n, top->Rba, s2u, s->v1);
@@ -721,7 +734,7 @@ static void add_nonscat_bot(int n, const layer_t* top, const layer_t* bot, layer
 /* src/adding.hpp:246-258 */
 static void add_layers(int n, const layer_t* top, const layer_t* bot, layer_t* ans,
                        addops_t* ops, scratch_t* s) {
-  memset(ans->su, 0, sizeof(double) * (size_t)(6 * n));
+  memset(ans->su, 0, sizeof(real_t) * (size_t)(6 * n));
   if (!top->scat) add_nonscat_top(n, top, bot, ans, ops, s);
   else if (!bot->scat) add_nonscat_bot(n, top, bot, ans, ops, s);
   else add_general(n, top, bot, ans, ops, s);
@@ -731,19 +744,19 @@ static void add_layers(int n, const layer_t* top, const layer_t* bot, layer_t* a
 /* interface intensities (src/solver.cpp:24-73)                              */
 /* ------------------------------------------------------------------------ */
 
-static void interface_up(int n, const layer_t* top, const layer_t* base, const double* Itd,
-                         const double* Ibu, double* out, double* lu_keep, int* perm_keep,
+static void interface_up(int n, const layer_t* top, const layer_t* base, const real_t* Itd,
+                         const real_t* Ibu, real_t* out, real_t* lu_keep, int* perm_keep,
                          scratch_t* s) {
-  double* M = s->t1;
+  real_t* M = s->t1;
   mm(n, base->Rab, top->Rba, M);
   for (int e = 0; e < n * n; ++e) M[e] = -M[e];
   for (int i = 0; i < n; ++i) M[i * n + i] += 1.0;
-  double* term1 = s->t2;        /* vectors held in matrix scratch */
-  double* TtopI = s->t2 + n;
-  double* term2 = s->t2 + 2 * n;
-  double* sdt = s->t2 + 3 * n;
-  double* term3 = s->t2 + 4 * n;
-  double* rhs = s->t2 + 5 * n;
+  real_t* term1 = s->t2;        /* vectors held in matrix scratch */
+  real_t* TtopI = s->t2 + n;
+  real_t* term2 = s->t2 + 2 * n;
+  real_t* sdt = s->t2 + 3 * n;
+  real_t* term3 = s->t2 + 4 * n;
+  real_t* rhs = s->t2 + 5 * n;
   mv(n, base->Tba, Ibu, term1);
   mv(n, top->Tab, Itd, TtopI);
   mv(n, base->Rab, TtopI, term2);
@@ -752,7 +765,7 @@ static void interface_up(int n, const layer_t* top, const layer_t* base, const d
   for (int i = 0; i < n; ++i)
     rhs[i] = term1[i] + term2[i] + term3[i] + base->su[i] + base->sus[i];
   if (lu_keep) {
-    memcpy(lu_keep, M, sizeof(double) * (size_t)(n * n));
+    memcpy(lu_keep, M, sizeof(real_t) * (size_t)(n * n));
     lu_factor(n, lu_keep, perm_keep);
     lu_solve(n, lu_keep, perm_keep, rhs, out);
   } else {
@@ -762,18 +775,18 @@ static void interface_up(int n, const layer_t* top, const layer_t* base, const d
 }
 
 static void interface_down(int n, const layer_t* top, const layer_t* base,
-                           const double* Itd, const double* Ibu, double* out,
-                           double* lu_keep, int* perm_keep, scratch_t* s) {
-  double* M = s->t1;
+                           const real_t* Itd, const real_t* Ibu, real_t* out,
+                           real_t* lu_keep, int* perm_keep, scratch_t* s) {
+  real_t* M = s->t1;
   mm(n, top->Rba, base->Rab, M);
   for (int e = 0; e < n * n; ++e) M[e] = -M[e];
   for (int i = 0; i < n; ++i) M[i * n + i] += 1.0;
-  double* term1 = s->t2;
-  double* TbI = s->t2 + n;
-  double* term2 = s->t2 + 2 * n;
-  double* sut = s->t2 + 3 * n;
-  double* term3 = s->t2 + 4 * n;
-  double* rhs = s->t2 + 5 * n;
+  real_t* term1 = s->t2;
+  real_t* TbI = s->t2 + n;
+  real_t* term2 = s->t2 + 2 * n;
+  real_t* sut = s->t2 + 3 * n;
+  real_t* term3 = s->t2 + 4 * n;
+  real_t* rhs = s->t2 + 5 * n;
   mv(n, top->Tab, Itd, term1);
   mv(n, base->Tba, Ibu, TbI);
   mv(n, top->Rba, TbI, term2);
@@ -782,7 +795,7 @@ static void interface_down(int n, const layer_t* top, const layer_t* base,
   for (int i = 0; i < n; ++i)
     rhs[i] = term1[i] + term2[i] + term3[i] + top->sd[i] + top->sds[i];
   if (lu_keep) {
-    memcpy(lu_keep, M, sizeof(double) * (size_t)(n * n));
+    memcpy(lu_keep, M, sizeof(real_t) * (size_t)(n * n));
     lu_factor(n, lu_keep, perm_keep);
     lu_solve(n, lu_keep, perm_keep, rhs, out);
   } else {
@@ -854,15 +867,15 @@ static int validate(const orc_config* c, char* err, int errlen) {
 
 typedef struct {
   int n, nlay, ltot, has_surface;
-  double *mu, *wt;
-  double *B, *tau_used;
+  real_t *mu, *wt;
+  real_t *B, *tau_used;
   layer_t *layers, *rbase, *rtop; /* layers[ltot], rbase/rtop[ltot+1] */
-  double *mem_layers, *mem_rbase, *mem_rtop;
+  real_t *mem_layers, *mem_rbase, *mem_rtop;
   addops_t *rbase_ops, *rtop_ops;
-  double *iface_up_lu, *iface_dn_lu; /* [(nlay+1)][n*n] */
+  real_t *iface_up_lu, *iface_dn_lu; /* [(nlay+1)][n*n] */
   int *iface_up_p, *iface_dn_p;      /* [(nlay+1)][ORC_MAX_N] */
   char* iface_has;
-  double* ops_mem;
+  real_t* ops_mem;
 } column_t;
 
 static void column_free(column_t* c) {
@@ -882,8 +895,8 @@ static void column_free(column_t* c) {
   free(c->ops_mem);
 }
 
-static void temperature_jacobian(const orc_config* cfg, column_t* C, const double* omega_in,
-                                 orc_output* out);
+static void temperature_jacobian(const orc_config* cfg, column_t* C, const real_t* omega,
+                                 const real_t* temp, int have_plev, orc_output* out);
 
 int orc_solve(const orc_config* cfg_in, orc_output* out, char* err, int errlen) {
   int rc = validate(cfg_in, err, errlen);
@@ -895,73 +908,69 @@ int orc_solve(const orc_config* cfg_in, orc_output* out, char* err, int errlen) 
   const size_t nn2 = (size_t)(n * n);
 
   /* index_from_bottom reversal (src/solver.cpp:361-372) */
-  double* tau = (double*)malloc(sizeof(double) * (size_t)nlay);
-  double* omg = (double*)malloc(sizeof(double) * (size_t)nlay);
-  double* mom = (double*)malloc(sizeof(double) * (size_t)(nlay * M));
-  double* temp = NULL;
-  double* plev = NULL;
+  real_t* tau = (real_t*)malloc(sizeof(real_t) * (size_t)nlay);
+  real_t* omg = (real_t*)malloc(sizeof(real_t) * (size_t)nlay);
+  real_t* mom = (real_t*)malloc(sizeof(real_t) * (size_t)(nlay * M));
+  real_t* temp = NULL;
+  real_t* plev = NULL;
   for (int l = 0; l < nlay; ++l) {
     int src = rev ? nlay - 1 - l : l;
     tau[l] = cfg_in->delta_tau[src];
     omg[l] = cfg_in->single_scat_albedo[src];
-    memcpy(mom + (size_t)l * M, cfg_in->phase_moments + (size_t)src * M,
-           sizeof(double) * (size_t)M);
+    for (int m = 0; m < M; ++m)
+      mom[(size_t)l * M + m] = cfg_in->phase_moments[(size_t)src * M + m];
   }
   if (cfg_in->use_thermal_emission) {
-    temp = (double*)malloc(sizeof(double) * (size_t)(nlay + 1));
+    temp = (real_t*)malloc(sizeof(real_t) * (size_t)(nlay + 1));
     for (int l = 0; l <= nlay; ++l) temp[l] = cfg_in->temperature[rev ? nlay - l : l];
   }
   if (cfg_in->planck_levels) {
-    plev = (double*)malloc(sizeof(double) * (size_t)(nlay + 1));
+    plev = (real_t*)malloc(sizeof(real_t) * (size_t)(nlay + 1));
     for (int l = 0; l <= nlay; ++l) plev[l] = cfg_in->planck_levels[rev ? nlay - l : l];
   }
-  orc_config cfgv = *cfg_in;
-  cfgv.delta_tau = tau;
-  cfgv.single_scat_albedo = omg;
-  cfgv.phase_moments = mom;
-  cfgv.temperature = temp;
-  cfgv.planck_levels = plev;
-  const orc_config* cfg = &cfgv;
+  /* scalars and flags from the caller; the (reversed) arrays are tau, omg,
+   * mom, temp and plev above, in the internal arithmetic type */
+  const orc_config* cfg = cfg_in;
 
   column_t C;
   memset(&C, 0, sizeof C);
   C.n = n;
   C.nlay = nlay;
-  C.mu = (double*)malloc(sizeof(double) * (size_t)(2 * n));
+  C.mu = (real_t*)malloc(sizeof(real_t) * (size_t)(2 * n));
   C.wt = C.mu + n;
-  orc_gauss_legendre(n, C.mu, C.wt);
-  const double* mu = C.mu;
-  const double* wt = C.wt;
-  double xfac_sum = 0.0;
+  gl_r(n, C.mu, C.wt);
+  const real_t* mu = C.mu;
+  const real_t* wt = C.wt;
+  real_t xfac_sum = 0.0;
   for (int i = 0; i < n; ++i) xfac_sum += mu[i] * wt[i];
-  double xfac = 0.5 / xfac_sum;
+  real_t xfac = 0.5 / xfac_sum;
 
   /* Planck values (src/solver.cpp:385-419) */
-  C.B = (double*)calloc((size_t)(2 * nlay + 1), sizeof(double));
+  C.B = (real_t*)calloc((size_t)(2 * nlay + 1), sizeof(real_t));
   C.tau_used = C.B + nlay + 1;
-  double* B = C.B;
-  double B_surface = 0.0, B_top_emission = 0.0;
+  real_t* B = C.B;
+  real_t B_surface = 0.0, B_top_emission = 0.0;
   if (cfg->use_thermal_emission) {
     for (int l = 0; l <= nlay; ++l) {
-      rc = orc_planck(cfg->wavenumber_low, cfg->wavenumber_high, cfg->temperature[l], &B[l]);
+      rc = planck_r(cfg->wavenumber_low, cfg->wavenumber_high, temp[l], &B[l]);
       if (rc) goto planck_fail;
     }
     if (cfg->surface_temperature >= 0.0) {
-      rc = orc_planck(cfg->wavenumber_low, cfg->wavenumber_high, cfg->surface_temperature,
+      rc = planck_r(cfg->wavenumber_low, cfg->wavenumber_high, cfg->surface_temperature,
                       &B_surface);
       if (rc) goto planck_fail;
     } else {
       B_surface = B[nlay];
     }
     if (cfg->top_temperature >= 0.0) {
-      rc = orc_planck(cfg->wavenumber_low, cfg->wavenumber_high, cfg->top_temperature,
+      rc = planck_r(cfg->wavenumber_low, cfg->wavenumber_high, cfg->top_temperature,
                       &B_top_emission);
       if (rc) goto planck_fail;
     } else {
       B_top_emission = B[0];
     }
-  } else if (cfg->planck_levels) {
-    for (int l = 0; l <= nlay; ++l) B[l] = cfg->planck_levels[l];
+  } else if (plev) {
+    for (int l = 0; l <= nlay; ++l) B[l] = plev[l];
     B_surface = cfg->surface_emission;
     B_top_emission = cfg->top_emission;
   } else {
@@ -979,9 +988,9 @@ int orc_solve(const orc_config* cfg_in, orc_output* out, char* err, int errlen) 
     C.ltot = ltot;
     C.has_surface = has_surface;
     size_t ld = layer_doubles(n);
-    C.mem_layers = (double*)malloc(sizeof(double) * ld * (size_t)ltot);
-    C.mem_rbase = (double*)malloc(sizeof(double) * ld * (size_t)(ltot + 1));
-    C.mem_rtop = (double*)malloc(sizeof(double) * ld * (size_t)(ltot + 1));
+    C.mem_layers = (real_t*)malloc(sizeof(real_t) * ld * (size_t)ltot);
+    C.mem_rbase = (real_t*)malloc(sizeof(real_t) * ld * (size_t)(ltot + 1));
+    C.mem_rtop = (real_t*)malloc(sizeof(real_t) * ld * (size_t)(ltot + 1));
     C.layers = (layer_t*)malloc(sizeof(layer_t) * (size_t)ltot);
     C.rbase = (layer_t*)malloc(sizeof(layer_t) * (size_t)(ltot + 1));
     C.rtop = (layer_t*)malloc(sizeof(layer_t) * (size_t)(ltot + 1));
@@ -993,29 +1002,29 @@ int orc_solve(const orc_config* cfg_in, orc_output* out, char* err, int errlen) 
       layer_default(&C.rtop[l], n);
     }
 
-    double* Ppp = (double*)malloc(sizeof(double) * (nn2 * 2 + (size_t)(4 * n)));
-    double* Ppm = Ppp + nn2;
-    double* pps = Ppm + nn2;
-    double* pms = pps + n;
-    double* chis = (double*)malloc(sizeof(double) * (size_t)(M + twoM + 1));
-    double* Pl = (double*)malloc(sizeof(double) * (size_t)((M + twoM + 1) * n));
-    double tau_cumulative = 0.0;
+    real_t* Ppp = (real_t*)malloc(sizeof(real_t) * (nn2 * 2 + (size_t)(4 * n)));
+    real_t* Ppm = Ppp + nn2;
+    real_t* pps = Ppm + nn2;
+    real_t* pms = pps + n;
+    real_t* chis = (real_t*)malloc(sizeof(real_t) * (size_t)(M + twoM + 1));
+    real_t* Pl = (real_t*)malloc(sizeof(real_t) * (size_t)((M + twoM + 1) * n));
+    real_t tau_cumulative = 0.0;
     for (int l = 0; l < nlay; ++l) {
-      double tau_layer = cfg->delta_tau[l];
-      double omega_layer = cfg->single_scat_albedo[l];
-      const double* chi_full = cfg->phase_moments + (size_t)l * M;
-      memset(Ppp, 0, sizeof(double) * (nn2 * 2 + (size_t)(4 * n)));
+      real_t tau_layer = tau[l];
+      real_t omega_layer = omg[l];
+      const real_t* chi_full = mom + (size_t)l * M;
+      memset(Ppp, 0, sizeof(real_t) * (nn2 * 2 + (size_t)(4 * n)));
       int has_solar_phase = 0;
       if (omega_layer > 0.0 && tau_layer > 0.0) {
-        const double* chi = chi_full;
+        const real_t* chi = chi_full;
         int Lm = M;
         if (cfg->use_delta_m) {
-          double f = (M > twoM) ? chi_full[twoM] : 0.0;
+          real_t f = (M > twoM) ? chi_full[twoM] : 0.0;
           if (f > 1e-12 && f < 1.0 - 1e-12) {
-            double omega_f = omega_layer * f;
-            tau_layer = (1.0 - omega_f) * cfg->delta_tau[l];
+            real_t omega_f = omega_layer * f;
+            tau_layer = (1.0 - omega_f) * tau[l];
             omega_layer = omega_layer * (1.0 - f) / (1.0 - omega_f);
-            double inv_1mf = 1.0 / (1.0 - f);
+            real_t inv_1mf = 1.0 / (1.0 - f);
             for (int ll = 0; ll < twoM; ++ll) chis[ll] = (chi_full[ll] - f) * inv_1mf;
             chi = chis;
             Lm = twoM;
@@ -1023,7 +1032,7 @@ int orc_solve(const orc_config* cfg_in, orc_output* out, char* err, int errlen) 
             Lm = (M < twoM) ? M : twoM;
           }
         }
-        orc_legendre_table(Lm, mu, n, Pl);
+        legtab_r(Lm, mu, n, Pl);
         phase_from_table(chi, Lm, Pl, n, wt, Ppp, Ppm);
         if (has_solar) {
           solar_from_table(chi, Lm, Pl, n, wt, cfg->solar_mu, pps, pms);
@@ -1044,11 +1053,11 @@ int orc_solve(const orc_config* cfg_in, orc_output* out, char* err, int errlen) 
     /* Lambertian surface layer (src/solver.cpp:516-551) */
     if (has_surface) {
       layer_t* s = &C.layers[nlay];
-      memset(s->Rab, 0, sizeof(double) * ld);
-      double A = cfg->surface_albedo;
+      memset(s->Rab, 0, sizeof(real_t) * ld);
+      real_t A = cfg->surface_albedo;
       for (int i = 0; i < n; ++i) {
         for (int j = 0; j < n; ++j) {
-          double r = 2.0 * A * mu[j] * wt[j] * xfac;
+          real_t r = 2.0 * A * mu[j] * wt[j] * xfac;
           s->Rab[i * n + j] = r;
           s->Rba[i * n + j] = r;
         }
@@ -1062,18 +1071,18 @@ int orc_solve(const orc_config* cfg_in, orc_output* out, char* err, int errlen) 
     }
 
     int want_jac = cfg->compute_temperature_jacobian &&
-                   (cfg->use_thermal_emission || cfg->planck_levels);
+                   (cfg->use_thermal_emission || plev != NULL);
     if (want_jac) {
       C.rbase_ops = (addops_t*)calloc((size_t)ltot, sizeof(addops_t));
       C.rtop_ops = (addops_t*)calloc((size_t)ltot, sizeof(addops_t));
-      C.ops_mem = (double*)malloc(sizeof(double) * nn2 * (size_t)(4 * ltot));
+      C.ops_mem = (real_t*)malloc(sizeof(real_t) * nn2 * (size_t)(4 * ltot));
       for (int l = 0; l < ltot; ++l) {
         C.rbase_ops[l].lu1 = C.ops_mem + nn2 * (size_t)(4 * l);
         C.rbase_ops[l].lu2 = C.rbase_ops[l].lu1 + nn2;
         C.rtop_ops[l].lu1 = C.rbase_ops[l].lu2 + nn2;
         C.rtop_ops[l].lu2 = C.rtop_ops[l].lu1 + nn2;
       }
-      C.iface_up_lu = (double*)malloc(sizeof(double) * nn2 * (size_t)(2 * (nlay + 1)));
+      C.iface_up_lu = (real_t*)malloc(sizeof(real_t) * nn2 * (size_t)(2 * (nlay + 1)));
       C.iface_dn_lu = C.iface_up_lu + nn2 * (size_t)(nlay + 1);
       C.iface_up_p = (int*)malloc(sizeof(int) * (size_t)(2 * (nlay + 1) * ORC_MAX_N));
       C.iface_dn_p = C.iface_up_p + (nlay + 1) * ORC_MAX_N;
@@ -1081,11 +1090,11 @@ int orc_solve(const orc_config* cfg_in, orc_output* out, char* err, int errlen) 
     }
 
     scratch_t S;
-    double* sblock;
+    real_t* sblock;
     scratch_alloc(&S, n, &sblock);
 
     /* composites from the bottom (src/solver.cpp:560-572) */
-    memcpy(C.rbase[1].Rab, C.layers[ltot - 1].Rab, sizeof(double) * ld);
+    memcpy(C.rbase[1].Rab, C.layers[ltot - 1].Rab, sizeof(real_t) * ld);
     C.rbase[1].scat = C.layers[ltot - 1].scat;
     for (int l = 1; l < ltot; ++l) {
       int k = ltot - 1 - l;
@@ -1093,20 +1102,20 @@ int orc_solve(const orc_config* cfg_in, orc_output* out, char* err, int errlen) 
                  want_jac ? &C.rbase_ops[l] : NULL, &S);
     }
     /* composites from the top (src/solver.cpp:574-583) */
-    memcpy(C.rtop[1].Rab, C.layers[0].Rab, sizeof(double) * ld);
+    memcpy(C.rtop[1].Rab, C.layers[0].Rab, sizeof(real_t) * ld);
     C.rtop[1].scat = C.layers[0].scat;
     for (int l = 1; l < ltot; ++l)
       add_layers(n, &C.rtop[l], &C.layers[l], &C.rtop[l + 1],
                  want_jac ? &C.rtop_ops[l] : NULL, &S);
 
     /* boundary intensities (src/solver.cpp:585-603) */
-    double Itd[ORC_MAX_N], Ibu[ORC_MAX_N];
+    real_t Itd[ORC_MAX_N], Ibu[ORC_MAX_N];
     for (int i = 0; i < n; ++i) Itd[i] = B_top_emission;
     for (int i = 0; i < n; ++i) Ibu[i] = 0.0;
     if (cfg->use_diffusion_lower_bc) {
-      double B_bottom = B[nlay];
-      double dtau_last = C.tau_used[nlay - 1];
-      double dB_dtau = (dtau_last > 0.0) ? (B_bottom - B[nlay - 1]) / dtau_last : 0.0;
+      real_t B_bottom = B[nlay];
+      real_t dtau_last = C.tau_used[nlay - 1];
+      real_t dB_dtau = (dtau_last > 0.0) ? (B_bottom - B[nlay - 1]) / dtau_last : 0.0;
       for (int i = 0; i < n; ++i) Ibu[i] = B_bottom + mu[i] * dB_dtau;
     } else if (!has_surface) {
       for (int i = 0; i < n; ++i) Ibu[i] = B_surface;
@@ -1114,18 +1123,18 @@ int orc_solve(const orc_config* cfg_in, orc_output* out, char* err, int errlen) 
 
     /* outputs (src/solver.cpp:605-700) */
     int nif = nlay + 1;
-    double* res = (double*)calloc((size_t)(5 * nif), sizeof(double));
-    double *fu = res, *fd = res + nif, *mi = res + 2 * nif, *fdiv = res + 3 * nif,
+    real_t* res = (real_t*)calloc((size_t)(5 * nif), sizeof(real_t));
+    real_t *fu = res, *fd = res + nif, *mi = res + 2 * nif, *fdiv = res + 3 * nif,
            *fdir = res + 4 * nif;
     if (cfg->solar_flux > 0.0 && cfg->solar_mu > 0.0) {
-      double tau_cum = 0.0;
+      real_t tau_cum = 0.0;
       fdir[0] = cfg->solar_flux * cfg->solar_mu;
       for (int l = 0; l < nlay; ++l) {
         tau_cum += C.tau_used[l];
         fdir[l + 1] = cfg->solar_flux * cfg->solar_mu * exp(-tau_cum / cfg->solar_mu);
       }
     }
-    double Iup[ORC_MAX_N], Idn[ORC_MAX_N], tmpv[ORC_MAX_N];
+    real_t Iup[ORC_MAX_N], Idn[ORC_MAX_N], tmpv[ORC_MAX_N];
     {
       layer_t* full = &C.rbase[ltot];
       mv(n, full->Rab, Itd, Iup);
@@ -1163,21 +1172,21 @@ int orc_solve(const orc_config* cfg_in, orc_output* out, char* err, int errlen) 
       }
     }
     if (cfg->solar_flux > 0.0 && cfg->solar_mu > 0.0) {
-      double inv_4pi_mu0 = 1.0 / (4.0 * PI * cfg->solar_mu);
+      real_t inv_4pi_mu0 = 1.0 / (4.0 * PI * cfg->solar_mu);
       for (int l = 0; l < nif; ++l) mi[l] += fdir[l] * inv_4pi_mu0;
     }
     for (int l = 0; l < nif; ++l) {
       int lyr = (l == 0) ? 0 : l - 1;
-      fdiv[l] = 4.0 * PI * (1.0 - cfg->single_scat_albedo[lyr]) * (mi[l] - B[l]);
+      fdiv[l] = 4.0 * PI * (1.0 - omg[lyr]) * (mi[l] - B[l]);
     }
 
-    if (want_jac) temperature_jacobian(cfg, &C, cfg->single_scat_albedo, out);
+    if (want_jac) temperature_jacobian(cfg, &C, omg, temp, plev != NULL, out);
 
     double* dst[5] = {out->flux_up, out->flux_down, out->mean_intensity,
                       out->flux_divergence, out->flux_direct};
     for (int a = 0; a < 5; ++a)
       if (dst[a])
-        for (int l = 0; l < nif; ++l) dst[a][l] = res[a * nif + (rev ? nif - 1 - l : l)];
+        for (int l = 0; l < nif; ++l) dst[a][l] = (double)res[a * nif + (rev ? nif - 1 - l : l)];
     free(res);
     free(sblock);
   }
@@ -1204,35 +1213,35 @@ planck_fail:
 /* ------------------------------------------------------------------------ */
 
 /* blocks are [n][ndof] row-major */
-static void temperature_jacobian(const orc_config* cfg, column_t* C, const double* omega,
-                                 orc_output* out) {
+static void temperature_jacobian(const orc_config* cfg, column_t* C, const real_t* omega,
+                                 const real_t* temp, int have_plev, orc_output* out) {
   const int n = C->n, nlay = C->nlay, ltot = C->ltot, has_surface = C->has_surface;
   const int thermal = cfg->use_thermal_emission;
   const int nif = nlay + 1, ndof = nlay + 2, surf_dof = nlay + 1;
-  const double A = cfg->surface_albedo;
-  const double *mu = C->mu, *wt = C->wt;
+  const real_t A = cfg->surface_albedo;
+  const real_t *mu = C->mu, *wt = C->wt;
   const int surface_independent =
-      has_surface && (thermal ? (cfg->surface_temperature >= 0.0) : (cfg->planck_levels != NULL));
+      has_surface && (thermal ? (cfg->surface_temperature >= 0.0) : have_plev);
   const int surf_col = surface_independent ? surf_dof : nlay;
   const size_t bs = (size_t)(n * ndof);
 
-  double* mem = (double*)calloc(bs * (size_t)(4 * (ltot + 1) + 12), sizeof(double));
-  double* dRbu = mem;
-  double* dRbd = dRbu + bs * (ltot + 1);
-  double* dRtu = dRbd + bs * (ltot + 1);
-  double* dRtd = dRtu + bs * (ltot + 1);
-  double* su = dRtd + bs * (ltot + 1);
-  double* sd = su + bs;
-  double* tU = sd + bs;
-  double* tD = tU + bs;
-  double* dItd = tD + bs;
-  double* dIbu = dItd + bs;
-  double* dIup = dIbu + bs;
-  double* dIdn = dIup + bs;
-  double* tmp1 = dIdn + bs;
-  double* tmp2 = tmp1 + bs;
-  double* colv = tmp2 + bs;       /* n */
-  double* colx = colv + n;        /* n */
+  real_t* mem = (real_t*)calloc(bs * (size_t)(4 * (ltot + 1) + 12), sizeof(real_t));
+  real_t* dRbu = mem;
+  real_t* dRbd = dRbu + bs * (ltot + 1);
+  real_t* dRtu = dRbd + bs * (ltot + 1);
+  real_t* dRtd = dRtu + bs * (ltot + 1);
+  real_t* su = dRtd + bs * (ltot + 1);
+  real_t* sd = su + bs;
+  real_t* tU = sd + bs;
+  real_t* tD = tU + bs;
+  real_t* dItd = tD + bs;
+  real_t* dIbu = dItd + bs;
+  real_t* dIup = dIbu + bs;
+  real_t* dIdn = dIup + bs;
+  real_t* tmp1 = dIdn + bs;
+  real_t* tmp2 = tmp1 + bs;
+  real_t* colv = tmp2 + bs;       /* n */
+  real_t* colx = colv + n;        /* n */
   (void)colx;
 
 #define BLK(P, l) ((P) + bs * (size_t)(l))
@@ -1240,8 +1249,8 @@ static void temperature_jacobian(const orc_config* cfg, column_t* C, const doubl
   /* seedLayer (src/solver.cpp:150-165) */
 #define SEED(j, dU, dD)                                                   \
   do {                                                                    \
-    memset(dU, 0, sizeof(double) * bs);                                   \
-    memset(dD, 0, sizeof(double) * bs);                                   \
+    memset(dU, 0, sizeof(real_t) * bs);                                   \
+    memset(dD, 0, sizeof(real_t) * bs);                                   \
     if (has_surface && (j) == nlay) {                                     \
       for (int i = 0; i < n; ++i) (dU)[i * ndof + surf_col] = (1.0 - A);  \
     } else {                                                              \
@@ -1258,12 +1267,12 @@ static void temperature_jacobian(const orc_config* cfg, column_t* C, const doubl
   /* Y = M X for M [n][n], X [n][ndof] */
 #define MB(Mat, X, Y)                                                     \
   do {                                                                    \
-    const double* mb_a_ = (Mat);                                          \
-    const double* mb_x_ = (X);                                            \
-    double* mb_y_ = (Y);                                                  \
+    const real_t* mb_a_ = (Mat);                                          \
+    const real_t* mb_x_ = (X);                                            \
+    real_t* mb_y_ = (Y);                                                  \
     for (int mb_i_ = 0; mb_i_ < n; ++mb_i_)                               \
       for (int mb_m_ = 0; mb_m_ < ndof; ++mb_m_) {                        \
-        double mb_s_ = 0.0;                                               \
+        real_t mb_s_ = 0.0;                                               \
         for (int mb_k_ = 0; mb_k_ < n; ++mb_k_)                           \
           mb_s_ += mb_a_[mb_i_ * n + mb_k_] * mb_x_[mb_k_ * ndof + mb_m_]; \
         mb_y_[mb_i_ * ndof + mb_m_] = mb_s_;                              \
@@ -1275,7 +1284,7 @@ static void temperature_jacobian(const orc_config* cfg, column_t* C, const doubl
   do {                                                                    \
     for (int m = 0; m < ndof; ++m) {                                      \
       for (int i = 0; i < n; ++i) colv[i] = (X)[i * ndof + m];            \
-      double xs[ORC_MAX_N];                                               \
+      real_t xs[ORC_MAX_N];                                               \
       lu_solve(n, (LU), (P), colv, xs);                                   \
       for (int i = 0; i < n; ++i) (X)[i * ndof + m] = xs[i];              \
     }                                                                     \
@@ -1318,7 +1327,7 @@ static void temperature_jacobian(const orc_config* cfg, column_t* C, const doubl
   if (thermal && cfg->top_temperature < 0.0)
     for (int i = 0; i < n; ++i) dItd[i * ndof + 0] = 1.0;
   if (!has_surface && cfg->use_diffusion_lower_bc) {
-    double dtau_last = C->tau_used[nlay - 1];
+    real_t dtau_last = C->tau_used[nlay - 1];
     if (dtau_last > 0.0) {
       for (int i = 0; i < n; ++i) {
         dIbu[i * ndof + nlay] = 1.0 + mu[i] / dtau_last;
@@ -1330,13 +1339,13 @@ static void temperature_jacobian(const orc_config* cfg, column_t* C, const doubl
   }
 
   size_t jsz = (size_t)(nif * ndof);
-  double* J = (double*)calloc(jsz * 4, sizeof(double));
-  double *Ju = J, *Jd = J + jsz, *Jm = J + 2 * jsz, *Jdv = J + 3 * jsz;
+  real_t* J = (real_t*)calloc(jsz * 4, sizeof(real_t));
+  real_t *Ju = J, *Jd = J + jsz, *Jm = J + 2 * jsz, *Jdv = J + 3 * jsz;
 
 #define REDUCE(k, U, D)                                                   \
   do {                                                                    \
     for (int m = 0; m < ndof; ++m) {                                      \
-      double fu = 0.0, fd = 0.0, jm = 0.0;                                \
+      real_t fu = 0.0, fd = 0.0, jm = 0.0;                                \
       for (int i = 0; i < n; ++i) {                                       \
         fu += 2.0 * PI * wt[i] * mu[i] * (U)[i * ndof + m];               \
         fd += 2.0 * PI * wt[i] * mu[i] * (D)[i * ndof + m];               \
@@ -1352,7 +1361,7 @@ static void temperature_jacobian(const orc_config* cfg, column_t* C, const doubl
     layer_t* full = &C->rbase[ltot];
     MB(full->Rab, dItd, dIup);
     MB(full->Tba, dIbu, tmp1);
-    double* dr = BLK(dRbu, ltot);
+    real_t* dr = BLK(dRbu, ltot);
     for (size_t e = 0; e < bs; ++e) dIup[e] = dIup[e] + tmp1[e] + dr[e];
     REDUCE(0, dIup, dItd);
   }
@@ -1366,7 +1375,7 @@ static void temperature_jacobian(const orc_config* cfg, column_t* C, const doubl
       MB(base->Rab, tmp1, tmp2);
       for (size_t e = 0; e < bs; ++e) dIup[e] += tmp2[e];
       MB(base->Rab, BLK(dRtd, n_top), tmp2);
-      double* drb = BLK(dRbu, n_base);
+      real_t* drb = BLK(dRbu, n_base);
       for (size_t e = 0; e < bs; ++e) dIup[e] += tmp2[e] + drb[e];
       LUB(C->iface_up_lu + (size_t)(n * n) * l, C->iface_up_p + l * ORC_MAX_N, dIup);
 
@@ -1375,16 +1384,16 @@ static void temperature_jacobian(const orc_config* cfg, column_t* C, const doubl
       MB(top->Rba, tmp1, tmp2);
       for (size_t e = 0; e < bs; ++e) dIdn[e] += tmp2[e];
       MB(top->Rba, BLK(dRbu, n_base), tmp2);
-      double* drt = BLK(dRtd, n_top);
+      real_t* drt = BLK(dRtd, n_top);
       for (size_t e = 0; e < bs; ++e) dIdn[e] += tmp2[e] + drt[e];
       LUB(C->iface_dn_lu + (size_t)(n * n) * l, C->iface_dn_p + l * ORC_MAX_N, dIdn);
     } else {
       layer_t* top = &C->rtop[n_top];
       MB(top->Tab, dItd, dIdn);
       MB(top->Rba, dIbu, tmp1);
-      double* drt = BLK(dRtd, n_top);
+      real_t* drt = BLK(dRtd, n_top);
       for (size_t e = 0; e < bs; ++e) dIdn[e] += tmp1[e] + drt[e];
-      memcpy(dIup, dIbu, sizeof(double) * bs);
+      memcpy(dIup, dIbu, sizeof(real_t) * bs);
     }
     REDUCE(l, dIup, dIdn);
   }
@@ -1392,20 +1401,20 @@ static void temperature_jacobian(const orc_config* cfg, column_t* C, const doubl
   /* flux-divergence Jacobian (src/solver.cpp:297-305) */
   for (int k = 0; k < nif; ++k) {
     int lyr = (k == 0) ? 0 : k - 1;
-    double pref = 4.0 * PI * (1.0 - omega[lyr]);
+    real_t pref = 4.0 * PI * (1.0 - omega[lyr]);
     for (int m = 0; m < ndof; ++m)
       Jdv[k * ndof + m] = pref * (Jm[k * ndof + m] - ((m == k) ? 1.0 : 0.0));
   }
 
   /* Planck chain rule (src/solver.cpp:307-327) */
-  double* chain = (double*)malloc(sizeof(double) * (size_t)ndof);
+  real_t* chain = (real_t*)malloc(sizeof(real_t) * (size_t)ndof);
   for (int m = 0; m < ndof; ++m) chain[m] = 1.0;
   if (thermal) {
     for (int m = 0; m <= nlay; ++m)
-      orc_planck_deriv(cfg->wavenumber_low, cfg->wavenumber_high, cfg->temperature[m],
+      planckd_r(cfg->wavenumber_low, cfg->wavenumber_high, temp[m],
                        &chain[m]);
     if (surface_independent)
-      orc_planck_deriv(cfg->wavenumber_low, cfg->wavenumber_high, cfg->surface_temperature,
+      planckd_r(cfg->wavenumber_low, cfg->wavenumber_high, cfg->surface_temperature,
                        &chain[surf_dof]);
     else
       chain[surf_dof] = 0.0;
@@ -1427,7 +1436,7 @@ static void temperature_jacobian(const orc_config* cfg, column_t* C, const doubl
       int ks = rev ? nif - 1 - k : k;
       for (int m = 0; m < ndof; ++m) {
         int ms = (rev && m <= nlay) ? nlay - m : m;
-        dst[a][k * ndof + m] = J[a * jsz + ks * ndof + ms];
+        dst[a][k * ndof + m] = (double)J[a * jsz + ks * ndof + ms];
       }
     }
   }
@@ -1519,4 +1528,95 @@ int orc_solve_batch(const orc_batch* b, int num_threads, char* err, int errlen) 
   free(jobs);
   free(th);
   return rc;
+}
+
+/* ------------------------------------------------------------------------ */
+/* exported building blocks (double interface)                               */
+/* ------------------------------------------------------------------------ */
+
+static real_t* to_r(const double* x, int n) {
+  if (!x) return NULL;
+  real_t* r = (real_t*)malloc(sizeof(real_t) * (size_t)(n > 0 ? n : 1));
+  for (int i = 0; i < n; ++i) r[i] = x[i];
+  return r;
+}
+
+static void from_r(double* dst, const real_t* src, int n) {
+  for (int i = 0; i < n; ++i) dst[i] = (double)src[i];
+}
+
+void orc_gauss_legendre(int n, double* nodes, double* weights) {
+  real_t* a = to_r(nodes, n);
+  real_t* b = to_r(weights, n);
+  gl_r(n, a, b);
+  from_r(nodes, a, n);
+  from_r(weights, b, n);
+  free(a);
+  free(b);
+}
+
+void orc_legendre_table(int L, const double* x, int nx, double* Pl) {
+  real_t* xr = to_r(x, nx);
+  real_t* P = (real_t*)malloc(sizeof(real_t) * (size_t)(L * nx + 1));
+  legtab_r(L, xr, nx, P);
+  from_r(Pl, P, L * nx);
+  free(xr);
+  free(P);
+}
+
+void orc_phase_matrices(const double* chi, int L, const double* mu, const double* wt, int n,
+                        double* Ppp, double* Ppm) {
+  real_t *c = to_r(chi, L), *m = to_r(mu, n), *w = to_r(wt, n);
+  real_t* A = (real_t*)malloc(sizeof(real_t) * (size_t)(2 * n * n));
+  phase_r(c, L, m, w, n, A, A + n * n);
+  from_r(Ppp, A, n * n);
+  from_r(Ppm, A + n * n, n * n);
+  free(c); free(m); free(w); free(A);
+}
+
+void orc_solar_phase_vectors(const double* chi, int L, const double* mu, const double* wt,
+                             int n, double mu0, double* p_plus, double* p_minus) {
+  real_t *c = to_r(chi, L), *m = to_r(mu, n), *w = to_r(wt, n);
+  real_t* v = (real_t*)malloc(sizeof(real_t) * (size_t)(2 * n));
+  solar_r(c, L, m, w, n, mu0, v, v + n);
+  from_r(p_plus, v, n);
+  from_r(p_minus, v + n, n);
+  free(c); free(m); free(w); free(v);
+}
+
+int orc_planck(double lo, double hi, double T, double* out) {
+  real_t r = 0;
+  int rc = planck_r(lo, hi, T, &r);
+  *out = (double)r;
+  return rc;
+}
+
+int orc_planck_deriv(double lo, double hi, double T, double* out) {
+  real_t r = 0;
+  int rc = planckd_r(lo, hi, T, &r);
+  *out = (double)r;
+  return rc;
+}
+
+int orc_doubling_count(double tau, double omega) { return nn_r(tau, omega); }
+
+int orc_doubling(int n, double tau, double omega, double B_top, double B_bottom,
+                 const double* Ppp, const double* Ppm, const double* mu, const double* wt,
+                 double solar_flux, double solar_mu, double tau_cumulative,
+                 const double* p_plus_solar, const double* p_minus_solar, double* R, double* T,
+                 double* s_up, double* s_down, double* s_up_solar, double* s_down_solar,
+                 double* j_p, double* j_q) {
+  real_t *pp = to_r(Ppp, n * n), *pm = to_r(Ppm, n * n), *m = to_r(mu, n), *w = to_r(wt, n);
+  real_t *ps = to_r(p_plus_solar, n), *ms = to_r(p_minus_solar, n);
+  real_t* o = (real_t*)malloc(sizeof(real_t) * (size_t)(2 * n * n + 6 * n));
+  int scat = doubling_r(n, tau, omega, B_top, B_bottom, pp, pm, m, w, solar_flux, solar_mu,
+                        tau_cumulative, ps, ms, o, o + n * n, o + 2 * n * n, o + 2 * n * n + n,
+                        o + 2 * n * n + 2 * n, o + 2 * n * n + 3 * n, o + 2 * n * n + 4 * n,
+                        o + 2 * n * n + 5 * n);
+  from_r(R, o, n * n);
+  from_r(T, o + n * n, n * n);
+  double* vs[6] = {s_up, s_down, s_up_solar, s_down_solar, j_p, j_q};
+  for (int k = 0; k < 6; ++k) from_r(vs[k], o + 2 * n * n + k * n, n);
+  free(pp); free(pm); free(m); free(w); free(ps); free(ms); free(o);
+  return scat;
 }

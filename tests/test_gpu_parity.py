@@ -1,0 +1,220 @@
+"""Plain FP64 HIP path vs the CPU oracle, through the C-ABI.
+
+Two references run on the same inputs: the FP64 oracle (a restatement of the
+reference solver) and the same oracle code in x87 long double (64-bit
+mantissa), which resolves the exact solution of the discrete problem to far
+below FP64 rounding.  On thick, near-conservative layers the FP64 reference
+algorithm itself carries up to ~5e-9 relative rounding error (measured against
+the extended solve), so the bar is accuracy class, per wavenumber w and output:
+
+    err_gpu(w) <= max(1e-10, 4 * err_fp64_oracle(w))
+
+with err = max over interfaces of |x - x_ext| / |x_ext| (an absolute floor of
+1e-12 x max|column| covers exact zeros).  flux_divergence = 4 pi (1-omega)(J-B)
+is judged on the scale of its terms, 4 pi (1-omega)(|J| + |B|), because J ~ B
+cancels.  The GPU-vs-FP64-oracle difference is printed alongside.
+"""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+import oracle  # noqa: E402
+from addingdoublingrt_amd import BatchConfig, solve_batch, workloads  # noqa: E402
+from addingdoublingrt_amd.solver import planck_levels  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+OUTS = ("flux_up", "flux_down", "mean_intensity", "flux_direct", "flux_divergence")
+RTOL = 1e-10
+CLASS_FACTOR = 4.0
+ATOL_REL = 1e-12
+
+
+def _col_err(x, ref, scale):
+    d = np.abs(x - ref)
+    return np.max(d / scale, axis=1)
+
+
+def _scale(k, ext, ssa_up=None, B=None):
+    if k != "flux_divergence":
+        colmax = np.abs(ext).max(axis=1, keepdims=True)
+        return np.maximum(np.abs(ext), ATOL_REL * np.where(colmax > 0, colmax, 1.0))
+    return None
+
+
+def compare(gpu, ref, ext, label, div_scale):
+    worst_g, worst_vs_orc = 0.0, 0.0
+    for k in OUTS:
+        g = gpu[k].detach().cpu().numpy() if hasattr(gpu[k], "cpu") else gpu[k]
+        r, e = ref[k], ext[k]
+        sc = div_scale if k == "flux_divergence" else _scale(k, e)
+        sc = np.where(sc > 0, sc, 1.0)
+        eg = _col_err(g, e, sc)
+        eo = _col_err(r, e, sc)
+        bound = np.maximum(RTOL, CLASS_FACTOR * eo)
+        worst_g = max(worst_g, float(eg.max()))
+        worst_vs_orc = max(worst_vs_orc, float(_col_err(g, r, sc).max()))
+        bad = np.where(eg > bound)[0]
+        if len(bad):
+            w = bad[0]
+            l = int(np.argmax(np.abs(g[w] - e[w]) / sc[w]))
+            raise AssertionError(
+                f"{label}: {k} w={w} l={l}: gpu err {eg[w]:.2e} > bound {bound[w]:.2e} "
+                f"(fp64 oracle err {eo[w]:.2e}); gpu={g[w, l]!r} ext={e[w, l]!r} "
+                f"orc={r[w, l]!r}; {len(bad)} columns")
+    print(f"{label}: gpu err vs extended {worst_g:.2e}; gpu vs fp64 oracle {worst_vs_orc:.2e}")
+    return worst_g
+
+
+def run_both(tau, ssa, mom, cfg: BatchConfig, B=None, Bs=None, Bt=None, check=True, label="",
+             **kw):
+    dev = "cuda"
+    t = lambda a: None if a is None else torch.as_tensor(np.ascontiguousarray(a), dtype=torch.float64,
+                                                        device=dev)
+    out = solve_batch(t(tau), t(ssa), t(mom), cfg, planck_levels=t(B), surface_emission=t(Bs),
+                      top_emission=t(Bt), **kw)
+    torch.cuda.synchronize()
+    okw = dict(planck_levels=B, surface_emission=cfg.surface_emission if Bs is None else Bs,
+               top_emission=cfg.top_emission if Bt is None else Bt,
+               surface_albedo=cfg.surface_albedo, solar_flux=cfg.solar_flux,
+               solar_mu=cfg.solar_mu, use_delta_m=cfg.use_delta_m,
+               use_diffusion_lower_bc=cfg.use_diffusion_lower_bc,
+               index_from_bottom=cfg.index_from_bottom, spectrum_scaling=cfg.spectrum_scaling)
+    ref = oracle.solve_batch(tau, ssa, mom, cfg.num_quadrature, **okw)
+    ext = oracle.solve_batch(tau, ssa, mom, cfg.num_quadrature, extended=True, **okw)
+    if check:
+        W, L = tau.shape
+        om = ssa[:, ::-1] if cfg.index_from_bottom else ssa
+        lyr = np.concatenate([[0], np.arange(L)])
+        pref = 4 * np.pi * (1.0 - om[:, lyr])
+        if cfg.index_from_bottom:
+            pref = pref[:, ::-1]
+        Bl = np.zeros((W, L + 1)) if B is None else B
+        div_scale = pref * (np.abs(ext["mean_intensity"]) + np.abs(Bl))
+        compare(out, ref, ext, label, div_scale)
+    return out, ref, ext
+
+
+def columns_inputs(cols):
+    d = workloads.device_inputs(cols, "cuda")
+    host = {k: (None if v is None else v.cpu().numpy()) for k, v in d.items()}
+    return host
+
+
+@pytest.mark.parametrize("k,W", [(1, 96), (2, 24), (3, 6), (4, 64), (5, 24)])
+def test_parity_benchmark_configs(k, W):
+    cols = workloads.make(k, W=W)
+    h = columns_inputs(cols)
+    run_both(cols.delta_tau, cols.single_scat_albedo, cols.phase_moments,
+             workloads.batch_config(cols), h["planck_levels"], h["surface_emission"],
+             h["top_emission"], label=cols.name)
+
+
+def _edge_batch(n, W=13, L=9, seed=0, M=None, delta_m=False):
+    rng = np.random.default_rng(seed)
+    M = M or (2 * n + (1 if delta_m else 0))
+    tau = 10.0 ** rng.uniform(-3, 1.3, (W, L))
+    ssa = rng.uniform(0, 1, (W, L))
+    tau[0, 1] = 0.0          # transparent
+    tau[1, :] = 0.0          # fully transparent column
+    ssa[2, 3] = 0.0          # pure absorption
+    ssa[3, :] = 0.0          # non-scattering column
+    ssa[4, 2] = 1.0          # conservative
+    ssa[5, :] = 1.0
+    tau[6, 4] = 0.5          # exact power of two (doubling count boundary)
+    tau[6, 5] = 1.0
+    tau[6, 6] = 8.0
+    ssa[7, :3] = [0.01, 0.1, 0.009999]  # ipow0 thresholds
+    g = rng.uniform(0.3, 0.9, (W, L))
+    mom = g[..., None] ** np.arange(M)
+    B = rng.uniform(0.5, 2.0, (W, L + 1))
+    return tau, ssa, mom, B
+
+
+@pytest.mark.parametrize("n", [2, 4, 8, 16, 32, 3, 6, 12])
+def test_parity_edge_layers(n):
+    tau, ssa, mom, B = _edge_batch(n, W=9 if n == 32 else 13)
+    cfg = BatchConfig(num_quadrature=n, surface_albedo=0.3, solar_flux=2.0, solar_mu=0.7)
+    run_both(tau, ssa, mom, cfg, B, B[:, -1].copy(), B[:, 0].copy(), label=f"edge N={n}")
+
+
+@pytest.mark.parametrize("opt", ["delta_m", "diffusion", "reversed", "no_surface", "black_cold",
+                                 "shared_moments", "scaling", "albedo_only", "solar_only"])
+def test_parity_options(opt):
+    n = 8
+    tau, ssa, mom, B = _edge_batch(n, delta_m=(opt == "delta_m"), seed=3)
+    W = tau.shape[0]
+    cfg = dict(num_quadrature=n, surface_albedo=0.2, solar_flux=1.0, solar_mu=0.45)
+    Bs, Bt = B[:, -1].copy(), B[:, 0].copy()
+    if opt == "delta_m":
+        cfg["use_delta_m"] = True
+    elif opt == "diffusion":
+        cfg["use_diffusion_lower_bc"] = True
+    elif opt == "reversed":
+        cfg["index_from_bottom"] = True
+    elif opt == "no_surface":
+        cfg["surface_albedo"] = 0.0
+        Bs = np.zeros(W)
+    elif opt == "black_cold":
+        cfg["surface_albedo"] = 0.0
+        Bs = np.zeros(W)
+        Bt = np.zeros(W)
+        B = None
+    elif opt == "shared_moments":
+        mom = mom[0].copy()
+    elif opt == "scaling":
+        cfg["spectrum_scaling"] = 0.37
+    elif opt == "albedo_only":
+        B, Bs, Bt = None, None, None
+        cfg["solar_flux"] = 0.0
+        cfg["top_emission"] = 0.25
+    elif opt == "solar_only":
+        B, Bs, Bt = None, None, None
+    run_both(tau, ssa, mom, BatchConfig(**cfg), B, Bs, Bt, label=f"option {opt}")
+
+
+def test_chunked_workspace_matches_single_chunk():
+    cols = workloads.make(1, W=200)
+    t = lambda a: torch.as_tensor(a, dtype=torch.float64, device="cuda")
+    cfg = workloads.batch_config(cols)
+    a = solve_batch(t(cols.delta_tau), t(cols.single_scat_albedo), t(cols.phase_moments), cfg)
+    b = solve_batch(t(cols.delta_tau), t(cols.single_scat_albedo), t(cols.phase_moments), cfg,
+                    max_workspace_bytes=3 * 1024 * 1024)
+    torch.cuda.synchronize()
+    for k in OUTS:
+        assert torch.equal(a[k], b[k]), k
+
+
+def test_empty_batch_and_validation():
+    from addingdoublingrt_amd import AdrtError
+    t = lambda a: torch.as_tensor(a, dtype=torch.float64, device="cuda")
+    cfg = BatchConfig(num_quadrature=4)
+    mom = t(np.ones((3, 8)))
+    out = solve_batch(t(np.zeros((0, 3))), t(np.zeros((0, 3))), mom, cfg)
+    assert out["flux_up"].shape == (0, 4)
+    tau = np.full((2, 3), 0.5)
+    tau[1, 2] = -1.0
+    with pytest.raises(AdrtError) as e:
+        solve_batch(t(tau), t(np.full((2, 3), 0.5)), mom, cfg, validate=True)
+    assert e.value.code == 6
+    with pytest.raises(AdrtError) as e:
+        solve_batch(t(np.full((2, 3), 0.5)), t(np.full((2, 3), 1.5)), mom, cfg, validate=True)
+    assert e.value.code == 7
+
+
+def test_device_planck_matches_oracle():
+    rng = np.random.default_rng(5)
+    W = 40
+    lo = np.concatenate([np.linspace(500, 2400, 20), [0.01, 700.0, 10.0], rng.uniform(100, 3000, 17)])
+    hi = lo + np.concatenate([np.full(20, 0.12), [50000.0, 0.0, 20.0], rng.uniform(1, 500, 17)])
+    T = rng.uniform(150, 330, (W, 7))
+    T[0, 0] = 0.0
+    t = lambda a: torch.as_tensor(a, dtype=torch.float64, device="cuda")
+    B = planck_levels(t(lo), t(hi), t(T)).cpu().numpy()
+    dB = planck_levels(t(lo), t(hi), t(T), derivative=True).cpu().numpy()
+    for w in range(W):
+        for l in range(7):
+            rb = oracle.planck(lo[w], hi[w], T[w, l])
+            rd = oracle.planck_deriv(lo[w], hi[w], T[w, l])
+            assert abs(B[w, l] - rb) <= 1e-13 * abs(rb), (w, l, B[w, l], rb)
+            assert abs(dB[w, l] - rd) <= 1e-12 * abs(rd) + 1e-300, (w, l, dB[w, l], rd)
