@@ -17,6 +17,7 @@
 
 #include "../../include/adrt_hip.h"
 #include "adrt_fwd.hpp"
+#include "adrt_fwd16.hpp"
 
 namespace {
 
@@ -290,12 +291,21 @@ int launch_chunk(const adrt::dev::FwdArgs& a, adrt_workspace* ws, hipStream_t st
   prep_kernel<<<(a.W + 255) / 256, 256, 0, stream>>>(a);
   prof_end(ws, e, stream);
   const long tasks = (long)a.W * a.L;
-  e = prof_begin(ws, 1, stream);
-  layer_kernel<N><<<(unsigned)((tasks + GPB - 1) / GPB), 256, 0, stream>>>(a);
-  prof_end(ws, e, stream);
-  e = prof_begin(ws, 2, stream);
-  adding_kernel<N><<<(unsigned)((a.W + GPB - 1) / GPB), 256, 0, stream>>>(a);
-  prof_end(ws, e, stream);
+  if constexpr (N == 16) {  // one wave per problem on the matrix cores
+    e = prof_begin(ws, 1, stream);
+    w16::layer16_kernel<<<(unsigned)((tasks + 3) / 4), 256, 0, stream>>>(a);
+    prof_end(ws, e, stream);
+    e = prof_begin(ws, 2, stream);
+    w16::adding16_kernel<<<(unsigned)((a.W + 3) / 4), 256, 0, stream>>>(a);
+    prof_end(ws, e, stream);
+  } else {
+    e = prof_begin(ws, 1, stream);
+    layer_kernel<N><<<(unsigned)((tasks + GPB - 1) / GPB), 256, 0, stream>>>(a);
+    prof_end(ws, e, stream);
+    e = prof_begin(ws, 2, stream);
+    adding_kernel<N><<<(unsigned)((a.W + GPB - 1) / GPB), 256, 0, stream>>>(a);
+    prof_end(ws, e, stream);
+  }
   return hipGetLastError() == hipSuccess ? ADRT_OK : ADRT_ERR_HIP;
 }
 

@@ -1,0 +1,674 @@
+// adrt_fwd16.hpp -- forward kernels for N = 16 on the matrix cores (gfx950).
+//
+// Same pipeline and arithmetic as adrt_fwd.hpp (the column-per-lane VALU
+// kernels used for the other quadrature orders), laid out one problem per
+// wave with the doubling / adding products on v_mfma_f64_16x16x4_f64.
+// Records in the layer store hold R and T in register (lane) order.
+#pragma once
+
+#include "adrt_fwd.hpp"
+#include "adrt_wave16.hpp"
+
+namespace adrt {
+namespace dev {
+namespace w16 {
+
+struct Lds16 {
+  static constexpr int tile = 0, tile2 = TILE, fv = 2 * TILE, pv = fv + N, prow = pv + N,
+                       vs = prow + 128;  // 4 vector slots of N
+  static constexpr int size = vs + 4 * N;
+};
+
+constexpr int REC16 = 2 * N * N + 2 * N;  // R, T (lane order), s_up, s_down
+constexpr int RBR16 = 2 * N * N + N;      // R_ab, T_ba (lane order), s_up
+
+// ---------------------------------------------------------------------------
+// layer setup + doubling, one wave per (wavenumber, layer)
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) layer16_kernel(FwdArgs a) {
+  using R = double;
+  __shared__ R smem[4 * Lds16::size];
+  const int wv = threadIdx.x >> 6;
+  const long task = (long)blockIdx.x * 4 + wv;
+  const int L_ = a.L;
+  if (task >= (long)a.W * L_) return;
+  const Lane L;
+  const int w = (int)(task / L_), l = (int)(task % L_);
+  const long gw = (long)(a.w0 + w);
+  R* sm = smem + wv * Lds16::size;
+  R* tile = sm + Lds16::tile;
+  R* tile2 = sm + Lds16::tile2;
+  R* fv = sm + Lds16::fv;
+  R* pv = sm + Lds16::pv;
+  R* prow = sm + Lds16::prow;
+  R* v0 = sm + Lds16::vs;
+  R* v1 = v0 + N;
+  R* v2 = v1 + N;
+  R* v3 = v2 + N;
+  R* out = a.lay + task * REC16;
+
+  const double tau = a.tau_s[task];
+  const double omega_in = a.omega_s[task];
+  const int mode = a.mode[task];
+  const double Btop = a.B ? a.B[gw * (L_ + 1) + lev(a, l)] : 0.0;
+  const double Bbot = a.B ? a.B[gw * (L_ + 1) + lev(a, l + 1)] : 0.0;
+  const double B_bar = (Bbot + Btop) / 2.0;
+  const double B_d = (tau > 0.0) ? (Bbot - Btop) / tau : 0.0;
+  const int j = L.j;
+  const double mu_j = a.mu[j], wt_j = a.wt[j];
+
+  if (tau <= 0.0 || omega_in <= 0.0) {
+    // transparent (src/doubling.hpp:52-53) or pure absorption (:56-77)
+    double trans = 1.0, su = 0.0, sd = 0.0;
+    if (tau > 0.0) {
+      const double tex = -tau / mu_j;
+      trans = (tex > -200.0) ? exp(tex) : 0.0;
+      const double omt = 1.0 - trans;
+      const double slope = mu_j * omt - 0.5 * tau * (1.0 + trans);
+      su = B_bar * omt + B_d * slope;
+      sd = B_bar * omt - B_d * slope;
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      out[L.l * 4 + r] = 0.0;
+      out[N * N + L.l * 4 + r] = (row<R>(L.h, r) == j) ? trans : 0.0;
+    }
+    if (L.h == 0) {
+      out[2 * N * N + j] = su;
+      out[2 * N * N + N + j] = sd;
+    }
+    if (L.l == 0) a.lflag[task] = 0;
+    return;
+  }
+
+  // ---- phase matrices (src/phase_matrix.hpp:61-181) ----
+  const double omega = omega_in > 1.0 ? 1.0 : omega_in;
+  const double con = 2.0 * omega * kPi;
+  const int li = lin(a, l);
+  const double* chi = a.shared_moments ? a.mom + (long)li * a.M : a.mom + (gw * L_ + li) * a.M;
+  const int twoM = 2 * a.n_real;
+  int Lm = a.M;
+  double f = 0.0, inv1mf = 1.0;
+  if (mode == 3) {
+    f = a.fdm[task];
+    inv1mf = 1.0 / (1.0 - f);
+    Lm = twoM;
+  } else if (mode == 2) {
+    Lm = a.M < twoM ? a.M : twoM;
+  }
+  double Ppp[4] = {0, 0, 0, 0}, Ppm[4] = {0, 0, 0, 0};
+  double sign = 1.0, spj = 0.0, smj = 0.0;
+  int rows[4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) rows[r] = row<R>(L.h, r);
+  for (int ll = 0; ll < Lm; ++ll) {
+    const double chil = (mode == 3) ? (chi[ll] - f) * inv1mf : chi[ll];
+    const double cl = (2 * ll + 1) * chil;
+    const double* P = a.Pl + ll * N;
+    const double plj = P[j];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const double term = cl * P[rows[r]] * plj;
+      Ppp[r] += term;
+      Ppm[r] += sign * term;
+    }
+    if (a.has_solar) {
+      const double term = cl * plj * a.Pl0[ll];
+      spj += term;
+      smj += sign * term;
+    }
+    sign = -sign;
+  }
+  // Hansen column normalisation, rows summed in order 0..15
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    Ppp[r] = Ppp[r] / (2.0 * kPi);
+    Ppm[r] = Ppm[r] / (2.0 * kPi);
+    tile[j * TS + rows[r]] = (Ppp[r] + Ppm[r]) * a.wt[rows[r]];
+  }
+  wave_sync();
+  double csum = 0.0;
+#pragma unroll
+  for (int i = 0; i < N; ++i) csum += tile[j * TS + i];
+  wave_sync();
+  if (csum > 0.0) {
+    const double corr = 1.0 / (2.0 * kPi * csum);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) { Ppp[r] *= corr; Ppm[r] *= corr; }
+  }
+  double pplus = 0.0, pminus = 0.0;
+  if (a.has_solar) {
+    pplus = spj / (2.0 * kPi);
+    pminus = smj / (2.0 * kPi);
+    const double s = colsum<R>((pplus + pminus) * wt_j, L, pv);
+    if (s > 0.0) {
+      const double corr = 1.0 / (2.0 * kPi * s);
+      pplus *= corr;
+      pminus *= corr;
+    }
+  }
+
+  // ---- thin-layer initialisation (src/doubling.hpp:83-147) ----
+  const int nn0 = (int)(log(tau) / log(2.0)) + ipow0(omega);
+  const int nn = nn0 < 1 ? 1 : nn0;
+  const double tau0 = tau * ldexp(1.0, -nn);
+  Mat<R> Rm, Tm;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int i = rows[r];
+    const double mui = a.mu[i];
+    const double gpp = (((i == j) ? 1.0 : 0.0) + (-con) * (Ppp[r] * wt_j)) / mui;
+    const double gpm = con * (Ppm[r] * wt_j) / mui;
+    Tm.v[r] = ((i == j) ? 1.0 : 0.0) - tau0 * gpp;
+    Rm.v[r] = tau0 * gpm;
+  }
+  double y = (1.0 - omega) * tau0 / mu_j;
+  double z = 0.0;
+  double sus = 0.0, sds = 0.0, gam = 0.0;
+  if (a.has_solar) {
+    const double Ftop = a.F0 * exp(-a.tau_cum[(long)w * (L_ + 1) + l] / a.mu0);
+    const double base = omega * tau0 / mu_j * Ftop;
+    sus = base * pminus;
+    sds = base * pplus;
+    gam = exp(-tau0 / a.mu0);
+  }
+  double gk = 0.5 * tau0;
+
+  // ---- doubling recursion (src/doubling.hpp:149-209) ----
+  for (int k = 0; k < nn; ++k) {
+    Mat<R> A, X, Y;
+    put<R>(tile, L, Rm);
+    if (L.h == 0) {
+      v0[j] = sds;
+      v1[j] = sus;
+    }
+    wave_sync();
+    eye_minus_mul<R>(tile, Rm, L, A);  // I - R^2
+    double b1 = 0.0, b2 = 0.0;
+    if (a.has_solar) {
+      double bs[2];
+      matvecs<R, 2>(tile, v0, L, bs);  // R s_down_sol, R s_up_sol
+      b1 = bs[0];
+      b2 = bs[1];
+    }
+    X = Tm;
+    right_solve<R>(A, X, L, fv, pv, prow, tile2);  // TG = T (I - R^2)^-1
+    put<R>(tile2, L, X);
+    const double zpgy = z + gk * y;
+    if (L.h == 0) {
+      v0[j] = zpgy;
+      v1[j] = y;
+      v2[j] = b1 + gam * sus;
+      v3[j] = gam * b2 + sds;
+    }
+    wave_sync();
+    mul<R>(tile2, Rm, L, Y);  // TGR = TG R
+    mul<R>(tile2, Tm, L, A);  // T_new = TG T
+    double a1, a3, c1 = 0.0, c2 = 0.0;
+    if (a.has_solar) {
+      double s4[4];
+      matvecs<R, 4>(tile2, v0, L, s4);
+      a1 = s4[0];
+      a3 = s4[1];
+      c1 = s4[2];
+      c2 = s4[3];
+    } else {
+      double s2[2];
+      matvecs<R, 2>(tile2, v0, L, s2);
+      a1 = s2[0];
+      a3 = s2[1];
+    }
+    put<R>(tile, L, Y);
+    wave_sync();
+    mul_add<R>(tile, Tm, Rm, L, Rm);  // R_new = R + TGR T
+    double s2[2];
+    matvecs<R, 2>(tile, v0, L, s2);
+    const double a2 = s2[0], a4 = s2[1];
+    wave_sync();
+    Tm = A;
+    const double znew = (a1 - a2) + z - gk * y;
+    const double ynew = a3 + a4 + y;
+    z = znew;
+    y = ynew;
+    if (a.has_solar) {
+      sus = c1 + sus;
+      sds = c2 + gam * sds;
+      gam = gam * gam;
+    }
+    gk = 2.0 * gk;
+  }
+
+  // ---- layer assembly (src/doubling.hpp:211-229), sources combined ----
+  gstore<R>(out, L, Rm);
+  gstore<R>(out + N * N, L, Tm);
+  if (L.h == 0) {
+    out[2 * N * N + j] = (y * B_bar + z * B_d) + sus;
+    out[2 * N * N + N + j] = (y * B_bar - z * B_d) + sds;
+  }
+  if (L.l == 0) a.lflag[task] = 1;
+}
+
+// ---------------------------------------------------------------------------
+// adding sweeps + interfaces, one wave per wavenumber
+// ---------------------------------------------------------------------------
+struct Comp16 {
+  Mat<double> Rab, Rba, Tab, Tba;
+  double su, sd;
+  int scat;
+};
+
+struct Ws16 {
+  double *tile, *tile2, *fv, *pv, *prow, *v0, *v1, *v2, *v3;
+};
+
+// rbase step: new = add(top = stored layer, bot = C)  (src/adding.hpp:246-258)
+__device__ void add16_on_top(Comp16& C, const double* rec, int lscat, const Lane& L,
+                             const Ws16& s) {
+  using R = double;
+  const int j = L.j;
+  const R tsu = rec[2 * N * N + j];
+  const R tsd = rec[2 * N * N + N + j];
+  Mat<R> Rt, Tt, A, X;
+  if (!lscat) {
+    // NonscatTop (src/adding.hpp:117-174): t = diag(top.T_ab), read from the record
+    gload<R>(rec + N * N, L, Tt);
+    const R tj = diag<R>(Tt, L, s.fv);
+    if (L.h == 0) {
+      s.v0[j] = tj;
+      s.v1[j] = tsd;
+    }
+    put<R>(s.tile, L, C.Rab);
+    put<R>(s.tile2, L, C.Tab);
+    wave_sync();
+    const R rbc = matvec<R>(s.tile, s.v1, L);   // bot.R_ab s1_down
+    const R tbc = matvec<R>(s.tile2, s.v1, L);  // bot.T_ab s1_down
+    R tv[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) tv[r] = s.v0[row<R>(L.h, r)];
+    wave_sync();
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      C.Rab.v[r] = tv[r] * C.Rab.v[r] * tj;
+      C.Tab.v[r] = C.Tab.v[r] * tj;
+      C.Tba.v[r] = tv[r] * C.Tba.v[r];
+    }
+    const R nsu = tsu + tj * (C.su + rbc);
+    const R nsd = C.sd + tbc;
+    C.su = nsu;
+    C.sd = nsd;
+    return;
+  }
+  gload<R>(rec, L, Rt);
+  gload<R>(rec + N * N, L, Tt);
+  if (!C.scat) {
+    // NonscatBot (src/adding.hpp:181-239): t = diag(bot.T_ab)
+    const R tj = diag<R>(C.Tab, L, s.fv);
+    if (L.h == 0) {
+      s.v0[j] = tj;
+      s.v1[j] = C.su;
+    }
+    put<R>(s.tile, L, Tt);
+    put<R>(s.tile2, L, Rt);
+    wave_sync();
+    const R tba = matvec<R>(s.tile, s.v1, L);   // top.T_ba s2_up
+    const R rba = matvec<R>(s.tile2, s.v1, L);  // top.R_ba s2_up
+    R tv[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) tv[r] = s.v0[row<R>(L.h, r)];
+    wave_sync();
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      C.Rab.v[r] = Rt.v[r];
+      C.Rba.v[r] = tv[r] * Rt.v[r] * tj;
+      C.Tab.v[r] = tv[r] * Tt.v[r];
+      C.Tba.v[r] = Tt.v[r] * tj;
+    }
+    const R nsu = tsu + tba;
+    const R nsd = C.sd + tj * (tsd + rba);
+    C.su = nsu;
+    C.sd = nsd;
+    C.scat = 1;
+    return;
+  }
+  // General (src/adding.hpp:69-110)
+  put<R>(s.tile, L, C.Rab);
+  if (L.h == 0) s.v0[j] = tsd;
+  wave_sync();
+  eye_minus_mul<R>(s.tile, Rt, L, A);                     // A1 = I - bot.R_ab top.R_ba
+  const R rhs_up = C.su + matvec<R>(s.tile, s.v0, L);     // s2_up + R_bc s1_down
+  wave_sync();
+  X = Tt;
+  right_solve<R>(A, X, L, s.fv, s.pv, s.prow, s.tile2);   // D1 = top.T_ba A1^-1
+  put<R>(s.tile, L, Rt);
+  if (L.h == 0) s.v1[j] = C.su;
+  wave_sync();
+  eye_minus_mul<R>(s.tile, C.Rab, L, A);                  // A2 = I - top.R_ba bot.R_ab
+  const R rhs_dn = tsd + matvec<R>(s.tile, s.v1, L);      // s1_down + R_ba s2_up
+  wave_sync();
+  right_solve<R>(A, C.Tab, L, s.fv, s.pv, s.prow, s.tile2);  // D2 = bot.T_ab A2^-1
+  if (L.h == 0) {
+    s.v2[j] = rhs_up;
+    s.v3[j] = rhs_dn;
+  }
+  put<R>(s.tile, L, C.Tab);  // D2
+  put<R>(s.tile2, L, X);     // D1
+  wave_sync();
+  mul<R>(s.tile, Rt, L, A);                                // D2 top.R_ba
+  const R nsd = C.sd + matvec<R>(s.tile, s.v3, L);
+  mul<R>(s.tile, Tt, L, C.Tab);                           // T_ab = D2 top.T_ab
+  Mat<R> B;
+  mul<R>(s.tile2, C.Rab, L, B);                           // D1 bot.R_ab
+  const R nsu = tsu + matvec<R>(s.tile2, s.v2, L);
+  mul<R>(s.tile2, C.Tba, L, X);                           // D1 bot.T_ba (new T_ba)
+  wave_sync();
+  put<R>(s.tile, L, A);
+  put<R>(s.tile2, L, B);
+  wave_sync();
+  mul_add<R>(s.tile, C.Tba, C.Rba, L, C.Rba);             // R_ba = bot.R_ba + (D2 R_ba) T_ba
+  mul_add<R>(s.tile2, Tt, Rt, L, C.Rab);                  // R_ab = top.R_ab + (D1 R_ab) T_ab
+  wave_sync();
+  C.Tba = X;
+  C.su = nsu;
+  C.sd = nsd;
+  C.scat = 1;
+}
+
+// rtop step: new = add(top = C, bot = stored layer)
+__device__ void add16_below(Comp16& C, const double* rec, int lscat, const Lane& L,
+                            const Ws16& s) {
+  using R = double;
+  const int j = L.j;
+  const R bsu = rec[2 * N * N + j];
+  const R bsd = rec[2 * N * N + N + j];
+  Mat<R> Rb, Tb, A, X;
+  gload<R>(rec, L, Rb);
+  gload<R>(rec + N * N, L, Tb);
+  if (!C.scat) {
+    // NonscatTop with top = C: t = diag(C.T_ab)
+    const R tj = diag<R>(C.Tab, L, s.fv);
+    if (L.h == 0) {
+      s.v0[j] = tj;
+      s.v1[j] = C.sd;
+    }
+    put<R>(s.tile, L, Rb);
+    put<R>(s.tile2, L, Tb);
+    wave_sync();
+    const R rbc = matvec<R>(s.tile, s.v1, L);   // bot.R_ab s1_down
+    const R tbc = matvec<R>(s.tile2, s.v1, L);  // bot.T_ab s1_down
+    R tv[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) tv[r] = s.v0[row<R>(L.h, r)];
+    wave_sync();
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      C.Rab.v[r] = tv[r] * Rb.v[r] * tj;
+      C.Rba.v[r] = Rb.v[r];
+      C.Tab.v[r] = Tb.v[r] * tj;
+      C.Tba.v[r] = tv[r] * Tb.v[r];
+    }
+    const R nsu = C.su + tj * (bsu + rbc);
+    const R nsd = bsd + tbc;
+    C.su = nsu;
+    C.sd = nsd;
+    C.scat = lscat;
+    return;
+  }
+  if (!lscat) {
+    // NonscatBot with bot = layer: t = diag(layer T)
+    const R tj = diag<R>(Tb, L, s.fv);
+    if (L.h == 0) {
+      s.v0[j] = tj;
+      s.v1[j] = bsu;
+    }
+    put<R>(s.tile, L, C.Tba);
+    put<R>(s.tile2, L, C.Rba);
+    wave_sync();
+    const R tba = matvec<R>(s.tile, s.v1, L);   // top.T_ba s2_up
+    const R rba = matvec<R>(s.tile2, s.v1, L);  // top.R_ba s2_up
+    R tv[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) tv[r] = s.v0[row<R>(L.h, r)];
+    wave_sync();
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      C.Rba.v[r] = tv[r] * C.Rba.v[r] * tj;
+      C.Tab.v[r] = tv[r] * C.Tab.v[r];
+      C.Tba.v[r] = C.Tba.v[r] * tj;
+    }
+    const R nsu = C.su + tba;
+    const R nsd = bsd + tj * (C.sd + rba);
+    C.su = nsu;
+    C.sd = nsd;
+    return;
+  }
+  // General with top = C, bot = layer
+  put<R>(s.tile, L, Rb);
+  if (L.h == 0) s.v0[j] = C.sd;
+  wave_sync();
+  eye_minus_mul<R>(s.tile, C.Rba, L, A);                  // A1 = I - bot.R_ab top.R_ba
+  const R rhs_up = bsu + matvec<R>(s.tile, s.v0, L);      // s2_up + R_bc s1_down
+  wave_sync();
+  right_solve<R>(A, C.Tba, L, s.fv, s.pv, s.prow, s.tile2);  // D1 = top.T_ba A1^-1
+  put<R>(s.tile, L, C.Rba);
+  if (L.h == 0) s.v1[j] = bsu;
+  wave_sync();
+  eye_minus_mul<R>(s.tile, Rb, L, A);                     // A2 = I - top.R_ba bot.R_ab
+  const R rhs_dn = C.sd + matvec<R>(s.tile, s.v1, L);     // s1_down + R_ba s2_up
+  wave_sync();
+  X = Tb;
+  right_solve<R>(A, X, L, s.fv, s.pv, s.prow, s.tile2);   // D2 = bot.T_ab A2^-1
+  if (L.h == 0) {
+    s.v2[j] = rhs_up;
+    s.v3[j] = rhs_dn;
+  }
+  put<R>(s.tile, L, C.Tba);  // D1
+  put<R>(s.tile2, L, X);     // D2
+  wave_sync();
+  mul<R>(s.tile, Rb, L, A);                                // D1 bot.R_ab
+  const R nsu = C.su + matvec<R>(s.tile, s.v2, L);
+  mul<R>(s.tile, Tb, L, C.Tba);                           // T_ba = D1 bot.T_ba
+  Mat<R> B;
+  mul<R>(s.tile2, C.Rba, L, B);                           // D2 top.R_ba
+  const R nsd = bsd + matvec<R>(s.tile2, s.v3, L);
+  mul<R>(s.tile2, C.Tab, L, X);                           // T_ab = D2 top.T_ab
+  wave_sync();
+  put<R>(s.tile, L, A);
+  put<R>(s.tile2, L, B);
+  wave_sync();
+  mul_add<R>(s.tile, C.Tab, C.Rab, L, C.Rab);             // R_ab = top.R_ab + (D1 R_ab) T_ab
+  mul_add<R>(s.tile2, Tb, Rb, L, C.Rba);                  // R_ba = bot.R_ba + (D2 R_ba) T_ba
+  wave_sync();
+  C.Tab = X;
+  C.su = nsu;
+  C.sd = nsd;
+  C.scat = 1;
+}
+
+__global__ void __launch_bounds__(256) adding16_kernel(FwdArgs a) {
+  using R = double;
+  __shared__ R smem[4 * Lds16::size];
+  const int wv = threadIdx.x >> 6;
+  const int w = blockIdx.x * 4 + wv;
+  if (w >= a.W) return;
+  const Lane L;
+  const int j = L.j;
+  const long gw = (long)(a.w0 + w);
+  const int Lay = a.L;
+  R* sm = smem + wv * Lds16::size;
+  Ws16 s{sm + Lds16::tile, sm + Lds16::tile2, sm + Lds16::fv, sm + Lds16::pv, sm + Lds16::prow,
+         sm + Lds16::vs, sm + Lds16::vs + N, sm + Lds16::vs + 2 * N, sm + Lds16::vs + 3 * N};
+  const double* lay = a.lay + (long)w * Lay * REC16;
+  const int* lflag = a.lflag + (long)w * Lay;
+  double* rbw = a.rb + (long)w * Lay * RBR16;
+  const double mu_j = a.mu[j], wt_j = a.wt[j];
+  const double* Bw = a.B ? a.B + gw * (Lay + 1) : nullptr;
+  auto Blev = [&](int l) -> double { return Bw ? Bw[lev(a, l)] : 0.0; };
+
+  // boundary conditions (src/solver.cpp:385-419, 516-551, 585-603)
+  const double Bs = a.surf_w ? a.surf_w[gw] : a.surf_em;
+  const double Btop = a.top_w ? a.top_w[gw] : a.top_em;
+  const double A = a.albedo;
+  const int has_surface = !a.diffusion && (A > 0.0 || Bs > 0.0);
+  const int ltot = Lay + (has_surface ? 1 : 0);
+  const double tau_tot = a.tau_cum[(long)w * (Lay + 1) + Lay];
+  const double Itd = Btop;
+  double Ibu = 0.0;
+  if (a.diffusion) {
+    const double Bb = Blev(Lay);
+    const double dt = a.tau_s[(long)w * Lay + Lay - 1];
+    const double dBdt = (dt > 0.0) ? (Bb - Blev(Lay - 1)) / dt : 0.0;
+    Ibu = Bb + mu_j * dBdt;
+  } else if (!has_surface) {
+    Ibu = Bs;
+  }
+
+  Comp16 C;
+  // ---- bottom-up sweep (src/solver.cpp:560-572) ----
+  if (has_surface) {
+    const double rr = 2.0 * A * mu_j * wt_j * a.xfac;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      C.Rab.v[r] = (row<R>(L.h, r) < a.n_real) ? rr : 0.0;
+      C.Rba.v[r] = C.Rab.v[r];
+      C.Tab.v[r] = 0.0;
+      C.Tba.v[r] = 0.0;
+    }
+    double ssol = 0.0;
+    if (a.has_solar && A > 0.0) ssol = (A / kPi) * a.F0 * a.mu0 * exp(-tau_tot / a.mu0);
+    C.su = (1.0 - A) * Bs + ssol;
+    C.sd = 0.0;
+    C.scat = A > 0.0;
+  } else {
+    const double* rec = lay + (long)(Lay - 1) * REC16;
+    gload<R>(rec, L, C.Rab);
+    gload<R>(rec + N * N, L, C.Tab);
+    C.Rba = C.Rab;
+    C.Tba = C.Tab;
+    C.su = rec[2 * N * N + j];
+    C.sd = rec[2 * N * N + N + j];
+    C.scat = lflag[Lay - 1];
+  }
+  for (int n = 1;; ++n) {  // C == rbase[n]
+    if (n < ltot) {
+      double* o = rbw + (long)(n - 1) * RBR16;
+      gstore<R>(o, L, C.Rab);
+      gstore<R>(o + N * N, L, C.Tba);
+      if (L.h == 0) o[2 * N * N + j] = C.su;
+    }
+    if (n == ltot) break;
+    const int k = ltot - 1 - n;
+    add16_on_top(C, lay + (long)k * REC16, lflag[k], L, s);
+  }
+
+  auto emit = [&](int l, double Iup, double Idn) {
+    const double c = 2.0 * kPi * wt_j * mu_j;
+    const double fu = colsum<R>(c * Iup, L, s.pv);
+    const double fdn = colsum<R>(c * Idn, L, s.pv);
+    double J = colsum<R>(0.5 * wt_j * (Iup + Idn), L, s.pv);
+    if (L.l == 0) {
+      const long o = gw * (Lay + 1) + lev(a, l);
+      double fdir = 0.0;
+      if (a.has_solar) {
+        fdir = (l == 0) ? a.F0 * a.mu0
+                        : a.F0 * a.mu0 * exp(-a.tau_cum[(long)w * (Lay + 1) + l] / a.mu0);
+        J += fdir * (1.0 / (4.0 * kPi * a.mu0));
+      }
+      a.fu[o] = fu * a.scaling;
+      a.fd[o] = fdn;
+      a.mi[o] = J;
+      a.fdir[o] = fdir;
+      if (a.fdiv) {
+        const int lyr = (l == 0) ? 0 : l - 1;
+        const double om = a.ssa[gw * Lay + lin(a, lyr)];
+        a.fdiv[o] = 4.0 * kPi * (1.0 - om) * (J - Blev(l));
+      }
+    }
+  };
+
+  // ---- TOA (src/solver.cpp:628-642) ----
+  {
+    put<R>(s.tile, L, C.Rab);
+    put<R>(s.tile2, L, C.Tba);
+    if (L.h == 0) {
+      s.v0[j] = Itd;
+      s.v1[j] = Ibu;
+    }
+    wave_sync();
+    const double rI = matvec<R>(s.tile, s.v0, L);
+    const double tI = matvec<R>(s.tile2, s.v1, L);
+    wave_sync();
+    emit(0, rI + (tI + C.su), Itd);
+  }
+
+  // ---- top-down sweep with interfaces (src/solver.cpp:574-583, 644-678) ----
+  {
+    gload<R>(lay, L, C.Rab);
+    gload<R>(lay + N * N, L, C.Tab);
+    C.Rba = C.Rab;
+    C.Tba = C.Tab;
+    C.su = lay[2 * N * N + j];
+    C.sd = lay[2 * N * N + N + j];
+    C.scat = lflag[0];
+  }
+  for (int l = 1; l <= Lay; ++l) {  // C == rtop[l]
+    const int n_base = ltot - l;
+    double Iup, Idn;
+    if (n_base > 0) {
+      const double* o = rbw + (long)(n_base - 1) * RBR16;
+      const double sbu = o[2 * N * N + j];
+      Mat<R> Rb, Tb, M;
+      gload<R>(o, L, Rb);
+      gload<R>(o + N * N, L, Tb);
+      // top.T_ab I_top_down ; base.T_ba I_bot_up
+      put<R>(s.tile, L, C.Tab);
+      put<R>(s.tile2, L, Tb);
+      if (L.h == 0) {
+        s.v0[j] = Itd;
+        s.v1[j] = Ibu;
+      }
+      wave_sync();
+      const double tti = matvec<R>(s.tile, s.v0, L);
+      const double tbi = matvec<R>(s.tile2, s.v1, L);
+      wave_sync();
+      // M_up = I - base.R_ab top.R_ba ; terms 2 and 3 (src/solver.cpp:24-46)
+      put<R>(s.tile, L, Rb);
+      put<R>(s.tile2, L, C.Rba);
+      if (L.h == 0) {
+        s.v0[j] = tti;
+        s.v1[j] = C.sd;
+        s.v2[j] = tbi;
+        s.v3[j] = sbu;
+      }
+      wave_sync();
+      eye_minus_mul<R>(s.tile, C.Rba, L, M);
+      double t23[2];
+      matvecs<R, 2>(s.tile, s.v0, L, t23);
+      double u23[2];
+      matvecs<R, 2>(s.tile2, s.v2, L, u23);
+      Mat<R> M2;
+      eye_minus_mul<R>(s.tile2, Rb, L, M2);  // M_dn = I - top.R_ba base.R_ab
+      wave_sync();
+      Iup = left_solve<R>(M, tbi + t23[0] + t23[1] + sbu, L, s.fv, s.pv, s.prow, s.tile);
+      Idn = left_solve<R>(M2, tti + u23[0] + u23[1] + C.sd, L, s.fv, s.pv, s.prow, s.tile);
+    } else {
+      put<R>(s.tile, L, C.Tab);
+      put<R>(s.tile2, L, C.Rba);
+      if (L.h == 0) {
+        s.v0[j] = Itd;
+        s.v1[j] = Ibu;
+      }
+      wave_sync();
+      const double tti = matvec<R>(s.tile, s.v0, L);
+      const double rbi = matvec<R>(s.tile2, s.v1, L);
+      wave_sync();
+      Idn = tti + (rbi + C.sd);
+      Iup = Ibu;
+    }
+    emit(l, Iup, Idn);
+    if (l < Lay) add16_below(C, lay + (long)l * REC16, lflag[l], L, s);
+  }
+}
+
+}  // namespace w16
+}  // namespace dev
+}  // namespace adrt
