@@ -14,8 +14,7 @@ namespace dev {
 namespace w16 {
 
 struct Lds16 {
-  static constexpr int tile = 0, tile2 = TILE, fv = 2 * TILE, pv = fv + N, prow = pv + N,
-                       vs = prow + 128;  // 4 vector slots of N
+  static constexpr int tile = 0, tile2 = TILE, pv = 2 * TILE, vs = pv + N;  // 4 vector slots
   static constexpr int size = vs + 4 * N;
 };
 
@@ -25,7 +24,7 @@ constexpr int RBR16 = 2 * N * N + N;      // R_ab, T_ba (lane order), s_up
 // ---------------------------------------------------------------------------
 // layer setup + doubling, one wave per (wavenumber, layer)
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) layer16_kernel(FwdArgs a) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) layer16_kernel(FwdArgs a) {
   using R = double;
   __shared__ R smem[4 * Lds16::size];
   const int wv = threadIdx.x >> 6;
@@ -38,9 +37,7 @@ __global__ void __launch_bounds__(256) layer16_kernel(FwdArgs a) {
   R* sm = smem + wv * Lds16::size;
   R* tile = sm + Lds16::tile;
   R* tile2 = sm + Lds16::tile2;
-  R* fv = sm + Lds16::fv;
   R* pv = sm + Lds16::pv;
-  R* prow = sm + Lds16::prow;
   R* v0 = sm + Lds16::vs;
   R* v1 = v0 + N;
   R* v2 = v1 + N;
@@ -192,7 +189,8 @@ __global__ void __launch_bounds__(256) layer16_kernel(FwdArgs a) {
       b2 = bs[1];
     }
     X = Tm;
-    right_solve<R>(A, X, L, fv, pv, prow, tile2);  // TG = T (I - R^2)^-1
+    right_solve<R>(A, X, L);  // TG = T (I - R^2)^-1
+    wave_sync();
     put<R>(tile2, L, X);
     const double zpgy = z + gk * y;
     if (L.h == 0) {
@@ -258,7 +256,7 @@ struct Comp16 {
 };
 
 struct Ws16 {
-  double *tile, *tile2, *fv, *pv, *prow, *v0, *v1, *v2, *v3;
+  double *tile, *tile2, *pv, *v0, *v1, *v2, *v3;
 };
 
 // rbase step: new = add(top = stored layer, bot = C)  (src/adding.hpp:246-258)
@@ -272,7 +270,7 @@ __device__ void add16_on_top(Comp16& C, const double* rec, int lscat, const Lane
   if (!lscat) {
     // NonscatTop (src/adding.hpp:117-174): t = diag(top.T_ab), read from the record
     gload<R>(rec + N * N, L, Tt);
-    const R tj = diag<R>(Tt, L, s.fv);
+    const R tj = diag<R>(Tt, L);
     if (L.h == 0) {
       s.v0[j] = tj;
       s.v1[j] = tsd;
@@ -302,7 +300,7 @@ __device__ void add16_on_top(Comp16& C, const double* rec, int lscat, const Lane
   gload<R>(rec + N * N, L, Tt);
   if (!C.scat) {
     // NonscatBot (src/adding.hpp:181-239): t = diag(bot.T_ab)
-    const R tj = diag<R>(C.Tab, L, s.fv);
+    const R tj = diag<R>(C.Tab, L);
     if (L.h == 0) {
       s.v0[j] = tj;
       s.v1[j] = C.su;
@@ -338,14 +336,14 @@ __device__ void add16_on_top(Comp16& C, const double* rec, int lscat, const Lane
   const R rhs_up = C.su + matvec<R>(s.tile, s.v0, L);     // s2_up + R_bc s1_down
   wave_sync();
   X = Tt;
-  right_solve<R>(A, X, L, s.fv, s.pv, s.prow, s.tile2);   // D1 = top.T_ba A1^-1
+  right_solve<R>(A, X, L);   // D1 = top.T_ba A1^-1
   put<R>(s.tile, L, Rt);
   if (L.h == 0) s.v1[j] = C.su;
   wave_sync();
   eye_minus_mul<R>(s.tile, C.Rab, L, A);                  // A2 = I - top.R_ba bot.R_ab
   const R rhs_dn = tsd + matvec<R>(s.tile, s.v1, L);      // s1_down + R_ba s2_up
   wave_sync();
-  right_solve<R>(A, C.Tab, L, s.fv, s.pv, s.prow, s.tile2);  // D2 = bot.T_ab A2^-1
+  right_solve<R>(A, C.Tab, L);  // D2 = bot.T_ab A2^-1
   if (L.h == 0) {
     s.v2[j] = rhs_up;
     s.v3[j] = rhs_dn;
@@ -385,7 +383,7 @@ __device__ void add16_below(Comp16& C, const double* rec, int lscat, const Lane&
   gload<R>(rec + N * N, L, Tb);
   if (!C.scat) {
     // NonscatTop with top = C: t = diag(C.T_ab)
-    const R tj = diag<R>(C.Tab, L, s.fv);
+    const R tj = diag<R>(C.Tab, L);
     if (L.h == 0) {
       s.v0[j] = tj;
       s.v1[j] = C.sd;
@@ -415,7 +413,7 @@ __device__ void add16_below(Comp16& C, const double* rec, int lscat, const Lane&
   }
   if (!lscat) {
     // NonscatBot with bot = layer: t = diag(layer T)
-    const R tj = diag<R>(Tb, L, s.fv);
+    const R tj = diag<R>(Tb, L);
     if (L.h == 0) {
       s.v0[j] = tj;
       s.v1[j] = bsu;
@@ -448,7 +446,7 @@ __device__ void add16_below(Comp16& C, const double* rec, int lscat, const Lane&
   eye_minus_mul<R>(s.tile, C.Rba, L, A);                  // A1 = I - bot.R_ab top.R_ba
   const R rhs_up = bsu + matvec<R>(s.tile, s.v0, L);      // s2_up + R_bc s1_down
   wave_sync();
-  right_solve<R>(A, C.Tba, L, s.fv, s.pv, s.prow, s.tile2);  // D1 = top.T_ba A1^-1
+  right_solve<R>(A, C.Tba, L);  // D1 = top.T_ba A1^-1
   put<R>(s.tile, L, C.Rba);
   if (L.h == 0) s.v1[j] = bsu;
   wave_sync();
@@ -456,7 +454,7 @@ __device__ void add16_below(Comp16& C, const double* rec, int lscat, const Lane&
   const R rhs_dn = C.sd + matvec<R>(s.tile, s.v1, L);     // s1_down + R_ba s2_up
   wave_sync();
   X = Tb;
-  right_solve<R>(A, X, L, s.fv, s.pv, s.prow, s.tile2);   // D2 = bot.T_ab A2^-1
+  right_solve<R>(A, X, L);   // D2 = bot.T_ab A2^-1
   if (L.h == 0) {
     s.v2[j] = rhs_up;
     s.v3[j] = rhs_dn;
@@ -484,7 +482,7 @@ __device__ void add16_below(Comp16& C, const double* rec, int lscat, const Lane&
   C.scat = 1;
 }
 
-__global__ void __launch_bounds__(256) adding16_kernel(FwdArgs a) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) adding16_kernel(FwdArgs a) {
   using R = double;
   __shared__ R smem[4 * Lds16::size];
   const int wv = threadIdx.x >> 6;
@@ -495,7 +493,7 @@ __global__ void __launch_bounds__(256) adding16_kernel(FwdArgs a) {
   const long gw = (long)(a.w0 + w);
   const int Lay = a.L;
   R* sm = smem + wv * Lds16::size;
-  Ws16 s{sm + Lds16::tile, sm + Lds16::tile2, sm + Lds16::fv, sm + Lds16::pv, sm + Lds16::prow,
+  Ws16 s{sm + Lds16::tile, sm + Lds16::tile2, sm + Lds16::pv,
          sm + Lds16::vs, sm + Lds16::vs + N, sm + Lds16::vs + 2 * N, sm + Lds16::vs + 3 * N};
   const double* lay = a.lay + (long)w * Lay * REC16;
   const int* lflag = a.lflag + (long)w * Lay;
@@ -648,8 +646,8 @@ __global__ void __launch_bounds__(256) adding16_kernel(FwdArgs a) {
       Mat<R> M2;
       eye_minus_mul<R>(s.tile2, Rb, L, M2);  // M_dn = I - top.R_ba base.R_ab
       wave_sync();
-      Iup = left_solve<R>(M, tbi + t23[0] + t23[1] + sbu, L, s.fv, s.pv, s.prow, s.tile);
-      Idn = left_solve<R>(M2, tti + u23[0] + u23[1] + C.sd, L, s.fv, s.pv, s.prow, s.tile);
+      Iup = left_solve<R>(M, tbi + t23[0] + t23[1] + sbu, L, s.tile);
+      Idn = left_solve<R>(M2, tti + u23[0] + u23[1] + C.sd, L, s.tile);
     } else {
       put<R>(s.tile, L, C.Tab);
       put<R>(s.tile2, L, C.Rba);

@@ -18,6 +18,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <utility>
+
 #include "adrt_device.hpp"
 
 namespace adrt {
@@ -51,6 +53,10 @@ __device__ __forceinline__ f4 mfma(float a, float b, f4 c) {
 
 template <typename R>
 using vec4 = typename std::conditional<sizeof(R) == 8, d4, f4>::type;
+
+// keeps the scheduler from interleaving neighbouring primitives (bounds the
+// number of values in flight, hence the register count and the occupancy)
+__device__ __forceinline__ void fence_sched() { __builtin_amdgcn_sched_barrier(0); }
 
 struct Lane {
   int l, j, h;
@@ -92,6 +98,7 @@ __device__ __forceinline__ void gstore(R* g, const Lane& L, const Mat<R>& m) {
 // C = X Y with X in an LDS tile, Y in registers
 template <typename R>
 __device__ __forceinline__ void mul(const R* X, const Mat<R>& Y, const Lane& L, Mat<R>& C) {
+  fence_sched();
   vec4<R> acc = {R(0), R(0), R(0), R(0)};
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
@@ -100,12 +107,14 @@ __device__ __forceinline__ void mul(const R* X, const Mat<R>& Y, const Lane& L, 
   }
 #pragma unroll
   for (int r = 0; r < 4; ++r) C.v[r] = acc[r];
+  fence_sched();
 }
 
 // C = Z + X Y
 template <typename R>
 __device__ __forceinline__ void mul_add(const R* X, const Mat<R>& Y, const Mat<R>& Z,
                                         const Lane& L, Mat<R>& C) {
+  fence_sched();
   vec4<R> acc = {Z.v[0], Z.v[1], Z.v[2], Z.v[3]};
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
@@ -114,6 +123,7 @@ __device__ __forceinline__ void mul_add(const R* X, const Mat<R>& Y, const Mat<R
   }
 #pragma unroll
   for (int r = 0; r < 4; ++r) C.v[r] = acc[r];
+  fence_sched();
 }
 
 // C = I - X Y
@@ -129,111 +139,138 @@ __device__ __forceinline__ void eye_minus_mul(const R* X, const Mat<R>& Y, const
 // (X v)_j for X in a tile and v in an LDS slot of 16 values
 template <typename R>
 __device__ __forceinline__ R matvec(const R* X, const R* v, const Lane& L) {
+  fence_sched();
   R s = R(0);
-#pragma unroll
+#pragma unroll 4
   for (int k = 0; k < N; ++k) s = fma_(X[k * TS + L.j], v[k], s);
+  fence_sched();
   return s;
 }
 
 template <typename R, int NV>
 __device__ __forceinline__ void matvecs(const R* X, const R* v, const Lane& L, R (&s)[NV]) {
+  fence_sched();
 #pragma unroll
   for (int q = 0; q < NV; ++q) s[q] = R(0);
-#pragma unroll
+#pragma unroll 4
   for (int k = 0; k < N; ++k) {
     const R x = X[k * TS + L.j];
 #pragma unroll
     for (int q = 0; q < NV; ++q) s[q] = fma_(x, v[q * N + k], s[q]);
   }
+  fence_sched();
+}
+
+// ---- cross-lane helpers (no LDS traffic, no barrier) ----------------------
+__device__ __forceinline__ double shfl(double v, int lane) {  // value of `lane`
+  const long long b = __builtin_bit_cast(long long, v);
+  const int lo = __builtin_amdgcn_ds_bpermute(lane << 2, (int)b);
+  const int hi = __builtin_amdgcn_ds_bpermute(lane << 2, (int)(b >> 32));
+  return __builtin_bit_cast(double, ((long long)hi << 32) | (long long)(unsigned)lo);
+}
+__device__ __forceinline__ float shfl(float v, int lane) {
+  return __builtin_bit_cast(float,
+                            __builtin_amdgcn_ds_bpermute(lane << 2, __builtin_bit_cast(int, v)));
+}
+__device__ __forceinline__ double readlane(double v, int lane) {  // lane is wave-uniform
+  const long long b = __builtin_bit_cast(long long, v);
+  const int lo = __builtin_amdgcn_readlane((int)b, lane);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+  return __builtin_bit_cast(double, ((long long)hi << 32) | (long long)(unsigned)lo);
+}
+__device__ __forceinline__ float readlane(float v, int lane) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), lane));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp(double v) {
+  const long long b = __builtin_bit_cast(long long, v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+  return __builtin_bit_cast(double, ((long long)hi << 32) | (long long)(unsigned)lo);
+}
+template <int CTRL>
+__device__ __forceinline__ float dpp(float v) {
+  return __builtin_bit_cast(
+      float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, 0xF, 0xF, false));
+}
+// maximum over the 16 lanes of each DPP row (lanes 16h .. 16h+15), in every lane
+template <typename R>
+__device__ __forceinline__ R row_max(R v) {
+  v = __builtin_fmax(v, dpp<0xB1>(v));   // quad_perm [1,0,3,2]
+  v = __builtin_fmax(v, dpp<0x4E>(v));   // quad_perm [2,3,0,1]
+  v = __builtin_fmax(v, dpp<0x141>(v));  // row_half_mirror
+  v = __builtin_fmax(v, dpp<0x140>(v));  // row_mirror
+  return v;
 }
 
 // diagonal entry M(j, j) of column j, for all four lanes of the column
 template <typename R>
-__device__ __forceinline__ R diag(const Mat<R>& m, const Lane& L, R* slot) {
-  R d = R(0);
-  bool own = false;
-#pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    const bool hit = row<R>(L.h, r) == L.j;
-    d = hit ? m.v[r] : d;
-    own = own || hit;
-  }
-  if (own) slot[L.j] = d;
-  wave_sync();
-  const R out = slot[L.j];
-  wave_sync();
-  return out;
+__device__ __forceinline__ R diag(const Mat<R>& m, const Lane& L) {
+  // owner of M(j, j) is lane (j, h*) with row(h*, r*) == j
+  int oh, orr;
+  if constexpr (sizeof(R) == 8) { oh = L.j & 3; orr = L.j >> 2; }
+  else { oh = L.j >> 2; orr = L.j & 3; }
+  const R sel = orr == 0 ? m.v[0] : orr == 1 ? m.v[1] : orr == 2 ? m.v[2] : m.v[3];
+  return shfl(sel, L.j + 16 * oh);
 }
 
-// X A = T for X, in place on T (Gauss-Jordan on [A^T | T^T], row partial
-// pivoting as an LU of A^T, corrected-reciprocal scaling of the pivot row).
-// Lane (j, h) holds its entries of column j of A (= row j of A^T) and of T.
-// The pivot index is wave-uniform (one problem per wave).
-// LDS: fv (16), pv (16), prow (2 x 64 in lane order), perm tile (TILE).
+// One Gauss-Jordan step k on [A^T | T^T] (row partial pivoting as an LU of
+// A^T; the pivot is the first row of largest magnitude among those not yet
+// used, and the pivot row is scaled with the corrected reciprocal).  Lane
+// (j, h) holds row j of A^T (= column j of A) at columns row(h, .).
+template <typename R, int k>
+__device__ __forceinline__ int gj_pivot(const Mat<R>& A, const Lane& L, bool used, R& f, R& fp) {
+  constexpr int kh = (sizeof(R) == 8) ? (k & 3) : (k >> 2);
+  constexpr int kr = (sizeof(R) == 8) ? (k >> 2) : (k & 3);
+  const R own = A.v[kr];                           // A(k, j), meaningful in row kh
+  const R key = used ? R(-1) : __builtin_fabs(own);
+  const R m = row_max<R>(key);
+  const unsigned long long hit = __builtin_amdgcn_ballot_w64(key == m);
+  const int p = __builtin_ctzll((hit >> (16 * kh)) & 0xFFFFull);
+  f = shfl(own, L.j + 16 * kh);                    // multiplier of row j
+  fp = readlane(own, p + 16 * kh);                 // pivot
+  return p;
+}
+
+// X A = T for X, in place on T.  No LDS: pivots by DPP row maxima and a
+// ballot, the pivot row by lane permutes, and the final row permutation by
+// one permute per register.
 template <typename R>
-__device__ __forceinline__ void right_solve(Mat<R>& A, Mat<R>& T, const Lane& L, R* fv, R* pv,
-                                            R* prow, R* perm) {
+__device__ __forceinline__ void right_solve(Mat<R>& A, Mat<R>& T, const Lane& L) {
+  fence_sched();
   bool used = false;
-  int mystep = 0;
-#pragma unroll
-  for (int k = 0; k < N; ++k) {
-    // owner of A(k, j): row(h, r) == k
-    int kh, kr;
-    if constexpr (sizeof(R) == 8) { kh = k % 4; kr = k / 4; }
-    else { kh = k / 4; kr = k % 4; }
-    if (L.h == kh) {
-      const R f = A.v[kr];
-      fv[L.j] = f;
-      pv[L.j] = used ? R(-1) : __builtin_fabs(f);
-    }
-    wave_sync();
-    const R f = fv[L.j];
-    int p = 0;
-    R best = pv[0];
-#pragma unroll
-    for (int i = 1; i < N; ++i) {
-      const R v = pv[i];
-      const bool gt = v > best;
-      best = gt ? v : best;
-      p = gt ? i : p;
-    }
-    p = __builtin_amdgcn_readfirstlane(p);
+  int src = 0;  // lane column holding row j of the result
+  auto step = [&](auto kc) {
+    constexpr int k = decltype(kc)::value;
+    R f, fp;
+    const int p = gj_pivot<R, k>(A, L, used, f, fp);
+    const R inv = R(1) / fp;
     const bool piv = (L.j == p);
-    if (piv) {  // the four lanes of the pivot column scale and publish the pivot row
-      const R inv = R(1) / f;
+    const int pl = p + 16 * L.h;
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        A.v[r] = div_by(A.v[r], f, inv);
-        T.v[r] = div_by(T.v[r], f, inv);
-        prow[L.h * 4 + r] = A.v[r];
-        prow[64 + L.h * 4 + r] = T.v[r];
-      }
-      used = true;
-      mystep = k;
+    for (int r = 0; r < 4; ++r) {
+      const R pa = div_by(shfl(A.v[r], pl), fp, inv);
+      const R pt = div_by(shfl(T.v[r], pl), fp, inv);
+      A.v[r] = piv ? pa : fma_(-f, pa, A.v[r]);
+      T.v[r] = piv ? pt : fma_(-f, pt, T.v[r]);
     }
-    wave_sync();
-    if (!piv) {
+    used = used || piv;
+    src = (L.j == k) ? p : src;
+  };
+  [&]<int... K>(std::integer_sequence<int, K...>) {
+    (step(std::integral_constant<int, K>{}), ...);
+  }(std::make_integer_sequence<int, N>{});
+  // row mystep of the reduced system is row mystep of X^T: lane (j, h) takes
+  // column j of X from the lane column pivoted at step j
+  const int sl = src + 16 * L.h;
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        A.v[r] = fma_(-f, prow[L.h * 4 + r], A.v[r]);
-        T.v[r] = fma_(-f, prow[64 + L.h * 4 + r], T.v[r]);
-      }
-    }
-    wave_sync();
-  }
-  // lane (j, h) holds rows row(h,.) of column `mystep` of X
-#pragma unroll
-  for (int r = 0; r < 4; ++r) perm[mystep * TS + row<R>(L.h, r)] = T.v[r];
-  wave_sync();
-#pragma unroll
-  for (int r = 0; r < 4; ++r) T.v[r] = perm[L.j * TS + row<R>(L.h, r)];
-  wave_sync();
+  for (int r = 0; r < 4; ++r) T.v[r] = shfl(T.v[r], sl);
+  fence_sched();
 }
 
-// x = M^-1 b: M by registers, b element j; returns x_j (all lanes of column j)
+// x = M^-1 b: M by registers, b element j (all lanes of column j); returns x_j
 template <typename R>
-__device__ __forceinline__ R left_solve(const Mat<R>& M, R b, const Lane& L, R* fv, R* pv,
-                                        R* prow, R* tile) {
+__device__ __forceinline__ R left_solve(const Mat<R>& M, R b, const Lane& L, R* tile) {
   // x^T M^T = b^T: right solve with A = M^T (lane j needs column j of M^T)
   put<R>(tile, L, M);
   wave_sync();
@@ -241,56 +278,29 @@ __device__ __forceinline__ R left_solve(const Mat<R>& M, R b, const Lane& L, R* 
   get_t<R>(tile, L, A);
   wave_sync();
   bool used = false;
-  int mystep = 0;
+  int src = 0;
   R t0 = b;
-#pragma unroll
-  for (int k = 0; k < N; ++k) {
-    int kh, kr;
-    if constexpr (sizeof(R) == 8) { kh = k % 4; kr = k / 4; }
-    else { kh = k / 4; kr = k % 4; }
-    if (L.h == kh) {
-      const R f = A.v[kr];
-      fv[L.j] = f;
-      pv[L.j] = used ? R(-1) : __builtin_fabs(f);
-    }
-    wave_sync();
-    const R f = fv[L.j];
-    int p = 0;
-    R best = pv[0];
-#pragma unroll
-    for (int i = 1; i < N; ++i) {
-      const R v = pv[i];
-      const bool gt = v > best;
-      best = gt ? v : best;
-      p = gt ? i : p;
-    }
-    p = __builtin_amdgcn_readfirstlane(p);
+  auto step = [&](auto kc) {
+    constexpr int k = decltype(kc)::value;
+    R f, fp;
+    const int p = gj_pivot<R, k>(A, L, used, f, fp);
+    const R inv = R(1) / fp;
     const bool piv = (L.j == p);
-    if (piv) {
-      const R inv = R(1) / f;
+    const int pl = p + 16 * L.h;
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        A.v[r] = div_by(A.v[r], f, inv);
-        prow[L.h * 4 + r] = A.v[r];
-      }
-      t0 = div_by(t0, f, inv);
-      if (L.h == 0) prow[64] = t0;
-      used = true;
-      mystep = k;
+    for (int r = 0; r < 4; ++r) {
+      const R pa = div_by(shfl(A.v[r], pl), fp, inv);
+      A.v[r] = piv ? pa : fma_(-f, pa, A.v[r]);
     }
-    wave_sync();
-    if (!piv) {
-#pragma unroll
-      for (int r = 0; r < 4; ++r) A.v[r] = fma_(-f, prow[L.h * 4 + r], A.v[r]);
-      t0 = fma_(-f, prow[64], t0);
-    }
-    wave_sync();
-  }
-  if (L.h == 0) fv[mystep] = t0;
-  wave_sync();
-  const R x = fv[L.j];
-  wave_sync();
-  return x;
+    const R ptv = div_by(readlane(t0, p), fp, inv);
+    t0 = piv ? ptv : fma_(-f, ptv, t0);
+    used = used || piv;
+    src = (L.j == k) ? p : src;
+  };
+  [&]<int... K>(std::integer_sequence<int, K...>) {
+    (step(std::integral_constant<int, K>{}), ...);
+  }(std::make_integer_sequence<int, N>{});
+  return shfl(t0, src + 16 * L.h);
 }
 
 // ordered sum over the 16 columns (one value per column), as the reference's
@@ -300,7 +310,7 @@ __device__ __forceinline__ R colsum(R v, const Lane& L, R* slot) {
   if (L.h == 0) slot[L.j] = v;
   wave_sync();
   R s = R(0);
-#pragma unroll
+#pragma unroll 4
   for (int i = 0; i < N; ++i) s += slot[i];
   wave_sync();
   return s;
