@@ -3,13 +3,16 @@
 The compute path is ``libadrt_hip.so`` (C-ABI: include/adrt_hip.h); this
 package is its host-side face:
 
-* ``solve_batch`` / ``BatchConfig`` -- batched columns on device tensors
-* ``ADConfig`` / ``solve``          -- the per-column drop-in of the
-  reference's ``adrt::ADConfig`` / ``adrt::solve``
+* ``solve_batch`` / ``BatchConfig`` -- batched forward solve on device tensors
+* ``adjoint_batch``                 -- hand-written reverse pass (gradients)
+* ``solve``                         -- differentiable solve (torch.autograd)
+* ``planck_levels``                 -- device band Planck, differentiable in T
 * ``workloads``                     -- synthetic inputs of the benchmark configs
 """
 from ._lib import AdrtError, AdrtLibraryError, lib  # noqa: F401
-from .solver import BatchConfig, Workspace, gauss_legendre, planck, planck_deriv, solve_batch  # noqa: F401
+from .solver import (BatchConfig, Workspace, adjoint_batch, gauss_legendre, planck,  # noqa: F401
+                     planck_deriv, planck_levels, solve, solve_batch)
 
-__all__ = ["AdrtError", "AdrtLibraryError", "BatchConfig", "Workspace", "gauss_legendre",
-           "planck", "planck_deriv", "solve_batch", "lib"]
+__all__ = ["AdrtError", "AdrtLibraryError", "BatchConfig", "Workspace", "adjoint_batch",
+           "gauss_legendre", "planck", "planck_deriv", "planck_levels", "solve", "solve_batch",
+           "lib"]

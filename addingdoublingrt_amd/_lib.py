@@ -15,6 +15,8 @@ LIB_PATH = os.path.join(_HERE, "libadrt_hip.so")
 ADRT_OK = 0
 PRECISION_F64 = 0
 PRECISION_MIXED = 1
+WORKSPACE_ADJOINT = 1
+PROFILE_KINDS = 6
 
 _dp = C.POINTER(C.c_double)
 
@@ -58,10 +60,34 @@ class DeviceOutputs(C.Structure):
     ]
 
 
+class DeviceCotangents(C.Structure):
+    _fields_ = [
+        ("flux_up", C.c_void_p), ("flux_down", C.c_void_p), ("mean_intensity", C.c_void_p),
+        ("flux_direct", C.c_void_p), ("flux_divergence", C.c_void_p),
+    ]
+
+
+class DeviceGradients(C.Structure):
+    _fields_ = [
+        ("delta_tau", C.c_void_p), ("single_scat_albedo", C.c_void_p),
+        ("phase_moments", C.c_void_p), ("planck_levels", C.c_void_p),
+        ("surface_emission", C.c_void_p), ("top_emission", C.c_void_p),
+        ("surface_albedo", C.c_void_p),
+    ]
+
+
 # Every symbol include/adrt_hip.h declares, with its ctypes signature.
 SIGNATURES = {
     "adrt_workspace_create": (C.c_int, [C.POINTER(BatchConfig), C.c_size_t, C.POINTER(C.c_void_p)]),
+    "adrt_workspace_create_ex": (C.c_int, [C.POINTER(BatchConfig), C.c_size_t, C.c_int,
+                                           C.POINTER(C.c_void_p)]),
     "adrt_workspace_destroy": (C.c_int, [C.c_void_p]),
+    "adrt_workspace_generation": (C.c_int, [C.c_void_p, C.POINTER(C.c_ulonglong)]),
+    "adrt_adjoint_batch": (C.c_int, [C.POINTER(BatchConfig), C.POINTER(DeviceInputs),
+                                     C.POINTER(DeviceOutputs), C.POINTER(DeviceCotangents),
+                                     C.POINTER(DeviceGradients), C.c_int, C.c_ulonglong,
+                                     C.c_void_p, C.c_void_p]),
+    "adrt_adjoint_status": (C.c_int, [C.c_void_p, C.c_void_p]),
     "adrt_workspace_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_size_t), C.POINTER(C.c_int)]),
     "adrt_solve_batch": (C.c_int, [C.POINTER(BatchConfig), C.POINTER(DeviceInputs),
                                    C.POINTER(DeviceOutputs), C.c_int, C.c_void_p, C.c_void_p]),

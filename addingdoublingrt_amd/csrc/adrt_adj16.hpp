@@ -1,0 +1,681 @@
+// adrt_adj16.hpp -- reverse pass (adjoint) of the N = 16 solve.
+//
+// The reference has no adjoint; this differentiates the same forward
+// algorithm (src/solver.cpp:348-722, src/doubling.hpp:29-230,
+// src/phase_matrix.hpp:61-181) in reverse mode:
+//
+//  1. adding16_kernel<true> solves the transposed interface system (see
+//     adrt_fwd16.hpp) for the adjoint vectors: per interface l, D_l (the
+//     multiplier of the equation of I_up(l)) and U_l (that of I_dn(l)).
+//  2. layer16_adjoint_kernel, one wave per (wavenumber, layer): the
+//     cotangents of the layer record are outer products of forward and adjoint
+//     interface vectors,
+//        R^ = D_l I_dn(l)^T + U_{l+1} I_up(l+1)^T,
+//        T^ = D_l I_up(l+1)^T + U_{l+1} I_dn(l)^T,  s_up^ = D_l, s_dn^ = U_{l+1};
+//     the doubling is recomputed from the thin-layer start with per-step
+//     checkpoints of R, T and (I - R^2)^-1 in a per-wave slab, then walked
+//     backwards with matrix-core products only; the thin-layer start, the
+//     Hansen normalisation and the Legendre sums are then differentiated.
+//  3. column16_adjoint_kernel, one thread per wavenumber: the cumulative
+//     optical depth (solar beam at layer tops, direct flux, surface beam),
+//     boundary and surface terms, level Planck values and the delta-M chain.
+//  4. moments16_reduce_kernel sums moment gradients over wavenumbers when the
+//     moments are shared.
+#pragma once
+
+#include "adrt_fwd16.hpp"
+
+namespace adrt {
+namespace dev {
+namespace w16 {
+
+struct AdjArgs {
+  double* slab;   // per-wave checkpoints [nwaves][nn_cap][SLAB]
+  int nwaves;     // waves of the persistent layer grid
+  int nn_cap;     // doubling steps a slab holds
+  int* status;    // bit 0: a layer needed more doubling steps than nn_cap
+  // per-task results, chunk-local [W][L]
+  double* g_taus;  // d/d scaled tau (layer terms; column terms added later)
+  double* g_omgs;  // d/d scaled omega
+  double* g_Bt;    // d/d B at the layer's top level
+  double* g_Bb;    // d/d B at the layer's bottom level
+  double* g_Ftop;  // d/d solar flux reaching the layer top
+  double* g_f;     // d/d delta-M fraction through the scaled moments
+  double* g_chi;   // [W][L][M] moment gradients (shared moments only)
+  // gradients in the caller's layout; null = not wanted
+  double* d_tau;     // [W][L]
+  double* d_ssa;     // [W][L]
+  double* d_mom;     // [W][L][M], or [L][M] for shared moments (accumulated over chunks)
+  double* d_B;       // [W][L+1]
+  double* d_surf;    // [W]
+  double* d_top;     // [W]
+  double* d_albedo;  // [W] per-wavenumber contributions
+};
+
+constexpr int SLAB = 3 * N * N + 4 * N + 2;  // R, T, (I-R^2)^-1 (register order); y, z, s_up, s_dn; gamma
+
+struct LdsA {
+  static constexpr int t0 = 0, t1 = TILE, t2 = 2 * TILE, t3 = 3 * TILE, vs = 4 * TILE;
+  static constexpr int size = vs + 8 * N;  // 8 vector slots; the last one for column sums
+};
+
+__device__ void layer16_adjoint_task(const FwdArgs& a, const AdjArgs& b, long task, const Lane& L,
+                                     double* sm, double* slab) {
+  using R = double;
+  const int Lay = a.L;
+  const int w = (int)(task / Lay), l = (int)(task % Lay);
+  const long gw = (long)(a.w0 + w);
+  const int j = L.j;
+  R* t0 = sm + LdsA::t0;
+  R* t1 = sm + LdsA::t1;
+  R* t2 = sm + LdsA::t2;
+  R* t3 = sm + LdsA::t3;
+  R* vs = sm + LdsA::vs;
+  R* csl = vs + 7 * N;  // column-sum slot
+
+  const double tau = a.tau_s[task];
+  const double omega_in = a.omega_s[task];
+  const int mode = a.mode[task];
+  const double Btop = a.B ? a.B[gw * (Lay + 1) + lev(a, l)] : 0.0;
+  const double Bbot = a.B ? a.B[gw * (Lay + 1) + lev(a, l + 1)] : 0.0;
+  const double B_bar = (Bbot + Btop) / 2.0;
+  const double B_d = (tau > 0.0) ? (Bbot - Btop) / tau : 0.0;
+  const double mu_j = a.mu[j], wt_j = a.wt[j];
+  const int li = lin(a, l);
+  double* chi_out = a.shared_moments ? b.g_chi + task * a.M
+                                     : (b.d_mom ? b.d_mom + (gw * Lay + li) * a.M : nullptr);
+
+  // cotangents of the layer record
+  const double* Iw = a.Iv + (long)w * (Lay + 1) * 2 * N;
+  const double* Aw = a.Av + (long)w * (Lay + 1) * 2 * N;
+  const double* Dl = Aw + (long)l * 2 * N + N;
+  const double* U1 = Aw + (long)(l + 1) * 2 * N;
+  const double idn_j = Iw[(long)l * 2 * N + N + j];
+  const double iup_j = Iw[(long)(l + 1) * 2 * N + j];
+  Mat<R> Rb, Tb;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int i = row<R>(L.h, r);
+    Rb.v[r] = Dl[i] * idn_j + U1[i] * iup_j;
+    Tb.v[r] = Dl[i] * iup_j + U1[i] * idn_j;
+  }
+  const double sub = Dl[j], sdb = U1[j];
+
+  double g_tau = 0.0, g_om = 0.0, g_Bt = 0.0, g_Bb = 0.0, g_F = 0.0, g_fr = 0.0;
+  auto finish = [&](int nchi) {  // moments nchi..M-1 get zero
+    if (chi_out)
+      for (int m = nchi + L.l; m < a.M; m += 64) chi_out[m] = 0.0;
+    if (L.l == 0) {
+      b.g_taus[task] = g_tau;
+      b.g_omgs[task] = g_om;
+      b.g_Bt[task] = g_Bt;
+      b.g_Bb[task] = g_Bb;
+      b.g_Ftop[task] = g_F;
+      b.g_f[task] = g_fr;
+    }
+  };
+
+  if (tau <= 0.0) {  // transparent: no dependence
+    finish(0);
+    return;
+  }
+  if (omega_in <= 0.0) {
+    // pure absorption (src/doubling.hpp:56-77): T = diag(t), s = B_bar (1-t) +- B_d slope
+    const double tex = -tau / mu_j;
+    const bool live = tex > -200.0;
+    const double t = live ? exp(tex) : 0.0;
+    const double dt = live ? -t / mu_j : 0.0;
+    const double omt = 1.0 - t;
+    const double slope = mu_j * omt - 0.5 * tau * (1.0 + t);
+    const double dslope = -mu_j * dt - 0.5 * (1.0 + t) - 0.5 * tau * dt;
+    const double dBd = -B_d / tau;
+    const double tbj = diag<R>(Tb, L);
+    const double dsu = -B_bar * dt + dBd * slope + B_d * dslope;
+    const double dsd = -B_bar * dt - dBd * slope - B_d * dslope;
+    g_tau = colsum<R>(tbj * dt + sub * dsu + sdb * dsd, L, csl);
+    const double gbar = colsum<R>((sub + sdb) * omt, L, csl);
+    const double gd = colsum<R>((sub - sdb) * slope, L, csl);
+    g_Bt = 0.5 * gbar - gd / tau;
+    g_Bb = 0.5 * gbar + gd / tau;
+    finish(0);
+    return;
+  }
+
+  // ---- phase matrices, keeping the sums before normalisation ----
+  const double omega = omega_in > 1.0 ? 1.0 : omega_in;
+  const double con = 2.0 * omega * kPi;
+  const double* chi = a.shared_moments ? a.mom + (long)li * a.M : a.mom + (gw * Lay + li) * a.M;
+  const int twoM = 2 * a.n_real;
+  int Lm = a.M;
+  double f = 0.0, inv1mf = 1.0;
+  if (mode == 3) {
+    f = a.fdm[task];
+    inv1mf = 1.0 / (1.0 - f);
+    Lm = twoM;
+  } else if (mode == 2) {
+    Lm = a.M < twoM ? a.M : twoM;
+  }
+  const bool solar = a.has_solar != 0;
+  double Qpp[4] = {0, 0, 0, 0}, Qpm[4] = {0, 0, 0, 0};  // unnormalised, / 2 pi
+  double sign = 1.0, spj = 0.0, smj = 0.0;
+  int rows[4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) rows[r] = row<R>(L.h, r);
+  for (int ll = 0; ll < Lm; ++ll) {
+    const double chil = (mode == 3) ? (chi[ll] - f) * inv1mf : chi[ll];
+    const double cl = (2 * ll + 1) * chil;
+    const double* P = a.Pl + ll * N;
+    const double plj = P[j];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const double term = cl * P[rows[r]] * plj;
+      Qpp[r] += term;
+      Qpm[r] += sign * term;
+    }
+    if (solar) {
+      const double term = cl * plj * a.Pl0[ll];
+      spj += term;
+      smj += sign * term;
+    }
+    sign = -sign;
+  }
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    Qpp[r] = Qpp[r] / (2.0 * kPi);
+    Qpm[r] = Qpm[r] / (2.0 * kPi);
+    t0[j * TS + rows[r]] = (Qpp[r] + Qpm[r]) * a.wt[rows[r]];
+  }
+  wave_sync();
+  double csum = 0.0;
+#pragma unroll
+  for (int i = 0; i < N; ++i) csum += t0[j * TS + i];
+  wave_sync();
+  const double corr = csum > 0.0 ? 1.0 / (2.0 * kPi * csum) : 1.0;
+  double Ppp[4], Ppm[4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    Ppp[r] = csum > 0.0 ? Qpp[r] * corr : Qpp[r];
+    Ppm[r] = csum > 0.0 ? Qpm[r] * corr : Qpm[r];
+  }
+  double qplus = 0.0, qminus = 0.0, ssol = 0.0, corr_s = 1.0, pplus = 0.0, pminus = 0.0;
+  if (solar) {
+    qplus = spj / (2.0 * kPi);
+    qminus = smj / (2.0 * kPi);
+    ssol = colsum<R>((qplus + qminus) * wt_j, L, csl);
+    corr_s = ssol > 0.0 ? 1.0 / (2.0 * kPi * ssol) : 1.0;
+    pplus = ssol > 0.0 ? qplus * corr_s : qplus;
+    pminus = ssol > 0.0 ? qminus * corr_s : qminus;
+  }
+
+  // ---- thin-layer start (src/doubling.hpp:83-147) ----
+  const int nn0 = (int)(log(tau) / log(2.0)) + ipow0(omega);
+  const int nn = nn0 < 1 ? 1 : nn0;
+  if (nn > b.nn_cap) {
+    if (L.l == 0) atomicOr(b.status, 1);
+    finish(0);
+    return;
+  }
+  const double tau0 = tau * ldexp(1.0, -nn);
+  Mat<R> Rm, Tm;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int i = rows[r];
+    const double mui = a.mu[i];
+    const double gpp = (((i == j) ? 1.0 : 0.0) + (-con) * (Ppp[r] * wt_j)) / mui;
+    const double gpm = con * (Ppm[r] * wt_j) / mui;
+    Tm.v[r] = ((i == j) ? 1.0 : 0.0) - tau0 * gpp;
+    Rm.v[r] = tau0 * gpm;
+  }
+  double y = (1.0 - omega) * tau0 / mu_j;
+  double z = 0.0;
+  double sus = 0.0, sds = 0.0, gam = 0.0, Ftop = 0.0, gam0 = 0.0;
+  if (solar) {
+    Ftop = a.F0 * exp(-a.tau_cum[(long)w * (Lay + 1) + l] / a.mu0);
+    const double base = omega * tau0 / mu_j * Ftop;
+    sus = base * pminus;
+    sds = base * pplus;
+    gam0 = exp(-tau0 / a.mu0);
+    gam = gam0;
+  }
+  double gk = 0.5 * tau0;
+  R* s0 = vs;
+  R* s1 = vs + N;
+  R* s2 = vs + 2 * N;
+  R* s3 = vs + 3 * N;
+  R* s4 = vs + 4 * N;
+  R* s5 = vs + 5 * N;
+  R* s6 = vs + 6 * N;
+
+  // ---- doubling recomputed with checkpoints; (I-R^2)^-1 by Gauss-Jordan ----
+  for (int k = 0; k < nn; ++k) {
+    R* st = slab + (long)k * SLAB;
+    gstore<R>(st, L, Rm);
+    gstore<R>(st + N * N, L, Tm);
+    if (L.h == 0) {
+      st[3 * N * N + j] = y;
+      st[3 * N * N + N + j] = z;
+      st[3 * N * N + 2 * N + j] = sus;
+      st[3 * N * N + 3 * N + j] = sds;
+    }
+    if (L.l == 0) st[3 * N * N + 4 * N] = gam;
+    Mat<R> A, Ai, TG, Y, X;
+    put<R>(t0, L, Rm);
+    if (L.h == 0) {
+      s0[j] = sds;
+      s1[j] = sus;
+    }
+    wave_sync();
+    eye_minus_mul<R>(t0, Rm, L, A);  // I - R^2
+    double b1 = 0.0, b2 = 0.0;
+    if (solar) {
+      double bs[2];
+      matvecs<R, 2>(t0, s0, L, bs);
+      b1 = bs[0];
+      b2 = bs[1];
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) Ai.v[r] = (rows[r] == j) ? 1.0 : 0.0;
+    right_solve<R>(A, Ai, L);  // (I - R^2)^-1
+    gstore<R>(st + 2 * N * N, L, Ai);
+    wave_sync();
+    put<R>(t1, L, Tm);
+    wave_sync();
+    mul<R>(t1, Ai, L, TG);  // TG = T (I - R^2)^-1
+    put<R>(t2, L, TG);
+    if (L.h == 0) {
+      s0[j] = z + gk * y;
+      s1[j] = y;
+      s2[j] = b1 + gam * sus;
+      s3[j] = gam * b2 + sds;
+    }
+    wave_sync();
+    mul<R>(t2, Rm, L, Y);  // TGR
+    mul<R>(t2, Tm, L, X);  // T_new = TG T
+    double a1, a3, c1 = 0.0, c2 = 0.0;
+    if (solar) {
+      double s4v[4];
+      matvecs<R, 4>(t2, s0, L, s4v);
+      a1 = s4v[0];
+      a3 = s4v[1];
+      c1 = s4v[2];
+      c2 = s4v[3];
+    } else {
+      double s2v[2];
+      matvecs<R, 2>(t2, s0, L, s2v);
+      a1 = s2v[0];
+      a3 = s2v[1];
+    }
+    put<R>(t0, L, Y);
+    wave_sync();
+    mul_add<R>(t0, Tm, Rm, L, Rm);  // R_new = R + TGR T
+    double s2v[2];
+    matvecs<R, 2>(t0, s0, L, s2v);
+    wave_sync();
+    Tm = X;
+    const double znew = (a1 - s2v[0]) + z - gk * y;
+    const double ynew = a3 + s2v[1] + y;
+    z = znew;
+    y = ynew;
+    if (solar) {
+      sus = c1 + sus;
+      sds = c2 + gam * sds;
+      gam = gam * gam;
+    }
+    gk = 2.0 * gk;
+  }
+
+  // ---- layer assembly, reversed: s_up = y B_bar + z B_d + s_up_sol, s_dn = y B_bar - z B_d + s_dn_sol
+  double yb = (sub + sdb) * B_bar, zb = (sub - sdb) * B_d, susb = sub, sdsb = sdb;
+  {
+    const double gbar = colsum<R>((sub + sdb) * y, L, csl);
+    const double gd = colsum<R>((sub - sdb) * z, L, csl);
+    g_Bt = 0.5 * gbar - gd / tau;
+    g_Bb = 0.5 * gbar + gd / tau;
+    g_tau = -gd * B_d / tau;
+  }
+  double gamb = 0.0, gb = 0.0;  // per-column parts of the scalar cotangents
+  Mat<R> Rbar = Rb, Tbar = Tb;
+
+  // ---- doubling steps in reverse ----
+  for (int k = nn - 1; k >= 0; --k) {
+    const R* st = slab + (long)k * SLAB;
+    Mat<R> Rk, Tk, Ai, TG, TGR, Tt, Rt;
+    gload<R>(st, L, Rk);
+    gload<R>(st + N * N, L, Tk);
+    gload<R>(st + 2 * N * N, L, Ai);
+    const double yk = st[3 * N * N + j], zk = st[3 * N * N + N + j];
+    const double susk = st[3 * N * N + 2 * N + j], sdsk = st[3 * N * N + 3 * N + j];
+    const double gm = st[3 * N * N + 4 * N];
+    const double g = 0.5 * tau0 * ldexp(1.0, k);
+    put<R>(t0, L, Tk);
+    put<R>(t2, L, Rk);
+    if (L.h == 0) {
+      s0[j] = zb;
+      s1[j] = yb;
+      s2[j] = susb;
+      s3[j] = sdsb;
+      s4[j] = sdsk;
+      s5[j] = susk;
+    }
+    wave_sync();
+    mul<R>(t0, Ai, L, TG);
+    get_t<R>(t0, L, Tt);
+    get_t<R>(t2, L, Rt);
+    wave_sync();
+    put<R>(t1, L, TG);
+    wave_sync();
+    mul<R>(t1, Rk, L, TGR);
+    const double u = zk + g * yk;
+    // vector recursions, reversed
+    const double x_zb = tmatvec<R>(TG, s0, L);
+    const double x_yb = tmatvec<R>(TG, s1, L);
+    const double r_zb = tmatvec<R>(TGR, s0, L);
+    const double r_yb = tmatvec<R>(TGR, s1, L);
+    const double ub = x_zb - r_zb;
+    const double yb_n = ((yb - g * zb) + (x_yb + r_yb)) + g * ub;
+    const double zb_n = zb + ub;
+    gb = 2.0 * gb + (ub - zb) * yk;
+    double v2 = 0.0, v3 = 0.0, b1b = 0.0, b2b = 0.0, susb_n = susb, sdsb_n = sdsb;
+    if (solar) {
+      const double v2b = tmatvec<R>(TG, s2, L);
+      const double v3b = tmatvec<R>(TG, s3, L);
+      double bs[2];
+      matvecs<R, 2>(t2, s4, L, bs);  // R s_dn, R s_up
+      v2 = bs[0] + gm * susk;
+      v3 = gm * bs[1] + sdsk;
+      b1b = v2b;
+      b2b = gm * v3b;
+      wave_sync();
+      if (L.h == 0) {
+        s6[j] = b1b;
+        s0[j] = b2b;  // s0 is free again (read above)
+      }
+      wave_sync();
+      const double rt_b1 = tmatvec<R>(Rk, s6, L);
+      const double rt_b2 = tmatvec<R>(Rk, s0, L);
+      susb_n = (susb + gm * v2b) + rt_b2;
+      sdsb_n = (gm * sdsb + v3b) + rt_b1;
+      gamb = 2.0 * gm * gamb + ((sdsb * sdsk + v2b * susk) + v3b * bs[1]);
+    }
+    // matrices, reversed
+    vec4<R> acc = zero4<R>();
+    put<R>(t0, L, Rbar);
+    put<R>(t3, L, Tbar);
+    wave_sync();
+    outer4<R>(acc, -zb, yb, 0.0, 0.0, u, yk, 0.0, 0.0, L);
+    mac<R>(acc, t0, Tt, L);  // TGR^ = R'^ T^T + vector terms
+    const Mat<R> TGRb = mat_of<R>(acc);
+    acc = zero4<R>();
+    outer4<R>(acc, zb, yb, susb, sdsb, u, yk, v2, v3, L);
+    mac<R>(acc, t3, Tt, L);  // TG^ = T'^ T^T + ...
+    wave_sync();
+    put<R>(t0, L, TGRb);
+    wave_sync();
+    mac<R>(acc, t0, Rt, L);  // ... + TGR^ R^T
+    const Mat<R> TGb = mat_of<R>(acc);
+    wave_sync();
+    put<R>(t3, L, Ai);
+    wave_sync();
+    Mat<R> AiT;
+    get_t<R>(t3, L, AiT);
+    wave_sync();
+    put<R>(t3, L, TGb);
+    wave_sync();
+    acc = zero4<R>();
+    mac<R>(acc, t3, AiT, L);  // W = TG^ (I - R^2)^-T
+    const Mat<R> W = mat_of<R>(acc);
+    acc = zero4<R>();
+    mac_t<R>(acc, t1, W, L);  // Abn = -A^ = TG^T W
+    const Mat<R> Abn = mat_of<R>(acc);
+    acc = acc_of<R>(Rbar);
+    outer4<R>(acc, b1b, b2b, 0.0, 0.0, sdsk, susk, 0.0, 0.0, L);
+    mac_t<R>(acc, t1, TGRb, L);  // + TG^T TGR^
+    wave_sync();
+    put<R>(t0, L, Abn);
+    wave_sync();
+    mac<R>(acc, t0, Rt, L);     // + Abn R^T
+    mac_t<R>(acc, t2, Abn, L);  // + R^T Abn
+    const Mat<R> Rbar_n = mat_of<R>(acc);
+    wave_sync();
+    put<R>(t3, L, TGR);
+    wave_sync();
+    acc = acc_of<R>(W);
+    mac_t<R>(acc, t3, Rbar, L);  // + TGR^T R'^
+    mac_t<R>(acc, t1, Tbar, L);  // + TG^T T'^
+    Tbar = mat_of<R>(acc);
+    Rbar = Rbar_n;
+    yb = yb_n;
+    zb = zb_n;
+    susb = susb_n;
+    sdsb = sdsb_n;
+    wave_sync();
+  }
+
+  // ---- thin-layer start, reversed ----
+  const double g_sum = colsum<R>(gb, L, csl);
+  const double gam_sum = solar ? colsum<R>(gamb, L, csl) : 0.0;
+  double tb_l = 0.0, cb_l = 0.0, ob_l = 0.0, fb_l = 0.0;
+  Mat<R> Pb_pp, Pb_pm;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int i = rows[r];
+    const double mui = a.mu[i];
+    const double dij = (i == j) ? 1.0 : 0.0;
+    const double tb = Tbar.v[r], rb = Rbar.v[r];
+    tb_l += tb * (-(dij - con * Ppp[r] * wt_j) / mui) + rb * (con * Ppm[r] * wt_j / mui);
+    cb_l += (tb * Ppp[r] + rb * Ppm[r]) * (tau0 * wt_j / mui);
+    Pb_pp.v[r] = tb * (tau0 * con * wt_j / mui);
+    Pb_pm.v[r] = rb * (tau0 * con * wt_j / mui);
+  }
+  double pplusb = 0.0, pminusb = 0.0;
+  if (L.h == 0) {
+    tb_l += yb * (1.0 - omega) / mu_j;
+    ob_l += -yb * tau0 / mu_j;
+  }
+  if (solar) {
+    const double base = omega * tau0 / mu_j * Ftop;
+    const double baseb = susb * pminus + sdsb * pplus;
+    pminusb = susb * base;
+    pplusb = sdsb * base;
+    if (L.h == 0) {
+      ob_l += baseb * tau0 * Ftop / mu_j;
+      tb_l += baseb * omega * Ftop / mu_j;
+      fb_l += baseb * omega * tau0 / mu_j;
+    }
+  }
+  double taub0 = wave_sum<R>(tb_l, L) + 0.5 * g_sum;
+  if (solar) taub0 -= gam_sum * gam0 / a.mu0;
+  const double conb = wave_sum<R>(cb_l, L);
+  g_om = wave_sum<R>(ob_l, L) + 2.0 * kPi * conb;
+  if (omega_in > 1.0) g_om = 0.0;  // clamped to 1
+  g_F = solar ? wave_sum<R>(fb_l, L) : 0.0;
+  g_tau += taub0 * ldexp(1.0, -nn);
+
+  // ---- Hansen normalisation and Legendre sums, reversed ----
+  double Qb_pp[4], Qb_pm[4];
+  if (csum > 0.0) {
+    double p = 0.0;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) p += Pb_pp.v[r] * Qpp[r] + Pb_pm.v[r] * Qpm[r];
+    const double corrb = col_reduce<R>(p, L);
+    const double csumb = -corrb * corr / csum;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      Qb_pp[r] = Pb_pp.v[r] * corr + csumb * a.wt[rows[r]];
+      Qb_pm[r] = Pb_pm.v[r] * corr + csumb * a.wt[rows[r]];
+    }
+  } else {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      Qb_pp[r] = Pb_pp.v[r];
+      Qb_pm[r] = Pb_pm.v[r];
+    }
+  }
+  double qplusb = pplusb, qminusb = pminusb;
+  if (solar && ssol > 0.0) {
+    const double corrb = colsum<R>(pplusb * qplus + pminusb * qminus, L, csl);
+    const double sb = -corrb * corr_s / ssol;
+    qplusb = pplusb * corr_s + sb * wt_j;
+    qminusb = pminusb * corr_s + sb * wt_j;
+  }
+  sign = 1.0;
+  for (int ll = 0; ll < Lm; ++ll) {
+    const double* P = a.Pl + ll * N;
+    const double plj = P[j];
+    double q = 0.0;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) q += P[rows[r]] * (Qb_pp[r] + sign * Qb_pm[r]);
+    double part = q * plj;
+    if (solar && L.h == 0) part += a.Pl0[ll] * plj * (qplusb + sign * qminusb);
+    const double chisb = wave_sum<R>(part, L) * ((2 * ll + 1) / (2.0 * kPi));
+    if (mode == 3) g_fr += chisb * (chi[ll] - 1.0) * inv1mf * inv1mf;
+    if (chi_out && L.l == 0) chi_out[ll] = (mode == 3) ? chisb * inv1mf : chisb;
+    sign = -sign;
+  }
+  finish(Lm);
+}
+
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 8)))
+layer16_adjoint_kernel(FwdArgs a, AdjArgs b) {
+  __shared__ double smem[4 * LdsA::size];
+  const int wv = threadIdx.x >> 6;
+  const int gwave = blockIdx.x * 4 + wv;
+  const Lane L;
+  double* sm = smem + wv * LdsA::size;
+  double* slab = b.slab + (long)gwave * b.nn_cap * SLAB;
+  const long ntask = (long)a.W * a.L;
+  for (long task = gwave; task < ntask; task += b.nwaves) layer16_adjoint_task(a, b, task, L, sm, slab);
+}
+
+// one thread per wavenumber
+__global__ void __launch_bounds__(256) column16_adjoint_kernel(FwdArgs a, AdjArgs b) {
+  const int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= a.W) return;
+  const long gw = (long)(a.w0 + w);
+  const int Lay = a.L;
+  const double* Aw = a.Av + (long)w * (Lay + 1) * 2 * N;
+  const double* Iw = a.Iv + (long)w * (Lay + 1) * 2 * N;
+  double* gts = b.g_taus + (long)w * Lay;
+  const double* Bw = a.B ? a.B + gw * (Lay + 1) : nullptr;
+  auto Blev = [&](int l) -> double { return Bw ? Bw[lev(a, l)] : 0.0; };
+  const double Bs = a.surf_w ? a.surf_w[gw] : a.surf_em;
+  const double A = a.albedo;
+  const int has_surface = !a.diffusion && (A > 0.0 || Bs > 0.0);
+  const double tau_tot = a.tau_cum[(long)w * (Lay + 1) + Lay];
+  const double* DL = Aw + (long)Lay * 2 * N + N;
+  double sumD = 0.0, sumDn = 0.0, sumU0 = 0.0, muwI = 0.0, muD = 0.0;
+  for (int i = 0; i < N; ++i) {
+    sumD += DL[i];
+    if (i < a.n_real) sumDn += DL[i];
+    sumU0 += Aw[i];
+    muwI += a.mu[i] * a.wt[i] * Iw[(long)Lay * 2 * N + N + i];
+    muD += DL[i] * a.mu[i];
+  }
+  // cumulative optical depth: solar beam at layer tops, direct flux and beam
+  // term of J at interfaces, surface beam; tau_cum(l) = sum of scaled tau above l
+  if (a.has_solar) {
+    double run = 0.0;
+    for (int l = Lay; l >= 1; --l) {
+      const double ex = exp(-a.tau_cum[(long)w * (Lay + 1) + l] / a.mu0);
+      const long o = gw * (Lay + 1) + lev(a, l);
+      double cdir = a.cdir ? a.cdir[o] : 0.0;
+      double cJ = a.cJ ? a.cJ[o] : 0.0;
+      if (a.cdiv) cJ += a.cdiv[o] * (4.0 * kPi * (1.0 - a.ssa[gw * Lay + lin(a, l - 1)]));
+      cdir += cJ / (4.0 * kPi * a.mu0);
+      double tcb = -cdir * a.F0 * ex;                                   // F_dir = F0 mu0 ex
+      if (l < Lay) tcb -= b.g_Ftop[(long)w * Lay + l] * a.F0 * ex / a.mu0;  // F_top = F0 ex
+      if (l == Lay && has_surface && A > 0.0) tcb -= sumD * (A / kPi) * a.F0 * ex;
+      run += tcb;
+      gts[l - 1] += run;
+    }
+  }
+  // boundaries (src/solver.cpp:385-419, 516-551)
+  if (b.d_top) b.d_top[gw] = sumU0;
+  double dBL = 0.0, dBLm1 = 0.0;
+  if (a.diffusion) {
+    const double dt = a.tau_s[(long)w * Lay + Lay - 1];
+    if (dt > 0.0) {
+      dBL = sumD + muD / dt;
+      dBLm1 = -muD / dt;
+      gts[Lay - 1] += -muD * (Blev(Lay) - Blev(Lay - 1)) / (dt * dt);
+    } else {
+      dBL = sumD;
+    }
+    if (b.d_surf) b.d_surf[gw] = 0.0;
+    if (b.d_albedo) b.d_albedo[gw] = 0.0;
+  } else {
+    if (b.d_surf) b.d_surf[gw] = (1.0 - A) * sumD;
+    if (b.d_albedo) {
+      double g = 2.0 * a.xfac * sumDn * muwI - Bs * sumD;
+      if (a.has_solar) g += sumD * a.F0 * a.mu0 * exp(-tau_tot / a.mu0) / kPi;
+      b.d_albedo[gw] = g;
+    }
+  }
+  // level Planck values
+  if (b.d_B) {
+    for (int l = 0; l <= Lay; ++l) {
+      double v = 0.0;
+      if (l >= 1) v += b.g_Bb[(long)w * Lay + l - 1];
+      if (l < Lay) v += b.g_Bt[(long)w * Lay + l];
+      const long o = gw * (Lay + 1) + lev(a, l);
+      if (a.cdiv) {
+        const int lyr = (l == 0) ? 0 : l - 1;
+        v -= a.cdiv[o] * (4.0 * kPi * (1.0 - a.ssa[gw * Lay + lin(a, lyr)]));
+      }
+      if (l == Lay) v += dBL;
+      if (l == Lay - 1) v += dBLm1;
+      b.d_B[o] = v;
+    }
+  }
+  // divergence omega, delta-M chain (src/solver.cpp:445-486)
+  for (int l = 0; l < Lay; ++l) {
+    const long t = (long)w * Lay + l;
+    const int li = lin(a, l);
+    const double taus_b = gts[l], oms_b = b.g_omgs[t];
+    const double tau = a.tau[gw * Lay + li], om = a.ssa[gw * Lay + li];
+    double omb = 0.0, taub;
+    if (a.cdiv) {
+      const long o1 = gw * (Lay + 1) + lev(a, l + 1);
+      omb -= a.cdiv[o1] * 4.0 * kPi * (a.mi[o1] - Blev(l + 1));
+      if (l == 0) {
+        const long o0 = gw * (Lay + 1) + lev(a, 0);
+        omb -= a.cdiv[o0] * 4.0 * kPi * (a.mi[o0] - Blev(0));
+      }
+    }
+    if (a.mode[t] == 3) {
+      const double f = a.fdm[t], d = 1.0 - om * f;
+      taub = taus_b * d;
+      omb += oms_b * (1.0 - f) / (d * d) - taus_b * f * tau;
+      const double fb = b.g_f[t] + oms_b * om * (om - 1.0) / (d * d) - taus_b * om * tau;
+      double* co = a.shared_moments ? b.g_chi + t * a.M
+                                    : (b.d_mom ? b.d_mom + (gw * Lay + li) * a.M : nullptr);
+      if (co) co[2 * a.n_real] += fb;
+    } else {
+      taub = taus_b;
+      omb += oms_b;
+    }
+    if (b.d_tau) b.d_tau[gw * Lay + li] = taub;
+    if (b.d_ssa) b.d_ssa[gw * Lay + li] = omb;
+  }
+}
+
+// shared moments: d_mom[l'][m] += sum over the chunk's wavenumbers, in order
+__global__ void __launch_bounds__(256) moments16_reduce_kernel(FwdArgs a, AdjArgs b) {
+  const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= (long)a.L * a.M) return;
+  const int l = (int)(k / a.M), m = (int)(k % a.M);
+  double s = 0.0;
+  for (int w = 0; w < a.W; ++w) s += b.g_chi[((long)w * a.L + l) * a.M + m];
+  b.d_mom[(long)lin(a, l) * a.M + m] += s;
+}
+
+__global__ void fill_kernel(double* p, long n, double v) {
+  const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) p[k] = v;
+}
+
+__global__ void fill_int_kernel(int* p, int v) { *p = v; }
+
+}  // namespace w16
+}  // namespace dev
+}  // namespace adrt

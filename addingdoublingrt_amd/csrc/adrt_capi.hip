@@ -17,7 +17,7 @@
 
 #include "../../include/adrt_hip.h"
 #include "adrt_fwd.hpp"
-#include "adrt_fwd16.hpp"
+#include "adrt_adj16.hpp"
 
 namespace {
 
@@ -210,12 +210,20 @@ struct adrt_workspace {
   double *tau_s = nullptr, *omega_s = nullptr, *tau_cum = nullptr, *fdm = nullptr;
   double *lay = nullptr, *rb = nullptr;
   int *mode = nullptr, *lflag = nullptr, *flag = nullptr;
+  // reverse pass (ADRT_WORKSPACE_ADJOINT)
+  int flags = 0;
+  double *Iv = nullptr, *Av = nullptr, *g_taus = nullptr, *g_omgs = nullptr, *g_Bt = nullptr,
+         *g_Bb = nullptr, *g_Ftop = nullptr, *g_f = nullptr, *g_chi = nullptr, *slab = nullptr;
+  int* status = nullptr;
+  int nwaves = 0, nn_cap = 0;
+  unsigned long long generation = 0;  // forward solves so far
+  bool kept_valid = false;            // the last forward's state (one chunk) is still held
   double xfac = 0.0;
   double tables_mu0 = -1.0;
   bool tables_ready = false;
   std::vector<double> h_mu, h_wt;
   // opt-in kernel timing: event pairs recorded on the caller's stream around
-  // every launch (kind 0 prep, 1 layer/doubling, 2 adding)
+  // every launch (kinds as adrt_profile_read)
   bool profiling = false;
   std::vector<hipEvent_t> ev_pool;
   std::vector<int> ev_kind;  // one per recorded pair
@@ -224,13 +232,18 @@ struct adrt_workspace {
 
 namespace {
 
-size_t per_wavenumber_bytes(int L, int Np) {
+size_t per_wavenumber_bytes(int L, int Np, int M, bool adjoint) {
   const size_t d = sizeof(double);
   size_t b = 0;
   b += 4 * (size_t)L * d + (size_t)(L + 1) * d;      // tau_s, omega_s, fdm, spare, tau_cum
   b += 2 * (size_t)L * sizeof(int);                   // mode, lflag
   b += (size_t)L * (2 * Np * Np + 2 * Np) * d;        // layer store
   b += (size_t)L * (2 * Np * Np + Np) * d;            // base composites
+  if (adjoint) {
+    b += 2 * (size_t)(L + 1) * 2 * Np * d;            // forward and adjoint interface vectors
+    b += 6 * (size_t)L * d;                           // per-layer gradient terms
+    b += (size_t)L * M * d;                           // per-layer moment gradients
+  }
   return b;
 }
 
@@ -296,7 +309,7 @@ int launch_chunk(const adrt::dev::FwdArgs& a, adrt_workspace* ws, hipStream_t st
     w16::layer16_kernel<<<(unsigned)((tasks + 3) / 4), 256, 0, stream>>>(a);
     prof_end(ws, e, stream);
     e = prof_begin(ws, 2, stream);
-    w16::adding16_kernel<<<(unsigned)((a.W + 3) / 4), 256, 0, stream>>>(a);
+    w16::adding16_kernel<false><<<(unsigned)((a.W + 3) / 4), 256, 0, stream>>>(a);
     prof_end(ws, e, stream);
   } else {
     e = prof_begin(ws, 1, stream);
@@ -314,15 +327,23 @@ int launch_chunk(const adrt::dev::FwdArgs& a, adrt_workspace* ws, hipStream_t st
 extern "C" {
 
 int adrt_workspace_create(const adrt_batch_config* cfg, size_t max_bytes, adrt_workspace** out) {
+  return adrt_workspace_create_ex(cfg, max_bytes, 0, out);
+}
+
+int adrt_workspace_create_ex(const adrt_batch_config* cfg, size_t max_bytes, int flags,
+                             adrt_workspace** out) {
   if (!out) return ADRT_ERR_NULL;
   *out = nullptr;
   int rc = check_config(cfg);
   if (rc) return rc;
+  const bool adjoint = (flags & ADRT_WORKSPACE_ADJOINT) != 0;
+  if (adjoint && cfg->num_quadrature > 16) return ADRT_ERR_ADJOINT;
   adrt_workspace* ws = new (std::nothrow) adrt_workspace();
   if (!ws) return ADRT_ERR_WORKSPACE;
+  ws->flags = flags;
   ws->L = cfg->num_layers;
   ws->n_real = cfg->num_quadrature;
-  ws->Np = padded_order(cfg->num_quadrature);
+  ws->Np = adjoint ? 16 : padded_order(cfg->num_quadrature);
   ws->M = cfg->num_moments;
   ws->Lmax = std::max(cfg->num_moments, 2 * cfg->num_quadrature);
   ws->h_mu.assign(ws->n_real, 0.0);
@@ -331,14 +352,30 @@ int adrt_workspace_create(const adrt_batch_config* cfg, size_t max_bytes, adrt_w
   double s = 0.0;
   for (int i = 0; i < ws->n_real; ++i) s += ws->h_mu[i] * ws->h_wt[i];
   ws->xfac = 0.5 / s;  // src/solver.cpp:378-383
+  if (hipGetDevice(&ws->device) != hipSuccess) {
+    delete ws;
+    return ADRT_ERR_HIP;
+  }
+  size_t slab_bytes = 0;
+  if (adjoint) {
+    // persistent reverse-layer grid: 4 waves per SIMD, 4 SIMDs per CU
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ws->device) !=
+            hipSuccess || cus <= 0)
+      cus = 256;
+    ws->nwaves = cus * 16;
+    ws->nn_cap = 64;
+    slab_bytes = (size_t)ws->nwaves * ws->nn_cap * adrt::dev::w16::SLAB * sizeof(double);
+  }
   if (max_bytes == 0) max_bytes = (size_t)4 << 30;
-  const size_t per_w = per_wavenumber_bytes(ws->L, ws->Np);
-  const size_t fixed = align256((size_t)(2 * ws->Np + ws->Lmax * ws->Np + ws->Lmax) * 8) + 256;
-  long chunk = max_bytes > fixed ? (long)((max_bytes - fixed) / (per_w + 2048)) : 1;
+  const size_t per_w = per_wavenumber_bytes(ws->L, ws->Np, ws->M, adjoint);
+  const size_t fixed =
+      align256((size_t)(2 * ws->Np + ws->Lmax * ws->Np + ws->Lmax) * 8) + 512 + slab_bytes;
+  long chunk = max_bytes > fixed ? (long)((max_bytes - fixed) / (per_w + 4096)) : 1;
   const int W = std::max(cfg->num_wavenumbers, 1);
   chunk = std::max(1L, std::min(chunk, (long)W));
   ws->chunk = (int)chunk;
-  const size_t L = ws->L, Np = ws->Np, C = chunk;
+  const size_t L = ws->L, Np = ws->Np, C = chunk, M = ws->M;
   size_t off = 0;
   auto take = [&](size_t b) {
     size_t o = off;
@@ -351,7 +388,16 @@ int adrt_workspace_create(const adrt_batch_config* cfg, size_t max_bytes, adrt_w
   const size_t o_mode = take(C * L * 4), o_lflag = take(C * L * 4), o_flag = take(4);
   const size_t o_lay = take(C * L * (2 * Np * Np + 2 * Np) * 8);
   const size_t o_rb = take(C * L * (2 * Np * Np + Np) * 8);
-  if (hipGetDevice(&ws->device) != hipSuccess || hipMalloc((void**)&ws->base, off) != hipSuccess) {
+  size_t o_iv = 0, o_av = 0, o_g = 0, o_chi = 0, o_slab = 0, o_status = 0;
+  if (adjoint) {
+    o_iv = take(C * (L + 1) * 2 * Np * 8);
+    o_av = take(C * (L + 1) * 2 * Np * 8);
+    o_g = take(6 * C * L * 8);
+    o_chi = take(C * L * M * 8);
+    o_slab = take(slab_bytes);
+    o_status = take(4);
+  }
+  if (hipMalloc((void**)&ws->base, off) != hipSuccess) {
     delete ws;
     return ADRT_ERR_HIP;
   }
@@ -369,6 +415,25 @@ int adrt_workspace_create(const adrt_batch_config* cfg, size_t max_bytes, adrt_w
   ws->flag = (int*)(ws->base + o_flag);
   ws->lay = (double*)(ws->base + o_lay);
   ws->rb = (double*)(ws->base + o_rb);
+  if (adjoint) {
+    ws->Iv = (double*)(ws->base + o_iv);
+    ws->Av = (double*)(ws->base + o_av);
+    double* g = (double*)(ws->base + o_g);
+    ws->g_taus = g;
+    ws->g_omgs = g + C * L;
+    ws->g_Bt = g + 2 * C * L;
+    ws->g_Bb = g + 3 * C * L;
+    ws->g_Ftop = g + 4 * C * L;
+    ws->g_f = g + 5 * C * L;
+    ws->g_chi = (double*)(ws->base + o_chi);
+    ws->slab = (double*)(ws->base + o_slab);
+    ws->status = (int*)(ws->base + o_status);
+    if (hipMemset(ws->status, 0, sizeof(int)) != hipSuccess) {
+      hipFree(ws->base);
+      delete ws;
+      return ADRT_ERR_HIP;
+    }
+  }
   *out = ws;
   return ADRT_OK;
 }
@@ -389,28 +454,12 @@ int adrt_workspace_info(const adrt_workspace* ws, size_t* bytes, int* chunk) {
   return ADRT_OK;
 }
 
-int adrt_solve_batch(const adrt_batch_config* cfg, const adrt_device_inputs* in,
-                     const adrt_device_outputs* out, adrt_precision precision,
-                     adrt_workspace* ws, hipStream_t stream) {
-  int rc = check_config(cfg);
-  if (rc) return rc;
-  if (!in || !out) return ADRT_ERR_NULL;
-  if (cfg->num_wavenumbers == 0) return ADRT_OK;  // empty batch: nothing to read or write
-  if (!in->delta_tau) return ADRT_ERR_DELTA_TAU;
-  if (!in->single_scat_albedo) return ADRT_ERR_SSA;
-  if (!in->phase_moments) return ADRT_ERR_MOMENTS;
-  if (cfg->use_diffusion_lower_bc && !in->planck_levels) return ADRT_ERR_DIFFUSION_BC;
-  if (!out->flux_up || !out->flux_down || !out->mean_intensity || !out->flux_direct)
-    return ADRT_ERR_OUTPUTS;
-  if (precision != ADRT_PRECISION_F64) return ADRT_ERR_PRECISION;
-  if (!ws) return ADRT_ERR_WORKSPACE;
-  if (ws->L != cfg->num_layers || ws->n_real != cfg->num_quadrature || ws->M != cfg->num_moments)
-    return ADRT_ERR_WORKSPACE;
-  if (cfg->num_wavenumbers == 0) return ADRT_OK;
-  const bool has_solar = cfg->solar_flux > 0.0 && cfg->solar_mu > 0.0;
-  rc = upload_tables(ws, has_solar ? cfg->solar_mu : 1.0, stream);
-  if (rc) return rc;
+}  // extern "C"
 
+namespace {
+
+adrt::dev::FwdArgs make_args(const adrt_batch_config* cfg, const adrt_device_inputs* in,
+                             const adrt_device_outputs* out, adrt_workspace* ws, bool has_solar) {
   adrt::dev::FwdArgs a{};
   a.L = cfg->num_layers;
   a.M = cfg->num_moments;
@@ -450,22 +499,164 @@ int adrt_solve_batch(const adrt_batch_config* cfg, const adrt_device_inputs* in,
   a.lay = ws->lay;
   a.lflag = ws->lflag;
   a.rb = ws->rb;
+  a.Iv = ws->Iv;  // kept for the reverse pass (adjoint workspaces only)
+  a.Av = ws->Av;
+  return a;
+}
+
+int forward_chunk(const adrt::dev::FwdArgs& a, adrt_workspace* ws, hipStream_t stream) {
+  switch (ws->Np) {
+#ifndef ADRT_ONLY_N
+    case 2: return launch_chunk<2>(a, ws, stream);
+    case 4: return launch_chunk<4>(a, ws, stream);
+    case 8: return launch_chunk<8>(a, ws, stream);
+    case 16: return launch_chunk<16>(a, ws, stream);
+    case 32: return launch_chunk<32>(a, ws, stream);
+#else
+    case ADRT_ONLY_N: return launch_chunk<ADRT_ONLY_N>(a, ws, stream);
+#endif
+    default: return ADRT_ERR_QUADRATURE_MAX;
+  }
+}
+
+int check_solve_args(const adrt_batch_config* cfg, const adrt_device_inputs* in,
+                     const adrt_device_outputs* out, adrt_precision precision,
+                     adrt_workspace* ws) {
+  if (!in->delta_tau) return ADRT_ERR_DELTA_TAU;
+  if (!in->single_scat_albedo) return ADRT_ERR_SSA;
+  if (!in->phase_moments) return ADRT_ERR_MOMENTS;
+  if (cfg->use_diffusion_lower_bc && !in->planck_levels) return ADRT_ERR_DIFFUSION_BC;
+  if (!out->flux_up || !out->flux_down || !out->mean_intensity || !out->flux_direct)
+    return ADRT_ERR_OUTPUTS;
+  if (precision != ADRT_PRECISION_F64) return ADRT_ERR_PRECISION;
+  if (!ws) return ADRT_ERR_WORKSPACE;
+  if (ws->L != cfg->num_layers || ws->n_real != cfg->num_quadrature || ws->M != cfg->num_moments)
+    return ADRT_ERR_WORKSPACE;
+  return ADRT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int adrt_solve_batch(const adrt_batch_config* cfg, const adrt_device_inputs* in,
+                     const adrt_device_outputs* out, adrt_precision precision,
+                     adrt_workspace* ws, hipStream_t stream) {
+  int rc = check_config(cfg);
+  if (rc) return rc;
+  if (!in || !out) return ADRT_ERR_NULL;
+  if (cfg->num_wavenumbers == 0) return ADRT_OK;  // empty batch: nothing to read or write
+  rc = check_solve_args(cfg, in, out, precision, ws);
+  if (rc) return rc;
+  const bool has_solar = cfg->solar_flux > 0.0 && cfg->solar_mu > 0.0;
+  rc = upload_tables(ws, has_solar ? cfg->solar_mu : 1.0, stream);
+  if (rc) return rc;
+  adrt::dev::FwdArgs a = make_args(cfg, in, out, ws, has_solar);
+  ws->kept_valid = false;
   for (int w0 = 0; w0 < cfg->num_wavenumbers; w0 += ws->chunk) {
     a.w0 = w0;
     a.W = std::min(ws->chunk, cfg->num_wavenumbers - w0);
-    switch (ws->Np) {
-#ifndef ADRT_ONLY_N
-      case 2: rc = launch_chunk<2>(a, ws, stream); break;
-      case 4: rc = launch_chunk<4>(a, ws, stream); break;
-      case 8: rc = launch_chunk<8>(a, ws, stream); break;
-      case 16: rc = launch_chunk<16>(a, ws, stream); break;
-      case 32: rc = launch_chunk<32>(a, ws, stream); break;
-#else
-      case ADRT_ONLY_N: rc = launch_chunk<ADRT_ONLY_N>(a, ws, stream); break;
-#endif
-      default: rc = ADRT_ERR_QUADRATURE_MAX;
-    }
+    rc = forward_chunk(a, ws, stream);
     if (rc) return rc;
+  }
+  ws->generation += 1;
+  ws->kept_valid = ws->Iv != nullptr && cfg->num_wavenumbers <= ws->chunk;
+  return ADRT_OK;
+}
+
+int adrt_workspace_generation(const adrt_workspace* ws, unsigned long long* generation) {
+  if (!ws || !generation) return ADRT_ERR_NULL;
+  *generation = ws->generation;
+  return ADRT_OK;
+}
+
+int adrt_adjoint_batch(const adrt_batch_config* cfg, const adrt_device_inputs* in,
+                       const adrt_device_outputs* out, const adrt_device_cotangents* cot,
+                       const adrt_device_gradients* grad, adrt_precision precision,
+                       unsigned long long forward_generation, adrt_workspace* ws,
+                       hipStream_t stream) {
+  using namespace adrt::dev;
+  int rc = check_config(cfg);
+  if (rc) return rc;
+  if (!in || !out || !cot || !grad) return ADRT_ERR_NULL;
+  if (cfg->num_wavenumbers == 0) return ADRT_OK;
+  rc = check_solve_args(cfg, in, out, precision, ws);
+  if (rc) return rc;
+  if (!ws->Iv || ws->Np != 16) return ADRT_ERR_ADJOINT;
+  const bool has_solar = cfg->solar_flux > 0.0 && cfg->solar_mu > 0.0;
+  rc = upload_tables(ws, has_solar ? cfg->solar_mu : 1.0, stream);
+  if (rc) return rc;
+  FwdArgs a = make_args(cfg, in, out, ws, has_solar);
+  a.cfu = cot->flux_up;
+  a.cfd = cot->flux_down;
+  a.cJ = cot->mean_intensity;
+  a.cdir = cot->flux_direct;
+  a.cdiv = cot->flux_divergence;
+  w16::AdjArgs b{};
+  b.slab = ws->slab;
+  b.nwaves = ws->nwaves;
+  b.nn_cap = ws->nn_cap;
+  b.status = ws->status;
+  b.g_taus = ws->g_taus;
+  b.g_omgs = ws->g_omgs;
+  b.g_Bt = ws->g_Bt;
+  b.g_Bb = ws->g_Bb;
+  b.g_Ftop = ws->g_Ftop;
+  b.g_f = ws->g_f;
+  b.g_chi = ws->g_chi;
+  b.d_tau = grad->delta_tau;
+  b.d_ssa = grad->single_scat_albedo;
+  b.d_mom = grad->phase_moments;
+  b.d_B = grad->planck_levels;
+  b.d_surf = grad->surface_emission;
+  b.d_top = grad->top_emission;
+  b.d_albedo = grad->surface_albedo;
+  const bool shared = in->phase_moments_shared != 0;
+  if (shared && b.d_mom) {  // accumulated over chunks
+    const long n = (long)cfg->num_layers * cfg->num_moments;
+    w16::fill_kernel<<<(unsigned)((n + 255) / 256), 256, 0, stream>>>(b.d_mom, n, 0.0);
+  }
+  const bool reuse = forward_generation != 0 && ws->kept_valid &&
+                     forward_generation == ws->generation && cfg->num_wavenumbers <= ws->chunk;
+  for (int w0 = 0; w0 < cfg->num_wavenumbers; w0 += ws->chunk) {
+    a.w0 = w0;
+    a.W = std::min(ws->chunk, cfg->num_wavenumbers - w0);
+    if (!reuse) {
+      rc = forward_chunk(a, ws, stream);
+      if (rc) return rc;
+      ws->kept_valid = false;
+    }
+    int e = prof_begin(ws, 3, stream);
+    w16::adding16_kernel<true><<<(unsigned)((a.W + 3) / 4), 256, 0, stream>>>(a);
+    prof_end(ws, e, stream);
+    const long tasks = (long)a.W * a.L;
+    const long blocks = std::min((tasks + 3) / 4, (long)ws->nwaves / 4);
+    b.nwaves = (int)(blocks * 4);
+    e = prof_begin(ws, 4, stream);
+    w16::layer16_adjoint_kernel<<<(unsigned)blocks, 256, 0, stream>>>(a, b);
+    prof_end(ws, e, stream);
+    e = prof_begin(ws, 5, stream);
+    w16::column16_adjoint_kernel<<<(unsigned)((a.W + 255) / 256), 256, 0, stream>>>(a, b);
+    if (shared && b.d_mom) {
+      const long n = (long)a.L * a.M;
+      w16::moments16_reduce_kernel<<<(unsigned)((n + 255) / 256), 256, 0, stream>>>(a, b);
+    }
+    prof_end(ws, e, stream);
+    if (hipGetLastError() != hipSuccess) return ADRT_ERR_HIP;
+  }
+  return ADRT_OK;
+}
+
+int adrt_adjoint_status(adrt_workspace* ws, hipStream_t stream) {
+  if (!ws) return ADRT_ERR_NULL;
+  if (!ws->status) return ADRT_ERR_ADJOINT;
+  int h = 0;
+  ADRT_HIP_TRY(hipMemcpyAsync(&h, ws->status, sizeof(int), hipMemcpyDeviceToHost, stream));
+  ADRT_HIP_TRY(hipStreamSynchronize(stream));
+  if (h) {
+    adrt::dev::w16::fill_int_kernel<<<1, 1, 0, stream>>>(ws->status, 0);
+    ADRT_HIP_TRY(hipStreamSynchronize(stream));
+    return ADRT_ERR_DOUBLING_STEPS;
   }
   return ADRT_OK;
 }
@@ -516,7 +707,7 @@ int adrt_profile(adrt_workspace* ws, int enable) {
 
 int adrt_profile_read(adrt_workspace* ws, double* ms, int* launches) {
   if (!ws || !ms) return ADRT_ERR_NULL;
-  for (int k = 0; k < 3; ++k) {
+  for (int k = 0; k < ADRT_PROFILE_KINDS; ++k) {
     ms[k] = 0.0;
     if (launches) launches[k] = 0;
   }
@@ -579,6 +770,10 @@ const char* adrt_status_string(int s) {
     case ADRT_ERR_HIP: return "HIP runtime error";
     case ADRT_ERR_PRECISION: return "unknown or unsupported precision";
     case ADRT_ERR_NULL: return "NULL argument";
+    case ADRT_ERR_ADJOINT:
+      return "reverse pass needs a workspace created with ADRT_WORKSPACE_ADJOINT (num_quadrature <= 16)";
+    case ADRT_ERR_DOUBLING_STEPS:
+      return "a layer needs more doubling steps than the reverse pass checkpoints (delta_tau too large)";
     default: return "unknown status";
   }
 }

@@ -60,6 +60,14 @@ struct FwdArgs {
   double* lay;      // [W][L][2N^2 + 2N]   R, T (column-major), s_up, s_down
   int* lflag;       // [W][L]   is_scattering
   double* rb;       // [W][L][2N^2 + N]    base composites R_ab, T_ba, s_up (n_base = 1..L)
+  // reverse pass (N = 16 kernels)
+  double* Iv;         // [W][L+1][2][N] forward interface intensities (up, down); null = not kept
+  double* Av;         // [W][L+1][2][N] adjoint interface vectors (up, down)
+  const double* cfu;  // [W][L+1] cotangents of the outputs in the caller's order; null = zero
+  const double* cfd;
+  const double* cJ;
+  const double* cdir;
+  const double* cdiv;
 };
 
 __device__ __forceinline__ int lin(const FwdArgs& a, int l) { return a.rev ? a.L - 1 - l : l; }

@@ -248,6 +248,14 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))
 
 // ---------------------------------------------------------------------------
 // adding sweeps + interfaces, one wave per wavenumber
+//
+// The interface intensities solve the two-stream system
+//   I_up(l)   = R_l I_dn(l) + T_l I_up(l+1) + s_up(l)
+//   I_dn(l+1) = T_l I_dn(l) + R_l I_up(l+1) + s_dn(l)
+// with I_dn(0) and (no surface) I_up(L) given; the adding sweeps and the
+// interface solves are a direct solver for it.  Its transpose has the same
+// form with R_l^T, T_l^T, the output cotangents as sources and the roles of
+// up and down exchanged, so the reverse pass runs this kernel with ADJ = true.
 // ---------------------------------------------------------------------------
 struct Comp16 {
   Mat<double> Rab, Rba, Tab, Tba;
@@ -259,17 +267,80 @@ struct Ws16 {
   double *tile, *tile2, *pv, *v0, *v1, *v2, *v3;
 };
 
-// rbase step: new = add(top = stored layer, bot = C)  (src/adding.hpp:246-258)
-__device__ void add16_on_top(Comp16& C, const double* rec, int lscat, const Lane& L,
+// cotangents of I_up(l) and I_dn(l) (element j) from the output cotangents:
+// flux_up = scaling 2 pi sum w mu I_up, flux_down = 2 pi sum w mu I_dn,
+// J = 1/2 sum w (I_up + I_dn) + beam, divergence = 4 pi (1 - omega)(J - B)
+__device__ __forceinline__ void ibar16(const FwdArgs& a, long gw, int l, double mu_j, double wt_j,
+                                       double& up, double& dn) {
+  const long o = gw * (a.L + 1) + lev(a, l);
+  const double cfu = a.cfu ? a.cfu[o] : 0.0;
+  const double cfd = a.cfd ? a.cfd[o] : 0.0;
+  double cJ = a.cJ ? a.cJ[o] : 0.0;
+  if (a.cdiv) {
+    const int lyr = (l == 0) ? 0 : l - 1;
+    cJ += a.cdiv[o] * (4.0 * kPi * (1.0 - a.ssa[gw * a.L + lin(a, lyr)]));
+  }
+  const double c = 2.0 * kPi * wt_j * mu_j;
+  up = cfu * a.scaling * c + 0.5 * wt_j * cJ;
+  dn = cfd * c + 0.5 * wt_j * cJ;
+}
+
+// layer l of the forward system, or of its transpose (ADJ): transposed R and
+// T, source up = cotangent of I_dn(l), source down = cotangent of I_up(l+1).
+// The matrices are read on demand so that they are live only where used.
+template <bool ADJ>
+struct LayerRef {
+  const FwdArgs& a;
+  const double* rec;
+  long gw;
+  double mu_j, wt_j;
+  int scat;
+  __device__ void mats(const Lane& L, Mat<double>& R, Mat<double>& T) const {
+    if constexpr (ADJ) {
+      gload_t<double>(rec, L, R);
+      gload_t<double>(rec + N * N, L, T);
+    } else {
+      gload<double>(rec, L, R);
+      gload<double>(rec + N * N, L, T);
+    }
+  }
+  __device__ void trans(const Lane& L, Mat<double>& T) const {
+    if constexpr (ADJ) gload_t<double>(rec + N * N, L, T);
+    else gload<double>(rec + N * N, L, T);
+  }
+  __device__ void sources(const Lane& L, int l, double& su, double& sd) const {
+    if constexpr (ADJ) {
+      double u, d;
+      ibar16(a, gw, l, mu_j, wt_j, u, d);
+      su = d;
+      ibar16(a, gw, l + 1, mu_j, wt_j, u, d);
+      sd = u;
+    } else {
+      su = rec[2 * N * N + L.j];
+      sd = rec[2 * N * N + N + L.j];
+    }
+  }
+};
+
+template <bool ADJ>
+__device__ __forceinline__ LayerRef<ADJ> layer_ref(const FwdArgs& a, const double* lay,
+                                                   const int* lflag, long gw, int l, double mu_j,
+                                                   double wt_j) {
+  return LayerRef<ADJ>{a, lay + (long)l * REC16, gw, mu_j, wt_j, lflag[l]};
+}
+
+// rbase step: new = add(top = layer y, bot = C)  (src/adding.hpp:246-258)
+template <bool ADJ>
+__device__ void add16_on_top(Comp16& C, const LayerRef<ADJ>& y, int l, const Lane& L,
                              const Ws16& s) {
   using R = double;
   const int j = L.j;
-  const R tsu = rec[2 * N * N + j];
-  const R tsd = rec[2 * N * N + N + j];
+  R tsu, tsd;
+  y.sources(L, l, tsu, tsd);
   Mat<R> Rt, Tt, A, X;
-  if (!lscat) {
-    // NonscatTop (src/adding.hpp:117-174): t = diag(top.T_ab), read from the record
-    gload<R>(rec + N * N, L, Tt);
+  if (!y.scat) {
+    // NonscatTop (src/adding.hpp:117-174): t = diag(top.T_ab)
+    y.trans(L, Tt);
     const R tj = diag<R>(Tt, L);
     if (L.h == 0) {
       s.v0[j] = tj;
@@ -296,8 +367,7 @@ __device__ void add16_on_top(Comp16& C, const double* rec, int lscat, const Lane
     C.sd = nsd;
     return;
   }
-  gload<R>(rec, L, Rt);
-  gload<R>(rec + N * N, L, Tt);
+  y.mats(L, Rt, Tt);
   if (!C.scat) {
     // NonscatBot (src/adding.hpp:181-239): t = diag(bot.T_ab)
     const R tj = diag<R>(C.Tab, L);
@@ -336,14 +406,14 @@ __device__ void add16_on_top(Comp16& C, const double* rec, int lscat, const Lane
   const R rhs_up = C.su + matvec<R>(s.tile, s.v0, L);     // s2_up + R_bc s1_down
   wave_sync();
   X = Tt;
-  right_solve<R>(A, X, L);   // D1 = top.T_ba A1^-1
+  right_solve<R>(A, X, L);                                // D1 = top.T_ba A1^-1
   put<R>(s.tile, L, Rt);
   if (L.h == 0) s.v1[j] = C.su;
   wave_sync();
   eye_minus_mul<R>(s.tile, C.Rab, L, A);                  // A2 = I - top.R_ba bot.R_ab
   const R rhs_dn = tsd + matvec<R>(s.tile, s.v1, L);      // s1_down + R_ba s2_up
   wave_sync();
-  right_solve<R>(A, C.Tab, L);  // D2 = bot.T_ab A2^-1
+  right_solve<R>(A, C.Tab, L);                            // D2 = bot.T_ab A2^-1
   if (L.h == 0) {
     s.v2[j] = rhs_up;
     s.v3[j] = rhs_dn;
@@ -371,16 +441,16 @@ __device__ void add16_on_top(Comp16& C, const double* rec, int lscat, const Lane
   C.scat = 1;
 }
 
-// rtop step: new = add(top = C, bot = stored layer)
-__device__ void add16_below(Comp16& C, const double* rec, int lscat, const Lane& L,
+// rtop step: new = add(top = C, bot = layer y)
+template <bool ADJ>
+__device__ void add16_below(Comp16& C, const LayerRef<ADJ>& y, int l, const Lane& L,
                             const Ws16& s) {
   using R = double;
   const int j = L.j;
-  const R bsu = rec[2 * N * N + j];
-  const R bsd = rec[2 * N * N + N + j];
+  R bsu, bsd;
+  y.sources(L, l, bsu, bsd);
   Mat<R> Rb, Tb, A, X;
-  gload<R>(rec, L, Rb);
-  gload<R>(rec + N * N, L, Tb);
+  y.mats(L, Rb, Tb);
   if (!C.scat) {
     // NonscatTop with top = C: t = diag(C.T_ab)
     const R tj = diag<R>(C.Tab, L);
@@ -408,10 +478,10 @@ __device__ void add16_below(Comp16& C, const double* rec, int lscat, const Lane&
     const R nsd = bsd + tbc;
     C.su = nsu;
     C.sd = nsd;
-    C.scat = lscat;
+    C.scat = y.scat;
     return;
   }
-  if (!lscat) {
+  if (!y.scat) {
     // NonscatBot with bot = layer: t = diag(layer T)
     const R tj = diag<R>(Tb, L);
     if (L.h == 0) {
@@ -446,7 +516,7 @@ __device__ void add16_below(Comp16& C, const double* rec, int lscat, const Lane&
   eye_minus_mul<R>(s.tile, C.Rba, L, A);                  // A1 = I - bot.R_ab top.R_ba
   const R rhs_up = bsu + matvec<R>(s.tile, s.v0, L);      // s2_up + R_bc s1_down
   wave_sync();
-  right_solve<R>(A, C.Tba, L);  // D1 = top.T_ba A1^-1
+  right_solve<R>(A, C.Tba, L);                            // D1 = top.T_ba A1^-1
   put<R>(s.tile, L, C.Rba);
   if (L.h == 0) s.v1[j] = bsu;
   wave_sync();
@@ -454,7 +524,7 @@ __device__ void add16_below(Comp16& C, const double* rec, int lscat, const Lane&
   const R rhs_dn = C.sd + matvec<R>(s.tile, s.v1, L);     // s1_down + R_ba s2_up
   wave_sync();
   X = Tb;
-  right_solve<R>(A, X, L);   // D2 = bot.T_ab A2^-1
+  right_solve<R>(A, X, L);                                // D2 = bot.T_ab A2^-1
   if (L.h == 0) {
     s.v2[j] = rhs_up;
     s.v3[j] = rhs_dn;
@@ -482,7 +552,9 @@ __device__ void add16_below(Comp16& C, const double* rec, int lscat, const Lane&
   C.scat = 1;
 }
 
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) adding16_kernel(FwdArgs a) {
+template <bool ADJ>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8)))
+adding16_kernel(FwdArgs a) {
   using R = double;
   __shared__ R smem[4 * Lds16::size];
   const int wv = threadIdx.x >> 6;
@@ -501,6 +573,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))
   const double mu_j = a.mu[j], wt_j = a.wt[j];
   const double* Bw = a.B ? a.B + gw * (Lay + 1) : nullptr;
   auto Blev = [&](int l) -> double { return Bw ? Bw[lev(a, l)] : 0.0; };
+  double* vec = ADJ ? (a.Av ? a.Av + (long)w * (Lay + 1) * 2 * N : nullptr)
+                    : (a.Iv ? a.Iv + (long)w * (Lay + 1) * 2 * N : nullptr);
 
   // boundary conditions (src/solver.cpp:385-419, 516-551, 585-603)
   const double Bs = a.surf_w ? a.surf_w[gw] : a.surf_em;
@@ -509,42 +583,57 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))
   const int has_surface = !a.diffusion && (A > 0.0 || Bs > 0.0);
   const int ltot = Lay + (has_surface ? 1 : 0);
   const double tau_tot = a.tau_cum[(long)w * (Lay + 1) + Lay];
-  const double Itd = Btop;
-  double Ibu = 0.0;
-  if (a.diffusion) {
-    const double Bb = Blev(Lay);
-    const double dt = a.tau_s[(long)w * Lay + Lay - 1];
-    const double dBdt = (dt > 0.0) ? (Bb - Blev(Lay - 1)) / dt : 0.0;
-    Ibu = Bb + mu_j * dBdt;
-  } else if (!has_surface) {
-    Ibu = Bs;
+  double Itd, Ibu = 0.0;
+  if constexpr (ADJ) {
+    double u, d;
+    ibar16(a, gw, 0, mu_j, wt_j, u, d);
+    Itd = u;  // transposed system: the top boundary carries the cotangent of I_up(0)
+    ibar16(a, gw, Lay, mu_j, wt_j, u, d);
+    if (!has_surface) Ibu = d;  // ... and a fixed I_up(L) the cotangent of I_dn(L)
+  } else {
+    Itd = Btop;
+    if (a.diffusion) {
+      const double Bb = Blev(Lay);
+      const double dt = a.tau_s[(long)w * Lay + Lay - 1];
+      const double dBdt = (dt > 0.0) ? (Bb - Blev(Lay - 1)) / dt : 0.0;
+      Ibu = Bb + mu_j * dBdt;
+    } else if (!has_surface) {
+      Ibu = Bs;
+    }
   }
 
   Comp16 C;
   // ---- bottom-up sweep (src/solver.cpp:560-572) ----
   if (has_surface) {
-    const double rr = 2.0 * A * mu_j * wt_j * a.xfac;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      C.Rab.v[r] = (row<R>(L.h, r) < a.n_real) ? rr : 0.0;
+      const int i = row<R>(L.h, r);
+      if constexpr (ADJ)  // R_s^T(i, j) = R_s(j, i)
+        C.Rab.v[r] = (j < a.n_real) ? 2.0 * A * a.mu[i] * a.wt[i] * a.xfac : 0.0;
+      else
+        C.Rab.v[r] = (i < a.n_real) ? 2.0 * A * mu_j * wt_j * a.xfac : 0.0;
       C.Rba.v[r] = C.Rab.v[r];
       C.Tab.v[r] = 0.0;
       C.Tba.v[r] = 0.0;
     }
-    double ssol = 0.0;
-    if (a.has_solar && A > 0.0) ssol = (A / kPi) * a.F0 * a.mu0 * exp(-tau_tot / a.mu0);
-    C.su = (1.0 - A) * Bs + ssol;
+    if constexpr (ADJ) {
+      double u, d;
+      ibar16(a, gw, Lay, mu_j, wt_j, u, d);
+      C.su = d;
+    } else {
+      double ssol = 0.0;
+      if (a.has_solar && A > 0.0) ssol = (A / kPi) * a.F0 * a.mu0 * exp(-tau_tot / a.mu0);
+      C.su = (1.0 - A) * Bs + ssol;
+    }
     C.sd = 0.0;
     C.scat = A > 0.0;
   } else {
-    const double* rec = lay + (long)(Lay - 1) * REC16;
-    gload<R>(rec, L, C.Rab);
-    gload<R>(rec + N * N, L, C.Tab);
+    const LayerRef<ADJ> y = layer_ref<ADJ>(a, lay, lflag, gw, Lay - 1, mu_j, wt_j);
+    y.mats(L, C.Rab, C.Tab);
     C.Rba = C.Rab;
     C.Tba = C.Tab;
-    C.su = rec[2 * N * N + j];
-    C.sd = rec[2 * N * N + N + j];
-    C.scat = lflag[Lay - 1];
+    y.sources(L, Lay - 1, C.su, C.sd);
+    C.scat = y.scat;
   }
   for (int n = 1;; ++n) {  // C == rbase[n]
     if (n < ltot) {
@@ -555,30 +644,36 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))
     }
     if (n == ltot) break;
     const int k = ltot - 1 - n;
-    add16_on_top(C, lay + (long)k * REC16, lflag[k], L, s);
+    add16_on_top<ADJ>(C, layer_ref<ADJ>(a, lay, lflag, gw, k, mu_j, wt_j), k, L, s);
   }
 
   auto emit = [&](int l, double Iup, double Idn) {
-    const double c = 2.0 * kPi * wt_j * mu_j;
-    const double fu = colsum<R>(c * Iup, L, s.pv);
-    const double fdn = colsum<R>(c * Idn, L, s.pv);
-    double J = colsum<R>(0.5 * wt_j * (Iup + Idn), L, s.pv);
-    if (L.l == 0) {
-      const long o = gw * (Lay + 1) + lev(a, l);
-      double fdir = 0.0;
-      if (a.has_solar) {
-        fdir = (l == 0) ? a.F0 * a.mu0
-                        : a.F0 * a.mu0 * exp(-a.tau_cum[(long)w * (Lay + 1) + l] / a.mu0);
-        J += fdir * (1.0 / (4.0 * kPi * a.mu0));
-      }
-      a.fu[o] = fu * a.scaling;
-      a.fd[o] = fdn;
-      a.mi[o] = J;
-      a.fdir[o] = fdir;
-      if (a.fdiv) {
-        const int lyr = (l == 0) ? 0 : l - 1;
-        const double om = a.ssa[gw * Lay + lin(a, lyr)];
-        a.fdiv[o] = 4.0 * kPi * (1.0 - om) * (J - Blev(l));
+    if (vec && L.h == 0) {
+      vec[(long)l * 2 * N + j] = Iup;
+      vec[(long)l * 2 * N + N + j] = Idn;
+    }
+    if constexpr (!ADJ) {
+      const double c = 2.0 * kPi * wt_j * mu_j;
+      const double fu = colsum<R>(c * Iup, L, s.pv);
+      const double fdn = colsum<R>(c * Idn, L, s.pv);
+      double J = colsum<R>(0.5 * wt_j * (Iup + Idn), L, s.pv);
+      if (L.l == 0) {
+        const long o = gw * (Lay + 1) + lev(a, l);
+        double fdir = 0.0;
+        if (a.has_solar) {
+          fdir = (l == 0) ? a.F0 * a.mu0
+                          : a.F0 * a.mu0 * exp(-a.tau_cum[(long)w * (Lay + 1) + l] / a.mu0);
+          J += fdir * (1.0 / (4.0 * kPi * a.mu0));
+        }
+        a.fu[o] = fu * a.scaling;
+        a.fd[o] = fdn;
+        a.mi[o] = J;
+        a.fdir[o] = fdir;
+        if (a.fdiv) {
+          const int lyr = (l == 0) ? 0 : l - 1;
+          const double om = a.ssa[gw * Lay + lin(a, lyr)];
+          a.fdiv[o] = 4.0 * kPi * (1.0 - om) * (J - Blev(l));
+        }
       }
     }
   };
@@ -600,13 +695,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))
 
   // ---- top-down sweep with interfaces (src/solver.cpp:574-583, 644-678) ----
   {
-    gload<R>(lay, L, C.Rab);
-    gload<R>(lay + N * N, L, C.Tab);
+    const LayerRef<ADJ> y = layer_ref<ADJ>(a, lay, lflag, gw, 0, mu_j, wt_j);
+    y.mats(L, C.Rab, C.Tab);
     C.Rba = C.Rab;
     C.Tba = C.Tab;
-    C.su = lay[2 * N * N + j];
-    C.sd = lay[2 * N * N + N + j];
-    C.scat = lflag[0];
+    y.sources(L, 0, C.su, C.sd);
+    C.scat = y.scat;
   }
   for (int l = 1; l <= Lay; ++l) {  // C == rtop[l]
     const int n_base = ltot - l;
@@ -663,7 +757,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))
       Iup = Ibu;
     }
     emit(l, Iup, Idn);
-    if (l < Lay) add16_below(C, lay + (long)l * REC16, lflag[l], L, s);
+    if (l < Lay) add16_below<ADJ>(C, layer_ref<ADJ>(a, lay, lflag, gw, l, mu_j, wt_j), l, L, s);
   }
 }
 

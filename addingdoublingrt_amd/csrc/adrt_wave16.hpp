@@ -89,6 +89,18 @@ __device__ __forceinline__ void gload(const R* g, const Lane& L, Mat<R>& m) {
   for (int r = 0; r < 4; ++r) m.v[r] = g[L.l * 4 + r];
 }
 
+// the transpose of a matrix stored in register order: element (row(h, r), j) of
+// the transpose is element (j, row(h, r)) of the stored matrix, held by lane
+// row(h, r) + 16 h' at register r' with row(h', r') = j
+template <typename R>
+__device__ __forceinline__ void gload_t(const R* g, const Lane& L, Mat<R>& m) {
+  int hj, rj;
+  if constexpr (sizeof(R) == 8) { hj = L.j & 3; rj = L.j >> 2; }
+  else { hj = L.j >> 2; rj = L.j & 3; }
+#pragma unroll
+  for (int r = 0; r < 4; ++r) m.v[r] = g[(row<R>(L.h, r) + 16 * hj) * 4 + rj];
+}
+
 template <typename R>
 __device__ __forceinline__ void gstore(R* g, const Lane& L, const Mat<R>& m) {
 #pragma unroll
@@ -301,6 +313,76 @@ __device__ __forceinline__ R left_solve(const Mat<R>& M, R b, const Lane& L, R* 
     (step(std::integral_constant<int, K>{}), ...);
   }(std::make_integer_sequence<int, N>{});
   return shfl(t0, src + 16 * L.h);
+}
+
+// ---- reverse-pass building blocks -------------------------------------------
+
+// sum over the four lanes of column j (every lane of the column gets the same
+// value: the additions pair (h0 + h1) + (h2 + h3) in every lane)
+template <typename R>
+__device__ __forceinline__ R col_reduce(R p, const Lane& L) {
+  p += shfl(p, L.l ^ 16);
+  p += shfl(p, L.l ^ 32);
+  return p;
+}
+
+// sum over all 64 lanes (fixed order, identical in every lane)
+template <typename R>
+__device__ __forceinline__ R wave_sum(R p, const Lane& L) {
+#pragma unroll
+  for (int m = 1; m < 64; m <<= 1) p += shfl(p, L.l ^ m);
+  return p;
+}
+
+// (M^T v)_j with M in registers and v in an LDS slot of 16 values
+template <typename R>
+__device__ __forceinline__ R tmatvec(const Mat<R>& M, const R* v, const Lane& L) {
+  R p = R(0);
+#pragma unroll
+  for (int r = 0; r < 4; ++r) p = fma_(M.v[r], v[row<R>(L.h, r)], p);
+  return col_reduce<R>(p, L);
+}
+
+// acc += X Y (X in an LDS tile, Y in registers)
+template <typename R>
+__device__ __forceinline__ void mac(vec4<R>& acc, const R* X, const Mat<R>& Y, const Lane& L) {
+  fence_sched();
+#pragma unroll
+  for (int r = 0; r < 4; ++r) acc = mfma(X[row<R>(L.h, r) * TS + L.j], Y.v[r], acc);
+  fence_sched();
+}
+
+// acc += X^T Y (X in an LDS tile, read transposed)
+template <typename R>
+__device__ __forceinline__ void mac_t(vec4<R>& acc, const R* X, const Mat<R>& Y, const Lane& L) {
+  fence_sched();
+#pragma unroll
+  for (int r = 0; r < 4; ++r) acc = mfma(X[L.j * TS + row<R>(L.h, r)], Y.v[r], acc);
+  fence_sched();
+}
+
+// acc += sum_k a_k b_k^T for four vectors given element-wise (element j in
+// every lane of column j): one K = 4 matrix instruction, lane (j, h) feeding
+// a_h[j] and b_h[j]
+template <typename R>
+__device__ __forceinline__ void outer4(vec4<R>& acc, R a0, R a1, R a2, R a3, R b0, R b1, R b2,
+                                       R b3, const Lane& L) {
+  const R av = L.h == 0 ? a0 : L.h == 1 ? a1 : L.h == 2 ? a2 : a3;
+  const R bv = L.h == 0 ? b0 : L.h == 1 ? b1 : L.h == 2 ? b2 : b3;
+  acc = mfma(av, bv, acc);
+}
+
+template <typename R>
+__device__ __forceinline__ vec4<R> acc_of(const Mat<R>& m) {
+  return vec4<R>{m.v[0], m.v[1], m.v[2], m.v[3]};
+}
+template <typename R>
+__device__ __forceinline__ Mat<R> mat_of(const vec4<R>& a) {
+  return Mat<R>{{a[0], a[1], a[2], a[3]}};
+}
+template <typename R>
+__device__ __forceinline__ vec4<R> zero4() {
+  return vec4<R>{R(0), R(0), R(0), R(0)};
 }
 
 // ordered sum over the 16 columns (one value per column), as the reference's

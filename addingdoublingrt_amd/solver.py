@@ -37,13 +37,17 @@ class BatchConfig:
 class Workspace:
     """Owns an ``adrt_workspace`` (device scratch for one column shape)."""
 
-    def __init__(self, num_wavenumbers, num_layers, num_quadrature, num_moments, max_bytes=0):
+    def __init__(self, num_wavenumbers, num_layers, num_quadrature, num_moments, max_bytes=0,
+                 adjoint=False):
         cfg = _lib.BatchConfig(num_wavenumbers=num_wavenumbers, num_layers=num_layers,
                                num_quadrature=num_quadrature, num_moments=num_moments,
                                surface_albedo=0.0, solar_mu=1.0, spectrum_scaling=1.0)
         h = C.c_void_p()
-        _lib.check(_lib.lib().adrt_workspace_create(C.byref(cfg), int(max_bytes), C.byref(h)))
+        flags = _lib.WORKSPACE_ADJOINT if adjoint else 0
+        _lib.check(_lib.lib().adrt_workspace_create_ex(C.byref(cfg), int(max_bytes), flags,
+                                                       C.byref(h)))
         self.handle = h
+        self.adjoint = bool(adjoint)
         self.shape = (num_layers, num_quadrature, num_moments)
         b = C.c_size_t()
         ch = C.c_int()
@@ -51,6 +55,11 @@ class Workspace:
         self.bytes = b.value
         self.chunk = ch.value
         self.created_for = num_wavenumbers
+
+    def generation(self) -> int:
+        g = C.c_ulonglong()
+        _lib.check(_lib.lib().adrt_workspace_generation(self.handle, C.byref(g)))
+        return g.value
 
     def __del__(self):
         h = getattr(self, "handle", None)
@@ -66,14 +75,17 @@ _ws_cache: dict = {}
 _ws_lock = threading.Lock()
 
 
-def get_workspace(num_wavenumbers, num_layers, num_quadrature, num_moments, max_bytes=0):
-    key = (torch.cuda.current_device(), num_layers, num_quadrature, num_moments, int(max_bytes))
+def get_workspace(num_wavenumbers, num_layers, num_quadrature, num_moments, max_bytes=0,
+                  adjoint=False):
+    key = (torch.cuda.current_device(), num_layers, num_quadrature, num_moments, int(max_bytes),
+           bool(adjoint))
     with _ws_lock:
         ws = _ws_cache.get(key)
         # a workspace serves any batch size (the library chunks); rebuild it
         # only when its chunk was capped by a smaller earlier batch
         if ws is None or (ws.chunk < num_wavenumbers and ws.chunk == ws.created_for):
-            ws = Workspace(num_wavenumbers, num_layers, num_quadrature, num_moments, max_bytes)
+            ws = Workspace(num_wavenumbers, num_layers, num_quadrature, num_moments, max_bytes,
+                           adjoint)
             _ws_cache[key] = ws
         return ws
 
@@ -88,21 +100,12 @@ def _dev_ptr(t, name, shape=None):
     return C.c_void_p(t.data_ptr())
 
 
-def solve_batch(delta_tau, single_scat_albedo, phase_moments, config: BatchConfig,
-                planck_levels=None, surface_emission=None, top_emission=None,
-                precision="f64", out=None, validate=False, max_workspace_bytes=0):
-    """Solve ``W`` independent columns; returns the five ``[W][L+1]`` outputs.
-
-    delta_tau, single_scat_albedo: ``[W][L]``; phase_moments: ``[L][M]`` (shared
-    by every wavenumber) or ``[W][L][M]``; planck_levels: ``[W][L+1]`` or None;
-    surface_emission / top_emission: ``[W]`` tensors or None (then the scalars
-    of ``config`` apply).
-    """
+def _prepare(delta_tau, single_scat_albedo, phase_moments, config: BatchConfig,
+             planck_levels, surface_emission, top_emission):
     W, L = delta_tau.shape
     shared = phase_moments.dim() == 2
     M = phase_moments.shape[-1]
     mshape = (L, M) if shared else (W, L, M)
-    prec = {"f64": _lib.PRECISION_F64, "mixed": _lib.PRECISION_MIXED}[precision]
     cfg = _lib.BatchConfig(
         num_wavenumbers=W, num_layers=L, num_quadrature=config.num_quadrature, num_moments=M,
         use_delta_m=int(config.use_delta_m), use_diffusion_lower_bc=int(config.use_diffusion_lower_bc),
@@ -118,18 +121,149 @@ def solve_batch(delta_tau, single_scat_albedo, phase_moments, config: BatchConfi
         planck_levels=_dev_ptr(planck_levels, "planck_levels", (W, L + 1)),
         surface_emission=_dev_ptr(surface_emission, "surface_emission", (W,)),
         top_emission=_dev_ptr(top_emission, "top_emission", (W,)))
+    return W, L, M, shared, cfg, inp
+
+
+def _outputs_struct(out, W, L):
+    return _lib.DeviceOutputs(**{k: _dev_ptr(out[k], k, (W, L + 1)) if out.get(k) is not None
+                                 else None for k in OUTPUT_NAMES})
+
+
+_PREC = {"f64": _lib.PRECISION_F64, "mixed": _lib.PRECISION_MIXED}
+
+
+def solve_batch(delta_tau, single_scat_albedo, phase_moments, config: BatchConfig,
+                planck_levels=None, surface_emission=None, top_emission=None,
+                precision="f64", out=None, validate=False, max_workspace_bytes=0,
+                adjoint_workspace=False):
+    """Solve ``W`` independent columns; returns the five ``[W][L+1]`` outputs.
+
+    delta_tau, single_scat_albedo: ``[W][L]``; phase_moments: ``[L][M]`` (shared
+    by every wavenumber) or ``[W][L][M]``; planck_levels: ``[W][L+1]`` or None;
+    surface_emission / top_emission: ``[W]`` tensors or None (then the scalars
+    of ``config`` apply).  With ``adjoint_workspace`` the solve runs on a
+    workspace that keeps what ``adjoint_batch`` needs; the returned dict then
+    also holds ``"_generation"``, the token that lets the reverse pass reuse it.
+    """
+    W, L, M, shared, cfg, inp = _prepare(delta_tau, single_scat_albedo, phase_moments, config,
+                                         planck_levels, surface_emission, top_emission)
     if out is None:
         out = {k: torch.empty((W, L + 1), dtype=torch.float64, device=delta_tau.device)
                for k in OUTPUT_NAMES}
-    o = _lib.DeviceOutputs(**{k: _dev_ptr(out[k], k, (W, L + 1)) if out.get(k) is not None
-                              else None for k in OUTPUT_NAMES})
-    ws = get_workspace(W, L, config.num_quadrature, M, max_workspace_bytes)
+    o = _outputs_struct(out, W, L)
+    ws = get_workspace(W, L, config.num_quadrature, M, max_workspace_bytes, adjoint_workspace)
     stream = C.c_void_p(torch.cuda.current_stream().cuda_stream)
     if validate:
         _lib.check(_lib.lib().adrt_validate_inputs(C.byref(cfg), C.byref(inp), ws.handle, stream))
-    _lib.check(_lib.lib().adrt_solve_batch(C.byref(cfg), C.byref(inp), C.byref(o), prec,
-                                           ws.handle, stream))
+    _lib.check(_lib.lib().adrt_solve_batch(C.byref(cfg), C.byref(inp), C.byref(o),
+                                           _PREC[precision], ws.handle, stream))
+    if adjoint_workspace:
+        out["_generation"] = ws.generation()
     return out
+
+
+GRADIENT_NAMES = ("delta_tau", "single_scat_albedo", "phase_moments", "planck_levels",
+                  "surface_emission", "top_emission", "surface_albedo")
+
+
+def adjoint_batch(delta_tau, single_scat_albedo, phase_moments, config: BatchConfig, outputs,
+                  cotangents, planck_levels=None, surface_emission=None, top_emission=None,
+                  precision="f64", forward_generation=0, max_workspace_bytes=0,
+                  want=GRADIENT_NAMES, check_status=True):
+    """Gradients of ``sum(cotangent * output)`` w.r.t. the inputs (``adrt_adjoint_batch``).
+
+    outputs: the dict ``solve_batch`` returned for the same inputs;
+    cotangents: dict of ``[W][L+1]`` tensors keyed like the outputs (missing =
+    zero).  Returns a dict keyed by ``GRADIENT_NAMES`` (the ones in ``want``):
+    ``[W][L]``, moments in their input shape, ``[W][L+1]``, and ``[W]`` for the
+    two emissions and the per-column parts of the surface-albedo gradient.
+    """
+    W, L, M, shared, cfg, inp = _prepare(delta_tau, single_scat_albedo, phase_moments, config,
+                                         planck_levels, surface_emission, top_emission)
+    dev = delta_tau.device
+    shapes = {"delta_tau": (W, L), "single_scat_albedo": (W, L),
+              "phase_moments": (L, M) if shared else (W, L, M), "planck_levels": (W, L + 1),
+              "surface_emission": (W,), "top_emission": (W,), "surface_albedo": (W,)}
+    grads = {k: torch.empty(shapes[k], dtype=torch.float64, device=dev) for k in want}
+    g = _lib.DeviceGradients(**{k: C.c_void_p(v.data_ptr()) for k, v in grads.items()})
+    cot = {}
+    for k in OUTPUT_NAMES:
+        c = cotangents.get(k)
+        if c is not None:
+            c = c.to(dtype=torch.float64).contiguous()
+            cot[k] = c
+    ct = _lib.DeviceCotangents(**{k: _dev_ptr(v, "cotangent " + k, (W, L + 1))
+                                  for k, v in cot.items()})
+    o = _outputs_struct(outputs, W, L)
+    ws = get_workspace(W, L, config.num_quadrature, M, max_workspace_bytes, True)
+    stream = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    _lib.check(_lib.lib().adrt_adjoint_batch(C.byref(cfg), C.byref(inp), C.byref(o), C.byref(ct),
+                                             C.byref(g), _PREC[precision],
+                                             int(forward_generation), ws.handle, stream))
+    if check_status:
+        _lib.check(_lib.lib().adrt_adjoint_status(ws.handle, stream))
+    return grads
+
+
+class _SolveFunction(torch.autograd.Function):
+    """Forward solve with a hand-written reverse pass (adrt_adjoint_batch)."""
+
+    @staticmethod
+    def forward(ctx, config, precision, max_ws, delta_tau, single_scat_albedo, phase_moments,
+                planck_levels, surface_emission, top_emission, surface_albedo):
+        if surface_albedo is not None:
+            config = BatchConfig(**{**config.__dict__, "surface_albedo": float(surface_albedo)})
+        out = solve_batch(delta_tau, single_scat_albedo, phase_moments, config, planck_levels,
+                          surface_emission, top_emission, precision=precision,
+                          max_workspace_bytes=max_ws, adjoint_workspace=True)
+        gen = out.pop("_generation")
+        ctx.config, ctx.precision, ctx.max_ws, ctx.generation = config, precision, max_ws, gen
+        ctx.has = (planck_levels is not None, surface_emission is not None,
+                   top_emission is not None, surface_albedo is not None)
+        outs = tuple(out[k] for k in OUTPUT_NAMES)
+        ctx.save_for_backward(delta_tau, single_scat_albedo, phase_moments, planck_levels,
+                              surface_emission, top_emission, *outs)
+        ctx.mark_non_differentiable(outs[3])  # flux_direct depends on tau only through the beam
+        return outs
+
+    @staticmethod
+    def backward(ctx, *gouts):
+        tau, ssa, mom, B, Bs, Bt, *outs = ctx.saved_tensors
+        out = dict(zip(OUTPUT_NAMES, outs))
+        cot = {k: g for k, g in zip(OUTPUT_NAMES, gouts) if g is not None}
+        hasB, hasBs, hasBt, hasA = ctx.has
+        want = ["delta_tau", "single_scat_albedo", "phase_moments"]
+        if hasB:
+            want.append("planck_levels")
+        if hasBs:
+            want.append("surface_emission")
+        if hasBt:
+            want.append("top_emission")
+        if hasA:
+            want.append("surface_albedo")
+        g = adjoint_batch(tau, ssa, mom, ctx.config, out, cot, B, Bs, Bt,
+                          precision=ctx.precision, forward_generation=ctx.generation,
+                          max_workspace_bytes=ctx.max_ws, want=tuple(want))
+        gA = g["surface_albedo"].sum() if hasA else None
+        return (None, None, None, g["delta_tau"], g["single_scat_albedo"], g["phase_moments"],
+                g.get("planck_levels"), g.get("surface_emission"), g.get("top_emission"), gA)
+
+
+def solve(delta_tau, single_scat_albedo, phase_moments, config: BatchConfig, planck_levels=None,
+          surface_emission=None, top_emission=None, surface_albedo=None, precision="f64",
+          max_workspace_bytes=0):
+    """Differentiable batched solve (``torch.autograd``); returns the five outputs as a dict.
+
+    Gradients flow to delta_tau, single_scat_albedo, phase_moments,
+    planck_levels, surface_emission, top_emission and ``surface_albedo`` (a
+    0-dim tensor that replaces ``config.surface_albedo``) through the
+    hand-written reverse pass.  flux_direct is not differentiable here (it is
+    the closed form F0 mu0 exp(-tau_cum / mu0)).
+    """
+    outs = _SolveFunction.apply(config, precision, max_workspace_bytes, delta_tau,
+                                single_scat_albedo, phase_moments, planck_levels,
+                                surface_emission, top_emission, surface_albedo)
+    return dict(zip(OUTPUT_NAMES, outs))
 
 
 def gauss_legendre(n: int):
@@ -153,12 +287,7 @@ def planck_deriv(lo: float, hi: float, T: float) -> float:
     return out.value
 
 
-def planck_levels(wn_lo, wn_hi, temperature, derivative=False):
-    """Band Planck (or dB/dT) at every level on the device.
-
-    wn_lo, wn_hi: ``[W]`` band edges; temperature: ``[L+1]`` (shared) or
-    ``[W][L+1]``.  Returns ``[W][L+1]`` (src/planck.cpp:35-177).
-    """
+def _planck_levels_raw(wn_lo, wn_hi, temperature, derivative):
     W = wn_lo.shape[0]
     shared = temperature.dim() == 1
     nlev = temperature.shape[-1]
@@ -167,6 +296,34 @@ def planck_levels(wn_lo, wn_hi, temperature, derivative=False):
     stream = C.c_void_p(torch.cuda.current_stream().cuda_stream)
     _lib.check(_lib.lib().adrt_planck_levels(
         _dev_ptr(wn_lo, "wn_lo", (W,)), _dev_ptr(wn_hi, "wn_hi", (W,)),
-        _dev_ptr(temperature, "temperature"), int(shared), W, nlev, int(bool(derivative)),
-        _dev_ptr(out, "out"), ws.handle, stream))
+        _dev_ptr(temperature.contiguous(), "temperature"), int(shared), W, nlev,
+        int(bool(derivative)), _dev_ptr(out, "out"), ws.handle, stream))
     return out
+
+
+class _PlanckFunction(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, wn_lo, wn_hi, temperature):
+        ctx.save_for_backward(wn_lo, wn_hi, temperature)
+        return _planck_levels_raw(wn_lo, wn_hi, temperature, False)
+
+    @staticmethod
+    def backward(ctx, g):
+        lo, hi, T = ctx.saved_tensors
+        dB = _planck_levels_raw(lo, hi, T.detach(), True)
+        gT = g * dB
+        if T.dim() == 1:
+            gT = gT.sum(dim=0)
+        return None, None, gT
+
+
+def planck_levels(wn_lo, wn_hi, temperature, derivative=False):
+    """Band Planck (or dB/dT) at every level on the device.
+
+    wn_lo, wn_hi: ``[W]`` band edges; temperature: ``[L+1]`` (shared) or
+    ``[W][L+1]``.  Returns ``[W][L+1]`` (src/planck.cpp:35-177).  Differentiable
+    in ``temperature`` (chain rule with the device dB/dT).
+    """
+    if derivative:
+        return _planck_levels_raw(wn_lo, wn_hi, temperature, True)
+    return _PlanckFunction.apply(wn_lo, wn_hi, temperature)

@@ -115,8 +115,8 @@ def main():
         step()
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
-    ms = (C.c_double * 3)()
-    nl = (C.c_int * 3)()
+    ms = (C.c_double * _lib.PROFILE_KINDS)()
+    nl = (C.c_int * _lib.PROFILE_KINDS)()
     _lib.check(L_.adrt_profile_read(ws.handle, ms, nl))
     _lib.check(L_.adrt_profile(ws.handle, 0))
 

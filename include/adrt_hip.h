@@ -55,6 +55,8 @@ extern "C" {
 #define ADRT_ERR_HIP 21                /* a HIP runtime call failed */
 #define ADRT_ERR_PRECISION 22          /* unknown precision value */
 #define ADRT_ERR_NULL 23               /* NULL argument */
+#define ADRT_ERR_ADJOINT 24            /* workspace not created with ADRT_WORKSPACE_ADJOINT, or N > 16 */
+#define ADRT_ERR_DOUBLING_STEPS 25     /* a layer needs more doubling steps than the reverse pass holds */
 
 #define ADRT_MAX_QUADRATURE 32
 
@@ -100,13 +102,45 @@ typedef struct adrt_device_outputs {
   double* flux_divergence;  /* may be NULL */
 } adrt_device_outputs;
 
+/* Cotangents of the outputs (dLoss/d output), [W][L+1] each in the layout of
+ * adrt_device_outputs; NULL = zero. */
+typedef struct adrt_device_cotangents {
+  const double* flux_up;
+  const double* flux_down;
+  const double* mean_intensity;
+  const double* flux_direct;
+  const double* flux_divergence;
+} adrt_device_cotangents;
+
+/* Gradients written by adrt_adjoint_batch, in the layout of the inputs; NULL =
+ * not wanted.  Every non-NULL array is overwritten. */
+typedef struct adrt_device_gradients {
+  double* delta_tau;           /* [W][L] */
+  double* single_scat_albedo;  /* [W][L] */
+  double* phase_moments;       /* [L][M] if the moments are shared (summed over W), else [W][L][M] */
+  double* planck_levels;       /* [W][L+1] */
+  double* surface_emission;    /* [W] (per column, whether or not the input was per column) */
+  double* top_emission;        /* [W] */
+  double* surface_albedo;      /* [W] per-column parts; the albedo gradient is their sum */
+} adrt_device_gradients;
+
 typedef struct adrt_workspace adrt_workspace;
+
+/* Workspace flags. */
+#define ADRT_WORKSPACE_ADJOINT 1 /* keep what the reverse pass needs (N <= 16) */
 
 /* Create a workspace for columns of this shape (num_layers, num_quadrature,
  * num_moments).  max_bytes bounds its device memory (0 = 4 GiB); the batch is
  * processed in chunks of wavenumbers that fit.  Allocates device memory. */
 int adrt_workspace_create(const adrt_batch_config* cfg, size_t max_bytes,
                           adrt_workspace** ws);
+/* As adrt_workspace_create, with flags (ADRT_WORKSPACE_ADJOINT).  With the
+ * adjoint flag the columns are solved at quadrature order 16 (orders below 16
+ * padded with zero-weight nodes, which leaves the results unchanged) and the
+ * workspace also holds interface intensities, per-layer gradient terms and
+ * the doubling checkpoints of the reverse pass. */
+int adrt_workspace_create_ex(const adrt_batch_config* cfg, size_t max_bytes, int flags,
+                             adrt_workspace** ws);
 int adrt_workspace_destroy(adrt_workspace* ws);
 /* Device bytes held and wavenumbers per chunk. */
 int adrt_workspace_info(const adrt_workspace* ws, size_t* bytes, int* chunk_wavenumbers);
@@ -115,6 +149,32 @@ int adrt_workspace_info(const adrt_workspace* ws, size_t* bytes, int* chunk_wave
 int adrt_solve_batch(const adrt_batch_config* cfg, const adrt_device_inputs* in,
                      const adrt_device_outputs* out, adrt_precision precision,
                      adrt_workspace* ws, hipStream_t stream);
+
+/* Counter of adrt_solve_batch calls on this workspace; pass the value read
+ * right after a forward solve to adrt_adjoint_batch to let it reuse that
+ * solve's layer records and interface intensities. */
+int adrt_workspace_generation(const adrt_workspace* ws, unsigned long long* generation);
+
+/* Reverse pass: gradients of sum(cotangent * output) with respect to the
+ * inputs, for the columns solved by adrt_solve_batch with the same cfg and in.
+ * out must hold that solve's outputs (mean_intensity is read; when the
+ * forward is recomputed they are written again with the same values).  The
+ * forward state kept in the workspace is reused when forward_generation equals
+ * the workspace generation and the whole batch fit one chunk; otherwise each
+ * chunk's forward is recomputed (checkpointing: layer records and interface
+ * intensities are kept per chunk, doubling steps are always recomputed).
+ * The reference has no adjoint; its derivative feature is the forward-mode
+ * temperature Jacobian of src/solver.cpp:117-341.  Asynchronous on stream;
+ * adrt_adjoint_status reports layers that exceeded the checkpoint depth. */
+int adrt_adjoint_batch(const adrt_batch_config* cfg, const adrt_device_inputs* in,
+                       const adrt_device_outputs* out, const adrt_device_cotangents* cot,
+                       const adrt_device_gradients* grad, adrt_precision precision,
+                       unsigned long long forward_generation, adrt_workspace* ws,
+                       hipStream_t stream);
+/* Synchronises the stream; ADRT_ERR_DOUBLING_STEPS if a layer of a reverse
+ * pass since the last call needed more doubling steps than the checkpoint slab
+ * holds (its gradients were written as zero), else ADRT_OK.  Clears the flag. */
+int adrt_adjoint_status(adrt_workspace* ws, hipStream_t stream);
 
 /* Checks delta_tau >= 0 and single_scat_albedo in [0,1] on the device and
  * returns ADRT_ERR_NEGATIVE_TAU / ADRT_ERR_SSA_RANGE (synchronises the stream). */
@@ -136,8 +196,11 @@ int adrt_planck_levels(const double* wn_lo, const double* wn_hi, const double* t
  * of adrt_solve_batch on this workspace is bracketed by HIP events on the
  * caller's stream.  adrt_profile_read synchronises those events and returns
  * the summed milliseconds and launch counts per kernel kind since the last
- * read: [0] prep, [1] layer setup + doubling, [2] adding + interfaces.
+ * read (ADRT_PROFILE_KINDS entries): [0] prep, [1] layer setup + doubling,
+ * [2] adding + interfaces, [3] reverse adding + interfaces, [4] reverse layer
+ * (recomputed doubling + its reverse), [5] reverse column terms + reductions.
  * Enabling allocates HIP events (host-side objects) on first use. */
+#define ADRT_PROFILE_KINDS 6
 int adrt_profile(adrt_workspace* ws, int enable);
 int adrt_profile_read(adrt_workspace* ws, double* ms, int* launches);
 
