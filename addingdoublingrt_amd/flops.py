@@ -9,12 +9,20 @@ Per column (N = quadrature order, counted as dense N x N x N products):
                                                       (src/solver.cpp:24-73)
 O(N^2) terms (vectors, diagonal scalings, phase setup) are not counted, and the
 count follows the reference algorithm whatever the kernels do internally.
+
+Reverse pass (no counterpart in the reference): every forward product is
+counted twice (dA = dC B^T, dB = A^T dC) and every right solve's reverse as a
+solve plus a product with the transposed inverse (2 + 2) N^3; the doubling
+steps recomputed from the checkpoints are not counted, so the reverse doubling
+is (8 + 8 + 4) N^3 = 20 N^3 per step.  The adding reverse is one adding solve
+of the transposed system, counted as the forward adding and interfaces.
 """
 from __future__ import annotations
 
 import numpy as np
 
 DOUBLING = 2 + 2 / 3 + 2 + 2 + 2 + 2
+DOUBLING_REVERSE = 2 * (2 + 2 + 2 + 2) + (2 + 2)
 GENERAL_ADD = 2 + 2 + 4 / 3 + 4 + 8 + 4
 INTERFACE = 4 + 4 / 3
 
@@ -71,4 +79,5 @@ def forward_flops(tau, ssa, mom, N, use_delta_m=False, surface_albedo=0.0,
     add = GENERAL_ADD * n3 * n_general
     itf = INTERFACE * n3 * n_iface
     return dict(doubling=dbl, adding=add, interfaces=itf, total=dbl + add + itf,
-                doubling_steps=int(nn.sum()), general_adds=n_general, interface_solves=n_iface)
+                doubling_steps=int(nn.sum()), general_adds=n_general, interface_solves=n_iface,
+                doubling_reverse=DOUBLING_REVERSE * n3 * float(nn.sum()))

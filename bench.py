@@ -1,19 +1,25 @@
 """Benchmark of the batched adding-doubling solve on one MI355X.
 
-One step = one solve of the full column set of the headline configuration
-(65536 wavenumbers x 100 layers, N = 16, solar mu0 = 0.3 + thermal 300 -> 180 K,
-albedo 0.15, 32 shared Rayleigh/HG moments per layer) with every input already
-resident in HBM.  Prints one JSON line (see the keys below).
+One step = one forward solve plus one reverse pass (gradients of a weighted
+sum of flux_up, flux_down and mean_intensity at every interface with respect
+to delta_tau, single_scat_albedo, the phase moments, the level Planck values,
+surface and top emission and the surface albedo) over the full column set of
+the headline configuration: 65536 wavenumbers x 100 layers, N = 16, solar
+mu0 = 0.3 + thermal 300 -> 180 K, albedo 0.15, 32 Rayleigh/HG moments per layer
+shared by all wavenumbers, with every input already resident in HBM.  The
+workspace holds the whole batch, so the reverse pass reuses the forward's
+layer records and recomputes only the doubling steps.  Prints one JSON line.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--wavenumbers W]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--wavenumbers W] [--form f64]
 
-roofline: the dominant kernel is the layer kernel (phase setup + doubling);
-its algorithmic work is (2 + 2/3 + 2 + 2 + 2 + 2) N^3 flops per doubling step
-summed over the scattering layers of the batch (counted from the inputs), its
-time comes from HIP events recorded by the library around each of its launches
-on the solve stream, and the peak is MI355X FP64 (78.6 TFLOP/s).
-cpu_baseline: the FP64 CPU oracle (a restatement of the reference solver) on a
-bounded sample of the same columns, on this host's cores.
+roofline: the dominant kernel of the step, with its algorithmic work from
+addingdoublingrt_amd/flops.py (forward doubling (2 + 2/3 + 2 + 2 + 2 + 2) N^3
+per step, reverse doubling 20 N^3 per step, counted from the inputs) over its
+time from HIP events the library records around each launch on the solve
+stream; peak = MI355X FP64 78.6 TFLOP/s.
+cpu_baseline: the FP64 CPU oracle (a restatement of the reference solver,
+forward only: the reference has no adjoint) on a bounded sample of the same
+columns, on this host's cores.
 """
 from __future__ import annotations
 
@@ -29,6 +35,8 @@ sys.path.insert(0, ROOT)
 
 METRIC = "column-wavenumbers/s fwd+adjoint, mixed vs plain FP64 at ≤1e-6 rel flux err"
 FP64_PEAK_TFLOPS = 78.6
+KINDS = ("prep", "layer_doubling", "adding_interfaces", "reverse_adding_interfaces",
+         "reverse_layer_doubling", "reverse_column_terms")
 
 
 def parse():
@@ -38,16 +46,15 @@ def parse():
     p.add_argument("--warmup", type=int, default=1)
     p.add_argument("--wavenumbers", type=int, default=65536)
     p.add_argument("--config", type=int, default=5)
+    p.add_argument("--form", choices=("f64",), default="f64")
     p.add_argument("--cpu-seconds", type=float, default=12.0)
     p.add_argument("--no-cpu", action="store_true")
-    p.add_argument("--workspace-gb", type=float, default=8.0)
+    p.add_argument("--workspace-gb", type=float, default=96.0)
     return p.parse_args()
 
 
 def cpu_baseline(cols, planck_host, seconds):
     """FP64 oracle on a bounded sample of the same columns (forward only)."""
-    import numpy as np
-
     import oracle
 
     threads = min(os.cpu_count() or 1, 16)
@@ -83,10 +90,9 @@ def main():
               "so more GPUs would each solve their own share of the wavenumbers.",
               file=sys.stderr)
         sys.exit(2)
-    import numpy as np
     import torch
 
-    from addingdoublingrt_amd import _lib, flops, solve_batch, workloads
+    from addingdoublingrt_amd import _lib, adjoint_batch, flops, solve_batch, workloads
     from addingdoublingrt_amd.solver import get_workspace
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
@@ -97,18 +103,27 @@ def main():
     ws_bytes = int(a.workspace_gb * (1 << 30))
     out = {k: torch.empty((W, L + 1), dtype=torch.float64, device="cuda")
            for k in ("flux_up", "flux_down", "mean_intensity", "flux_direct", "flux_divergence")}
+    g = torch.Generator(device="cuda").manual_seed(5)
+    cot = {k: torch.rand((W, L + 1), dtype=torch.float64, device="cuda", generator=g)
+           for k in ("flux_up", "flux_down", "mean_intensity")}
+    inputs = (d["delta_tau"], d["single_scat_albedo"], d["phase_moments"], cfg)
+    kw = dict(planck_levels=d["planck_levels"], surface_emission=d["surface_emission"],
+              top_emission=d["top_emission"])
+    grads = {}
 
     def step():
-        solve_batch(d["delta_tau"], d["single_scat_albedo"], d["phase_moments"], cfg,
-                    planck_levels=d["planck_levels"], surface_emission=d["surface_emission"],
-                    top_emission=d["top_emission"], out=out, max_workspace_bytes=ws_bytes)
+        o = solve_batch(*inputs, **kw, out=out, max_workspace_bytes=ws_bytes,
+                        adjoint_workspace=True)
+        gen = o.pop("_generation")
+        grads.update(adjoint_batch(*inputs, out, cot, **kw, forward_generation=gen,
+                                   max_workspace_bytes=ws_bytes, check_status=False))
 
     for _ in range(a.warmup):
         step()
     torch.cuda.synchronize()
-    ws = get_workspace(W, L, cols.N, cols.phase_moments.shape[-1], ws_bytes)
-    L_ = _lib.lib()
-    _lib.check(L_.adrt_profile(ws.handle, 1))
+    ws = get_workspace(W, L, cols.N, cols.phase_moments.shape[-1], ws_bytes, True)
+    lib = _lib.lib()
+    _lib.check(lib.adrt_profile(ws.handle, 1))
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(a.steps):
@@ -117,17 +132,26 @@ def main():
     elapsed = time.perf_counter() - t0
     ms = (C.c_double * _lib.PROFILE_KINDS)()
     nl = (C.c_int * _lib.PROFILE_KINDS)()
-    _lib.check(L_.adrt_profile_read(ws.handle, ms, nl))
-    _lib.check(L_.adrt_profile(ws.handle, 0))
+    _lib.check(lib.adrt_profile_read(ws.handle, ms, nl))
+    _lib.check(lib.adrt_profile(ws.handle, 0))
+    stream = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    _lib.check(lib.adrt_adjoint_status(ws.handle, stream))
 
     ms_per_step = elapsed * 1e3 / a.steps
     Bs_host = d["surface_emission"].cpu().numpy() if d["surface_emission"] is not None else None
     work = flops.forward_flops(cols.delta_tau, cols.single_scat_albedo, cols.phase_moments,
                                cols.N, cols.use_delta_m, cols.surface_albedo, Bs_host,
                                cols.use_diffusion_lower_bc)
-    layer_ms_per_launch = ms[1] / max(nl[1], 1)
-    flops_per_launch = work["doubling"] / (nl[1] / a.steps)
-    achieved = flops_per_launch / (layer_ms_per_launch * 1e-3) / 1e12
+    per_kind = {k: ms[i] / a.steps for i, k in enumerate(KINDS)}
+    # dominant kernel and its algorithmic work
+    model = {1: ("layer16_kernel: phase setup + doubling (forward)", work["doubling"]),
+             4: ("layer16_adjoint_kernel: recomputed doubling + its reverse",
+                 work["doubling_reverse"])}
+    dom = max(model, key=lambda k: ms[k])
+    name, fl_step = model[dom]
+    launches = nl[dom] / a.steps
+    avg_ms = ms[dom] / max(nl[dom], 1)
+    achieved = (fl_step / launches) / (avg_ms * 1e-3) / 1e12
     res = {
         "metric": METRIC,
         "value": W / (ms_per_step * 1e-3),
@@ -144,26 +168,23 @@ def main():
         "config": {
             "workload": f"config {a.config}: {cols.name}, {W} wavenumbers x {L} layers, "
                         f"N={cols.N}",
-            "form": "plain FP64 forward (adjoint and mixed forms not built yet)",
+            "form": "plain FP64 forward + reverse pass (the mixed-precision form is not built yet)",
             "num_wavenumbers": W, "num_layers": L, "num_quadrature": cols.N,
-            "algorithmic_gflop_per_step": work["total"] / 1e9,
+            "algorithmic_gflop_per_step": (2 * work["total"] + work["doubling_reverse"]
+                                           - work["doubling"]) / 1e9,
             "doubling_steps": work["doubling_steps"], "general_adds": work["general_adds"],
-            "kernel_ms_per_step": {"prep": ms[0] / a.steps, "layer_doubling": ms[1] / a.steps,
-                                   "adding_interfaces": ms[2] / a.steps},
+            "kernel_ms_per_step": per_kind,
         },
         "roofline": {
             "bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-            "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
-            "kernel": "layer_kernel (phase setup + doubling), FP64",
-            "launches_per_step": nl[1] / a.steps, "avg_launch_ms": layer_ms_per_launch,
+            "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None, "kernel": name,
+            "launches_per_step": launches, "avg_launch_ms": avg_ms,
         },
     }
     if not a.no_cpu:
-        ph = None
+        ph = (None, None, None)
         if cols.thermal:
             ph = (d["planck_levels"].cpu().numpy(), Bs_host, d["top_emission"].cpu().numpy())
-        else:
-            ph = (None, None, None)
         res["cpu_baseline"] = cpu_baseline(cols, ph, a.cpu_seconds)
     print(json.dumps(res))
 
