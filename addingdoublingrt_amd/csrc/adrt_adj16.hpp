@@ -59,6 +59,257 @@ struct LdsA {
   static constexpr int size = vs + 8 * N;  // 8 vector slots; the last one for column sums
 };
 
+// phase matrices and thin-layer scalars of a scattering layer, recomputed for
+// the reverse pass in the register layout of RowT, keeping the sums before the
+// Hansen normalisation (src/phase_matrix.hpp:61-181, src/doubling.hpp:83-147)
+template <typename RowT>
+struct Phase16 {
+  double omega, con, f, inv1mf, csum, corr, qplus, qminus, ssol, corr_s, pplus, pminus;
+  double tau0, Ftop, gam0;
+  double Qpp[4], Qpm[4], Ppp[4], Ppm[4];  // rows row<RowT>(h, r) of column j
+  const double* chi;
+  int Lm, nn, mode;
+  bool solar;
+};
+
+template <typename RowT>
+__device__ __noinline__ void phase16_recompute(const FwdArgs& a, long task, int w, int l,
+                                                  long gw, const Lane& L, double tau,
+                                                  double omega_in, int mode, double* dtile,
+                                                  double* csl, Phase16<RowT>& p) {
+  const int Lay = a.L, j = L.j;
+  const double wt_j = a.wt[j];
+  const int li = lin(a, l);
+  p.mode = mode;
+  p.omega = omega_in > 1.0 ? 1.0 : omega_in;
+  p.con = 2.0 * p.omega * kPi;
+  p.chi = a.shared_moments ? a.mom + (long)li * a.M : a.mom + (gw * Lay + li) * a.M;
+  const int twoM = 2 * a.n_real;
+  p.Lm = a.M;
+  p.f = 0.0;
+  p.inv1mf = 1.0;
+  if (mode == 3) {
+    p.f = a.fdm[task];
+    p.inv1mf = 1.0 / (1.0 - p.f);
+    p.Lm = twoM;
+  } else if (mode == 2) {
+    p.Lm = a.M < twoM ? a.M : twoM;
+  }
+  p.solar = a.has_solar != 0;
+  double spj = 0.0, smj = 0.0, sign = 1.0;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) p.Qpp[r] = p.Qpm[r] = 0.0;
+  for (int ll = 0; ll < p.Lm; ++ll) {
+    const double chil = (mode == 3) ? (p.chi[ll] - p.f) * p.inv1mf : p.chi[ll];
+    const double cl = (2 * ll + 1) * chil;
+    const double* P = a.Pl + ll * N;
+    const double plj = P[j];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const double term = cl * P[row<RowT>(L.h, r)] * plj;
+      p.Qpp[r] += term;
+      p.Qpm[r] += sign * term;
+    }
+    if (p.solar) {
+      const double term = cl * plj * a.Pl0[ll];
+      spj += term;
+      smj += sign * term;
+    }
+    sign = -sign;
+  }
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    p.Qpp[r] = p.Qpp[r] / (2.0 * kPi);
+    p.Qpm[r] = p.Qpm[r] / (2.0 * kPi);
+    dtile[j * TS + row<RowT>(L.h, r)] = (p.Qpp[r] + p.Qpm[r]) * a.wt[row<RowT>(L.h, r)];
+  }
+  wave_sync();
+  p.csum = 0.0;
+#pragma unroll
+  for (int i = 0; i < N; ++i) p.csum += dtile[j * TS + i];
+  wave_sync();
+  p.corr = p.csum > 0.0 ? 1.0 / (2.0 * kPi * p.csum) : 1.0;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    p.Ppp[r] = p.csum > 0.0 ? p.Qpp[r] * p.corr : p.Qpp[r];
+    p.Ppm[r] = p.csum > 0.0 ? p.Qpm[r] * p.corr : p.Qpm[r];
+  }
+  p.qplus = p.qminus = p.ssol = 0.0;
+  p.corr_s = 1.0;
+  p.pplus = p.pminus = 0.0;
+  if (p.solar) {
+    p.qplus = spj / (2.0 * kPi);
+    p.qminus = smj / (2.0 * kPi);
+    p.ssol = colsum<double>((p.qplus + p.qminus) * wt_j, L, csl);
+    p.corr_s = p.ssol > 0.0 ? 1.0 / (2.0 * kPi * p.ssol) : 1.0;
+    p.pplus = p.ssol > 0.0 ? p.qplus * p.corr_s : p.qplus;
+    p.pminus = p.ssol > 0.0 ? p.qminus * p.corr_s : p.qminus;
+  }
+  const int nn0 = (int)(log(tau) / log(2.0)) + ipow0(p.omega);
+  p.nn = nn0 < 1 ? 1 : nn0;
+  p.tau0 = tau * ldexp(1.0, -p.nn);
+  p.Ftop = 0.0;
+  p.gam0 = 0.0;
+  if (p.solar) {
+    p.Ftop = a.F0 * exp(-a.tau_cum[(long)w * (Lay + 1) + l] / a.mu0);
+    p.gam0 = exp(-p.tau0 / a.mu0);
+  }
+}
+
+// The thin-layer start, the Hansen normalisation and the Legendre sums,
+// reversed: from the cotangents of T0, R0 (rows row<RowT>(h, r)), y0 and the
+// solar start vectors, and the column sums of the g and gamma cotangents, to
+// the layer's tau, omega, solar-flux and moment gradients.
+template <typename RowT>
+__device__ __noinline__ void start16_reverse(const FwdArgs& a, const Lane& L,
+                                                const Phase16<RowT>& p, double omega_in,
+                                                const double (&Rb0)[4], const double (&Tb0)[4],
+                                                double yb, double susb, double sdsb,
+                                                double g_sum, double gam_sum, double* csl,
+                                                double* chi_out, double& g_tau, double& g_om,
+                                                double& g_F, double& g_fr) {
+  const int j = L.j;
+  const double mu_j = a.mu[j], wt_j = a.wt[j];
+  const double tau0 = p.tau0, con = p.con, omega = p.omega;
+  double tb_l = 0.0, cb_l = 0.0, ob_l = 0.0, fb_l = 0.0;
+  double Pb_pp[4], Pb_pm[4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int i = row<RowT>(L.h, r);
+    const double mui = a.mu[i];
+    const double dij = (i == j) ? 1.0 : 0.0;
+    const double tb = Tb0[r], rb = Rb0[r];
+    tb_l += tb * (-(dij - con * p.Ppp[r] * wt_j) / mui) + rb * (con * p.Ppm[r] * wt_j / mui);
+    cb_l += (tb * p.Ppp[r] + rb * p.Ppm[r]) * (tau0 * wt_j / mui);
+    Pb_pp[r] = tb * (tau0 * con * wt_j / mui);
+    Pb_pm[r] = rb * (tau0 * con * wt_j / mui);
+  }
+  double pplusb = 0.0, pminusb = 0.0;
+  if (L.h == 0) {
+    tb_l += yb * (1.0 - omega) / mu_j;
+    ob_l += -yb * tau0 / mu_j;
+  }
+  if (p.solar) {
+    const double base = omega * tau0 / mu_j * p.Ftop;
+    const double baseb = susb * p.pminus + sdsb * p.pplus;
+    pminusb = susb * base;
+    pplusb = sdsb * base;
+    if (L.h == 0) {
+      ob_l += baseb * tau0 * p.Ftop / mu_j;
+      tb_l += baseb * omega * p.Ftop / mu_j;
+      fb_l += baseb * omega * tau0 / mu_j;
+    }
+  }
+  double taub0 = wave_sum<double>(tb_l, L) + 0.5 * g_sum;
+  if (p.solar) taub0 -= gam_sum * p.gam0 / a.mu0;
+  const double conb = wave_sum<double>(cb_l, L);
+  g_om = wave_sum<double>(ob_l, L) + 2.0 * kPi * conb;
+  if (omega_in > 1.0) g_om = 0.0;  // clamped to 1
+  g_F = p.solar ? wave_sum<double>(fb_l, L) : 0.0;
+  g_tau += taub0 * ldexp(1.0, -p.nn);
+
+  // Hansen normalisation and Legendre sums
+  double Qb_pp[4], Qb_pm[4];
+  if (p.csum > 0.0) {
+    double q = 0.0;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) q += Pb_pp[r] * p.Qpp[r] + Pb_pm[r] * p.Qpm[r];
+    const double corrb = col_reduce<double>(q, L);
+    const double csumb = -corrb * p.corr / p.csum;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      Qb_pp[r] = Pb_pp[r] * p.corr + csumb * a.wt[row<RowT>(L.h, r)];
+      Qb_pm[r] = Pb_pm[r] * p.corr + csumb * a.wt[row<RowT>(L.h, r)];
+    }
+  } else {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      Qb_pp[r] = Pb_pp[r];
+      Qb_pm[r] = Pb_pm[r];
+    }
+  }
+  double qplusb = pplusb, qminusb = pminusb;
+  if (p.solar && p.ssol > 0.0) {
+    const double corrb = colsum<double>(pplusb * p.qplus + pminusb * p.qminus, L, csl);
+    const double sb = -corrb * p.corr_s / p.ssol;
+    qplusb = pplusb * p.corr_s + sb * wt_j;
+    qminusb = pminusb * p.corr_s + sb * wt_j;
+  }
+  double sign = 1.0;
+  for (int ll = 0; ll < p.Lm; ++ll) {
+    const double* P = a.Pl + ll * N;
+    const double plj = P[j];
+    double q = 0.0;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) q += P[row<RowT>(L.h, r)] * (Qb_pp[r] + sign * Qb_pm[r]);
+    double part = q * plj;
+    if (p.solar && L.h == 0) part += a.Pl0[ll] * plj * (qplusb + sign * qminusb);
+    const double chisb = wave_sum<double>(part, L) * ((2 * ll + 1) / (2.0 * kPi));
+    if (p.mode == 3) g_fr += chisb * (p.chi[ll] - 1.0) * p.inv1mf * p.inv1mf;
+    if (chi_out && L.l == 0) chi_out[ll] = (p.mode == 3) ? chisb * p.inv1mf : chisb;
+    sign = -sign;
+  }
+}
+
+// cotangents of a layer record from the interface vectors (see the file head)
+struct LayerCot16 {
+  const double* Dl;  // adjoint vector of the up-equation at l
+  const double* U1;  // adjoint vector of the down-equation at l+1
+  double idn_j, iup_j, sub, sdb;
+};
+
+__device__ __forceinline__ LayerCot16 layer_cot16(const FwdArgs& a, int w, int l, const Lane& L) {
+  const int Lay = a.L, j = L.j;
+  const double* Iw = a.Iv + (long)w * (Lay + 1) * 2 * N;
+  const double* Aw = a.Av + (long)w * (Lay + 1) * 2 * N;
+  LayerCot16 c;
+  c.Dl = Aw + (long)l * 2 * N + N;
+  c.U1 = Aw + (long)(l + 1) * 2 * N;
+  c.idn_j = Iw[(long)l * 2 * N + N + j];
+  c.iup_j = Iw[(long)(l + 1) * 2 * N + j];
+  c.sub = c.Dl[j];
+  c.sdb = c.U1[j];
+  return c;
+}
+
+// R^ and T^ entries at row i of column j
+__device__ __forceinline__ double cot_R(const LayerCot16& c, int i) {
+  return c.Dl[i] * c.idn_j + c.U1[i] * c.iup_j;
+}
+__device__ __forceinline__ double cot_T(const LayerCot16& c, int i) {
+  return c.Dl[i] * c.iup_j + c.U1[i] * c.idn_j;
+}
+
+// reverse of a transparent (no dependence) or purely absorbing layer
+// (src/doubling.hpp:52-77); returns false for a scattering layer
+__device__ __forceinline__ bool nonscat16_reverse(const FwdArgs& a, const Lane& L,
+                                                  const LayerCot16& c, double tau,
+                                                  double omega_in, double B_bar, double B_d,
+                                                  double* csl, double& g_tau, double& g_Bt,
+                                                  double& g_Bb) {
+  if (tau <= 0.0) return true;
+  if (omega_in > 0.0) return false;
+  const int j = L.j;
+  const double mu_j = a.mu[j];
+  const double tex = -tau / mu_j;
+  const bool live = tex > -200.0;
+  const double t = live ? exp(tex) : 0.0;
+  const double dt = live ? -t / mu_j : 0.0;
+  const double omt = 1.0 - t;
+  const double slope = mu_j * omt - 0.5 * tau * (1.0 + t);
+  const double dslope = -mu_j * dt - 0.5 * (1.0 + t) - 0.5 * tau * dt;
+  const double dBd = -B_d / tau;
+  const double tbj = cot_T(c, j);
+  const double dsu = -B_bar * dt + dBd * slope + B_d * dslope;
+  const double dsd = -B_bar * dt - dBd * slope - B_d * dslope;
+  g_tau = colsum<double>(tbj * dt + c.sub * dsu + c.sdb * dsd, L, csl);
+  const double gbar = colsum<double>((c.sub + c.sdb) * omt, L, csl);
+  const double gd = colsum<double>((c.sub - c.sdb) * slope, L, csl);
+  g_Bt = 0.5 * gbar - gd / tau;
+  g_Bb = 0.5 * gbar + gd / tau;
+  return true;
+}
+
 __device__ void layer16_adjoint_task(const FwdArgs& a, const AdjArgs& b, long task, const Lane& L,
                                      double* sm, double* slab) {
   using R = double;
@@ -80,27 +331,11 @@ __device__ void layer16_adjoint_task(const FwdArgs& a, const AdjArgs& b, long ta
   const double Bbot = a.B ? a.B[gw * (Lay + 1) + lev(a, l + 1)] : 0.0;
   const double B_bar = (Bbot + Btop) / 2.0;
   const double B_d = (tau > 0.0) ? (Bbot - Btop) / tau : 0.0;
-  const double mu_j = a.mu[j], wt_j = a.wt[j];
+  const double mu_j = a.mu[j];
   const int li = lin(a, l);
   double* chi_out = a.shared_moments ? b.g_chi + task * a.M
                                      : (b.d_mom ? b.d_mom + (gw * Lay + li) * a.M : nullptr);
-
-  // cotangents of the layer record
-  const double* Iw = a.Iv + (long)w * (Lay + 1) * 2 * N;
-  const double* Aw = a.Av + (long)w * (Lay + 1) * 2 * N;
-  const double* Dl = Aw + (long)l * 2 * N + N;
-  const double* U1 = Aw + (long)(l + 1) * 2 * N;
-  const double idn_j = Iw[(long)l * 2 * N + N + j];
-  const double iup_j = Iw[(long)(l + 1) * 2 * N + j];
-  Mat<R> Rb, Tb;
-#pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    const int i = row<R>(L.h, r);
-    Rb.v[r] = Dl[i] * idn_j + U1[i] * iup_j;
-    Tb.v[r] = Dl[i] * iup_j + U1[i] * idn_j;
-  }
-  const double sub = Dl[j], sdb = U1[j];
-
+  const LayerCot16 c = layer_cot16(a, w, l, L);
   double g_tau = 0.0, g_om = 0.0, g_Bt = 0.0, g_Bb = 0.0, g_F = 0.0, g_fr = 0.0;
   auto finish = [&](int nchi) {  // moments nchi..M-1 get zero
     if (chi_out)
@@ -114,128 +349,50 @@ __device__ void layer16_adjoint_task(const FwdArgs& a, const AdjArgs& b, long ta
       b.g_f[task] = g_fr;
     }
   };
-
-  if (tau <= 0.0) {  // transparent: no dependence
+  if (nonscat16_reverse(a, L, c, tau, omega_in, B_bar, B_d, csl, g_tau, g_Bt, g_Bb)) {
     finish(0);
     return;
   }
-  if (omega_in <= 0.0) {
-    // pure absorption (src/doubling.hpp:56-77): T = diag(t), s = B_bar (1-t) +- B_d slope
-    const double tex = -tau / mu_j;
-    const bool live = tex > -200.0;
-    const double t = live ? exp(tex) : 0.0;
-    const double dt = live ? -t / mu_j : 0.0;
-    const double omt = 1.0 - t;
-    const double slope = mu_j * omt - 0.5 * tau * (1.0 + t);
-    const double dslope = -mu_j * dt - 0.5 * (1.0 + t) - 0.5 * tau * dt;
-    const double dBd = -B_d / tau;
-    const double tbj = diag<R>(Tb, L);
-    const double dsu = -B_bar * dt + dBd * slope + B_d * dslope;
-    const double dsd = -B_bar * dt - dBd * slope - B_d * dslope;
-    g_tau = colsum<R>(tbj * dt + sub * dsu + sdb * dsd, L, csl);
-    const double gbar = colsum<R>((sub + sdb) * omt, L, csl);
-    const double gd = colsum<R>((sub - sdb) * slope, L, csl);
-    g_Bt = 0.5 * gbar - gd / tau;
-    g_Bb = 0.5 * gbar + gd / tau;
-    finish(0);
-    return;
-  }
-
-  // ---- phase matrices, keeping the sums before normalisation ----
-  const double omega = omega_in > 1.0 ? 1.0 : omega_in;
-  const double con = 2.0 * omega * kPi;
-  const double* chi = a.shared_moments ? a.mom + (long)li * a.M : a.mom + (gw * Lay + li) * a.M;
-  const int twoM = 2 * a.n_real;
-  int Lm = a.M;
-  double f = 0.0, inv1mf = 1.0;
-  if (mode == 3) {
-    f = a.fdm[task];
-    inv1mf = 1.0 / (1.0 - f);
-    Lm = twoM;
-  } else if (mode == 2) {
-    Lm = a.M < twoM ? a.M : twoM;
-  }
-  const bool solar = a.has_solar != 0;
-  double Qpp[4] = {0, 0, 0, 0}, Qpm[4] = {0, 0, 0, 0};  // unnormalised, / 2 pi
-  double sign = 1.0, spj = 0.0, smj = 0.0;
-  int rows[4];
-#pragma unroll
-  for (int r = 0; r < 4; ++r) rows[r] = row<R>(L.h, r);
-  for (int ll = 0; ll < Lm; ++ll) {
-    const double chil = (mode == 3) ? (chi[ll] - f) * inv1mf : chi[ll];
-    const double cl = (2 * ll + 1) * chil;
-    const double* P = a.Pl + ll * N;
-    const double plj = P[j];
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const double term = cl * P[rows[r]] * plj;
-      Qpp[r] += term;
-      Qpm[r] += sign * term;
-    }
-    if (solar) {
-      const double term = cl * plj * a.Pl0[ll];
-      spj += term;
-      smj += sign * term;
-    }
-    sign = -sign;
-  }
+  Mat<R> Rb, Tb;
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
-    Qpp[r] = Qpp[r] / (2.0 * kPi);
-    Qpm[r] = Qpm[r] / (2.0 * kPi);
-    t0[j * TS + rows[r]] = (Qpp[r] + Qpm[r]) * a.wt[rows[r]];
+    Rb.v[r] = cot_R(c, row<R>(L.h, r));
+    Tb.v[r] = cot_T(c, row<R>(L.h, r));
   }
-  wave_sync();
-  double csum = 0.0;
-#pragma unroll
-  for (int i = 0; i < N; ++i) csum += t0[j * TS + i];
-  wave_sync();
-  const double corr = csum > 0.0 ? 1.0 / (2.0 * kPi * csum) : 1.0;
-  double Ppp[4], Ppm[4];
-#pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    Ppp[r] = csum > 0.0 ? Qpp[r] * corr : Qpp[r];
-    Ppm[r] = csum > 0.0 ? Qpm[r] * corr : Qpm[r];
-  }
-  double qplus = 0.0, qminus = 0.0, ssol = 0.0, corr_s = 1.0, pplus = 0.0, pminus = 0.0;
-  if (solar) {
-    qplus = spj / (2.0 * kPi);
-    qminus = smj / (2.0 * kPi);
-    ssol = colsum<R>((qplus + qminus) * wt_j, L, csl);
-    corr_s = ssol > 0.0 ? 1.0 / (2.0 * kPi * ssol) : 1.0;
-    pplus = ssol > 0.0 ? qplus * corr_s : qplus;
-    pminus = ssol > 0.0 ? qminus * corr_s : qminus;
-  }
+  const double sub = c.sub, sdb = c.sdb;
 
-  // ---- thin-layer start (src/doubling.hpp:83-147) ----
-  const int nn0 = (int)(log(tau) / log(2.0)) + ipow0(omega);
-  const int nn = nn0 < 1 ? 1 : nn0;
+  Phase16<R> p;
+  phase16_recompute<R>(a, task, w, l, gw, L, tau, omega_in, mode, t0, csl, p);
+  const int nn = p.nn;
   if (nn > b.nn_cap) {
     if (L.l == 0) atomicOr(b.status, 1);
     finish(0);
     return;
   }
-  const double tau0 = tau * ldexp(1.0, -nn);
+  const double tau0 = p.tau0;
+  const bool solar = p.solar;
+  const double omega = p.omega, con = p.con;
+  int rows[4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) rows[r] = row<R>(L.h, r);
   Mat<R> Rm, Tm;
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
     const int i = rows[r];
     const double mui = a.mu[i];
-    const double gpp = (((i == j) ? 1.0 : 0.0) + (-con) * (Ppp[r] * wt_j)) / mui;
-    const double gpm = con * (Ppm[r] * wt_j) / mui;
+    const double gpp = (((i == j) ? 1.0 : 0.0) + (-con) * (p.Ppp[r] * a.wt[j])) / mui;
+    const double gpm = con * (p.Ppm[r] * a.wt[j]) / mui;
     Tm.v[r] = ((i == j) ? 1.0 : 0.0) - tau0 * gpp;
     Rm.v[r] = tau0 * gpm;
   }
   double y = (1.0 - omega) * tau0 / mu_j;
   double z = 0.0;
-  double sus = 0.0, sds = 0.0, gam = 0.0, Ftop = 0.0, gam0 = 0.0;
+  double sus = 0.0, sds = 0.0, gam = 0.0;
   if (solar) {
-    Ftop = a.F0 * exp(-a.tau_cum[(long)w * (Lay + 1) + l] / a.mu0);
-    const double base = omega * tau0 / mu_j * Ftop;
-    sus = base * pminus;
-    sds = base * pplus;
-    gam0 = exp(-tau0 / a.mu0);
-    gam = gam0;
+    const double base = omega * tau0 / mu_j * p.Ftop;
+    sus = base * p.pminus;
+    sds = base * p.pplus;
+    gam = p.gam0;
   }
   double gk = 0.5 * tau0;
   R* s0 = vs;
@@ -451,90 +608,22 @@ __device__ void layer16_adjoint_task(const FwdArgs& a, const AdjArgs& b, long ta
     wave_sync();
   }
 
-  // ---- thin-layer start, reversed ----
+  // ---- thin-layer start, Hansen normalisation and Legendre sums, reversed ----
   const double g_sum = colsum<R>(gb, L, csl);
   const double gam_sum = solar ? colsum<R>(gamb, L, csl) : 0.0;
-  double tb_l = 0.0, cb_l = 0.0, ob_l = 0.0, fb_l = 0.0;
-  Mat<R> Pb_pp, Pb_pm;
+  double Rb0[4], Tb0[4];
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
-    const int i = rows[r];
-    const double mui = a.mu[i];
-    const double dij = (i == j) ? 1.0 : 0.0;
-    const double tb = Tbar.v[r], rb = Rbar.v[r];
-    tb_l += tb * (-(dij - con * Ppp[r] * wt_j) / mui) + rb * (con * Ppm[r] * wt_j / mui);
-    cb_l += (tb * Ppp[r] + rb * Ppm[r]) * (tau0 * wt_j / mui);
-    Pb_pp.v[r] = tb * (tau0 * con * wt_j / mui);
-    Pb_pm.v[r] = rb * (tau0 * con * wt_j / mui);
+    Rb0[r] = Rbar.v[r];
+    Tb0[r] = Tbar.v[r];
   }
-  double pplusb = 0.0, pminusb = 0.0;
-  if (L.h == 0) {
-    tb_l += yb * (1.0 - omega) / mu_j;
-    ob_l += -yb * tau0 / mu_j;
-  }
-  if (solar) {
-    const double base = omega * tau0 / mu_j * Ftop;
-    const double baseb = susb * pminus + sdsb * pplus;
-    pminusb = susb * base;
-    pplusb = sdsb * base;
-    if (L.h == 0) {
-      ob_l += baseb * tau0 * Ftop / mu_j;
-      tb_l += baseb * omega * Ftop / mu_j;
-      fb_l += baseb * omega * tau0 / mu_j;
-    }
-  }
-  double taub0 = wave_sum<R>(tb_l, L) + 0.5 * g_sum;
-  if (solar) taub0 -= gam_sum * gam0 / a.mu0;
-  const double conb = wave_sum<R>(cb_l, L);
-  g_om = wave_sum<R>(ob_l, L) + 2.0 * kPi * conb;
-  if (omega_in > 1.0) g_om = 0.0;  // clamped to 1
-  g_F = solar ? wave_sum<R>(fb_l, L) : 0.0;
-  g_tau += taub0 * ldexp(1.0, -nn);
-
-  // ---- Hansen normalisation and Legendre sums, reversed ----
-  double Qb_pp[4], Qb_pm[4];
-  if (csum > 0.0) {
-    double p = 0.0;
-#pragma unroll
-    for (int r = 0; r < 4; ++r) p += Pb_pp.v[r] * Qpp[r] + Pb_pm.v[r] * Qpm[r];
-    const double corrb = col_reduce<R>(p, L);
-    const double csumb = -corrb * corr / csum;
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      Qb_pp[r] = Pb_pp.v[r] * corr + csumb * a.wt[rows[r]];
-      Qb_pm[r] = Pb_pm.v[r] * corr + csumb * a.wt[rows[r]];
-    }
-  } else {
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      Qb_pp[r] = Pb_pp.v[r];
-      Qb_pm[r] = Pb_pm.v[r];
-    }
-  }
-  double qplusb = pplusb, qminusb = pminusb;
-  if (solar && ssol > 0.0) {
-    const double corrb = colsum<R>(pplusb * qplus + pminusb * qminus, L, csl);
-    const double sb = -corrb * corr_s / ssol;
-    qplusb = pplusb * corr_s + sb * wt_j;
-    qminusb = pminusb * corr_s + sb * wt_j;
-  }
-  sign = 1.0;
-  for (int ll = 0; ll < Lm; ++ll) {
-    const double* P = a.Pl + ll * N;
-    const double plj = P[j];
-    double q = 0.0;
-#pragma unroll
-    for (int r = 0; r < 4; ++r) q += P[rows[r]] * (Qb_pp[r] + sign * Qb_pm[r]);
-    double part = q * plj;
-    if (solar && L.h == 0) part += a.Pl0[ll] * plj * (qplusb + sign * qminusb);
-    const double chisb = wave_sum<R>(part, L) * ((2 * ll + 1) / (2.0 * kPi));
-    if (mode == 3) g_fr += chisb * (chi[ll] - 1.0) * inv1mf * inv1mf;
-    if (chi_out && L.l == 0) chi_out[ll] = (mode == 3) ? chisb * inv1mf : chisb;
-    sign = -sign;
-  }
-  finish(Lm);
+  start16_reverse<R>(a, L, p, omega_in, Rb0, Tb0, yb, susb, sdsb, g_sum, gam_sum, csl, chi_out,
+                     g_tau, g_om, g_F, g_fr);
+  finish(p.Lm);
 }
 
+// REDO: only the layers the mixed-precision forward left to the FP64 kernel
+template <bool REDO>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 8)))
 layer16_adjoint_kernel(FwdArgs a, AdjArgs b) {
   __shared__ double smem[4 * LdsA::size];
@@ -544,7 +633,12 @@ layer16_adjoint_kernel(FwdArgs a, AdjArgs b) {
   double* sm = smem + wv * LdsA::size;
   double* slab = b.slab + (long)gwave * b.nn_cap * SLAB;
   const long ntask = (long)a.W * a.L;
-  for (long task = gwave; task < ntask; task += b.nwaves) layer16_adjoint_task(a, b, task, L, sm, slab);
+  for (long task = gwave; task < ntask; task += b.nwaves) {
+    if constexpr (REDO) {
+      if (!a.redo[task]) continue;
+    }
+    layer16_adjoint_task(a, b, task, L, sm, slab);
+  }
 }
 
 // one thread per wavenumber
@@ -659,13 +753,38 @@ __global__ void __launch_bounds__(256) column16_adjoint_kernel(FwdArgs a, AdjArg
   }
 }
 
-// shared moments: d_mom[l'][m] += sum over the chunk's wavenumbers, in order
-__global__ void __launch_bounds__(256) moments16_reduce_kernel(FwdArgs a, AdjArgs b) {
+// shared moments: d_mom[l'][m] += sum over the chunk's wavenumbers, in a fixed
+// order (deterministic): MOM_SLICES partial sums per (layer, moment) over
+// contiguous wavenumber slices (coalesced over m), then the slices in order.
+constexpr int MOM_SLICES = 64;
+
+__global__ void __launch_bounds__(256) moments16_partial_kernel(FwdArgs a, AdjArgs b,
+                                                                double* part) {
+  const int l = blockIdx.x, sl = blockIdx.y;
+  const int tm = threadIdx.x & 63, tw = threadIdx.x >> 6;
+  const int per = (a.W + MOM_SLICES - 1) / MOM_SLICES;
+  const int wb = sl * per, we = min(a.W, wb + per);
+  __shared__ double red[4][64];
+  for (int m0 = 0; m0 < a.M; m0 += 64) {
+    const int m = m0 + tm;
+    double s = 0.0;
+    if (m < a.M)
+      for (int w = wb + tw; w < we; w += 4) s += b.g_chi[((long)w * a.L + l) * a.M + m];
+    red[tw][tm] = s;
+    __syncthreads();
+    if (tw == 0 && m < a.M)
+      part[((long)l * MOM_SLICES + sl) * a.M + m] = ((red[0][tm] + red[1][tm]) + red[2][tm]) + red[3][tm];
+    __syncthreads();
+  }
+}
+
+__global__ void __launch_bounds__(256) moments16_reduce_kernel(FwdArgs a, AdjArgs b,
+                                                               const double* part) {
   const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= (long)a.L * a.M) return;
   const int l = (int)(k / a.M), m = (int)(k % a.M);
   double s = 0.0;
-  for (int w = 0; w < a.W; ++w) s += b.g_chi[((long)w * a.L + l) * a.M + m];
+  for (int sl = 0; sl < MOM_SLICES; ++sl) s += part[((long)l * MOM_SLICES + sl) * a.M + m];
   b.d_mom[(long)lin(a, l) * a.M + m] += s;
 }
 

@@ -18,6 +18,8 @@
 #include "../../include/adrt_hip.h"
 #include "adrt_fwd.hpp"
 #include "adrt_adj16.hpp"
+#include "adrt_mix16.hpp"
+#include "adrt_adjm16.hpp"
 
 namespace {
 
@@ -209,15 +211,17 @@ struct adrt_workspace {
   double *mu = nullptr, *wt = nullptr, *Pl = nullptr, *Pl0 = nullptr;
   double *tau_s = nullptr, *omega_s = nullptr, *tau_cum = nullptr, *fdm = nullptr;
   double *lay = nullptr, *rb = nullptr;
-  int *mode = nullptr, *lflag = nullptr, *flag = nullptr;
+  int *mode = nullptr, *lflag = nullptr, *flag = nullptr, *redo = nullptr;
   // reverse pass (ADRT_WORKSPACE_ADJOINT)
   int flags = 0;
   double *Iv = nullptr, *Av = nullptr, *g_taus = nullptr, *g_omgs = nullptr, *g_Bt = nullptr,
-         *g_Bb = nullptr, *g_Ftop = nullptr, *g_f = nullptr, *g_chi = nullptr, *slab = nullptr;
+         *g_Bb = nullptr, *g_Ftop = nullptr, *g_f = nullptr, *g_chi = nullptr, *slab = nullptr,
+         *mom_part = nullptr;
   int* status = nullptr;
   int nwaves = 0, nn_cap = 0;
   unsigned long long generation = 0;  // forward solves so far
   bool kept_valid = false;            // the last forward's state (one chunk) is still held
+  bool kept_mixed = false;            // ... and was computed by the mixed-precision form
   double xfac = 0.0;
   double tables_mu0 = -1.0;
   bool tables_ready = false;
@@ -236,7 +240,7 @@ size_t per_wavenumber_bytes(int L, int Np, int M, bool adjoint) {
   const size_t d = sizeof(double);
   size_t b = 0;
   b += 4 * (size_t)L * d + (size_t)(L + 1) * d;      // tau_s, omega_s, fdm, spare, tau_cum
-  b += 2 * (size_t)L * sizeof(int);                   // mode, lflag
+  b += 3 * (size_t)L * sizeof(int);                   // mode, lflag, redo
   b += (size_t)L * (2 * Np * Np + 2 * Np) * d;        // layer store
   b += (size_t)L * (2 * Np * Np + Np) * d;            // base composites
   if (adjoint) {
@@ -297,7 +301,8 @@ void prof_end(adrt_workspace* ws, int k, hipStream_t stream) {
 }
 
 template <int N>
-int launch_chunk(const adrt::dev::FwdArgs& a, adrt_workspace* ws, hipStream_t stream) {
+int launch_chunk(const adrt::dev::FwdArgs& a, adrt_workspace* ws, hipStream_t stream,
+                 bool mixed) {
   using namespace adrt::dev;
   constexpr int GPB = Geo<N>::GPB;  // lane groups per 256-thread block
   int e = prof_begin(ws, 0, stream);
@@ -306,12 +311,22 @@ int launch_chunk(const adrt::dev::FwdArgs& a, adrt_workspace* ws, hipStream_t st
   const long tasks = (long)a.W * a.L;
   if constexpr (N == 16) {  // one wave per problem on the matrix cores
     e = prof_begin(ws, 1, stream);
-    w16::layer16_kernel<<<(unsigned)((tasks + 3) / 4), 256, 0, stream>>>(a);
+    if (mixed) {
+      w16::layer16m_kernel<<<(unsigned)((tasks + 3) / 4), 256, 0, stream>>>(a);
+      // layers whose doubling start is outside FP32's reach, again in FP64
+      w16::layer16_kernel<true><<<(unsigned)((tasks + 3) / 4), 256, 0, stream>>>(a);
+    } else {
+      w16::layer16_kernel<false><<<(unsigned)((tasks + 3) / 4), 256, 0, stream>>>(a);
+    }
     prof_end(ws, e, stream);
     e = prof_begin(ws, 2, stream);
-    w16::adding16_kernel<false><<<(unsigned)((a.W + 3) / 4), 256, 0, stream>>>(a);
+    if (mixed)
+      w16::adding16_kernel<false, true><<<(unsigned)((a.W + 3) / 4), 256, 0, stream>>>(a);
+    else
+      w16::adding16_kernel<false, false><<<(unsigned)((a.W + 3) / 4), 256, 0, stream>>>(a);
     prof_end(ws, e, stream);
   } else {
+    if (mixed) return ADRT_ERR_PRECISION;
     e = prof_begin(ws, 1, stream);
     layer_kernel<N><<<(unsigned)((tasks + GPB - 1) / GPB), 256, 0, stream>>>(a);
     prof_end(ws, e, stream);
@@ -370,7 +385,8 @@ int adrt_workspace_create_ex(const adrt_batch_config* cfg, size_t max_bytes, int
   if (max_bytes == 0) max_bytes = (size_t)4 << 30;
   const size_t per_w = per_wavenumber_bytes(ws->L, ws->Np, ws->M, adjoint);
   const size_t fixed =
-      align256((size_t)(2 * ws->Np + ws->Lmax * ws->Np + ws->Lmax) * 8) + 512 + slab_bytes;
+      align256((size_t)(2 * ws->Np + ws->Lmax * ws->Np + ws->Lmax) * 8) + 512 + slab_bytes +
+      (adjoint ? align256((size_t)ws->L * adrt::dev::w16::MOM_SLICES * ws->M * 8) : 0);
   long chunk = max_bytes > fixed ? (long)((max_bytes - fixed) / (per_w + 4096)) : 1;
   const int W = std::max(cfg->num_wavenumbers, 1);
   chunk = std::max(1L, std::min(chunk, (long)W));
@@ -386,13 +402,15 @@ int adrt_workspace_create_ex(const adrt_batch_config* cfg, size_t max_bytes, int
   const size_t o_tau_s = take(C * L * 8), o_om = take(C * L * 8), o_fdm = take(C * L * 8);
   const size_t o_cum = take(C * (L + 1) * 8);
   const size_t o_mode = take(C * L * 4), o_lflag = take(C * L * 4), o_flag = take(4);
+  const size_t o_redo = take(C * L * 4);
   const size_t o_lay = take(C * L * (2 * Np * Np + 2 * Np) * 8);
   const size_t o_rb = take(C * L * (2 * Np * Np + Np) * 8);
-  size_t o_iv = 0, o_av = 0, o_g = 0, o_chi = 0, o_slab = 0, o_status = 0;
+  size_t o_iv = 0, o_av = 0, o_g = 0, o_chi = 0, o_slab = 0, o_status = 0, o_part = 0;
   if (adjoint) {
     o_iv = take(C * (L + 1) * 2 * Np * 8);
     o_av = take(C * (L + 1) * 2 * Np * 8);
     o_g = take(6 * C * L * 8);
+    o_part = take(L * adrt::dev::w16::MOM_SLICES * M * 8);
     o_chi = take(C * L * M * 8);
     o_slab = take(slab_bytes);
     o_status = take(4);
@@ -412,6 +430,7 @@ int adrt_workspace_create_ex(const adrt_batch_config* cfg, size_t max_bytes, int
   ws->tau_cum = (double*)(ws->base + o_cum);
   ws->mode = (int*)(ws->base + o_mode);
   ws->lflag = (int*)(ws->base + o_lflag);
+  ws->redo = (int*)(ws->base + o_redo);
   ws->flag = (int*)(ws->base + o_flag);
   ws->lay = (double*)(ws->base + o_lay);
   ws->rb = (double*)(ws->base + o_rb);
@@ -426,6 +445,7 @@ int adrt_workspace_create_ex(const adrt_batch_config* cfg, size_t max_bytes, int
     ws->g_Ftop = g + 4 * C * L;
     ws->g_f = g + 5 * C * L;
     ws->g_chi = (double*)(ws->base + o_chi);
+    ws->mom_part = (double*)(ws->base + o_part);
     ws->slab = (double*)(ws->base + o_slab);
     ws->status = (int*)(ws->base + o_status);
     if (hipMemset(ws->status, 0, sizeof(int)) != hipSuccess) {
@@ -498,22 +518,24 @@ adrt::dev::FwdArgs make_args(const adrt_batch_config* cfg, const adrt_device_inp
   a.mode = ws->mode;
   a.lay = ws->lay;
   a.lflag = ws->lflag;
+  a.redo = ws->redo;
   a.rb = ws->rb;
   a.Iv = ws->Iv;  // kept for the reverse pass (adjoint workspaces only)
   a.Av = ws->Av;
   return a;
 }
 
-int forward_chunk(const adrt::dev::FwdArgs& a, adrt_workspace* ws, hipStream_t stream) {
+int forward_chunk(const adrt::dev::FwdArgs& a, adrt_workspace* ws, hipStream_t stream,
+                  bool mixed) {
   switch (ws->Np) {
 #ifndef ADRT_ONLY_N
-    case 2: return launch_chunk<2>(a, ws, stream);
-    case 4: return launch_chunk<4>(a, ws, stream);
-    case 8: return launch_chunk<8>(a, ws, stream);
-    case 16: return launch_chunk<16>(a, ws, stream);
-    case 32: return launch_chunk<32>(a, ws, stream);
+    case 2: return launch_chunk<2>(a, ws, stream, mixed);
+    case 4: return launch_chunk<4>(a, ws, stream, mixed);
+    case 8: return launch_chunk<8>(a, ws, stream, mixed);
+    case 16: return launch_chunk<16>(a, ws, stream, mixed);
+    case 32: return launch_chunk<32>(a, ws, stream, mixed);
 #else
-    case ADRT_ONLY_N: return launch_chunk<ADRT_ONLY_N>(a, ws, stream);
+    case ADRT_ONLY_N: return launch_chunk<ADRT_ONLY_N>(a, ws, stream, mixed);
 #endif
     default: return ADRT_ERR_QUADRATURE_MAX;
   }
@@ -528,10 +550,13 @@ int check_solve_args(const adrt_batch_config* cfg, const adrt_device_inputs* in,
   if (cfg->use_diffusion_lower_bc && !in->planck_levels) return ADRT_ERR_DIFFUSION_BC;
   if (!out->flux_up || !out->flux_down || !out->mean_intensity || !out->flux_direct)
     return ADRT_ERR_OUTPUTS;
-  if (precision != ADRT_PRECISION_F64) return ADRT_ERR_PRECISION;
   if (!ws) return ADRT_ERR_WORKSPACE;
   if (ws->L != cfg->num_layers || ws->n_real != cfg->num_quadrature || ws->M != cfg->num_moments)
     return ADRT_ERR_WORKSPACE;
+  // the mixed form runs at quadrature order 16 (native 9..16, or padded on a
+  // reverse-pass workspace)
+  if (precision != ADRT_PRECISION_F64 && !(precision == ADRT_PRECISION_MIXED && ws->Np == 16))
+    return ADRT_ERR_PRECISION;
   return ADRT_OK;
 }
 
@@ -556,11 +581,12 @@ int adrt_solve_batch(const adrt_batch_config* cfg, const adrt_device_inputs* in,
   for (int w0 = 0; w0 < cfg->num_wavenumbers; w0 += ws->chunk) {
     a.w0 = w0;
     a.W = std::min(ws->chunk, cfg->num_wavenumbers - w0);
-    rc = forward_chunk(a, ws, stream);
+    rc = forward_chunk(a, ws, stream, precision == ADRT_PRECISION_MIXED);
     if (rc) return rc;
   }
   ws->generation += 1;
   ws->kept_valid = ws->Iv != nullptr && cfg->num_wavenumbers <= ws->chunk;
+  ws->kept_mixed = precision == ADRT_PRECISION_MIXED;
   return ADRT_OK;
 }
 
@@ -616,30 +642,43 @@ int adrt_adjoint_batch(const adrt_batch_config* cfg, const adrt_device_inputs* i
     const long n = (long)cfg->num_layers * cfg->num_moments;
     w16::fill_kernel<<<(unsigned)((n + 255) / 256), 256, 0, stream>>>(b.d_mom, n, 0.0);
   }
+  const bool mixed = precision == ADRT_PRECISION_MIXED;
+  if (mixed != ws->kept_mixed) forward_generation = 0;  // kept records are of the other form
   const bool reuse = forward_generation != 0 && ws->kept_valid &&
                      forward_generation == ws->generation && cfg->num_wavenumbers <= ws->chunk;
   for (int w0 = 0; w0 < cfg->num_wavenumbers; w0 += ws->chunk) {
     a.w0 = w0;
     a.W = std::min(ws->chunk, cfg->num_wavenumbers - w0);
     if (!reuse) {
-      rc = forward_chunk(a, ws, stream);
+      rc = forward_chunk(a, ws, stream, mixed);
       if (rc) return rc;
       ws->kept_valid = false;
     }
     int e = prof_begin(ws, 3, stream);
-    w16::adding16_kernel<true><<<(unsigned)((a.W + 3) / 4), 256, 0, stream>>>(a);
+    if (mixed)
+      w16::adding16_kernel<true, true><<<(unsigned)((a.W + 3) / 4), 256, 0, stream>>>(a);
+    else
+      w16::adding16_kernel<true, false><<<(unsigned)((a.W + 3) / 4), 256, 0, stream>>>(a);
     prof_end(ws, e, stream);
     const long tasks = (long)a.W * a.L;
     const long blocks = std::min((tasks + 3) / 4, (long)ws->nwaves / 4);
     b.nwaves = (int)(blocks * 4);
     e = prof_begin(ws, 4, stream);
-    w16::layer16_adjoint_kernel<<<(unsigned)blocks, 256, 0, stream>>>(a, b);
+    if (mixed) {
+      w16::layer16m_adjoint_kernel<<<(unsigned)blocks, 256, 0, stream>>>(a, b);
+      w16::layer16_adjoint_kernel<true><<<(unsigned)blocks, 256, 0, stream>>>(a, b);
+    } else {
+      w16::layer16_adjoint_kernel<false><<<(unsigned)blocks, 256, 0, stream>>>(a, b);
+    }
     prof_end(ws, e, stream);
     e = prof_begin(ws, 5, stream);
     w16::column16_adjoint_kernel<<<(unsigned)((a.W + 255) / 256), 256, 0, stream>>>(a, b);
     if (shared && b.d_mom) {
       const long n = (long)a.L * a.M;
-      w16::moments16_reduce_kernel<<<(unsigned)((n + 255) / 256), 256, 0, stream>>>(a, b);
+      double* part = ws->mom_part;
+      w16::moments16_partial_kernel<<<dim3((unsigned)a.L, w16::MOM_SLICES), 256, 0, stream>>>(
+          a, b, part);
+      w16::moments16_reduce_kernel<<<(unsigned)((n + 255) / 256), 256, 0, stream>>>(a, b, part);
     }
     prof_end(ws, e, stream);
     if (hipGetLastError() != hipSuccess) return ADRT_ERR_HIP;

@@ -59,6 +59,7 @@ struct FwdArgs {
   int* mode;        // [W][L]   0 no phase, 1 all M moments, 2 first min(M,2n), 3 delta-M
   double* lay;      // [W][L][2N^2 + 2N]   R, T (column-major), s_up, s_down
   int* lflag;       // [W][L]   is_scattering
+  int* redo;        // [W][L]   mixed form: layer recomputed by the FP64 kernel
   double* rb;       // [W][L][2N^2 + N]    base composites R_ab, T_ba, s_up (n_base = 1..L)
   // reverse pass (N = 16 kernels)
   double* Iv;         // [W][L+1][2][N] forward interface intensities (up, down); null = not kept

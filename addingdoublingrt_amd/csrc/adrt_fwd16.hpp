@@ -22,63 +22,27 @@ constexpr int REC16 = 2 * N * N + 2 * N;  // R, T (lane order), s_up, s_down
 constexpr int RBR16 = 2 * N * N + N;      // R_ab, T_ba (lane order), s_up
 
 // ---------------------------------------------------------------------------
-// layer setup + doubling, one wave per (wavenumber, layer)
+// phase matrices and thin-layer start of a scattering layer, in FP64, for the
+// register layout of R (src/phase_matrix.hpp:61-181, src/doubling.hpp:79-147)
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) layer16_kernel(FwdArgs a) {
-  using R = double;
-  __shared__ R smem[4 * Lds16::size];
-  const int wv = threadIdx.x >> 6;
-  const long task = (long)blockIdx.x * 4 + wv;
-  const int L_ = a.L;
-  if (task >= (long)a.W * L_) return;
-  const Lane L;
-  const int w = (int)(task / L_), l = (int)(task % L_);
-  const long gw = (long)(a.w0 + w);
-  R* sm = smem + wv * Lds16::size;
-  R* tile = sm + Lds16::tile;
-  R* tile2 = sm + Lds16::tile2;
-  R* pv = sm + Lds16::pv;
-  R* v0 = sm + Lds16::vs;
-  R* v1 = v0 + N;
-  R* v2 = v1 + N;
-  R* v3 = v2 + N;
-  R* out = a.lay + task * REC16;
+template <typename R>
+struct Start16 {
+  double Tv[4], Rv[4];  // T0, R0 at rows row<R>(h, r) of column j
+  double Sv[4];         // I - T0 = tau0 G++, without cancellation
+  double y, sus, sds;   // element j
+  double gam, gk, tau0;
+  int nn;
+};
 
-  const double tau = a.tau_s[task];
-  const double omega_in = a.omega_s[task];
-  const int mode = a.mode[task];
-  const double Btop = a.B ? a.B[gw * (L_ + 1) + lev(a, l)] : 0.0;
-  const double Bbot = a.B ? a.B[gw * (L_ + 1) + lev(a, l + 1)] : 0.0;
-  const double B_bar = (Bbot + Btop) / 2.0;
-  const double B_d = (tau > 0.0) ? (Bbot - Btop) / tau : 0.0;
+// dtile: a TILE of doubles (LDS), pv: 16 doubles (LDS)
+template <typename R>
+__device__ __forceinline__ void layer16_start(const FwdArgs& a, long task, int w, int l, long gw,
+                                              const Lane& L, double tau, double omega_in,
+                                              int mode, double* dtile, double* pv,
+                                              Start16<R>& st) {
+  const int L_ = a.L;
   const int j = L.j;
   const double mu_j = a.mu[j], wt_j = a.wt[j];
-
-  if (tau <= 0.0 || omega_in <= 0.0) {
-    // transparent (src/doubling.hpp:52-53) or pure absorption (:56-77)
-    double trans = 1.0, su = 0.0, sd = 0.0;
-    if (tau > 0.0) {
-      const double tex = -tau / mu_j;
-      trans = (tex > -200.0) ? exp(tex) : 0.0;
-      const double omt = 1.0 - trans;
-      const double slope = mu_j * omt - 0.5 * tau * (1.0 + trans);
-      su = B_bar * omt + B_d * slope;
-      sd = B_bar * omt - B_d * slope;
-    }
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      out[L.l * 4 + r] = 0.0;
-      out[N * N + L.l * 4 + r] = (row<R>(L.h, r) == j) ? trans : 0.0;
-    }
-    if (L.h == 0) {
-      out[2 * N * N + j] = su;
-      out[2 * N * N + N + j] = sd;
-    }
-    if (L.l == 0) a.lflag[task] = 0;
-    return;
-  }
-
-  // ---- phase matrices (src/phase_matrix.hpp:61-181) ----
   const double omega = omega_in > 1.0 ? 1.0 : omega_in;
   const double con = 2.0 * omega * kPi;
   const int li = lin(a, l);
@@ -121,12 +85,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))
   for (int r = 0; r < 4; ++r) {
     Ppp[r] = Ppp[r] / (2.0 * kPi);
     Ppm[r] = Ppm[r] / (2.0 * kPi);
-    tile[j * TS + rows[r]] = (Ppp[r] + Ppm[r]) * a.wt[rows[r]];
+    dtile[j * TS + rows[r]] = (Ppp[r] + Ppm[r]) * a.wt[rows[r]];
   }
   wave_sync();
   double csum = 0.0;
 #pragma unroll
-  for (int i = 0; i < N; ++i) csum += tile[j * TS + i];
+  for (int i = 0; i < N; ++i) csum += dtile[j * TS + i];
   wave_sync();
   if (csum > 0.0) {
     const double corr = 1.0 / (2.0 * kPi * csum);
@@ -137,39 +101,116 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))
   if (a.has_solar) {
     pplus = spj / (2.0 * kPi);
     pminus = smj / (2.0 * kPi);
-    const double s = colsum<R>((pplus + pminus) * wt_j, L, pv);
+    const double s = colsum<double>((pplus + pminus) * wt_j, L, pv);
     if (s > 0.0) {
       const double corr = 1.0 / (2.0 * kPi * s);
       pplus *= corr;
       pminus *= corr;
     }
   }
-
-  // ---- thin-layer initialisation (src/doubling.hpp:83-147) ----
+  // thin-layer initialisation (src/doubling.hpp:83-147)
   const int nn0 = (int)(log(tau) / log(2.0)) + ipow0(omega);
-  const int nn = nn0 < 1 ? 1 : nn0;
-  const double tau0 = tau * ldexp(1.0, -nn);
-  Mat<R> Rm, Tm;
+  st.nn = nn0 < 1 ? 1 : nn0;
+  st.tau0 = tau * ldexp(1.0, -st.nn);
+  const double tau0 = st.tau0;
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
     const int i = rows[r];
     const double mui = a.mu[i];
     const double gpp = (((i == j) ? 1.0 : 0.0) + (-con) * (Ppp[r] * wt_j)) / mui;
     const double gpm = con * (Ppm[r] * wt_j) / mui;
-    Tm.v[r] = ((i == j) ? 1.0 : 0.0) - tau0 * gpp;
-    Rm.v[r] = tau0 * gpm;
+    st.Tv[r] = ((i == j) ? 1.0 : 0.0) - tau0 * gpp;
+    st.Sv[r] = tau0 * gpp;
+    st.Rv[r] = tau0 * gpm;
   }
-  double y = (1.0 - omega) * tau0 / mu_j;
-  double z = 0.0;
-  double sus = 0.0, sds = 0.0, gam = 0.0;
+  st.y = (1.0 - omega) * tau0 / mu_j;
+  st.sus = 0.0;
+  st.sds = 0.0;
+  st.gam = 0.0;
   if (a.has_solar) {
     const double Ftop = a.F0 * exp(-a.tau_cum[(long)w * (L_ + 1) + l] / a.mu0);
     const double base = omega * tau0 / mu_j * Ftop;
-    sus = base * pminus;
-    sds = base * pplus;
-    gam = exp(-tau0 / a.mu0);
+    st.sus = base * pminus;
+    st.sds = base * pplus;
+    st.gam = exp(-tau0 / a.mu0);
   }
-  double gk = 0.5 * tau0;
+  st.gk = 0.5 * tau0;
+}
+
+// ---------------------------------------------------------------------------
+// layer setup + doubling, one wave per (wavenumber, layer).
+// REDO: only the layers the mixed-precision kernel marked (its FP32 recursion
+// cannot follow them, see adrt_mix16.hpp), written as mixed records: R and
+// S = I - T in FP32 register order of the FP32 matrix instruction, sources FP64.
+// ---------------------------------------------------------------------------
+template <bool REDO>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) layer16_kernel(FwdArgs a) {
+  using R = double;
+  __shared__ R smem[4 * Lds16::size];
+  const int wv = threadIdx.x >> 6;
+  const long task = (long)blockIdx.x * 4 + wv;
+  const int L_ = a.L;
+  if (task >= (long)a.W * L_) return;
+  if constexpr (REDO) {
+    if (!a.redo[task]) return;
+  }
+  const Lane L;
+  const int w = (int)(task / L_), l = (int)(task % L_);
+  const long gw = (long)(a.w0 + w);
+  R* sm = smem + wv * Lds16::size;
+  R* tile = sm + Lds16::tile;
+  R* tile2 = sm + Lds16::tile2;
+  R* pv = sm + Lds16::pv;
+  R* v0 = sm + Lds16::vs;
+  R* v1 = v0 + N;
+  R* v2 = v1 + N;
+  R* v3 = v2 + N;
+  R* out = a.lay + task * REC16;
+
+  const double tau = a.tau_s[task];
+  const double omega_in = a.omega_s[task];
+  const int mode = a.mode[task];
+  const double Btop = a.B ? a.B[gw * (L_ + 1) + lev(a, l)] : 0.0;
+  const double Bbot = a.B ? a.B[gw * (L_ + 1) + lev(a, l + 1)] : 0.0;
+  const double B_bar = (Bbot + Btop) / 2.0;
+  const double B_d = (tau > 0.0) ? (Bbot - Btop) / tau : 0.0;
+  const int j = L.j;
+  const double mu_j = a.mu[j];
+
+  if (tau <= 0.0 || omega_in <= 0.0) {
+    // transparent (src/doubling.hpp:52-53) or pure absorption (:56-77)
+    double trans = 1.0, su = 0.0, sd = 0.0;
+    if (tau > 0.0) {
+      const double tex = -tau / mu_j;
+      trans = (tex > -200.0) ? exp(tex) : 0.0;
+      const double omt = 1.0 - trans;
+      const double slope = mu_j * omt - 0.5 * tau * (1.0 + trans);
+      su = B_bar * omt + B_d * slope;
+      sd = B_bar * omt - B_d * slope;
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      out[L.l * 4 + r] = 0.0;
+      out[N * N + L.l * 4 + r] = (row<R>(L.h, r) == j) ? trans : 0.0;
+    }
+    if (L.h == 0) {
+      out[2 * N * N + j] = su;
+      out[2 * N * N + N + j] = sd;
+    }
+    if (L.l == 0) a.lflag[task] = 0;
+    return;
+  }
+
+  Start16<R> st;
+  layer16_start<R>(a, task, w, l, gw, L, tau, omega_in, mode, tile, pv, st);
+  const int nn = st.nn;
+  Mat<R> Rm, Tm;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    Tm.v[r] = st.Tv[r];
+    Rm.v[r] = st.Rv[r];
+  }
+  double y = st.y, z = 0.0, sus = st.sus, sds = st.sds, gam = st.gam, gk = st.gk;
 
   // ---- doubling recursion (src/doubling.hpp:149-209) ----
   for (int k = 0; k < nn; ++k) {
@@ -237,11 +278,29 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))
   }
 
   // ---- layer assembly (src/doubling.hpp:211-229), sources combined ----
-  gstore<R>(out, L, Rm);
-  gstore<R>(out + N * N, L, Tm);
-  if (L.h == 0) {
-    out[2 * N * N + j] = (y * B_bar + z * B_d) + sus;
-    out[2 * N * N + N + j] = (y * B_bar - z * B_d) + sds;
+  if constexpr (REDO) {
+    // to the FP32 register layout through LDS: R, then S = I - T
+    float* om = (float*)out;
+    put<R>(tile, L, Rm);
+    put<R>(tile2, L, Tm);
+    wave_sync();
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int i = row<float>(L.h, r);
+      om[L.l * 4 + r] = (float)tile[j * TS + i];
+      om[N * N + L.l * 4 + r] = (float)(((i == j) ? 1.0 : 0.0) - tile2[j * TS + i]);
+    }
+    if (L.h == 0) {
+      out[N * N + j] = (y * B_bar + z * B_d) + sus;
+      out[N * N + N + j] = (y * B_bar - z * B_d) + sds;
+    }
+  } else {
+    gstore<R>(out, L, Rm);
+    gstore<R>(out + N * N, L, Tm);
+    if (L.h == 0) {
+      out[2 * N * N + j] = (y * B_bar + z * B_d) + sus;
+      out[2 * N * N + N + j] = (y * B_bar - z * B_d) + sds;
+    }
   }
   if (L.l == 0) a.lflag[task] = 1;
 }
@@ -288,26 +347,43 @@ __device__ __forceinline__ void ibar16(const FwdArgs& a, long gw, int l, double 
 // layer l of the forward system, or of its transpose (ADJ): transposed R and
 // T, source up = cotangent of I_dn(l), source down = cotangent of I_up(l+1).
 // The matrices are read on demand so that they are live only where used.
-template <bool ADJ>
+// F32: records of the mixed-precision layer kernel (R and S = I - T in FP32
+// register order of the FP32 matrix instruction, rows 4h + r; sources FP64
+// after them); T is rebuilt as I - S in FP64.
+template <bool ADJ, bool F32>
 struct LayerRef {
   const FwdArgs& a;
   const double* rec;
   long gw;
   double mu_j, wt_j;
   int scat;
-  __device__ void mats(const Lane& L, Mat<double>& R, Mat<double>& T) const {
-    if constexpr (ADJ) {
-      gload_t<double>(rec, L, R);
-      gload_t<double>(rec + N * N, L, T);
+  // element (row<double>(h, r), j) of a stored N x N matrix, or of its
+  // transpose; eye_minus: the record holds I - M
+  __device__ static void load(const double* m, const Lane& L, bool trans, bool eye_minus,
+                              Mat<double>& M) {
+    if constexpr (F32) {
+      const float* f = (const float*)m;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = row<double>(L.h, r);
+        // FP32 element (p, c) sits in lane c + 16 (p >> 2), register p & 3
+        const double v = trans ? (double)f[(i + 16 * (L.j >> 2)) * 4 + (L.j & 3)]
+                               : (double)f[(L.j + 16 * (i >> 2)) * 4 + (i & 3)];
+        M.v[r] = eye_minus ? ((i == L.j) ? 1.0 : 0.0) - v : v;
+      }
     } else {
-      gload<double>(rec, L, R);
-      gload<double>(rec + N * N, L, T);
+      if (trans) gload_t<double>(m, L, M);
+      else gload<double>(m, L, M);
     }
   }
-  __device__ void trans(const Lane& L, Mat<double>& T) const {
-    if constexpr (ADJ) gload_t<double>(rec + N * N, L, T);
-    else gload<double>(rec + N * N, L, T);
+  __device__ const double* tmat() const {
+    return F32 ? (const double*)((const float*)rec + N * N) : rec + N * N;
   }
+  __device__ void mats(const Lane& L, Mat<double>& R, Mat<double>& T) const {
+    load(rec, L, ADJ, false, R);
+    load(tmat(), L, ADJ, true, T);
+  }
+  __device__ void trans(const Lane& L, Mat<double>& T) const { load(tmat(), L, ADJ, true, T); }
   __device__ void sources(const Lane& L, int l, double& su, double& sd) const {
     if constexpr (ADJ) {
       double u, d;
@@ -316,22 +392,23 @@ struct LayerRef {
       ibar16(a, gw, l + 1, mu_j, wt_j, u, d);
       sd = u;
     } else {
-      su = rec[2 * N * N + L.j];
-      sd = rec[2 * N * N + N + L.j];
+      const double* src = rec + (F32 ? N * N : 2 * N * N);
+      su = src[L.j];
+      sd = src[N + L.j];
     }
   }
 };
 
-template <bool ADJ>
-__device__ __forceinline__ LayerRef<ADJ> layer_ref(const FwdArgs& a, const double* lay,
-                                                   const int* lflag, long gw, int l, double mu_j,
-                                                   double wt_j) {
-  return LayerRef<ADJ>{a, lay + (long)l * REC16, gw, mu_j, wt_j, lflag[l]};
+template <bool ADJ, bool F32>
+__device__ __forceinline__ LayerRef<ADJ, F32> layer_ref(const FwdArgs& a, const double* lay,
+                                                        const int* lflag, long gw, int l,
+                                                        double mu_j, double wt_j) {
+  return LayerRef<ADJ, F32>{a, lay + (long)l * REC16, gw, mu_j, wt_j, lflag[l]};
 }
 
 // rbase step: new = add(top = layer y, bot = C)  (src/adding.hpp:246-258)
-template <bool ADJ>
-__device__ void add16_on_top(Comp16& C, const LayerRef<ADJ>& y, int l, const Lane& L,
+template <class LR>
+__device__ void add16_on_top(Comp16& C, const LR& y, int l, const Lane& L,
                              const Ws16& s) {
   using R = double;
   const int j = L.j;
@@ -442,8 +519,8 @@ __device__ void add16_on_top(Comp16& C, const LayerRef<ADJ>& y, int l, const Lan
 }
 
 // rtop step: new = add(top = C, bot = layer y)
-template <bool ADJ>
-__device__ void add16_below(Comp16& C, const LayerRef<ADJ>& y, int l, const Lane& L,
+template <class LR>
+__device__ void add16_below(Comp16& C, const LR& y, int l, const Lane& L,
                             const Ws16& s) {
   using R = double;
   const int j = L.j;
@@ -552,7 +629,7 @@ __device__ void add16_below(Comp16& C, const LayerRef<ADJ>& y, int l, const Lane
   C.scat = 1;
 }
 
-template <bool ADJ>
+template <bool ADJ, bool F32 = false>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8)))
 adding16_kernel(FwdArgs a) {
   using R = double;
@@ -628,7 +705,7 @@ adding16_kernel(FwdArgs a) {
     C.sd = 0.0;
     C.scat = A > 0.0;
   } else {
-    const LayerRef<ADJ> y = layer_ref<ADJ>(a, lay, lflag, gw, Lay - 1, mu_j, wt_j);
+    const auto y = layer_ref<ADJ, F32>(a, lay, lflag, gw, Lay - 1, mu_j, wt_j);
     y.mats(L, C.Rab, C.Tab);
     C.Rba = C.Rab;
     C.Tba = C.Tab;
@@ -644,7 +721,7 @@ adding16_kernel(FwdArgs a) {
     }
     if (n == ltot) break;
     const int k = ltot - 1 - n;
-    add16_on_top<ADJ>(C, layer_ref<ADJ>(a, lay, lflag, gw, k, mu_j, wt_j), k, L, s);
+    add16_on_top(C, layer_ref<ADJ, F32>(a, lay, lflag, gw, k, mu_j, wt_j), k, L, s);
   }
 
   auto emit = [&](int l, double Iup, double Idn) {
@@ -695,7 +772,7 @@ adding16_kernel(FwdArgs a) {
 
   // ---- top-down sweep with interfaces (src/solver.cpp:574-583, 644-678) ----
   {
-    const LayerRef<ADJ> y = layer_ref<ADJ>(a, lay, lflag, gw, 0, mu_j, wt_j);
+    const auto y = layer_ref<ADJ, F32>(a, lay, lflag, gw, 0, mu_j, wt_j);
     y.mats(L, C.Rab, C.Tab);
     C.Rba = C.Rab;
     C.Tba = C.Tab;
@@ -757,7 +834,7 @@ adding16_kernel(FwdArgs a) {
       Iup = Ibu;
     }
     emit(l, Iup, Idn);
-    if (l < Lay) add16_below<ADJ>(C, layer_ref<ADJ>(a, lay, lflag, gw, l, mu_j, wt_j), l, L, s);
+    if (l < Lay) add16_below(C, layer_ref<ADJ, F32>(a, lay, lflag, gw, l, mu_j, wt_j), l, L, s);
   }
 }
 

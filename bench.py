@@ -46,7 +46,7 @@ def parse():
     p.add_argument("--warmup", type=int, default=1)
     p.add_argument("--wavenumbers", type=int, default=65536)
     p.add_argument("--config", type=int, default=5)
-    p.add_argument("--form", choices=("f64",), default="f64")
+    p.add_argument("--form", choices=("f64", "mixed"), default="f64")
     p.add_argument("--cpu-seconds", type=float, default=12.0)
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--workspace-gb", type=float, default=96.0)
@@ -113,10 +113,11 @@ def main():
 
     def step():
         o = solve_batch(*inputs, **kw, out=out, max_workspace_bytes=ws_bytes,
-                        adjoint_workspace=True)
+                        adjoint_workspace=True, precision=a.form)
         gen = o.pop("_generation")
         grads.update(adjoint_batch(*inputs, out, cot, **kw, forward_generation=gen,
-                                   max_workspace_bytes=ws_bytes, check_status=False))
+                                   max_workspace_bytes=ws_bytes, check_status=False,
+                                   precision=a.form))
 
     for _ in range(a.warmup):
         step()
@@ -163,12 +164,14 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if a.form == "f64" else "f32+f64",
         "data": "synthetic (numpy PCG64 seed 20261203+k; Planck levels from the device kernel)",
         "config": {
             "workload": f"config {a.config}: {cols.name}, {W} wavenumbers x {L} layers, "
                         f"N={cols.N}",
-            "form": "plain FP64 forward + reverse pass (the mixed-precision form is not built yet)",
+            "form": ("plain FP64 forward + reverse pass" if a.form == "f64" else
+                     "mixed forward (FP32 doubling with FP64-refined solves, FP64 adding) "
+                     "+ reverse pass"),
             "num_wavenumbers": W, "num_layers": L, "num_quadrature": cols.N,
             "algorithmic_gflop_per_step": (2 * work["total"] + work["doubling_reverse"]
                                            - work["doubling"]) / 1e9,
