@@ -72,8 +72,25 @@ struct Phase16 {
   bool solar;
 };
 
+// Each wave's slab ends with a parking area for its lanes' Phase16: the phase
+// data are needed again only after both doubling loops, so they wait in memory
+// rather than in registers while the loops run.
+constexpr int PHASE_WORDS = (int)((sizeof(Phase16<double>) + 7) / 8);
+static_assert(sizeof(Phase16<double>) == sizeof(Phase16<float>), "one parking size");
+__host__ __device__ constexpr long slab_stride(int nn_cap) {
+  return (long)nn_cap * SLAB + 64 * PHASE_WORDS;
+}
 template <typename RowT>
-__device__ __noinline__ void phase16_recompute(const FwdArgs& a, long task, int w, int l,
+__device__ __forceinline__ void park(double* slab, int nn_cap, const Lane& L, const Phase16<RowT>& p) {
+  *(Phase16<RowT>*)(slab + (long)nn_cap * SLAB + L.l * PHASE_WORDS) = p;
+}
+template <typename RowT>
+__device__ __forceinline__ void unpark(const double* slab, int nn_cap, const Lane& L, Phase16<RowT>& p) {
+  p = *(const Phase16<RowT>*)(slab + (long)nn_cap * SLAB + L.l * PHASE_WORDS);
+}
+
+template <typename RowT>
+__device__ __forceinline__ void phase16_recompute(const FwdArgs& a, long task, int w, int l,
                                                   long gw, const Lane& L, double tau,
                                                   double omega_in, int mode, double* dtile,
                                                   double* csl, Phase16<RowT>& p) {
@@ -161,7 +178,7 @@ __device__ __noinline__ void phase16_recompute(const FwdArgs& a, long task, int 
 // solar start vectors, and the column sums of the g and gamma cotangents, to
 // the layer's tau, omega, solar-flux and moment gradients.
 template <typename RowT>
-__device__ __noinline__ void start16_reverse(const FwdArgs& a, const Lane& L,
+__device__ __forceinline__ void start16_reverse(const FwdArgs& a, const Lane& L,
                                                 const Phase16<RowT>& p, double omega_in,
                                                 const double (&Rb0)[4], const double (&Tb0)[4],
                                                 double yb, double susb, double sdsb,
@@ -395,6 +412,7 @@ __device__ void layer16_adjoint_task(const FwdArgs& a, const AdjArgs& b, long ta
     gam = p.gam0;
   }
   double gk = 0.5 * tau0;
+  park<R>(slab, b.nn_cap, L, p);
   R* s0 = vs;
   R* s1 = vs + N;
   R* s2 = vs + 2 * N;
@@ -422,7 +440,7 @@ __device__ void layer16_adjoint_task(const FwdArgs& a, const AdjArgs& b, long ta
       s1[j] = sus;
     }
     wave_sync();
-    eye_minus_mul<R>(t0, Rm, L, A);  // I - R^2
+    mul<R>(t0, Rm, L, A);  // P = R^2
     double b1 = 0.0, b2 = 0.0;
     if (solar) {
       double bs[2];
@@ -430,11 +448,11 @@ __device__ void layer16_adjoint_task(const FwdArgs& a, const AdjArgs& b, long ta
       b1 = bs[0];
       b2 = bs[1];
     }
-#pragma unroll
-    for (int r = 0; r < 4; ++r) Ai.v[r] = (rows[r] == j) ? 1.0 : 0.0;
-    right_solve<R>(A, Ai, L);  // (I - R^2)^-1
-    gstore<R>(st + 2 * N * N, L, Ai);
     wave_sync();
+    put<R>(t0, L, A);
+    wave_sync();
+    inverse_ns<R>(A, t0, t3, Ai, L);  // (I - R^2)^-1
+    gstore<R>(st + 2 * N * N, L, Ai);
     put<R>(t1, L, Tm);
     wave_sync();
     mul<R>(t1, Ai, L, TG);  // TG = T (I - R^2)^-1
@@ -617,6 +635,7 @@ __device__ void layer16_adjoint_task(const FwdArgs& a, const AdjArgs& b, long ta
     Rb0[r] = Rbar.v[r];
     Tb0[r] = Tbar.v[r];
   }
+  unpark<R>(slab, b.nn_cap, L, p);
   start16_reverse<R>(a, L, p, omega_in, Rb0, Tb0, yb, susb, sdsb, g_sum, gam_sum, csl, chi_out,
                      g_tau, g_om, g_F, g_fr);
   finish(p.Lm);
@@ -627,11 +646,11 @@ template <bool REDO>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 8)))
 layer16_adjoint_kernel(FwdArgs a, AdjArgs b) {
   __shared__ double smem[4 * LdsA::size];
-  const int wv = threadIdx.x >> 6;
+  const int wv = wave_index();
   const int gwave = blockIdx.x * 4 + wv;
   const Lane L;
   double* sm = smem + wv * LdsA::size;
-  double* slab = b.slab + (long)gwave * b.nn_cap * SLAB;
+  double* slab = b.slab + (long)gwave * slab_stride(b.nn_cap);
   const long ntask = (long)a.W * a.L;
   for (long task = gwave; task < ntask; task += b.nwaves) {
     if constexpr (REDO) {

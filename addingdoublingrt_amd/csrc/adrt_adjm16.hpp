@@ -121,6 +121,7 @@ __device__ void layer16m_adjoint_task(const FwdArgs& a, const AdjArgs& b, long t
     gam = p.gam0;
   }
   double gk = 0.5 * tau0;
+  park<R>(slab, b.nn_cap, L, p);
 
   // ---- the forward S-form doubling, recomputed with checkpoints ----
   for (int k = 0; k < nn; ++k) {
@@ -418,19 +419,20 @@ __device__ void layer16m_adjoint_task(const FwdArgs& a, const AdjArgs& b, long t
     Rb0[r] = Rbar.v[r];
     Tb0[r] = -(double)Sbar.v[r];  // T0 = I - S0
   }
+  unpark<R>(slab, b.nn_cap, L, p);
   start16_reverse<R>(a, L, p, omega_in, Rb0, Tb0, yb, susb, sdsb, g_sum, gam_sum, csl, chi_out,
                      g_tau, g_om, g_F, g_fr);
   finish(p.Lm);
 }
 
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8)))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 8)))
 layer16m_adjoint_kernel(FwdArgs a, AdjArgs b) {
   __shared__ double smem[4 * LdsAM::size];
-  const int wv = threadIdx.x >> 6;
+  const int wv = wave_index();
   const int gwave = blockIdx.x * 4 + wv;
   const Lane L;
   double* sm = smem + wv * LdsAM::size;
-  double* slab = b.slab + (long)gwave * b.nn_cap * SLAB;
+  double* slab = b.slab + (long)gwave * slab_stride(b.nn_cap);
   const long ntask = (long)a.W * a.L;
   for (long task = gwave; task < ntask; task += b.nwaves) {
     if (a.redo[task]) continue;  // FP64 layer (layer16_adjoint_kernel<true>)

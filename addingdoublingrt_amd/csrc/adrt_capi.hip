@@ -380,7 +380,7 @@ int adrt_workspace_create_ex(const adrt_batch_config* cfg, size_t max_bytes, int
       cus = 256;
     ws->nwaves = cus * 16;
     ws->nn_cap = 64;
-    slab_bytes = (size_t)ws->nwaves * ws->nn_cap * adrt::dev::w16::SLAB * sizeof(double);
+    slab_bytes = (size_t)ws->nwaves * adrt::dev::w16::slab_stride(ws->nn_cap) * sizeof(double);
   }
   if (max_bytes == 0) max_bytes = (size_t)4 << 30;
   const size_t per_w = per_wavenumber_bytes(ws->L, ws->Np, ws->M, adjoint);

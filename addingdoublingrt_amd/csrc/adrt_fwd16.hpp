@@ -147,7 +147,7 @@ template <bool REDO>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) layer16_kernel(FwdArgs a) {
   using R = double;
   __shared__ R smem[4 * Lds16::size];
-  const int wv = threadIdx.x >> 6;
+  const int wv = wave_index();
   const long task = (long)blockIdx.x * 4 + wv;
   const int L_ = a.L;
   if (task >= (long)a.W * L_) return;
@@ -221,7 +221,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))
       v1[j] = sus;
     }
     wave_sync();
-    eye_minus_mul<R>(tile, Rm, L, A);  // I - R^2
+    mul<R>(tile, Rm, L, Y);  // P = R^2
     double b1 = 0.0, b2 = 0.0;
     if (a.has_solar) {
       double bs[2];
@@ -229,8 +229,13 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))
       b1 = bs[0];
       b2 = bs[1];
     }
-    X = Tm;
-    right_solve<R>(A, X, L);  // TG = T (I - R^2)^-1
+    wave_sync();
+    put<R>(tile, L, Y);
+    wave_sync();
+    inverse_ns<R>(Y, tile, tile2, A, L);  // (I - R^2)^-1
+    put<R>(tile2, L, Tm);
+    wave_sync();
+    mul<R>(tile2, A, L, X);  // TG = T (I - R^2)^-1
     wave_sync();
     put<R>(tile2, L, X);
     const double zpgy = z + gk * y;
@@ -634,7 +639,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))
 adding16_kernel(FwdArgs a) {
   using R = double;
   __shared__ R smem[4 * Lds16::size];
-  const int wv = threadIdx.x >> 6;
+  const int wv = wave_index();
   const int w = blockIdx.x * 4 + wv;
   if (w >= a.W) return;
   const Lane L;

@@ -56,7 +56,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))
 layer16m_kernel(FwdArgs a) {
   using R = float;
   __shared__ double smem[4 * LdsM::size];
-  const int wv = threadIdx.x >> 6;
+  const int wv = wave_index();
   const long task = (long)blockIdx.x * 4 + wv;
   const int L_ = a.L;
   if (task >= (long)a.W * L_) return;

@@ -58,6 +58,12 @@ using vec4 = typename std::conditional<sizeof(R) == 8, d4, f4>::type;
 // number of values in flight, hence the register count and the occupancy)
 __device__ __forceinline__ void fence_sched() { __builtin_amdgcn_sched_barrier(0); }
 
+// index of this wave within its workgroup, as a wave-uniform (scalar) value:
+// task indices derived from it keep per-task loads and branches scalar
+__device__ __forceinline__ int wave_index() {
+  return __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+}
+
 struct Lane {
   int l, j, h;
   __device__ Lane() : l(threadIdx.x & 63), j(threadIdx.x & 15), h((threadIdx.x & 63) >> 4) {}
@@ -278,6 +284,86 @@ __device__ __forceinline__ void right_solve(Mat<R>& A, Mat<R>& T, const Lane& L)
 #pragma unroll
   for (int r = 0; r < 4; ++r) T.v[r] = shfl(T.v[r], sl);
   fence_sched();
+}
+
+// N max |E_ij| over the wave (bounds the infinity norm of E), wave-uniform
+template <typename R>
+__device__ __forceinline__ float residual_size(const Mat<R>& E) {
+  float m = 0.0f;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) m = __builtin_fmaxf(m, __builtin_fabsf((float)E.v[r]));
+  m = row_max<float>(m);
+  m = __builtin_fmaxf(__builtin_fmaxf(readlane(m, 0), readlane(m, 16)),
+                      __builtin_fmaxf(readlane(m, 32), readlane(m, 48)));
+  return m * (float)N;
+}
+
+// Residual sizes at which the Newton-Schulz inverse stops: the iterate is
+// taken as is, after one Newton step (residual e^2) or after one third-order
+// step (residual e^3), each leaving a residual below FP64 rounding.
+constexpr float NS_DONE = 1e-16f, NS_NEWTON = 1e-8f, NS_CUBIC = 1e-5f;
+constexpr int NS_MAX_STEPS = 8;
+
+// X = (I - P)^-1 for P in registers and in the tile tP (tX: scratch tile).
+// Newton-Schulz iteration on matrix cores from the Neumann start X0 = I + P,
+// whose residual E0 = I - (I - P) X0 is P^2:
+//   X <- X + X E          (new residual E^2)
+//   X <- X + X E + X E E  (new residual E^3, once e = N max|E| <= NS_CUBIC)
+// with E = I - X + P X recomputed from each iterate, so rounding does not
+// accumulate.  The doubling's thin layers give e ~ (tau0 |G|)^4 and stop at
+// once; where the start is poor (e > 1/2: (I - P) close to singular) or the
+// iteration has not finished in NS_MAX_STEPS steps, Gauss-Jordan with partial
+// pivoting (right_solve) solves X (I - P) = I instead.  Every branch is
+// wave-uniform.  Returns true if Gauss-Jordan ran.
+template <typename R>
+__device__ __forceinline__ bool inverse_ns(const Mat<R>& P, const R* tP, R* tX, Mat<R>& X,
+                                           const Lane& L) {
+  Mat<R> E;
+  mul<R>(tP, P, L, E);  // P^2
+#pragma unroll
+  for (int r = 0; r < 4; ++r) X.v[r] = ((row<R>(L.h, r) == L.j) ? R(1) : R(0)) + P.v[r];
+  float e = residual_size<R>(E);
+  for (int it = 0; e > NS_DONE; ++it) {
+    if (!(e <= 0.5f) || it == NS_MAX_STEPS) {
+      Mat<R> A;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const R d = (row<R>(L.h, r) == L.j) ? R(1) : R(0);
+        A.v[r] = d - P.v[r];
+        X.v[r] = d;
+      }
+      right_solve<R>(A, X, L);
+      return true;
+    }
+    put<R>(tX, L, X);
+    wave_sync();
+    if (e <= NS_NEWTON) {
+      mul_add<R>(tX, E, X, L, X);
+      wave_sync();
+      break;
+    }
+    Mat<R> XE;
+    mul<R>(tX, E, L, XE);
+    wave_sync();
+    if (e <= NS_CUBIC) {
+      put<R>(tX, L, XE);
+      wave_sync();
+#pragma unroll
+      for (int r = 0; r < 4; ++r) X.v[r] += XE.v[r];
+      mul_add<R>(tX, E, X, L, X);
+      wave_sync();
+      break;
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) X.v[r] += XE.v[r];
+    Mat<R> PX;
+    mul<R>(tP, X, L, PX);
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+      E.v[r] = (((row<R>(L.h, r) == L.j) ? R(1) : R(0)) - X.v[r]) + PX.v[r];
+    e = residual_size<R>(E);
+  }
+  return false;
 }
 
 // x = M^-1 b: M by registers, b element j (all lanes of column j); returns x_j
