@@ -480,22 +480,33 @@ __device__ void add16_on_top(Comp16& C, const LR& y, int l, const Lane& L,
     C.scat = 1;
     return;
   }
-  // General (src/adding.hpp:69-110)
+  // General (src/adding.hpp:69-110), the two inverses by inverse_ns
   put<R>(s.tile, L, C.Rab);
-  if (L.h == 0) s.v0[j] = tsd;
+  put<R>(s.tile2, L, Rt);
+  if (L.h == 0) {
+    s.v0[j] = tsd;
+    s.v1[j] = C.su;
+  }
   wave_sync();
-  eye_minus_mul<R>(s.tile, Rt, L, A);                     // A1 = I - bot.R_ab top.R_ba
+  mul<R>(s.tile, Rt, L, A);                               // P1 = bot.R_ab top.R_ba
   const R rhs_up = C.su + matvec<R>(s.tile, s.v0, L);     // s2_up + R_bc s1_down
+  const R rhs_dn = tsd + matvec<R>(s.tile2, s.v1, L);     // s1_down + R_ba s2_up
+  Mat<R> B;
+  mul<R>(s.tile2, C.Rab, L, B);                           // P2 = top.R_ba bot.R_ab
   wave_sync();
-  X = Tt;
-  right_solve<R>(A, X, L);                                // D1 = top.T_ba A1^-1
-  put<R>(s.tile, L, Rt);
-  if (L.h == 0) s.v1[j] = C.su;
+  put<R>(s.tile, L, A);
   wave_sync();
-  eye_minus_mul<R>(s.tile, C.Rab, L, A);                  // A2 = I - top.R_ba bot.R_ab
-  const R rhs_dn = tsd + matvec<R>(s.tile, s.v1, L);      // s1_down + R_ba s2_up
+  inverse_ns<R>(A, s.tile, s.tile2, X, L);                // A1^-1 = (I - P1)^-1
+  put<R>(s.tile, L, B);
+  put<R>(s.tile2, L, Tt);
   wave_sync();
-  right_solve<R>(A, C.Tab, L);                            // D2 = bot.T_ab A2^-1
+  mul<R>(s.tile2, X, L, X);                               // D1 = top.T_ba A1^-1
+  wave_sync();
+  inverse_ns<R>(B, s.tile, s.tile2, A, L);                // A2^-1 = (I - P2)^-1
+  put<R>(s.tile, L, C.Tab);
+  wave_sync();
+  mul<R>(s.tile, A, L, C.Tab);                            // D2 = bot.T_ab A2^-1
+  wave_sync();
   if (L.h == 0) {
     s.v2[j] = rhs_up;
     s.v3[j] = rhs_dn;
@@ -506,7 +517,6 @@ __device__ void add16_on_top(Comp16& C, const LR& y, int l, const Lane& L,
   mul<R>(s.tile, Rt, L, A);                                // D2 top.R_ba
   const R nsd = C.sd + matvec<R>(s.tile, s.v3, L);
   mul<R>(s.tile, Tt, L, C.Tab);                           // T_ab = D2 top.T_ab
-  Mat<R> B;
   mul<R>(s.tile2, C.Rab, L, B);                           // D1 bot.R_ab
   const R nsu = tsu + matvec<R>(s.tile2, s.v2, L);
   mul<R>(s.tile2, C.Tba, L, X);                           // D1 bot.T_ba (new T_ba)
@@ -591,22 +601,33 @@ __device__ void add16_below(Comp16& C, const LR& y, int l, const Lane& L,
     C.sd = nsd;
     return;
   }
-  // General with top = C, bot = layer
+  // General with top = C, bot = layer; the inverses as in add16_on_top
   put<R>(s.tile, L, Rb);
-  if (L.h == 0) s.v0[j] = C.sd;
+  put<R>(s.tile2, L, C.Rba);
+  if (L.h == 0) {
+    s.v0[j] = C.sd;
+    s.v1[j] = bsu;
+  }
   wave_sync();
-  eye_minus_mul<R>(s.tile, C.Rba, L, A);                  // A1 = I - bot.R_ab top.R_ba
+  mul<R>(s.tile, C.Rba, L, A);                            // P1 = bot.R_ab top.R_ba
   const R rhs_up = bsu + matvec<R>(s.tile, s.v0, L);      // s2_up + R_bc s1_down
+  const R rhs_dn = C.sd + matvec<R>(s.tile2, s.v1, L);    // s1_down + R_ba s2_up
+  Mat<R> B;
+  mul<R>(s.tile2, Rb, L, B);                              // P2 = top.R_ba bot.R_ab
   wave_sync();
-  right_solve<R>(A, C.Tba, L);                            // D1 = top.T_ba A1^-1
-  put<R>(s.tile, L, C.Rba);
-  if (L.h == 0) s.v1[j] = bsu;
+  put<R>(s.tile, L, A);
   wave_sync();
-  eye_minus_mul<R>(s.tile, Rb, L, A);                     // A2 = I - top.R_ba bot.R_ab
-  const R rhs_dn = C.sd + matvec<R>(s.tile, s.v1, L);     // s1_down + R_ba s2_up
+  inverse_ns<R>(A, s.tile, s.tile2, X, L);                // A1^-1
+  put<R>(s.tile, L, B);
+  put<R>(s.tile2, L, C.Tba);
   wave_sync();
-  X = Tb;
-  right_solve<R>(A, X, L);                                // D2 = bot.T_ab A2^-1
+  mul<R>(s.tile2, X, L, C.Tba);                           // D1 = top.T_ba A1^-1
+  wave_sync();
+  inverse_ns<R>(B, s.tile, s.tile2, A, L);                // A2^-1
+  put<R>(s.tile, L, Tb);
+  wave_sync();
+  mul<R>(s.tile, A, L, X);                                // D2 = bot.T_ab A2^-1
+  wave_sync();
   if (L.h == 0) {
     s.v2[j] = rhs_up;
     s.v3[j] = rhs_dn;
@@ -617,7 +638,6 @@ __device__ void add16_below(Comp16& C, const LR& y, int l, const Lane& L,
   mul<R>(s.tile, Rb, L, A);                                // D1 bot.R_ab
   const R nsu = C.su + matvec<R>(s.tile, s.v2, L);
   mul<R>(s.tile, Tb, L, C.Tba);                           // T_ba = D1 bot.T_ba
-  Mat<R> B;
   mul<R>(s.tile2, C.Rba, L, B);                           // D2 top.R_ba
   const R nsd = bsd + matvec<R>(s.tile2, s.v3, L);
   mul<R>(s.tile2, C.Tab, L, X);                           // T_ab = D2 top.T_ab
@@ -814,16 +834,35 @@ adding16_kernel(FwdArgs a) {
         s.v3[j] = sbu;
       }
       wave_sync();
-      eye_minus_mul<R>(s.tile, C.Rba, L, M);
+      mul<R>(s.tile, C.Rba, L, M);  // P_up = base.R_ab top.R_ba, M_up = I - P_up
       double t23[2];
       matvecs<R, 2>(s.tile, s.v0, L, t23);
       double u23[2];
       matvecs<R, 2>(s.tile2, s.v2, L, u23);
       Mat<R> M2;
-      eye_minus_mul<R>(s.tile2, Rb, L, M2);  // M_dn = I - top.R_ba base.R_ab
+      mul<R>(s.tile2, Rb, L, M2);  // P_dn = top.R_ba base.R_ab, M_dn = I - P_dn
+      const double b_up = tbi + t23[0] + t23[1] + sbu;
+      const double b_dn = tti + u23[0] + u23[1] + C.sd;
       wave_sync();
-      Iup = left_solve<R>(M, tbi + t23[0] + t23[1] + sbu, L, s.tile);
-      Idn = left_solve<R>(M2, tti + u23[0] + u23[1] + C.sd, L, s.tile);
+      put<R>(s.tile, L, M);
+      if (L.h == 0) {
+        s.v0[j] = b_up;
+        s.v1[j] = b_dn;
+      }
+      wave_sync();
+      Mat<R> X;
+      inverse_ns<R>(M, s.tile, s.tile2, X, L);  // M_up^-1
+      put<R>(s.tile, L, X);
+      wave_sync();
+      Iup = matvec<R>(s.tile, s.v0, L);
+      wave_sync();
+      put<R>(s.tile, L, M2);
+      wave_sync();
+      inverse_ns<R>(M2, s.tile, s.tile2, X, L);  // M_dn^-1
+      put<R>(s.tile, L, X);
+      wave_sync();
+      Idn = matvec<R>(s.tile, s.v1, L);
+      wave_sync();
     } else {
       put<R>(s.tile, L, C.Tab);
       put<R>(s.tile2, L, C.Rba);

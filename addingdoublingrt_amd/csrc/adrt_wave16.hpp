@@ -325,6 +325,7 @@ __device__ __forceinline__ bool inverse_ns(const Mat<R>& P, const R* tP, R* tX, 
   float e = residual_size<R>(E);
   for (int it = 0; e > NS_DONE; ++it) {
     if (!(e <= 0.5f) || it == NS_MAX_STEPS) {
+      wave_sync();
       Mat<R> A;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
@@ -363,6 +364,7 @@ __device__ __forceinline__ bool inverse_ns(const Mat<R>& P, const R* tP, R* tX, 
       E.v[r] = (((row<R>(L.h, r) == L.j) ? R(1) : R(0)) - X.v[r]) + PX.v[r];
     e = residual_size<R>(E);
   }
+  wave_sync();
   return false;
 }
 

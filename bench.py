@@ -50,6 +50,9 @@ def parse():
     p.add_argument("--cpu-seconds", type=float, default=12.0)
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--workspace-gb", type=float, default=96.0)
+    p.add_argument("--copy-mib", type=int, default=0,
+                   help="only run a plain device copy of this many MiB, three times (the "
+                        "reference kernel that scales the FETCH_SIZE / WRITE_SIZE counters)")
     return p.parse_args()
 
 
@@ -91,6 +94,15 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     import torch
+
+    if a.copy_mib:
+        src = torch.ones(a.copy_mib << 17, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": src.numel() * 8, "copies": 3}))
+        return
 
     from addingdoublingrt_amd import _lib, adjoint_batch, flops, solve_batch, workloads
     from addingdoublingrt_amd.solver import get_workspace
