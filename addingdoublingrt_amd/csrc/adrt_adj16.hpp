@@ -42,6 +42,7 @@ struct AdjArgs {
   double* g_Ftop;  // d/d solar flux reaching the layer top
   double* g_f;     // d/d delta-M fraction through the scaled moments
   double* g_chi;   // [W][L][M] moment gradients (shared moments only)
+  double* scot;    // [W][L][SCOT] cotangents of the doubling start (FP64 path)
   // gradients in the caller's layout; null = not wanted
   double* d_tau;     // [W][L]
   double* d_ssa;     // [W][L]
@@ -52,7 +53,13 @@ struct AdjArgs {
   double* d_albedo;  // [W] per-wavenumber contributions
 };
 
+// waves per SIMD of the persistent doubling-reverse grid (its register budget:
+// 512 / 3 -> 168 registers per lane)
+constexpr int REV_WAVES_PER_SIMD = 3;
 constexpr int SLAB = 3 * N * N + 4 * N + 2;  // R, T, (I-R^2)^-1 (register order); y, z, s_up, s_dn; gamma
+// cotangents of the doubling start: R0, T0 (register order); y0, s_up0, s_dn0;
+// column sums of the g and gamma cotangents
+constexpr int SCOT = 2 * N * N + 3 * N + 2;
 
 struct LdsA {
   static constexpr int t0 = 0, t1 = TILE, t2 = 2 * TILE, t3 = 3 * TILE, vs = 4 * TILE;
@@ -348,7 +355,6 @@ __device__ void layer16_adjoint_task(const FwdArgs& a, const AdjArgs& b, long ta
   const double Bbot = a.B ? a.B[gw * (Lay + 1) + lev(a, l + 1)] : 0.0;
   const double B_bar = (Bbot + Btop) / 2.0;
   const double B_d = (tau > 0.0) ? (Bbot - Btop) / tau : 0.0;
-  const double mu_j = a.mu[j];
   const int li = lin(a, l);
   double* chi_out = a.shared_moments ? b.g_chi + task * a.M
                                      : (b.d_mom ? b.d_mom + (gw * Lay + li) * a.M : nullptr);
@@ -378,41 +384,36 @@ __device__ void layer16_adjoint_task(const FwdArgs& a, const AdjArgs& b, long ta
   }
   const double sub = c.sub, sdb = c.sdb;
 
-  Phase16<R> p;
-  phase16_recompute<R>(a, task, w, l, gw, L, tau, omega_in, mode, t0, csl, p);
-  const int nn = p.nn;
+  // the thin-layer start as the forward computed it; its reverse (phase
+  // matrices, Hansen normalisation, Legendre sums) is start16_adjoint_kernel's
+  Mat<R> Rm, Tm;
+  double y, z = 0.0, sus, sds, gam, gk, tau0;
+  int nn;
+  {
+    Start16<R> st0;
+    layer16_start<R>(a, task, w, l, gw, L, tau, omega_in, mode, t0, csl, st0);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      Tm.v[r] = st0.Tv[r];
+      Rm.v[r] = st0.Rv[r];
+    }
+    y = st0.y;
+    sus = st0.sus;
+    sds = st0.sds;
+    gam = st0.gam;
+    gk = st0.gk;
+    tau0 = st0.tau0;
+    nn = st0.nn;
+  }
   if (nn > b.nn_cap) {
     if (L.l == 0) atomicOr(b.status, 1);
     finish(0);
     return;
   }
-  const double tau0 = p.tau0;
-  const bool solar = p.solar;
-  const double omega = p.omega, con = p.con;
+  const bool solar = a.has_solar != 0;
   int rows[4];
 #pragma unroll
   for (int r = 0; r < 4; ++r) rows[r] = row<R>(L.h, r);
-  Mat<R> Rm, Tm;
-#pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    const int i = rows[r];
-    const double mui = a.mu[i];
-    const double gpp = (((i == j) ? 1.0 : 0.0) + (-con) * (p.Ppp[r] * a.wt[j])) / mui;
-    const double gpm = con * (p.Ppm[r] * a.wt[j]) / mui;
-    Tm.v[r] = ((i == j) ? 1.0 : 0.0) - tau0 * gpp;
-    Rm.v[r] = tau0 * gpm;
-  }
-  double y = (1.0 - omega) * tau0 / mu_j;
-  double z = 0.0;
-  double sus = 0.0, sds = 0.0, gam = 0.0;
-  if (solar) {
-    const double base = omega * tau0 / mu_j * p.Ftop;
-    sus = base * p.pminus;
-    sds = base * p.pplus;
-    gam = p.gam0;
-  }
-  double gk = 0.5 * tau0;
-  park<R>(slab, b.nn_cap, L, p);
   R* s0 = vs;
   R* s1 = vs + N;
   R* s2 = vs + 2 * N;
@@ -626,24 +627,81 @@ __device__ void layer16_adjoint_task(const FwdArgs& a, const AdjArgs& b, long ta
     wave_sync();
   }
 
-  // ---- thin-layer start, Hansen normalisation and Legendre sums, reversed ----
+  // ---- cotangents of the doubling start, for start16_adjoint_kernel ----
   const double g_sum = colsum<R>(gb, L, csl);
   const double gam_sum = solar ? colsum<R>(gamb, L, csl) : 0.0;
+  double* sc = b.scot + task * SCOT;
+  gstore<R>(sc, L, Rbar);
+  gstore<R>(sc + N * N, L, Tbar);
+  if (L.h == 0) {
+    sc[2 * N * N + j] = yb;
+    sc[2 * N * N + N + j] = susb;
+    sc[2 * N * N + 2 * N + j] = sdsb;
+  }
+  if (L.l == 0) {
+    sc[2 * N * N + 3 * N] = g_sum;
+    sc[2 * N * N + 3 * N + 1] = gam_sum;
+    b.g_taus[task] = g_tau;  // completed by start16_adjoint_kernel
+    b.g_Bt[task] = g_Bt;
+    b.g_Bb[task] = g_Bb;
+  }
+}
+
+// The thin-layer start, the Hansen normalisation and the Legendre sums of
+// each scattering layer, reversed, from the start cotangents that
+// layer16_adjoint_kernel left in b.scot; one wave per (wavenumber, layer).
+// Kept apart from the doubling reverse so that kernel keeps to 128 registers.
+template <bool REDO>
+__global__ void __launch_bounds__(256) start16_adjoint_kernel(FwdArgs a, AdjArgs b) {
+  __shared__ double smem[4 * (TILE + N)];
+  const int wv = wave_index();
+  const long task = (long)blockIdx.x * 4 + wv;
+  const int Lay = a.L;
+  if (task >= (long)a.W * Lay) return;
+  if constexpr (REDO) {
+    if (!a.redo[task]) return;
+  }
+  const double tau = a.tau_s[task];
+  const double omega_in = a.omega_s[task];
+  if (tau <= 0.0 || omega_in <= 0.0) return;  // finished by layer16_adjoint_kernel
+  const Lane L;
+  const int j = L.j;
+  const int w = (int)(task / Lay), l = (int)(task % Lay);
+  const long gw = (long)(a.w0 + w);
+  double* dtile = smem + wv * (TILE + N);
+  double* csl = dtile + TILE;
+  Phase16<double> p;
+  phase16_recompute<double>(a, task, w, l, gw, L, tau, omega_in, a.mode[task], dtile, csl, p);
+  if (p.nn > b.nn_cap) return;  // reported through b.status
+  const double* sc = b.scot + task * SCOT;
   double Rb0[4], Tb0[4];
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
-    Rb0[r] = Rbar.v[r];
-    Tb0[r] = Tbar.v[r];
+    Rb0[r] = sc[L.l * 4 + r];
+    Tb0[r] = sc[N * N + L.l * 4 + r];
   }
-  unpark<R>(slab, b.nn_cap, L, p);
-  start16_reverse<R>(a, L, p, omega_in, Rb0, Tb0, yb, susb, sdsb, g_sum, gam_sum, csl, chi_out,
-                     g_tau, g_om, g_F, g_fr);
-  finish(p.Lm);
+  const double yb = sc[2 * N * N + j], susb = sc[2 * N * N + N + j];
+  const double sdsb = sc[2 * N * N + 2 * N + j];
+  const double g_sum = sc[2 * N * N + 3 * N], gam_sum = sc[2 * N * N + 3 * N + 1];
+  const int li = lin(a, l);
+  double* chi_out = a.shared_moments ? b.g_chi + task * a.M
+                                     : (b.d_mom ? b.d_mom + (gw * Lay + li) * a.M : nullptr);
+  double g_tau = b.g_taus[task], g_om = 0.0, g_F = 0.0, g_fr = 0.0;
+  start16_reverse<double>(a, L, p, omega_in, Rb0, Tb0, yb, susb, sdsb, g_sum, gam_sum, csl,
+                          chi_out, g_tau, g_om, g_F, g_fr);
+  if (chi_out)
+    for (int m = p.Lm + L.l; m < a.M; m += 64) chi_out[m] = 0.0;
+  if (L.l == 0) {
+    b.g_taus[task] = g_tau;
+    b.g_omgs[task] = g_om;
+    b.g_Ftop[task] = g_F;
+    b.g_f[task] = g_fr;
+  }
 }
 
 // REDO: only the layers the mixed-precision forward left to the FP64 kernel
 template <bool REDO>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 8)))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(REV_WAVES_PER_SIMD, 8)))
 layer16_adjoint_kernel(FwdArgs a, AdjArgs b) {
   __shared__ double smem[4 * LdsA::size];
   const int wv = wave_index();

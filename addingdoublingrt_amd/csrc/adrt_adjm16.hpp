@@ -425,7 +425,9 @@ __device__ void layer16m_adjoint_task(const FwdArgs& a, const AdjArgs& b, long t
   finish(p.Lm);
 }
 
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 8)))
+constexpr int MREV_WAVES_PER_SIMD = 2;  // register budget 256 per lane
+
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MREV_WAVES_PER_SIMD, 8)))
 layer16m_adjoint_kernel(FwdArgs a, AdjArgs b) {
   __shared__ double smem[4 * LdsAM::size];
   const int wv = wave_index();

@@ -69,8 +69,8 @@ void legendre_column(int L, double x, double* out) {
 }
 
 // ---- band Planck function (src/planck.cpp:35-131) --------------------------
-__host__ __device__ constexpr double kC2 = 1.438786, kSigma = 5.67032E-8, kVcut = 1.5;
-__host__ __device__ constexpr double kA1 = 1.0 / 3.0, kA2 = -1.0 / 8.0, kA3 = 1.0 / 60.0, kA4 = -1.0 / 5040.0,
+constexpr double kC2 = 1.438786, kSigma = 5.67032E-8, kVcut = 1.5;
+constexpr double kA1 = 1.0 / 3.0, kA2 = -1.0 / 8.0, kA3 = 1.0 / 60.0, kA4 = -1.0 / 5040.0,
                  kA5 = 1.0 / 272160.0, kA6 = -1.0 / 13305600.0;
 
 __host__ __device__ inline double plkernel(double v) { return v * v * v / (exp(v) - 1.0); }
@@ -215,7 +215,7 @@ struct adrt_workspace {
   // reverse pass (ADRT_WORKSPACE_ADJOINT)
   int flags = 0;
   double *Iv = nullptr, *Av = nullptr, *g_taus = nullptr, *g_omgs = nullptr, *g_Bt = nullptr,
-         *g_Bb = nullptr, *g_Ftop = nullptr, *g_f = nullptr, *g_chi = nullptr, *slab = nullptr,
+         *g_Bb = nullptr, *g_Ftop = nullptr, *g_f = nullptr, *g_chi = nullptr, *scot = nullptr, *slab = nullptr,
          *mom_part = nullptr;
   int* status = nullptr;
   int nwaves = 0, nn_cap = 0;
@@ -247,6 +247,8 @@ size_t per_wavenumber_bytes(int L, int Np, int M, bool adjoint) {
     b += 2 * (size_t)(L + 1) * 2 * Np * d;            // forward and adjoint interface vectors
     b += 6 * (size_t)L * d;                           // per-layer gradient terms
     b += (size_t)L * M * d;                           // per-layer moment gradients
+    if (Np == adrt::dev::w16::N)
+      b += (size_t)L * adrt::dev::w16::SCOT * d;      // doubling-start cotangents
   }
   return b;
 }
@@ -292,12 +294,12 @@ int prof_begin(adrt_workspace* ws, int kind, hipStream_t stream) {
   }
   const size_t k = ws->ev_used++;
   ws->ev_kind[k] = kind;
-  hipEventRecord(ws->ev_pool[2 * k], stream);
+  (void)hipEventRecord(ws->ev_pool[2 * k], stream);  // profiling only
   return (int)k;
 }
 
 void prof_end(adrt_workspace* ws, int k, hipStream_t stream) {
-  if (k >= 0) hipEventRecord(ws->ev_pool[2 * k + 1], stream);
+  if (k >= 0) (void)hipEventRecord(ws->ev_pool[2 * k + 1], stream);
 }
 
 template <int N>
@@ -373,12 +375,12 @@ int adrt_workspace_create_ex(const adrt_batch_config* cfg, size_t max_bytes, int
   }
   size_t slab_bytes = 0;
   if (adjoint) {
-    // persistent reverse-layer grid: 4 waves per SIMD, 4 SIMDs per CU
+    // persistent reverse-layer grid: REV_WAVES_PER_SIMD waves on each of 4 SIMDs per CU
     int cus = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ws->device) !=
             hipSuccess || cus <= 0)
       cus = 256;
-    ws->nwaves = cus * 16;
+    ws->nwaves = cus * 4 * adrt::dev::w16::REV_WAVES_PER_SIMD;
     ws->nn_cap = 64;
     slab_bytes = (size_t)ws->nwaves * adrt::dev::w16::slab_stride(ws->nn_cap) * sizeof(double);
   }
@@ -405,13 +407,15 @@ int adrt_workspace_create_ex(const adrt_batch_config* cfg, size_t max_bytes, int
   const size_t o_redo = take(C * L * 4);
   const size_t o_lay = take(C * L * (2 * Np * Np + 2 * Np) * 8);
   const size_t o_rb = take(C * L * (2 * Np * Np + Np) * 8);
-  size_t o_iv = 0, o_av = 0, o_g = 0, o_chi = 0, o_slab = 0, o_status = 0, o_part = 0;
+  size_t o_iv = 0, o_av = 0, o_g = 0, o_chi = 0, o_slab = 0, o_status = 0, o_part = 0,
+         o_scot = 0;
   if (adjoint) {
     o_iv = take(C * (L + 1) * 2 * Np * 8);
     o_av = take(C * (L + 1) * 2 * Np * 8);
     o_g = take(6 * C * L * 8);
     o_part = take(L * adrt::dev::w16::MOM_SLICES * M * 8);
     o_chi = take(C * L * M * 8);
+    o_scot = take(Np == adrt::dev::w16::N ? C * L * adrt::dev::w16::SCOT * 8 : 0);
     o_slab = take(slab_bytes);
     o_status = take(4);
   }
@@ -445,11 +449,12 @@ int adrt_workspace_create_ex(const adrt_batch_config* cfg, size_t max_bytes, int
     ws->g_Ftop = g + 4 * C * L;
     ws->g_f = g + 5 * C * L;
     ws->g_chi = (double*)(ws->base + o_chi);
+    ws->scot = (double*)(ws->base + o_scot);
     ws->mom_part = (double*)(ws->base + o_part);
     ws->slab = (double*)(ws->base + o_slab);
     ws->status = (int*)(ws->base + o_status);
     if (hipMemset(ws->status, 0, sizeof(int)) != hipSuccess) {
-      hipFree(ws->base);
+      (void)hipFree(ws->base);  // already failing
       delete ws;
       return ADRT_ERR_HIP;
     }
@@ -461,7 +466,7 @@ int adrt_workspace_create_ex(const adrt_batch_config* cfg, size_t max_bytes, int
 int adrt_workspace_destroy(adrt_workspace* ws) {
   if (!ws) return ADRT_OK;
   int rc = ADRT_OK;
-  for (hipEvent_t e : ws->ev_pool) hipEventDestroy(e);
+  for (hipEvent_t e : ws->ev_pool) (void)hipEventDestroy(e);
   if (ws->base && hipFree(ws->base) != hipSuccess) rc = ADRT_ERR_HIP;
   delete ws;
   return rc;
@@ -630,6 +635,7 @@ int adrt_adjoint_batch(const adrt_batch_config* cfg, const adrt_device_inputs* i
   b.g_Ftop = ws->g_Ftop;
   b.g_f = ws->g_f;
   b.g_chi = ws->g_chi;
+  b.scot = ws->scot;
   b.d_tau = grad->delta_tau;
   b.d_ssa = grad->single_scat_albedo;
   b.d_mom = grad->phase_moments;
@@ -664,11 +670,19 @@ int adrt_adjoint_batch(const adrt_batch_config* cfg, const adrt_device_inputs* i
     const long blocks = std::min((tasks + 3) / 4, (long)ws->nwaves / 4);
     b.nwaves = (int)(blocks * 4);
     e = prof_begin(ws, 4, stream);
+    const unsigned task_blocks = (unsigned)((tasks + 3) / 4);
     if (mixed) {
-      w16::layer16m_adjoint_kernel<<<(unsigned)blocks, 256, 0, stream>>>(a, b);
+      // its own persistent grid: MREV_WAVES_PER_SIMD waves per SIMD
+      w16::AdjArgs bm = b;
+      const long mblocks = std::min((tasks + 3) / 4, (long)ws->nwaves / w16::REV_WAVES_PER_SIMD *
+                                                         w16::MREV_WAVES_PER_SIMD / 4);
+      bm.nwaves = (int)(mblocks * 4);
+      w16::layer16m_adjoint_kernel<<<(unsigned)mblocks, 256, 0, stream>>>(a, bm);
       w16::layer16_adjoint_kernel<true><<<(unsigned)blocks, 256, 0, stream>>>(a, b);
+      w16::start16_adjoint_kernel<true><<<task_blocks, 256, 0, stream>>>(a, b);
     } else {
       w16::layer16_adjoint_kernel<false><<<(unsigned)blocks, 256, 0, stream>>>(a, b);
+      w16::start16_adjoint_kernel<false><<<task_blocks, 256, 0, stream>>>(a, b);
     }
     prof_end(ws, e, stream);
     e = prof_begin(ws, 5, stream);
