@@ -123,6 +123,7 @@ __device__ __forceinline__ void phase16_recompute(const FwdArgs& a, long task, i
   double spj = 0.0, smj = 0.0, sign = 1.0;
 #pragma unroll
   for (int r = 0; r < 4; ++r) p.Qpp[r] = p.Qpm[r] = 0.0;
+#pragma unroll 8
   for (int ll = 0; ll < p.Lm; ++ll) {
     const double chil = (mode == 3) ? (p.chi[ll] - p.f) * p.inv1mf : p.chi[ll];
     const double cl = (2 * ll + 1) * chil;
@@ -259,20 +260,36 @@ __device__ __forceinline__ void start16_reverse(const FwdArgs& a, const Lane& L,
     qplusb = pplusb * p.corr_s + sb * wt_j;
     qminusb = pminusb * p.corr_s + sb * wt_j;
   }
-  double sign = 1.0;
-  for (int ll = 0; ll < p.Lm; ++ll) {
-    const double* P = a.Pl + ll * N;
-    const double plj = P[j];
-    double q = 0.0;
+  // Legendre sums, 32 moments at a time: this lane's part of each, then one
+  // transposed reduction (lane l ends with moment c0 + bitrev5(l mod 32))
+  double gfr_part = 0.0;
+  for (int c0 = 0; c0 < p.Lm; c0 += 32) {
+    double v[32];
 #pragma unroll
-    for (int r = 0; r < 4; ++r) q += P[row<RowT>(L.h, r)] * (Qb_pp[r] + sign * Qb_pm[r]);
-    double part = q * plj;
-    if (p.solar && L.h == 0) part += a.Pl0[ll] * plj * (qplusb + sign * qminusb);
-    const double chisb = wave_sum<double>(part, L) * ((2 * ll + 1) / (2.0 * kPi));
-    if (p.mode == 3) g_fr += chisb * (p.chi[ll] - 1.0) * p.inv1mf * p.inv1mf;
-    if (chi_out && L.l == 0) chi_out[ll] = (p.mode == 3) ? chisb * p.inv1mf : chisb;
-    sign = -sign;
+    for (int q = 0; q < 32; ++q) {
+      const int ll = c0 + q;
+      double part = 0.0;
+      if (ll < p.Lm) {
+        const double sgn = (ll & 1) ? -1.0 : 1.0;
+        const double* P = a.Pl + ll * N;
+        const double plj = P[j];
+        double t = 0.0;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) t += P[row<RowT>(L.h, r)] * (Qb_pp[r] + sgn * Qb_pm[r]);
+        part = t * plj;
+        if (p.solar && L.h == 0) part += a.Pl0[ll] * plj * (qplusb + sgn * qminusb);
+      }
+      v[q] = part;
+    }
+    const double tot = transpose_reduce32(v, L);
+    const int ll = c0 + bitrev5(L.l & 31);
+    if (ll < p.Lm && L.l < 32) {
+      const double chisb = tot * ((2 * ll + 1) / (2.0 * kPi));
+      if (p.mode == 3) gfr_part += chisb * (p.chi[ll] - 1.0) * p.inv1mf * p.inv1mf;
+      if (chi_out) chi_out[ll] = (p.mode == 3) ? chisb * p.inv1mf : chisb;
+    }
   }
+  if (p.mode == 3) g_fr += wave_sum<double>(gfr_part, L);
 }
 
 // cotangents of a layer record from the interface vectors (see the file head)

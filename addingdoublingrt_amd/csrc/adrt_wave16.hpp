@@ -422,6 +422,33 @@ __device__ __forceinline__ R wave_sum(R p, const Lane& L) {
   return p;
 }
 
+// Sums over the 64 lanes of 32 values per lane at once: five butterfly stages
+// each halve the list a lane keeps (lanes with bit s set keep the upper half
+// and add their partner's copy of it), a sixth adds the two wave halves.  Lane
+// l returns the total of entry bitrev5(l mod 32).
+template <int S>
+__device__ __forceinline__ void butterfly_stage(double (&v)[32], const Lane& L) {
+  constexpr int half = 16 >> S;
+  const bool up = (L.l >> S) & 1;
+#pragma unroll
+  for (int q = 0; q < half; ++q) {
+    const double keep = up ? v[q + half] : v[q];
+    const double send = up ? v[q] : v[q + half];
+    v[q] = keep + shfl(send, L.l ^ (1 << S));
+  }
+}
+__device__ __forceinline__ double transpose_reduce32(double (&v)[32], const Lane& L) {
+  butterfly_stage<0>(v, L);
+  butterfly_stage<1>(v, L);
+  butterfly_stage<2>(v, L);
+  butterfly_stage<3>(v, L);
+  butterfly_stage<4>(v, L);
+  return v[0] + shfl(v[0], L.l ^ 32);
+}
+__device__ __forceinline__ int bitrev5(int m) {
+  return ((m & 1) << 4) | ((m & 2) << 2) | (m & 4) | ((m & 8) >> 2) | ((m & 16) >> 4);
+}
+
 // (M^T v)_j with M in registers and v in an LDS slot of 16 values
 template <typename R>
 __device__ __forceinline__ R tmatvec(const Mat<R>& M, const R* v, const Lane& L) {
