@@ -16,7 +16,7 @@ ADRT_OK = 0
 PRECISION_F64 = 0
 PRECISION_MIXED = 1
 WORKSPACE_ADJOINT = 1
-PROFILE_KINDS = 6
+PROFILE_KINDS = 7
 
 _dp = C.POINTER(C.c_double)
 

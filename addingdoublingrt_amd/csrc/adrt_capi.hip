@@ -679,9 +679,13 @@ int adrt_adjoint_batch(const adrt_batch_config* cfg, const adrt_device_inputs* i
       bm.nwaves = (int)(mblocks * 4);
       w16::layer16m_adjoint_kernel<<<(unsigned)mblocks, 256, 0, stream>>>(a, bm);
       w16::layer16_adjoint_kernel<true><<<(unsigned)blocks, 256, 0, stream>>>(a, b);
+      prof_end(ws, e, stream);
+      e = prof_begin(ws, 6, stream);
       w16::start16_adjoint_kernel<true><<<task_blocks, 256, 0, stream>>>(a, b);
     } else {
       w16::layer16_adjoint_kernel<false><<<(unsigned)blocks, 256, 0, stream>>>(a, b);
+      prof_end(ws, e, stream);
+      e = prof_begin(ws, 6, stream);
       w16::start16_adjoint_kernel<false><<<task_blocks, 256, 0, stream>>>(a, b);
     }
     prof_end(ws, e, stream);

@@ -36,7 +36,22 @@ sys.path.insert(0, ROOT)
 METRIC = "column-wavenumbers/s fwd+adjoint, mixed vs plain FP64 at ≤1e-6 rel flux err"
 FP64_PEAK_TFLOPS = 78.6
 KINDS = ("prep", "layer_doubling", "adding_interfaces", "reverse_adding_interfaces",
-         "reverse_layer_doubling", "reverse_column_terms")
+         "reverse_layer_doubling", "reverse_column_terms", "reverse_start_terms")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "fwd_adj_f64_n16_65536x100_hbm_traffic.md")
+
+
+def profiled_traffic(kernel, W, config, form):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary, for the workload it was measured on; None otherwise."""
+    if (W, config, form) != (65536, 5, "f64") or not os.path.exists(TRAFFIC_FILE):
+        return None
+    with open(TRAFFIC_FILE) as f:
+        for line in f:
+            if line.startswith("bench-traffic:"):
+                name, val = line[len("bench-traffic:"):].strip().rsplit(None, 1)
+                if name == kernel:
+                    return float(val)
+    return None
 
 
 def parse():
@@ -157,11 +172,17 @@ def main():
                                cols.use_diffusion_lower_bc)
     per_kind = {k: ms[i] / a.steps for i, k in enumerate(KINDS)}
     # dominant kernel and its algorithmic work
-    model = {1: ("layer16_kernel: phase setup + doubling (forward)", work["doubling"]),
-             4: ("layer16_adjoint_kernel: recomputed doubling + its reverse",
-                 work["doubling_reverse"])}
+    model = {1: ("layer16_kernel<false>", "phase setup + doubling (forward)", work["doubling"]),
+             4: ("layer16_adjoint_kernel<false>",
+                 "recomputed doubling + its reverse (reverse counted)", work["doubling_reverse"])}
+    if a.form == "mixed":
+        model = {1: ("layer16m_kernel", "mixed doubling (forward), FP64 flop count",
+                     work["doubling"]),
+                 4: ("layer16m_adjoint_kernel",
+                     "mixed recomputed doubling + reverse, FP64 flop count",
+                     work["doubling_reverse"])}
     dom = max(model, key=lambda k: ms[k])
-    name, fl_step = model[dom]
+    kname, name, fl_step = model[dom]
     launches = nl[dom] / a.steps
     avg_ms = ms[dom] / max(nl[dom], 1)
     achieved = (fl_step / launches) / (avg_ms * 1e-3) / 1e12
@@ -192,7 +213,9 @@ def main():
         },
         "roofline": {
             "bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-            "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None, "kernel": name,
+            "frac": achieved / FP64_PEAK_TFLOPS,
+            "traffic": profiled_traffic(kname, W, a.config, a.form),
+            "kernel": f"{kname}: {name}",
             "launches_per_step": launches, "avg_launch_ms": avg_ms,
         },
     }

@@ -198,9 +198,10 @@ int adrt_planck_levels(const double* wn_lo, const double* wn_hi, const double* t
  * the summed milliseconds and launch counts per kernel kind since the last
  * read (ADRT_PROFILE_KINDS entries): [0] prep, [1] layer setup + doubling,
  * [2] adding + interfaces, [3] reverse adding + interfaces, [4] reverse layer
- * (recomputed doubling + its reverse), [5] reverse column terms + reductions.
+ * doubling (recomputed doubling + its reverse), [5] reverse column terms +
+ * reductions, [6] reverse of the doubling start (phase matrices, Legendre sums).
  * Enabling allocates HIP events (host-side objects) on first use. */
-#define ADRT_PROFILE_KINDS 6
+#define ADRT_PROFILE_KINDS 7
 int adrt_profile(adrt_workspace* ws, int enable);
 int adrt_profile_read(adrt_workspace* ws, double* ms, int* launches);
 
