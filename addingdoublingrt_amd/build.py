@@ -34,5 +34,27 @@ def build(verbose: bool = False) -> str:
     return OUT
 
 
+ROOT = os.path.dirname(HERE)
+CPP_CHECK_SRC = os.path.join(ROOT, "tests", "cpp", "adrt_cpp_check.cpp")
+CPP_CHECK = os.path.join(ROOT, "tests", "cpp", "adrt_cpp_check")
+
+
+def build_cpp_check(verbose: bool = False) -> str:
+    """The test driver of the C++ host layer (include/adrt_hip.hpp), host code
+    linked against libadrt_hip.so and the HIP runtime."""
+    rocm = os.environ.get("ROCM_PATH", "/opt/rocm")
+    cmd = [os.environ.get("CXX", "g++"), "-std=c++17", "-O2", "-Wall", "-D__HIP_PLATFORM_AMD__",
+           "-I", os.path.join(ROOT, "include"), "-I", os.path.join(rocm, "include"),
+           CPP_CHECK_SRC, "-o", CPP_CHECK + ".tmp", "-L", HERE, "-ladrt_hip",
+           "-L", os.path.join(rocm, "lib"), "-lamdhip64",
+           "-Wl,-rpath,$ORIGIN/../../addingdoublingrt_amd", "-Wl,-rpath," + os.path.join(rocm, "lib")]
+    if verbose:
+        print(" ".join(cmd), file=sys.stderr)
+    subprocess.run(cmd, check=True)
+    os.replace(CPP_CHECK + ".tmp", CPP_CHECK)
+    return CPP_CHECK
+
+
 if __name__ == "__main__":
     print(build(verbose=True))
+    print(build_cpp_check(verbose=True))
