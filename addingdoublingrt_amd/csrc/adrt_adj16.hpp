@@ -120,52 +120,48 @@ __device__ __forceinline__ void phase16_recompute(const FwdArgs& a, long task, i
     p.Lm = a.M < twoM ? a.M : twoM;
   }
   p.solar = a.has_solar != 0;
-  double spj = 0.0, smj = 0.0, sign = 1.0;
-#pragma unroll
-  for (int r = 0; r < 4; ++r) p.Qpp[r] = p.Qpm[r] = 0.0;
-#pragma unroll 8
-  for (int ll = 0; ll < p.Lm; ++ll) {
-    const double chil = (mode == 3) ? (p.chi[ll] - p.f) * p.inv1mf : p.chi[ll];
-    const double cl = (2 * ll + 1) * chil;
-    const double* P = a.Pl + ll * N;
-    const double plj = P[j];
+  if (a.phase) {
+    // shared moments: the layer's phase record (phase16_kernel)
+    const double* ph = a.phase + (long)l * PH16;
+    p.csum = ph[PH_CSUM + j];
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const double term = cl * P[row<RowT>(L.h, r)] * plj;
-      p.Qpp[r] += term;
-      p.Qpm[r] += sign * term;
+      p.Qpp[r] = ph[row<RowT>(L.h, r) * N + j];
+      p.Qpm[r] = ph[PH_QPM + row<RowT>(L.h, r) * N + j];
     }
+    p.qplus = ph[PH_QP + j];
+    p.qminus = ph[PH_QM + j];
+    p.ssol = ph[PH_SSOL];
+  } else {
+    double spj, smj;
+    phase16_sums<RowT>(a, p.chi, mode, p.f, L, p.Qpp, p.Qpm, spj, smj);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      p.Qpp[r] = p.Qpp[r] / (2.0 * kPi);
+      p.Qpm[r] = p.Qpm[r] / (2.0 * kPi);
+      dtile[j * TS + row<RowT>(L.h, r)] = (p.Qpp[r] + p.Qpm[r]) * a.wt[row<RowT>(L.h, r)];
+    }
+    wave_sync();
+    p.csum = 0.0;
+#pragma unroll
+    for (int i = 0; i < N; ++i) p.csum += dtile[j * TS + i];
+    wave_sync();
+    p.qplus = p.qminus = p.ssol = 0.0;
     if (p.solar) {
-      const double term = cl * plj * a.Pl0[ll];
-      spj += term;
-      smj += sign * term;
+      p.qplus = spj / (2.0 * kPi);
+      p.qminus = smj / (2.0 * kPi);
+      p.ssol = colsum<double>((p.qplus + p.qminus) * wt_j, L, csl);
     }
-    sign = -sign;
   }
-#pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    p.Qpp[r] = p.Qpp[r] / (2.0 * kPi);
-    p.Qpm[r] = p.Qpm[r] / (2.0 * kPi);
-    dtile[j * TS + row<RowT>(L.h, r)] = (p.Qpp[r] + p.Qpm[r]) * a.wt[row<RowT>(L.h, r)];
-  }
-  wave_sync();
-  p.csum = 0.0;
-#pragma unroll
-  for (int i = 0; i < N; ++i) p.csum += dtile[j * TS + i];
-  wave_sync();
   p.corr = p.csum > 0.0 ? 1.0 / (2.0 * kPi * p.csum) : 1.0;
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
     p.Ppp[r] = p.csum > 0.0 ? p.Qpp[r] * p.corr : p.Qpp[r];
     p.Ppm[r] = p.csum > 0.0 ? p.Qpm[r] * p.corr : p.Qpm[r];
   }
-  p.qplus = p.qminus = p.ssol = 0.0;
   p.corr_s = 1.0;
   p.pplus = p.pminus = 0.0;
   if (p.solar) {
-    p.qplus = spj / (2.0 * kPi);
-    p.qminus = smj / (2.0 * kPi);
-    p.ssol = colsum<double>((p.qplus + p.qminus) * wt_j, L, csl);
     p.corr_s = p.ssol > 0.0 ? 1.0 / (2.0 * kPi * p.ssol) : 1.0;
     p.pplus = p.ssol > 0.0 ? p.qplus * p.corr_s : p.qplus;
     p.pminus = p.ssol > 0.0 ? p.qminus * p.corr_s : p.qminus;

@@ -216,7 +216,7 @@ struct adrt_workspace {
   int flags = 0;
   double *Iv = nullptr, *Av = nullptr, *g_taus = nullptr, *g_omgs = nullptr, *g_Bt = nullptr,
          *g_Bb = nullptr, *g_Ftop = nullptr, *g_f = nullptr, *g_chi = nullptr, *scot = nullptr, *slab = nullptr,
-         *mom_part = nullptr;
+         *mom_part = nullptr, *phase = nullptr;
   int* status = nullptr;
   int nwaves = 0, nn_cap = 0;
   unsigned long long generation = 0;  // forward solves so far
@@ -313,6 +313,7 @@ int launch_chunk(const adrt::dev::FwdArgs& a, adrt_workspace* ws, hipStream_t st
   const long tasks = (long)a.W * a.L;
   if constexpr (N == 16) {  // one wave per problem on the matrix cores
     e = prof_begin(ws, 1, stream);
+    if (a.phase) w16::phase16_kernel<<<(unsigned)a.L, 64, 0, stream>>>(a, ws->phase);
     if (mixed) {
       w16::layer16m_kernel<<<(unsigned)((tasks + 3) / 4), 256, 0, stream>>>(a);
       // layers whose doubling start is outside FP32's reach, again in FP64
@@ -386,8 +387,11 @@ int adrt_workspace_create_ex(const adrt_batch_config* cfg, size_t max_bytes, int
   }
   if (max_bytes == 0) max_bytes = (size_t)4 << 30;
   const size_t per_w = per_wavenumber_bytes(ws->L, ws->Np, ws->M, adjoint);
+  const size_t phase_bytes =
+      ws->Np == adrt::dev::w16::N ? (size_t)ws->L * adrt::dev::w16::PH16 * 8 : 0;
   const size_t fixed =
       align256((size_t)(2 * ws->Np + ws->Lmax * ws->Np + ws->Lmax) * 8) + 512 + slab_bytes +
+      align256(phase_bytes) +
       (adjoint ? align256((size_t)ws->L * adrt::dev::w16::MOM_SLICES * ws->M * 8) : 0);
   long chunk = max_bytes > fixed ? (long)((max_bytes - fixed) / (per_w + 4096)) : 1;
   const int W = std::max(cfg->num_wavenumbers, 1);
@@ -407,6 +411,7 @@ int adrt_workspace_create_ex(const adrt_batch_config* cfg, size_t max_bytes, int
   const size_t o_redo = take(C * L * 4);
   const size_t o_lay = take(C * L * (2 * Np * Np + 2 * Np) * 8);
   const size_t o_rb = take(C * L * (2 * Np * Np + Np) * 8);
+  const size_t o_phase = take(phase_bytes);
   size_t o_iv = 0, o_av = 0, o_g = 0, o_chi = 0, o_slab = 0, o_status = 0, o_part = 0,
          o_scot = 0;
   if (adjoint) {
@@ -438,6 +443,7 @@ int adrt_workspace_create_ex(const adrt_batch_config* cfg, size_t max_bytes, int
   ws->flag = (int*)(ws->base + o_flag);
   ws->lay = (double*)(ws->base + o_lay);
   ws->rb = (double*)(ws->base + o_rb);
+  ws->phase = phase_bytes ? (double*)(ws->base + o_phase) : nullptr;
   if (adjoint) {
     ws->Iv = (double*)(ws->base + o_iv);
     ws->Av = (double*)(ws->base + o_av);
@@ -527,6 +533,8 @@ adrt::dev::FwdArgs make_args(const adrt_batch_config* cfg, const adrt_device_inp
   a.rb = ws->rb;
   a.Iv = ws->Iv;  // kept for the reverse pass (adjoint workspaces only)
   a.Av = ws->Av;
+  // one phase record per layer when every wavenumber shares the moments
+  a.phase = (a.shared_moments && ws->phase) ? ws->phase : nullptr;
   return a;
 }
 

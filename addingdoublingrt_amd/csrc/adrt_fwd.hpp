@@ -51,6 +51,7 @@ struct FwdArgs {
   const double* wt;   // [N]
   const double* Pl;   // [Lmax][N]
   const double* Pl0;  // [Lmax] Legendre polynomials at mu0
+  const double* phase;  // [L][PH16] phase sums per layer when the moments are shared (N = 16), else null
   // workspace (chunk-local)
   double* tau_s;    // [W][L]   scaled tau used by doubling
   double* omega_s;  // [W][L]   scaled omega

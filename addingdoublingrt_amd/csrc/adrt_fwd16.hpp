@@ -34,6 +34,99 @@ struct Start16 {
   int nn;
 };
 
+// Phase sums of one layer, shared by every wavenumber when the moments are
+// (src/phase_matrix.hpp:61-181 before the Hansen normalisation): Qpp, Qpm
+// (row-major, divided by 2 pi), the column sums csum[j] = sum_i (Qpp + Qpm)(i, j)
+// w_i, the solar vectors q+, q- (divided by 2 pi) and ssol = sum_j (q+ + q-)_j w_j.
+constexpr int PH16 = 2 * N * N + 4 * N;
+constexpr int PH_QPM = N * N, PH_CSUM = 2 * N * N, PH_QP = 2 * N * N + N,
+              PH_QM = 2 * N * N + 2 * N, PH_SSOL = 2 * N * N + 3 * N;
+
+// Legendre sums of layer l's moments for the rows row<R>(h, r) of column j;
+// returns the number of moments used (Lm) and the delta-M fraction f.
+template <typename R>
+__device__ __forceinline__ int phase16_sums(const FwdArgs& a, const double* chi, int mode,
+                                            double f, const Lane& L, double (&Qpp)[4],
+                                            double (&Qpm)[4], double& spj, double& smj) {
+  const int j = L.j;
+  const int twoM = 2 * a.n_real;
+  int Lm = a.M;
+  double inv1mf = 1.0;
+  if (mode == 3) {
+    inv1mf = 1.0 / (1.0 - f);
+    Lm = twoM;
+  } else if (mode == 2) {
+    Lm = a.M < twoM ? a.M : twoM;
+  }
+  double sign = 1.0;
+  spj = smj = 0.0;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) Qpp[r] = Qpm[r] = 0.0;
+  for (int ll = 0; ll < Lm; ++ll) {
+    const double chil = (mode == 3) ? (chi[ll] - f) * inv1mf : chi[ll];
+    const double cl = (2 * ll + 1) * chil;
+    const double* P = a.Pl + ll * N;
+    const double plj = P[j];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const double term = cl * P[row<R>(L.h, r)] * plj;
+      Qpp[r] += term;
+      Qpm[r] += sign * term;
+    }
+    if (a.has_solar) {
+      const double term = cl * plj * a.Pl0[ll];
+      spj += term;
+      smj += sign * term;
+    }
+    sign = -sign;
+  }
+  return Lm;
+}
+
+// One wave per layer: the phase record of layer l (shared moments only).  The
+// delta-M mode of a scattering layer depends on its moments alone
+// (src/solver.cpp:445-486), so it is decided here as the prep kernel does.
+__global__ void __launch_bounds__(64) phase16_kernel(FwdArgs a, double* phase) {
+  __shared__ double tile[TILE];
+  const int l = blockIdx.x;
+  if (l >= a.L) return;
+  const Lane L;
+  const int j = L.j;
+  const int li = lin(a, l);
+  const double* chi = a.mom + (long)li * a.M;
+  const int twoM = 2 * a.n_real;
+  int mode = 1;
+  double f = 0.0;
+  if (a.delta_m) {
+    f = (a.M > twoM) ? chi[twoM] : 0.0;
+    mode = (f > 1e-12 && f < 1.0 - 1e-12) ? 3 : 2;
+  }
+  double Qpp[4], Qpm[4], spj, smj;
+  phase16_sums<double>(a, chi, mode, f, L, Qpp, Qpm, spj, smj);
+  double* ph = phase + (long)l * PH16;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int i = row<double>(L.h, r);
+    Qpp[r] = Qpp[r] / (2.0 * kPi);
+    Qpm[r] = Qpm[r] / (2.0 * kPi);
+    ph[i * N + j] = Qpp[r];
+    ph[PH_QPM + i * N + j] = Qpm[r];
+    tile[j * TS + i] = (Qpp[r] + Qpm[r]) * a.wt[i];
+  }
+  wave_sync();
+  double csum = 0.0;
+#pragma unroll
+  for (int i = 0; i < N; ++i) csum += tile[j * TS + i];
+  if (L.h == 0) {
+    ph[PH_CSUM + j] = csum;
+    ph[PH_QP + j] = spj / (2.0 * kPi);
+    ph[PH_QM + j] = smj / (2.0 * kPi);
+  }
+  wave_sync();
+  const double ssol = colsum<double>((spj / (2.0 * kPi) + smj / (2.0 * kPi)) * a.wt[j], L, tile);
+  if (L.l == 0) ph[PH_SSOL] = ssol;
+}
+
 // dtile: a TILE of doubles (LDS), pv: 16 doubles (LDS)
 template <typename R>
 __device__ __forceinline__ void layer16_start(const FwdArgs& a, long task, int w, int l, long gw,
@@ -47,65 +140,61 @@ __device__ __forceinline__ void layer16_start(const FwdArgs& a, long task, int w
   const double con = 2.0 * omega * kPi;
   const int li = lin(a, l);
   const double* chi = a.shared_moments ? a.mom + (long)li * a.M : a.mom + (gw * L_ + li) * a.M;
-  const int twoM = 2 * a.n_real;
-  int Lm = a.M;
-  double f = 0.0, inv1mf = 1.0;
-  if (mode == 3) {
-    f = a.fdm[task];
-    inv1mf = 1.0 / (1.0 - f);
-    Lm = twoM;
-  } else if (mode == 2) {
-    Lm = a.M < twoM ? a.M : twoM;
-  }
-  double Ppp[4] = {0, 0, 0, 0}, Ppm[4] = {0, 0, 0, 0};
-  double sign = 1.0, spj = 0.0, smj = 0.0;
+  const double f = (mode == 3) ? a.fdm[task] : 0.0;
+  double Ppp[4], Ppm[4];
   int rows[4];
 #pragma unroll
   for (int r = 0; r < 4; ++r) rows[r] = row<R>(L.h, r);
-  for (int ll = 0; ll < Lm; ++ll) {
-    const double chil = (mode == 3) ? (chi[ll] - f) * inv1mf : chi[ll];
-    const double cl = (2 * ll + 1) * chil;
-    const double* P = a.Pl + ll * N;
-    const double plj = P[j];
+  double pplus = 0.0, pminus = 0.0;
+  if (a.phase) {
+    // shared moments: the layer's phase record (phase16_kernel)
+    const double* ph = a.phase + (long)l * PH16;
+    const double csum = ph[PH_CSUM + j];
+    const double corr = csum > 0.0 ? 1.0 / (2.0 * kPi * csum) : 1.0;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const double term = cl * P[rows[r]] * plj;
-      Ppp[r] += term;
-      Ppm[r] += sign * term;
+      Ppp[r] = ph[rows[r] * N + j] * corr;
+      Ppm[r] = ph[PH_QPM + rows[r] * N + j] * corr;
     }
     if (a.has_solar) {
-      const double term = cl * plj * a.Pl0[ll];
-      spj += term;
-      smj += sign * term;
+      pplus = ph[PH_QP + j];
+      pminus = ph[PH_QM + j];
+      const double s = ph[PH_SSOL];
+      if (s > 0.0) {
+        const double c = 1.0 / (2.0 * kPi * s);
+        pplus *= c;
+        pminus *= c;
+      }
     }
-    sign = -sign;
-  }
-  // Hansen column normalisation, rows summed in order 0..15
+  } else {
+    double spj, smj;
+    phase16_sums<R>(a, chi, mode, f, L, Ppp, Ppm, spj, smj);
+    // Hansen column normalisation, rows summed in order 0..15
 #pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    Ppp[r] = Ppp[r] / (2.0 * kPi);
-    Ppm[r] = Ppm[r] / (2.0 * kPi);
-    dtile[j * TS + rows[r]] = (Ppp[r] + Ppm[r]) * a.wt[rows[r]];
-  }
-  wave_sync();
-  double csum = 0.0;
+    for (int r = 0; r < 4; ++r) {
+      Ppp[r] = Ppp[r] / (2.0 * kPi);
+      Ppm[r] = Ppm[r] / (2.0 * kPi);
+      dtile[j * TS + rows[r]] = (Ppp[r] + Ppm[r]) * a.wt[rows[r]];
+    }
+    wave_sync();
+    double csum = 0.0;
 #pragma unroll
-  for (int i = 0; i < N; ++i) csum += dtile[j * TS + i];
-  wave_sync();
-  if (csum > 0.0) {
-    const double corr = 1.0 / (2.0 * kPi * csum);
+    for (int i = 0; i < N; ++i) csum += dtile[j * TS + i];
+    wave_sync();
+    if (csum > 0.0) {
+      const double corr = 1.0 / (2.0 * kPi * csum);
 #pragma unroll
-    for (int r = 0; r < 4; ++r) { Ppp[r] *= corr; Ppm[r] *= corr; }
-  }
-  double pplus = 0.0, pminus = 0.0;
-  if (a.has_solar) {
-    pplus = spj / (2.0 * kPi);
-    pminus = smj / (2.0 * kPi);
-    const double s = colsum<double>((pplus + pminus) * wt_j, L, pv);
-    if (s > 0.0) {
-      const double corr = 1.0 / (2.0 * kPi * s);
-      pplus *= corr;
-      pminus *= corr;
+      for (int r = 0; r < 4; ++r) { Ppp[r] *= corr; Ppm[r] *= corr; }
+    }
+    if (a.has_solar) {
+      pplus = spj / (2.0 * kPi);
+      pminus = smj / (2.0 * kPi);
+      const double s = colsum<double>((pplus + pminus) * wt_j, L, pv);
+      if (s > 0.0) {
+        const double corr = 1.0 / (2.0 * kPi * s);
+        pplus *= corr;
+        pminus *= corr;
+      }
     }
   }
   // thin-layer initialisation (src/doubling.hpp:83-147)
