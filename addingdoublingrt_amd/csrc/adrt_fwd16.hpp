@@ -233,7 +233,7 @@ __device__ __forceinline__ void layer16_start(const FwdArgs& a, long task, int w
 // S = I - T in FP32 register order of the FP32 matrix instruction, sources FP64.
 // ---------------------------------------------------------------------------
 template <bool REDO>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) layer16_kernel(FwdArgs a) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) layer16_kernel(FwdArgs a) {
   using R = double;
   __shared__ R smem[4 * Lds16::size];
   const int wv = wave_index();
