@@ -325,12 +325,14 @@ __device__ __forceinline__ bool inverse_ns(const Mat<R>& P, const R* tP, R* tX, 
   float e = residual_size<R>(E);
   for (int it = 0; e > NS_DONE; ++it) {
     if (!(e <= 0.5f) || it == NS_MAX_STEPS) {
-      wave_sync();
+      // P again from its tile, so no register holds it through the iteration
       Mat<R> A;
+      get<R>(tP, L, A);
+      wave_sync();
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const R d = (row<R>(L.h, r) == L.j) ? R(1) : R(0);
-        A.v[r] = d - P.v[r];
+        A.v[r] = d - A.v[r];
         X.v[r] = d;
       }
       right_solve<R>(A, X, L);
