@@ -136,7 +136,7 @@ __device__ void layer16m_adjoint_task(const FwdArgs& a, const AdjArgs& b, long t
       st[MS_SD + j] = sds;
     }
     if (L.l == 0) st[MS_GAM] = gam;
-    Mat<R> A, Q, G, X;
+    Mat<R> Q, G, X;
     put<R>(tA, L, Rm);
     if (L.h == 0) {
       fs[j] = (float)sds;
@@ -151,9 +151,10 @@ __device__ void layer16m_adjoint_task(const FwdArgs& a, const AdjArgs& b, long t
       b1 = bs[0];
       b2 = bs[1];
     }
-#pragma unroll
-    for (int r = 0; r < 4; ++r) A.v[r] = ((row<R>(L.h, r) == j) ? 1.0f : 0.0f) - Q.v[r];
-    right_solve<R>(A, Q, L);
+    wave_sync();
+    put<R>(tA, L, Q);
+    wave_sync();
+    q_form_ns<R>(Q, tA, tB, L);
 #pragma unroll
     for (int r = 0; r < 4; ++r) td[j * TS + row<R>(L.h, r)] = (double)Rm.v[r];
     wave_sync();

@@ -132,7 +132,7 @@ layer16m_kernel(FwdArgs a) {
   double y = st.y, z = 0.0, sus = st.sus, sds = st.sds, gam = st.gam, gk = st.gk;
 
   for (int k = 0; k < nn; ++k) {
-    Mat<R> A, Q, G, X;
+    Mat<R> Q, G, X;
     // ---- Q = R^2 (I - R^2)^-1 ----
     put<R>(tf, L, Rm);
     if (L.h == 0) {
@@ -148,9 +148,10 @@ layer16m_kernel(FwdArgs a) {
       b1 = bs[0];
       b2 = bs[1];
     }
-#pragma unroll
-    for (int r = 0; r < 4; ++r) A.v[r] = ((row<R>(L.h, r) == j) ? 1.0f : 0.0f) - Q.v[r];
-    right_solve<R>(A, Q, L);  // Q0
+    wave_sync();
+    put<R>(tf, L, Q);
+    wave_sync();
+    q_form_ns<R>(Q, tf, tf2, L);  // Q0
     // FP64 residual E = R R - Q0 + Q0 (R R)
 #pragma unroll
     for (int r = 0; r < 4; ++r) td0[j * TS + row<R>(L.h, r)] = (double)Rm.v[r];

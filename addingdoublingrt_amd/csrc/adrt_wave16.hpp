@@ -370,6 +370,51 @@ __device__ __forceinline__ bool inverse_ns(const Mat<R>& P, const R* tP, R* tX, 
   return false;
 }
 
+// Q = P (I - P)^-1 = (I - P)^-1 - I, for P in Q on entry and in the tile tP
+// (tX: scratch tile), in Q form so that a small Q keeps its relative
+// precision in FP32: the Newton-Schulz iteration of X = I + Q from Q0 = P,
+//   E = P - Q + P Q  (= I - (I - P) X),   Q <- Q + E + Q E  (residual E^2),
+// until N max|E| <= 1e-5 N max|P| -- a relative accuracy that one refinement
+// step with an FP64 residual then carries to FP32 rounding.  E0 = P^2; an
+// E0 past 1/2 (I - P close to singular) or 8 steps fall back to Gauss-Jordan.
+template <typename R>
+__device__ __forceinline__ void q_form_ns(Mat<R>& Q, const R* tP, R* tX, const Lane& L) {
+  const float pmax = residual_size<R>(Q);
+  Mat<R> E;
+  mul<R>(tP, Q, L, E);  // P^2
+  float e = residual_size<R>(E);
+  for (int it = 0; e > 1e-5f * pmax; ++it) {
+    if (!(e <= 0.5f) || it == NS_MAX_STEPS) {
+      Mat<R> A;
+      get<R>(tP, L, A);
+      wave_sync();
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        Q.v[r] = A.v[r];
+        A.v[r] = ((row<R>(L.h, r) == L.j) ? R(1) : R(0)) - A.v[r];
+      }
+      right_solve<R>(A, Q, L);
+      return;
+    }
+    put<R>(tX, L, Q);
+    wave_sync();
+    {
+      Mat<R> Z;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) Z.v[r] = Q.v[r] + E.v[r];
+      mul_add<R>(tX, E, Z, L, Q);
+    }
+    wave_sync();
+    Mat<R> PQ, Pm;
+    mul<R>(tP, Q, L, PQ);
+    get<R>(tP, L, Pm);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) E.v[r] = (Pm.v[r] - Q.v[r]) + PQ.v[r];
+    e = residual_size<R>(E);
+  }
+  wave_sync();
+}
+
 // x = M^-1 b: M by registers, b element j (all lanes of column j); returns x_j
 template <typename R>
 __device__ __forceinline__ R left_solve(const Mat<R>& M, R b, const Lane& L, R* tile) {
