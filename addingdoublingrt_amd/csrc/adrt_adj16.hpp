@@ -664,7 +664,10 @@ __device__ void layer16_adjoint_task(const FwdArgs& a, const AdjArgs& b, long ta
 // each scattering layer, reversed, from the start cotangents that
 // layer16_adjoint_kernel left in b.scot; one wave per (wavenumber, layer).
 // Kept apart from the doubling reverse so that kernel keeps to 128 registers.
-template <bool REDO>
+// REDO: only the layers the mixed-precision forward left to the FP64 kernels;
+// RowT = float: the other layers of the mixed form (start cotangents in the
+// FP32 register order of layer16m_adjoint_kernel).
+template <bool REDO, typename RowT = double>
 __global__ void __launch_bounds__(256) start16_adjoint_kernel(FwdArgs a, AdjArgs b) {
   __shared__ double smem[4 * (TILE + N)];
   const int wv = wave_index();
@@ -673,18 +676,20 @@ __global__ void __launch_bounds__(256) start16_adjoint_kernel(FwdArgs a, AdjArgs
   if (task >= (long)a.W * Lay) return;
   if constexpr (REDO) {
     if (!a.redo[task]) return;
+  } else if constexpr (sizeof(RowT) == 4) {
+    if (a.redo[task]) return;
   }
   const double tau = a.tau_s[task];
   const double omega_in = a.omega_s[task];
-  if (tau <= 0.0 || omega_in <= 0.0) return;  // finished by layer16_adjoint_kernel
+  if (tau <= 0.0 || omega_in <= 0.0) return;  // finished by the doubling reverse
   const Lane L;
   const int j = L.j;
   const int w = (int)(task / Lay), l = (int)(task % Lay);
   const long gw = (long)(a.w0 + w);
   double* dtile = smem + wv * (TILE + N);
   double* csl = dtile + TILE;
-  Phase16<double> p;
-  phase16_recompute<double>(a, task, w, l, gw, L, tau, omega_in, a.mode[task], dtile, csl, p);
+  Phase16<RowT> p;
+  phase16_recompute<RowT>(a, task, w, l, gw, L, tau, omega_in, a.mode[task], dtile, csl, p);
   if (p.nn > b.nn_cap) return;  // reported through b.status
   const double* sc = b.scot + task * SCOT;
   double Rb0[4], Tb0[4];
@@ -700,8 +705,8 @@ __global__ void __launch_bounds__(256) start16_adjoint_kernel(FwdArgs a, AdjArgs
   double* chi_out = a.shared_moments ? b.g_chi + task * a.M
                                      : (b.d_mom ? b.d_mom + (gw * Lay + li) * a.M : nullptr);
   double g_tau = b.g_taus[task], g_om = 0.0, g_F = 0.0, g_fr = 0.0;
-  start16_reverse<double>(a, L, p, omega_in, Rb0, Tb0, yb, susb, sdsb, g_sum, gam_sum, csl,
-                          chi_out, g_tau, g_om, g_F, g_fr);
+  start16_reverse<RowT>(a, L, p, omega_in, Rb0, Tb0, yb, susb, sdsb, g_sum, gam_sum, csl,
+                        chi_out, g_tau, g_om, g_F, g_fr);
   if (chi_out)
     for (int m = p.Lm + L.l; m < a.M; m += 64) chi_out[m] = 0.0;
   if (L.l == 0) {

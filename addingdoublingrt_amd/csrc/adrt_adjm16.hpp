@@ -92,36 +92,33 @@ __device__ void layer16m_adjoint_task(const FwdArgs& a, const AdjArgs& b, long t
     return;
   }
 
-  Phase16<R> p;
-  phase16_recompute<R>(a, task, w, l, gw, L, tau, omega_in, mode, td, csl, p);
-  const int nn = p.nn;
+  // the thin-layer start as the forward computed it; its reverse is
+  // start16_adjoint_kernel<false, float>'s
+  Mat<R> Rm, Sm;
+  double y, z = 0.0, sus, sds, gam, gk, tau0;
+  int nn;
+  {
+    Start16<R> st0;
+    layer16_start<R>(a, task, w, l, gw, L, tau, omega_in, mode, td, csl, st0);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      Sm.v[r] = (float)st0.Sv[r];  // S0 = tau0 G++, formed directly
+      Rm.v[r] = (float)st0.Rv[r];
+    }
+    y = st0.y;
+    sus = st0.sus;
+    sds = st0.sds;
+    gam = st0.gam;
+    gk = st0.gk;
+    tau0 = st0.tau0;
+    nn = st0.nn;
+  }
   if (nn > b.nn_cap) {
     if (L.l == 0) atomicOr(b.status, 1);
     finish(0);
     return;
   }
-  const double tau0 = p.tau0;
-  const bool solar = p.solar;
-  // start: S0 = tau0 G++, R0 = tau0 G+- (FP64, then FP32)
-  Mat<R> Rm, Sm;
-#pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    const int i = row<R>(L.h, r);
-    const double mui = a.mu[i];
-    const double gpp = (((i == j) ? 1.0 : 0.0) + (-p.con) * (p.Ppp[r] * a.wt[j])) / mui;
-    const double gpm = p.con * (p.Ppm[r] * a.wt[j]) / mui;
-    Sm.v[r] = (float)(tau0 * gpp);
-    Rm.v[r] = (float)(tau0 * gpm);
-  }
-  double y = (1.0 - p.omega) * tau0 / mu_j, z = 0.0, sus = 0.0, sds = 0.0, gam = 0.0;
-  if (solar) {
-    const double base = p.omega * tau0 / mu_j * p.Ftop;
-    sus = base * p.pminus;
-    sds = base * p.pplus;
-    gam = p.gam0;
-  }
-  double gk = 0.5 * tau0;
-  park<R>(slab, b.nn_cap, L, p);
+  const bool solar = a.has_solar != 0;
 
   // ---- the forward S-form doubling, recomputed with checkpoints ----
   for (int k = 0; k < nn; ++k) {
@@ -411,22 +408,30 @@ __device__ void layer16m_adjoint_task(const FwdArgs& a, const AdjArgs& b, long t
     wave_sync();
   }
 
-  // ---- thin-layer start, Hansen normalisation and Legendre sums, reversed ----
+  // ---- cotangents of the doubling start, for start16_adjoint_kernel ----
   const double g_sum = colsum<double>(gb, L, csl);
   const double gam_sum = solar ? colsum<double>(gamb, L, csl) : 0.0;
-  double Rb0[4], Tb0[4];
+  double* sc = b.scot + task * SCOT;
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
-    Rb0[r] = Rbar.v[r];
-    Tb0[r] = -(double)Sbar.v[r];  // T0 = I - S0
+    sc[L.l * 4 + r] = (double)Rbar.v[r];
+    sc[N * N + L.l * 4 + r] = -(double)Sbar.v[r];  // T0 = I - S0
   }
-  unpark<R>(slab, b.nn_cap, L, p);
-  start16_reverse<R>(a, L, p, omega_in, Rb0, Tb0, yb, susb, sdsb, g_sum, gam_sum, csl, chi_out,
-                     g_tau, g_om, g_F, g_fr);
-  finish(p.Lm);
+  if (L.h == 0) {
+    sc[2 * N * N + j] = yb;
+    sc[2 * N * N + N + j] = susb;
+    sc[2 * N * N + 2 * N + j] = sdsb;
+  }
+  if (L.l == 0) {
+    sc[2 * N * N + 3 * N] = g_sum;
+    sc[2 * N * N + 3 * N + 1] = gam_sum;
+    b.g_taus[task] = g_tau;  // completed by start16_adjoint_kernel
+    b.g_Bt[task] = g_Bt;
+    b.g_Bb[task] = g_Bb;
+  }
 }
 
-constexpr int MREV_WAVES_PER_SIMD = 2;  // register budget 256 per lane
+constexpr int MREV_WAVES_PER_SIMD = 3;  // register budget 168 per lane
 
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MREV_WAVES_PER_SIMD, 8)))
 layer16m_adjoint_kernel(FwdArgs a, AdjArgs b) {
