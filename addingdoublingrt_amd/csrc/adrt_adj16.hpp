@@ -187,8 +187,8 @@ __device__ __forceinline__ void start16_reverse(const FwdArgs& a, const Lane& L,
                                                 const double (&Rb0)[4], const double (&Tb0)[4],
                                                 double yb, double susb, double sdsb,
                                                 double g_sum, double gam_sum, double* csl,
-                                                double* chi_out, double& g_tau, double& g_om,
-                                                double& g_F, double& g_fr) {
+                                                double* tiles, double* chi_out, double& g_tau,
+                                                double& g_om, double& g_F, double& g_fr) {
   const int j = L.j;
   const double mu_j = a.mu[j], wt_j = a.wt[j];
   const double tau0 = p.tau0, con = p.con, omega = p.omega;
@@ -256,34 +256,61 @@ __device__ __forceinline__ void start16_reverse(const FwdArgs& a, const Lane& L,
     qplusb = pplusb * p.corr_s + sb * wt_j;
     qminusb = pminusb * p.corr_s + sb * wt_j;
   }
-  // Legendre sums, 32 moments at a time: this lane's part of each, then one
-  // transposed reduction (lane l ends with moment c0 + bitrev5(l mod 32))
+  // Legendre sums on the matrix cores, 16 moments at a time: with
+  // S+- = Q^++ +- Q^+- (rows in the FP64 instruction order) and the block
+  // A(m, i) = P_{l0+m}(mu_i) in an LDS tile, U+- = A S+- holds
+  // sum_i P_l(mu_i) S+-(i, j) for l = l0 + m, and
+  //   chi^_l = (2l+1)/(2 pi) sum_j P_l(mu_j) (U(m, j) + P_l(mu0) w+-_j),
+  // w+- = q+^ +- q-^, the sign + for even l; the sum over j is a DPP row sum,
+  // so lane (j, h) ends with the moments l0 + h + 4r of its register r
+  double* tP = tiles;
+  double* tS = tiles + TILE;
+  Mat<double> Sp, Sn;
+  if constexpr (sizeof(RowT) == 8) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      Sp.v[r] = Qb_pp[r] + Qb_pm[r];
+      Sn.v[r] = Qb_pp[r] - Qb_pm[r];
+    }
+  } else {  // to the FP64 row order through LDS
+#pragma unroll
+    for (int r = 0; r < 4; ++r) tS[j * TS + row<RowT>(L.h, r)] = Qb_pp[r] + Qb_pm[r];
+    wave_sync();
+    get<double>(tS, L, Sp);
+    wave_sync();
+#pragma unroll
+    for (int r = 0; r < 4; ++r) tS[j * TS + row<RowT>(L.h, r)] = Qb_pp[r] - Qb_pm[r];
+    wave_sync();
+    get<double>(tS, L, Sn);
+    wave_sync();
+  }
+  const double wp = qplusb + qminusb, wn = qplusb - qminusb;
+  const bool odd = (L.h & 1) != 0;  // parity of l0 + h + 4r
   double gfr_part = 0.0;
-  for (int c0 = 0; c0 < p.Lm; c0 += 32) {
-    double v[32];
+  for (int l0 = 0; l0 < p.Lm; l0 += 16) {
 #pragma unroll
-    for (int q = 0; q < 32; ++q) {
-      const int ll = c0 + q;
-      double part = 0.0;
-      if (ll < p.Lm) {
-        const double sgn = (ll & 1) ? -1.0 : 1.0;
-        const double* P = a.Pl + ll * N;
-        const double plj = P[j];
-        double t = 0.0;
+    for (int r = 0; r < 4; ++r) {
+      const int ll = l0 + row<double>(L.h, r);
+      tP[j * TS + row<double>(L.h, r)] = (ll < p.Lm) ? a.Pl[ll * N + j] : 0.0;
+    }
+    wave_sync();
+    Mat<double> Up, Un;
+    mul<double>(tP, Sp, L, Up);
+    mul<double>(tP, Sn, L, Un);
 #pragma unroll
-        for (int r = 0; r < 4; ++r) t += P[row<RowT>(L.h, r)] * (Qb_pp[r] + sgn * Qb_pm[r]);
-        part = t * plj;
-        if (p.solar && L.h == 0) part += a.Pl0[ll] * plj * (qplusb + sgn * qminusb);
+    for (int r = 0; r < 4; ++r) {
+      const int m = row<double>(L.h, r), ll = l0 + m;
+      const bool live = ll < p.Lm;
+      double u = odd ? Un.v[r] : Up.v[r];
+      if (p.solar && live) u += a.Pl0[ll] * (odd ? wn : wp);
+      const double tot = row_sum<double>(tP[j * TS + m] * u);
+      if (live && j == 0) {
+        const double chisb = tot * ((2 * ll + 1) / (2.0 * kPi));
+        if (p.mode == 3) gfr_part += chisb * (p.chi[ll] - 1.0) * p.inv1mf * p.inv1mf;
+        if (chi_out) chi_out[ll] = (p.mode == 3) ? chisb * p.inv1mf : chisb;
       }
-      v[q] = part;
     }
-    const double tot = transpose_reduce32(v, L);
-    const int ll = c0 + bitrev5(L.l & 31);
-    if (ll < p.Lm && L.l < 32) {
-      const double chisb = tot * ((2 * ll + 1) / (2.0 * kPi));
-      if (p.mode == 3) gfr_part += chisb * (p.chi[ll] - 1.0) * p.inv1mf * p.inv1mf;
-      if (chi_out) chi_out[ll] = (p.mode == 3) ? chisb * p.inv1mf : chisb;
-    }
+    wave_sync();
   }
   if (p.mode == 3) g_fr += wave_sum<double>(gfr_part, L);
 }
@@ -669,7 +696,7 @@ __device__ void layer16_adjoint_task(const FwdArgs& a, const AdjArgs& b, long ta
 // FP32 register order of layer16m_adjoint_kernel).
 template <bool REDO, typename RowT = double>
 __global__ void __launch_bounds__(256) start16_adjoint_kernel(FwdArgs a, AdjArgs b) {
-  __shared__ double smem[4 * (TILE + N)];
+  __shared__ double smem[4 * (3 * TILE + N)];
   const int wv = wave_index();
   const long task = (long)blockIdx.x * 4 + wv;
   const int Lay = a.L;
@@ -686,8 +713,9 @@ __global__ void __launch_bounds__(256) start16_adjoint_kernel(FwdArgs a, AdjArgs
   const int j = L.j;
   const int w = (int)(task / Lay), l = (int)(task % Lay);
   const long gw = (long)(a.w0 + w);
-  double* dtile = smem + wv * (TILE + N);
+  double* dtile = smem + wv * (3 * TILE + N);
   double* csl = dtile + TILE;
+  double* tiles = csl + N;  // two tiles for the Legendre sums
   Phase16<RowT> p;
   phase16_recompute<RowT>(a, task, w, l, gw, L, tau, omega_in, a.mode[task], dtile, csl, p);
   if (p.nn > b.nn_cap) return;  // reported through b.status
@@ -706,7 +734,7 @@ __global__ void __launch_bounds__(256) start16_adjoint_kernel(FwdArgs a, AdjArgs
                                      : (b.d_mom ? b.d_mom + (gw * Lay + li) * a.M : nullptr);
   double g_tau = b.g_taus[task], g_om = 0.0, g_F = 0.0, g_fr = 0.0;
   start16_reverse<RowT>(a, L, p, omega_in, Rb0, Tb0, yb, susb, sdsb, g_sum, gam_sum, csl,
-                        chi_out, g_tau, g_om, g_F, g_fr);
+                        tiles, chi_out, g_tau, g_om, g_F, g_fr);
   if (chi_out)
     for (int m = p.Lm + L.l; m < a.M; m += 64) chi_out[m] = 0.0;
   if (L.l == 0) {

@@ -221,6 +221,18 @@ __device__ __forceinline__ R row_max(R v) {
   return v;
 }
 
+// sum over the 16 lanes of each DPP row, the same value in every lane of the
+// row (pairs, quads, half-row mirror, row mirror: each stage adds two equal-
+// order partial sums, so every lane rounds alike)
+template <typename R>
+__device__ __forceinline__ R row_sum(R v) {
+  v = v + dpp<0xB1>(v);
+  v = v + dpp<0x4E>(v);
+  v = v + dpp<0x141>(v);
+  v = v + dpp<0x140>(v);
+  return v;
+}
+
 // diagonal entry M(j, j) of column j, for all four lanes of the column
 template <typename R>
 __device__ __forceinline__ R diag(const Mat<R>& m, const Lane& L) {
@@ -467,33 +479,6 @@ __device__ __forceinline__ R wave_sum(R p, const Lane& L) {
 #pragma unroll
   for (int m = 1; m < 64; m <<= 1) p += shfl(p, L.l ^ m);
   return p;
-}
-
-// Sums over the 64 lanes of 32 values per lane at once: five butterfly stages
-// each halve the list a lane keeps (lanes with bit s set keep the upper half
-// and add their partner's copy of it), a sixth adds the two wave halves.  Lane
-// l returns the total of entry bitrev5(l mod 32).
-template <int S>
-__device__ __forceinline__ void butterfly_stage(double (&v)[32], const Lane& L) {
-  constexpr int half = 16 >> S;
-  const bool up = (L.l >> S) & 1;
-#pragma unroll
-  for (int q = 0; q < half; ++q) {
-    const double keep = up ? v[q + half] : v[q];
-    const double send = up ? v[q] : v[q + half];
-    v[q] = keep + shfl(send, L.l ^ (1 << S));
-  }
-}
-__device__ __forceinline__ double transpose_reduce32(double (&v)[32], const Lane& L) {
-  butterfly_stage<0>(v, L);
-  butterfly_stage<1>(v, L);
-  butterfly_stage<2>(v, L);
-  butterfly_stage<3>(v, L);
-  butterfly_stage<4>(v, L);
-  return v[0] + shfl(v[0], L.l ^ 32);
-}
-__device__ __forceinline__ int bitrev5(int m) {
-  return ((m & 1) << 4) | ((m & 2) << 2) | (m & 4) | ((m & 8) >> 2) | ((m & 16) >> 4);
 }
 
 // (M^T v)_j with M in registers and v in an LDS slot of 16 values
