@@ -138,6 +138,22 @@ def test_parity_edge_layers(n):
     run_both(tau, ssa, mom, cfg, B, B[:, -1].copy(), B[:, 0].copy(), label=f"edge N={n}")
 
 
+@pytest.mark.parametrize("n", [16, 8])
+def test_parity_thick_conservative(n):
+    """Thick, (near-)conservative layers: the doubling's I - R^2 approaches
+    singularity in the last steps, where the N = 16 kernels leave the
+    Newton-Schulz inverse for pivoted Gauss-Jordan."""
+    W, L = 6, 5
+    tau = np.array([[50.0, 200.0, 1000.0, 3.0, 0.7]] * W)
+    tau *= np.linspace(1.0, 1.3, W)[:, None]
+    ssa = np.array([[1.0, 1.0 - 1e-9, 0.9999, 1.0, 0.999999]] * W)
+    g = np.linspace(0.0, 0.85, W)
+    mom = np.repeat((g[:, None] ** np.arange(2 * n))[:, None, :], L, axis=1)
+    B = np.linspace(1.0, 2.0, L + 1)[None, :].repeat(W, axis=0)
+    cfg = BatchConfig(num_quadrature=n, surface_albedo=0.5, solar_flux=1.0, solar_mu=0.4)
+    run_both(tau, ssa, mom, cfg, B, B[:, -1].copy(), B[:, 0].copy(), label=f"thick N={n}")
+
+
 @pytest.mark.parametrize("opt", ["delta_m", "diffusion", "reversed", "no_surface", "black_cold",
                                  "shared_moments", "scaling", "albedo_only", "solar_only"])
 def test_parity_options(opt):
