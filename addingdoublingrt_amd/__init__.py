@@ -4,6 +4,8 @@ The compute path is ``libadrt_hip.so`` (C-ABI: include/adrt_hip.h); this
 package is its host-side face:
 
 * ``solve_batch`` / ``BatchConfig`` -- batched forward solve on device tensors
+* ``solve_batch_from_coefficients`` -- the same from raw absorption/scattering
+                                       coefficients, altitudes and temperatures
 * ``adjoint_batch``                 -- hand-written reverse pass (gradients)
 * ``solve``                         -- differentiable solve (torch.autograd)
 * ``planck_levels``                 -- device band Planck, differentiable in T
@@ -11,8 +13,9 @@ package is its host-side face:
 """
 from ._lib import AdrtError, AdrtLibraryError, lib  # noqa: F401
 from .solver import (BatchConfig, Workspace, adjoint_batch, gauss_legendre, planck,  # noqa: F401
-                     planck_deriv, planck_levels, solve, solve_batch)
+                     planck_deriv, planck_levels, solve, solve_batch,
+                     solve_batch_from_coefficients)
 
 __all__ = ["AdrtError", "AdrtLibraryError", "BatchConfig", "Workspace", "adjoint_batch",
            "gauss_legendre", "planck", "planck_deriv", "planck_levels", "solve", "solve_batch",
-           "lib"]
+           "solve_batch_from_coefficients", "lib"]

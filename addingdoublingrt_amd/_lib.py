@@ -76,6 +76,15 @@ class DeviceGradients(C.Structure):
     ]
 
 
+class DeviceCoefficients(C.Structure):
+    _fields_ = [
+        ("absorption_coeff", C.c_void_p), ("scattering_coeff", C.c_void_p),
+        ("altitude", C.c_void_p), ("temperature", C.c_void_p), ("wavenumber", C.c_void_p),
+        ("phase_moments", C.c_void_p), ("phase_moments_shared", C.c_int),
+        ("cloud_optical_depth", C.c_void_p),
+    ]
+
+
 # Every symbol include/adrt_hip.h declares, with its ctypes signature.
 SIGNATURES = {
     "adrt_workspace_create": (C.c_int, [C.POINTER(BatchConfig), C.c_size_t, C.POINTER(C.c_void_p)]),
@@ -91,6 +100,10 @@ SIGNATURES = {
     "adrt_workspace_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_size_t), C.POINTER(C.c_int)]),
     "adrt_solve_batch": (C.c_int, [C.POINTER(BatchConfig), C.POINTER(DeviceInputs),
                                    C.POINTER(DeviceOutputs), C.c_int, C.c_void_p, C.c_void_p]),
+    "adrt_solve_batch_from_coefficients": (C.c_int, [C.POINTER(BatchConfig),
+                                                     C.POINTER(DeviceCoefficients), C.c_double,
+                                                     C.c_double, C.POINTER(DeviceOutputs), C.c_int,
+                                                     C.c_void_p, C.c_void_p]),
     "adrt_validate_inputs": (C.c_int, [C.POINTER(BatchConfig), C.POINTER(DeviceInputs),
                                        C.c_void_p, C.c_void_p]),
     "adrt_planck_levels": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int,

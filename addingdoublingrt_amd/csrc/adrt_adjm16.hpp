@@ -69,7 +69,6 @@ __device__ void layer16m_adjoint_task(const FwdArgs& a, const AdjArgs& b, long t
   const double Bbot = a.B ? a.B[gw * (Lay + 1) + lev(a, l + 1)] : 0.0;
   const double B_bar = (Bbot + Btop) / 2.0;
   const double B_d = (tau > 0.0) ? (Bbot - Btop) / tau : 0.0;
-  const double mu_j = a.mu[j];
   const int li = lin(a, l);
   double* chi_out = a.shared_moments ? b.g_chi + task * a.M
                                      : (b.d_mom ? b.d_mom + (gw * Lay + li) * a.M : nullptr);

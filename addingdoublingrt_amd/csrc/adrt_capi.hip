@@ -192,6 +192,43 @@ __global__ void planck_levels_kernel(const double* lo, const double* hi, const d
   out[i] = b;
 }
 
+// Single-wavenumber Planck function in W m^-2 cm, the form and constants of the
+// retrieval code the raw entry serves (src_cuda/cuda_planck_single.cuh:18-34):
+// c1 = 2 h c^2 (CGS, scaled to W m^-2 cm), c2 = h c / k.
+__device__ __forceinline__ double planck_single(double T, double nu) {
+  constexpr double c1 = 1.19105e-08, c2 = 1.43879;
+  if (T < 1e-5) return 0.0;
+  return c1 * nu * nu * nu / expm1(c2 * nu / T);
+}
+
+// raw coefficients -> TOA-first tau, omega, level Planck and boundary emission
+// for the wavenumbers w0 .. w0+W-1 of a batch of nwav (src_cuda/cuda_solver.cu:1131-1182)
+__global__ void raw_optics_kernel(int W, int w0, int nwav, int L, const double* abs_c,
+                                  const double* sca_c, const double* alt, const double* T,
+                                  const double* wn, const double* cloud, double Ts, double Tt,
+                                  double* tau_o, double* ssa_o, double* B_o, double* Bs_o,
+                                  double* Bt_o) {
+  const int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= W) return;
+  const long gw = (long)w0 + w;
+  const double nu = wn[gw];
+  for (int l = 0; l < L; ++l) {
+    const int b = L - 1 - l;  // BOA-first layer index
+    const double dz = alt[b + 1] - alt[b];
+    const double ab = abs_c[(long)b * nwav + gw], at = abs_c[(long)(b + 1) * nwav + gw];
+    const double sb = sca_c[(long)b * nwav + gw], st = sca_c[(long)(b + 1) * nwav + gw];
+    double tau = dz * ((at + st) + (ab + sb)) * 0.5;
+    const double sdepth = dz * (st + sb) * 0.5;
+    if (cloud) tau += cloud[(long)b * nwav + gw];
+    if (tau < 0.0) tau = 0.0;
+    tau_o[(long)w * L + l] = tau;
+    ssa_o[(long)w * L + l] = tau > 0.0 ? sdepth / tau : 0.0;
+  }
+  for (int lev = 0; lev <= L; ++lev) B_o[(long)w * (L + 1) + lev] = planck_single(T[L - lev], nu);
+  Bs_o[w] = planck_single(Ts >= 0.0 ? Ts : T[0], nu);
+  Bt_o[w] = planck_single(Tt >= 0.0 ? Tt : T[L], nu);
+}
+
 __global__ void validate_kernel(const double* tau, const double* ssa, long n, int* flag) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -216,7 +253,7 @@ struct adrt_workspace {
   int flags = 0;
   double *Iv = nullptr, *Av = nullptr, *g_taus = nullptr, *g_omgs = nullptr, *g_Bt = nullptr,
          *g_Bb = nullptr, *g_Ftop = nullptr, *g_f = nullptr, *g_chi = nullptr, *scot = nullptr, *slab = nullptr,
-         *mom_part = nullptr, *phase = nullptr;
+         *mom_part = nullptr, *phase = nullptr, *raw = nullptr;
   int* status = nullptr;
   int nwaves = 0, nn_cap = 0;
   unsigned long long generation = 0;  // forward solves so far
@@ -243,6 +280,7 @@ size_t per_wavenumber_bytes(int L, int Np, int M, bool adjoint) {
   b += 3 * (size_t)L * sizeof(int);                   // mode, lflag, redo
   b += (size_t)L * (2 * Np * Np + 2 * Np) * d;        // layer store
   b += (size_t)L * (2 * Np * Np + Np) * d;            // base composites
+  b += (size_t)(3 * L + 3) * d;                       // raw-coefficient optics
   if (adjoint) {
     b += 2 * (size_t)(L + 1) * 2 * Np * d;            // forward and adjoint interface vectors
     b += 6 * (size_t)L * d;                           // per-layer gradient terms
@@ -412,6 +450,7 @@ int adrt_workspace_create_ex(const adrt_batch_config* cfg, size_t max_bytes, int
   const size_t o_lay = take(C * L * (2 * Np * Np + 2 * Np) * 8);
   const size_t o_rb = take(C * L * (2 * Np * Np + Np) * 8);
   const size_t o_phase = take(phase_bytes);
+  const size_t o_raw = take(C * (3 * L + 3) * 8);
   size_t o_iv = 0, o_av = 0, o_g = 0, o_chi = 0, o_slab = 0, o_status = 0, o_part = 0,
          o_scot = 0;
   if (adjoint) {
@@ -444,6 +483,7 @@ int adrt_workspace_create_ex(const adrt_batch_config* cfg, size_t max_bytes, int
   ws->lay = (double*)(ws->base + o_lay);
   ws->rb = (double*)(ws->base + o_rb);
   ws->phase = phase_bytes ? (double*)(ws->base + o_phase) : nullptr;
+  ws->raw = (double*)(ws->base + o_raw);
   if (adjoint) {
     ws->Iv = (double*)(ws->base + o_iv);
     ws->Av = (double*)(ws->base + o_av);
@@ -600,6 +640,55 @@ int adrt_solve_batch(const adrt_batch_config* cfg, const adrt_device_inputs* in,
   ws->generation += 1;
   ws->kept_valid = ws->Iv != nullptr && cfg->num_wavenumbers <= ws->chunk;
   ws->kept_mixed = precision == ADRT_PRECISION_MIXED;
+  return ADRT_OK;
+}
+
+int adrt_solve_batch_from_coefficients(const adrt_batch_config* cfg_in,
+                                       const adrt_device_coefficients* coef,
+                                       double surface_temperature, double top_temperature,
+                                       const adrt_device_outputs* out, adrt_precision precision,
+                                       adrt_workspace* ws, hipStream_t stream) {
+  if (!cfg_in || !coef || !out) return ADRT_ERR_NULL;
+  adrt_batch_config cfg = *cfg_in;
+  cfg.index_from_bottom = 0;
+  cfg.surface_emission = 0.0;
+  cfg.top_emission = 0.0;
+  int rc = check_config(&cfg);
+  if (rc) return rc;
+  const int W = cfg.num_wavenumbers, L = cfg.num_layers, M = cfg.num_moments;
+  if (W == 0) return ADRT_OK;
+  if (!coef->absorption_coeff || !coef->scattering_coeff || !coef->altitude ||
+      !coef->temperature || !coef->wavenumber || !coef->phase_moments)
+    return ADRT_ERR_NULL;
+  if (!ws || ws->L != L) return ADRT_ERR_WORKSPACE;
+  const long C = ws->chunk;
+  double* tau = ws->raw;
+  double* ssa = tau + C * L;
+  double* B = ssa + C * L;
+  double* Bs = B + C * (L + 1);
+  double* Bt = Bs + C;
+  for (int w0 = 0; w0 < W; w0 += ws->chunk) {
+    const int Wc = std::min(ws->chunk, W - w0);
+    raw_optics_kernel<<<(unsigned)((Wc + 127) / 128), 128, 0, stream>>>(
+        Wc, w0, W, L, coef->absorption_coeff, coef->scattering_coeff, coef->altitude,
+        coef->temperature, coef->wavenumber, coef->cloud_optical_depth, surface_temperature,
+        top_temperature, tau, ssa, B, Bs, Bt);
+    if (hipGetLastError() != hipSuccess) return ADRT_ERR_HIP;
+    adrt_batch_config cc = cfg;
+    cc.num_wavenumbers = Wc;
+    const double* mom = coef->phase_moments_shared
+                            ? coef->phase_moments
+                            : coef->phase_moments + (long)w0 * L * M;
+    adrt_device_inputs in{tau, ssa, mom, coef->phase_moments_shared, B, Bs, Bt};
+    const long o = (long)w0 * (L + 1);
+    adrt_device_outputs oc{out->flux_up ? out->flux_up + o : nullptr,
+                           out->flux_down ? out->flux_down + o : nullptr,
+                           out->mean_intensity ? out->mean_intensity + o : nullptr,
+                           out->flux_direct ? out->flux_direct + o : nullptr,
+                           out->flux_divergence ? out->flux_divergence + o : nullptr};
+    rc = adrt_solve_batch(&cc, &in, &oc, precision, ws, stream);
+    if (rc) return rc;
+  }
   return ADRT_OK;
 }
 

@@ -321,10 +321,13 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))
     wave_sync();
     put<R>(tile, L, Y);
     wave_sync();
-    inverse_ns<R>(Y, tile, tile2, A, L);  // (I - R^2)^-1
-    put<R>(tile2, L, Tm);
-    wave_sync();
-    mul<R>(tile2, A, L, X);  // TG = T (I - R^2)^-1
+    if (inverse_ns<R, false>(Y, tile, tile2, A, L)) {  // (I - R^2)^-1
+      right_solve_tile<R>(tile, Tm, X, L);              // or TG by a solve
+    } else {
+      put<R>(tile2, L, Tm);
+      wave_sync();
+      mul<R>(tile2, A, L, X);  // TG = T (I - R^2)^-1
+    }
     wave_sync();
     put<R>(tile2, L, X);
     const double zpgy = z + gk * y;
@@ -585,16 +588,24 @@ __device__ void add16_on_top(Comp16& C, const LR& y, int l, const Lane& L,
   wave_sync();
   put<R>(s.tile, L, A);
   wave_sync();
-  inverse_ns<R>(A, s.tile, s.tile2, X, L);                // A1^-1 = (I - P1)^-1
-  put<R>(s.tile, L, B);
-  put<R>(s.tile2, L, Tt);
+  if (inverse_ns<R, false>(A, s.tile, s.tile2, X, L)) {   // A1^-1 = (I - P1)^-1
+    right_solve_tile<R>(s.tile, Tt, X, L);                // or D1 by a solve
+    put<R>(s.tile, L, B);
+  } else {
+    put<R>(s.tile, L, B);
+    put<R>(s.tile2, L, Tt);
+    wave_sync();
+    mul<R>(s.tile2, X, L, X);                             // D1 = top.T_ba A1^-1
+  }
   wave_sync();
-  mul<R>(s.tile2, X, L, X);                               // D1 = top.T_ba A1^-1
-  wave_sync();
-  inverse_ns<R>(B, s.tile, s.tile2, A, L);                // A2^-1 = (I - P2)^-1
-  put<R>(s.tile, L, C.Tab);
-  wave_sync();
-  mul<R>(s.tile, A, L, C.Tab);                            // D2 = bot.T_ab A2^-1
+  if (inverse_ns<R, false>(B, s.tile, s.tile2, A, L)) {   // A2^-1 = (I - P2)^-1
+    right_solve_tile<R>(s.tile, C.Tab, A, L);             // or D2 by a solve
+    C.Tab = A;
+  } else {
+    put<R>(s.tile, L, C.Tab);
+    wave_sync();
+    mul<R>(s.tile, A, L, C.Tab);                          // D2 = bot.T_ab A2^-1
+  }
   wave_sync();
   if (L.h == 0) {
     s.v2[j] = rhs_up;
@@ -706,16 +717,24 @@ __device__ void add16_below(Comp16& C, const LR& y, int l, const Lane& L,
   wave_sync();
   put<R>(s.tile, L, A);
   wave_sync();
-  inverse_ns<R>(A, s.tile, s.tile2, X, L);                // A1^-1
-  put<R>(s.tile, L, B);
-  put<R>(s.tile2, L, C.Tba);
+  if (inverse_ns<R, false>(A, s.tile, s.tile2, X, L)) {   // A1^-1
+    right_solve_tile<R>(s.tile, C.Tba, X, L);             // or D1 by a solve
+    C.Tba = X;
+    put<R>(s.tile, L, B);
+  } else {
+    put<R>(s.tile, L, B);
+    put<R>(s.tile2, L, C.Tba);
+    wave_sync();
+    mul<R>(s.tile2, X, L, C.Tba);                         // D1 = top.T_ba A1^-1
+  }
   wave_sync();
-  mul<R>(s.tile2, X, L, C.Tba);                           // D1 = top.T_ba A1^-1
-  wave_sync();
-  inverse_ns<R>(B, s.tile, s.tile2, A, L);                // A2^-1
-  put<R>(s.tile, L, Tb);
-  wave_sync();
-  mul<R>(s.tile, A, L, X);                                // D2 = bot.T_ab A2^-1
+  if (inverse_ns<R, false>(B, s.tile, s.tile2, A, L)) {   // A2^-1
+    right_solve_tile<R>(s.tile, Tb, X, L);                // or D2 by a solve
+  } else {
+    put<R>(s.tile, L, Tb);
+    wave_sync();
+    mul<R>(s.tile, A, L, X);                              // D2 = bot.T_ab A2^-1
+  }
   wave_sync();
   if (L.h == 0) {
     s.v2[j] = rhs_up;
@@ -940,17 +959,23 @@ adding16_kernel(FwdArgs a) {
       }
       wave_sync();
       Mat<R> X;
-      inverse_ns<R>(M, s.tile, s.tile2, X, L);  // M_up^-1
-      put<R>(s.tile, L, X);
-      wave_sync();
-      Iup = matvec<R>(s.tile, s.v0, L);
+      if (inverse_ns<R, false>(M, s.tile, s.tile2, X, L)) {  // M_up^-1
+        Iup = left_solve_tile<R>(s.tile, b_up, L);            // or I_up by a solve
+      } else {
+        put<R>(s.tile, L, X);
+        wave_sync();
+        Iup = matvec<R>(s.tile, s.v0, L);
+      }
       wave_sync();
       put<R>(s.tile, L, M2);
       wave_sync();
-      inverse_ns<R>(M2, s.tile, s.tile2, X, L);  // M_dn^-1
-      put<R>(s.tile, L, X);
-      wave_sync();
-      Idn = matvec<R>(s.tile, s.v1, L);
+      if (inverse_ns<R, false>(M2, s.tile, s.tile2, X, L)) {  // M_dn^-1
+        Idn = left_solve_tile<R>(s.tile, b_dn, L);             // or I_dn by a solve
+      } else {
+        put<R>(s.tile, L, X);
+        wave_sync();
+        Idn = matvec<R>(s.tile, s.v1, L);
+      }
       wave_sync();
     } else {
       put<R>(s.tile, L, C.Tab);

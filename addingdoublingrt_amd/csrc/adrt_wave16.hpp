@@ -326,8 +326,12 @@ constexpr int NS_MAX_STEPS = 8;
 // once; where the start is poor (e > 1/2: (I - P) close to singular) or the
 // iteration has not finished in NS_MAX_STEPS steps, Gauss-Jordan with partial
 // pivoting (right_solve) solves X (I - P) = I instead.  Every branch is
-// wave-uniform.  Returns true if Gauss-Jordan ran.
-template <typename R>
+// wave-uniform.  Returns true if Gauss-Jordan ran.  With GJ = false the
+// fallback is left to the caller: the function returns true at once, X unset,
+// so that a caller with a right-hand side can solve for it directly (a solve
+// keeps the error of an ill-conditioned I - P at that of an LU solve; the
+// product with a computed inverse loses up to cond(I - P) more).
+template <typename R, bool GJ = true>
 __device__ __forceinline__ bool inverse_ns(const Mat<R>& P, const R* tP, R* tX, Mat<R>& X,
                                            const Lane& L) {
   Mat<R> E;
@@ -337,6 +341,7 @@ __device__ __forceinline__ bool inverse_ns(const Mat<R>& P, const R* tP, R* tX, 
   float e = residual_size<R>(E);
   for (int it = 0; e > NS_DONE; ++it) {
     if (!(e <= 0.5f) || it == NS_MAX_STEPS) {
+      if constexpr (!GJ) return true;
       // P again from its tile, so no register holds it through the iteration
       Mat<R> A;
       get<R>(tP, L, A);
@@ -380,6 +385,22 @@ __device__ __forceinline__ bool inverse_ns(const Mat<R>& P, const R* tP, R* tX, 
   }
   wave_sync();
   return false;
+}
+
+// D = T (I - P)^-1 for P in the tile tP, by Gauss-Jordan with partial pivoting
+// on D (I - P) = T (the fallback of inverse_ns<R, false>).
+template <typename R>
+__device__ __forceinline__ void right_solve_tile(const R* tP, const Mat<R>& T, Mat<R>& D,
+                                                 const Lane& L) {
+  Mat<R> A;
+  get<R>(tP, L, A);
+  wave_sync();
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    A.v[r] = ((row<R>(L.h, r) == L.j) ? R(1) : R(0)) - A.v[r];
+    D.v[r] = T.v[r];
+  }
+  right_solve<R>(A, D, L);
 }
 
 // Q = P (I - P)^-1 = (I - P)^-1 - I, for P in Q on entry and in the tile tP
@@ -427,15 +448,10 @@ __device__ __forceinline__ void q_form_ns(Mat<R>& Q, const R* tP, R* tX, const L
   wave_sync();
 }
 
-// x = M^-1 b: M by registers, b element j (all lanes of column j); returns x_j
+// x = M^-1 b for A = M^T in registers (destroyed), b element j (all lanes of
+// column j); returns x_j.  x^T M^T = b^T is a right solve with A = M^T.
 template <typename R>
-__device__ __forceinline__ R left_solve(const Mat<R>& M, R b, const Lane& L, R* tile) {
-  // x^T M^T = b^T: right solve with A = M^T (lane j needs column j of M^T)
-  put<R>(tile, L, M);
-  wave_sync();
-  Mat<R> A;
-  get_t<R>(tile, L, A);
-  wave_sync();
+__device__ __forceinline__ R left_solve_t(Mat<R>& A, R b, const Lane& L) {
   bool used = false;
   int src = 0;
   R t0 = b;
@@ -460,6 +476,30 @@ __device__ __forceinline__ R left_solve(const Mat<R>& M, R b, const Lane& L, R* 
     (step(std::integral_constant<int, K>{}), ...);
   }(std::make_integer_sequence<int, N>{});
   return shfl(t0, src + 16 * L.h);
+}
+
+// x = M^-1 b: M by registers (lane j needs column j of M^T, so M goes through
+// the tile)
+template <typename R>
+__device__ __forceinline__ R left_solve(const Mat<R>& M, R b, const Lane& L, R* tile) {
+  put<R>(tile, L, M);
+  wave_sync();
+  Mat<R> A;
+  get_t<R>(tile, L, A);
+  wave_sync();
+  return left_solve_t<R>(A, b, L);
+}
+
+// x = (I - P)^-1 b for P in the tile tP, by Gauss-Jordan (the fallback of
+// inverse_ns<R, false> for one right-hand side)
+template <typename R>
+__device__ __forceinline__ R left_solve_tile(const R* tP, R b, const Lane& L) {
+  Mat<R> A;
+  get_t<R>(tP, L, A);
+  wave_sync();
+#pragma unroll
+  for (int r = 0; r < 4; ++r) A.v[r] = ((row<R>(L.h, r) == L.j) ? R(1) : R(0)) - A.v[r];
+  return left_solve_t<R>(A, b, L);
 }
 
 // ---- reverse-pass building blocks -------------------------------------------

@@ -162,6 +162,57 @@ def solve_batch(delta_tau, single_scat_albedo, phase_moments, config: BatchConfi
     return out
 
 
+def solve_batch_from_coefficients(absorption_coeff, scattering_coeff, altitude, temperature,
+                                  wavenumber, phase_moments, config: BatchConfig,
+                                  cloud_optical_depth=None, surface_temperature=-1.0,
+                                  top_temperature=-1.0, precision="f64", out=None,
+                                  max_workspace_bytes=0):
+    """Solve from raw retrieval coefficients (``adrt_solve_batch_from_coefficients``).
+
+    Mirrors ``adrt::cuda::solveBatchFromCoefficients``
+    (src_cuda/cuda_solver.cuh:104-179): absorption_coeff / scattering_coeff
+    ``[L+1][W]`` and altitude / temperature ``[L+1]`` bottom of the atmosphere
+    first, wavenumber ``[W]``, cloud_optical_depth ``[L][W]`` (BOA first) or
+    None, phase_moments ``[L][M]`` or ``[W][L][M]`` (TOA first).  Layer optical
+    depths use the trapezoid rule, thermal sources the single-wavenumber
+    Planck function; a negative surface / top temperature means "use the
+    bottom / top level".  Returns the five ``[W][L+1]`` outputs, TOA first.
+    ``config.surface_emission``, ``top_emission`` and ``index_from_bottom`` are
+    not used.
+    """
+    W = wavenumber.shape[0]
+    L = altitude.shape[0] - 1
+    shared = phase_moments.dim() == 2
+    M = phase_moments.shape[-1]
+    cfg = _lib.BatchConfig(
+        num_wavenumbers=W, num_layers=L, num_quadrature=config.num_quadrature, num_moments=M,
+        use_delta_m=int(config.use_delta_m),
+        use_diffusion_lower_bc=int(config.use_diffusion_lower_bc), index_from_bottom=0,
+        surface_albedo=config.surface_albedo, surface_emission=0.0, top_emission=0.0,
+        solar_flux=config.solar_flux, solar_mu=config.solar_mu,
+        spectrum_scaling=config.spectrum_scaling)
+    coef = _lib.DeviceCoefficients(
+        absorption_coeff=_dev_ptr(absorption_coeff, "absorption_coeff", (L + 1, W)),
+        scattering_coeff=_dev_ptr(scattering_coeff, "scattering_coeff", (L + 1, W)),
+        altitude=_dev_ptr(altitude, "altitude", (L + 1,)),
+        temperature=_dev_ptr(temperature, "temperature", (L + 1,)),
+        wavenumber=_dev_ptr(wavenumber, "wavenumber", (W,)),
+        phase_moments=_dev_ptr(phase_moments, "phase_moments", (L, M) if shared else (W, L, M)),
+        phase_moments_shared=int(shared),
+        cloud_optical_depth=_dev_ptr(cloud_optical_depth, "cloud_optical_depth", (L, W)))
+    if out is None:
+        out = {k: torch.empty((W, L + 1), dtype=torch.float64, device=wavenumber.device)
+               for k in OUTPUT_NAMES}
+    o = _outputs_struct(out, W, L)
+    ws = get_workspace(W, L, config.num_quadrature, M, max_workspace_bytes,
+                       precision == "mixed")
+    stream = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    _lib.check(_lib.lib().adrt_solve_batch_from_coefficients(
+        C.byref(cfg), C.byref(coef), float(surface_temperature), float(top_temperature),
+        C.byref(o), _PREC[precision], ws.handle, stream))
+    return out
+
+
 GRADIENT_NAMES = ("delta_tau", "single_scat_albedo", "phase_moments", "planck_levels",
                   "surface_emission", "top_emission", "surface_albedo")
 

@@ -11,6 +11,9 @@
  *   adrt_solve_batch        <- adrt::cuda::solveBatch        src_cuda/cuda_solver.cuh:77-80
  *                              (FP64, every interface, not only TOA) and the
  *                              per-column adrt::solve        src/adding_doubling.hpp:258-269
+ *   adrt_solve_batch_from_coefficients, adrt_device_coefficients
+ *                           <- adrt::cuda::solveBatchFromCoefficients, RawDeviceData
+ *                                                            src_cuda/cuda_solver.cuh:104-179
  *   adrt_batch_config       <- adrt::cuda::BatchConfig       src_cuda/cuda_solver.cuh:18-40
  *   adrt_device_inputs/outputs <- adrt::cuda::DeviceData     src_cuda/cuda_solver.cuh:45-68
  *   adrt_workspace_*        <- adrt::cuda::SolverWorkspaceGPU src_cuda/cuda_solver.cuh:140-156
@@ -175,6 +178,39 @@ int adrt_adjoint_batch(const adrt_batch_config* cfg, const adrt_device_inputs* i
  * pass since the last call needed more doubling steps than the checkpoint slab
  * holds (its gradients were written as zero), else ADRT_OK.  Clears the flag. */
 int adrt_adjoint_status(adrt_workspace* ws, hipStream_t stream);
+
+/* Raw retrieval inputs (the BeAR format), caller-owned device memory, FP64.
+ * Coefficient arrays are level-major with the bottom of the atmosphere first,
+ * array[level * num_wavenumbers + w]; the solver takes layer l (TOA = 0) from
+ * levels L-1-l and L-l.  Replaces adrt::cuda::RawDeviceData
+ * (src_cuda/cuda_solver.cuh:104-134). */
+typedef struct adrt_device_coefficients {
+  const double* absorption_coeff;     /* [L+1][W] (cm^-1), BOA first */
+  const double* scattering_coeff;     /* [L+1][W] (cm^-1), BOA first */
+  const double* altitude;             /* [L+1] (cm), BOA first */
+  const double* temperature;          /* [L+1] (K), BOA first */
+  const double* wavenumber;           /* [W] (cm^-1) */
+  const double* phase_moments;        /* [L][M] if shared, else [W][L][M]; layers TOA first */
+  int phase_moments_shared;
+  const double* cloud_optical_depth;  /* [L][W] layer-major, BOA first, or NULL */
+} adrt_device_coefficients;
+
+/* Solve from raw coefficients (adrt::cuda::solveBatchFromCoefficients,
+ * src_cuda/cuda_solver.cuh:164-179, src_cuda/cuda_solver.cu:131-221,1131-1353):
+ * per wavenumber and layer, tau = dz (ext_bot + ext_top) / 2 + cloud tau
+ * (clamped at 0) with ext = absorption + scattering, omega = dz (scat_bot +
+ * scat_top) / 2 / tau (0 where tau = 0); level Planck values are the
+ * single-wavenumber B(T, nu) = c1 nu^3 / (exp(c2 nu / T) - 1) (0 below 1e-5 K);
+ * the surface emits B(surface_temperature) if surface_temperature >= 0, else
+ * B at the bottom level, and the top boundary receives B(top_temperature) if
+ * top_temperature >= 0, else B at the top level.  cfg->surface_emission,
+ * top_emission and index_from_bottom are not used: outputs are [W][L+1] at
+ * every interface, TOA first.  Asynchronous on stream. */
+int adrt_solve_batch_from_coefficients(const adrt_batch_config* cfg,
+                                       const adrt_device_coefficients* coef,
+                                       double surface_temperature, double top_temperature,
+                                       const adrt_device_outputs* out, adrt_precision precision,
+                                       adrt_workspace* ws, hipStream_t stream);
 
 /* Checks delta_tau >= 0 and single_scat_albedo in [0,1] on the device and
  * returns ADRT_ERR_NEGATIVE_TAU / ADRT_ERR_SSA_RANGE (synchronises the stream). */

@@ -670,6 +670,100 @@ inline void solveBatch(const BatchConfig& config, const DeviceData& data,
   column(data.flux_direct, o[3], surf);
 }
 
+/// Raw retrieval inputs (src_cuda/cuda_solver.cuh:104-134), FP64: coefficient
+/// arrays level-major array[level * nwav + wav] with the bottom of the
+/// atmosphere at index 0; phase moments TOA first as in DeviceData.
+struct RawDeviceData {
+  const double* absorption_coeff = nullptr;     ///< [nlev * nwav] (cm^-1)
+  const double* scattering_coeff = nullptr;     ///< [nlev * nwav] (cm^-1)
+  const double* altitude = nullptr;             ///< [nlev] (cm)
+  const double* temperature = nullptr;          ///< [nlev] (K)
+  const double* wavenumber = nullptr;           ///< [nwav] (cm^-1)
+  const double* phase_moments = nullptr;        ///< [nlay * nmom] or [nwav * nlay * nmom]
+  bool phase_moments_shared = false;
+  const double* cloud_optical_depth = nullptr;  ///< [nlay * nwav] (BOA first) or nullptr
+
+  bool all_interfaces = false;  ///< outputs [nwav * nlev] (TOA first) instead of TOA [nwav]
+  double* flux_up = nullptr;
+  double* flux_down = nullptr;
+  double* flux_direct = nullptr;      ///< at the surface (TOA mode) or every interface
+  double* mean_intensity = nullptr;   ///< all_interfaces only
+  double* flux_divergence = nullptr;  ///< all_interfaces only
+};
+
+/// Persistent workspace of the raw path (src_cuda/cuda_solver.cuh:140-156):
+/// the library workspace (which also holds the per-wavenumber optical depths,
+/// albedos and Planck values) and the all-interface outputs of TOA mode.
+struct SolverWorkspaceGPU {
+  adrt::detail::Workspace ws;
+  adrt::detail::DeviceBuffer out[4];
+};
+
+/// Solve from raw coefficients (src_cuda/cuda_solver.cuh:164-179).  Optical
+/// depths by the trapezoid rule, single-wavenumber Planck values, surface /
+/// top emission from config.surface_temperature / top_temperature when >= 0
+/// (else the bottom / top level), for every quadrature order alike; the
+/// band-Planck fields, use_thermal_emission, surface_emission, top_emission and
+/// index_from_bottom of config are not used.  Throws std::invalid_argument for
+/// invalid configurations.
+inline void solveBatchFromCoefficients(const BatchConfig& config, const RawDeviceData& data,
+                                       SolverWorkspaceGPU& w, hipStream_t stream = 0) {
+  using adrt::detail::check;
+  using adrt::detail::hip_check;
+  const int W = config.num_wavenumbers, L = config.num_layers, nif = L + 1;
+  if (W < 0) check(ADRT_ERR_NUM_WAVENUMBERS);
+  adrt_batch_config c{};
+  c.num_wavenumbers = W;
+  c.num_layers = L;
+  c.num_quadrature = config.num_quadrature;
+  c.num_moments = config.num_moments_max;
+  c.use_delta_m = config.use_delta_m;
+  c.use_diffusion_lower_bc = config.use_diffusion_lower_bc;
+  c.surface_albedo = config.surface_albedo;
+  c.solar_flux = config.solar_flux;
+  c.solar_mu = config.solar_mu;
+  c.spectrum_scaling = config.spectrum_scaling;
+  if (W == 0) return;
+  adrt_workspace* ws =
+      w.ws.get(c, config.precision == ADRT_PRECISION_MIXED ? ADRT_WORKSPACE_ADJOINT : 0);
+  const adrt_device_coefficients k{data.absorption_coeff, data.scattering_coeff, data.altitude,
+                                   data.temperature,      data.wavenumber,        data.phase_moments,
+                                   data.phase_moments_shared, data.cloud_optical_depth};
+  double* o[4];
+  for (int q = 0; q < 4; ++q) o[q] = w.out[q].get((std::size_t)W * nif);
+  if (data.all_interfaces) {
+    adrt_device_outputs out{data.flux_up ? data.flux_up : o[0],
+                            data.flux_down ? data.flux_down : o[1],
+                            data.mean_intensity ? data.mean_intensity : o[2],
+                            data.flux_direct ? data.flux_direct : o[3], data.flux_divergence};
+    check(adrt_solve_batch_from_coefficients(&c, &k, config.surface_temperature,
+                                             config.top_temperature, &out, config.precision, ws,
+                                             stream));
+    return;
+  }
+  adrt_device_outputs out{o[0], o[1], o[2], o[3], nullptr};
+  check(adrt_solve_batch_from_coefficients(&c, &k, config.surface_temperature,
+                                           config.top_temperature, &out, config.precision, ws,
+                                           stream));
+  auto column = [&](double* dst, const double* src, int level) {
+    if (dst)
+      hip_check(hipMemcpy2DAsync(dst, sizeof(double), src + level, nif * sizeof(double),
+                                 sizeof(double), W, hipMemcpyDeviceToDevice, stream),
+                "hipMemcpy2DAsync");
+  };
+  column(data.flux_up, o[0], 0);
+  column(data.flux_down, o[1], 0);
+  column(data.flux_direct, o[3], L);
+}
+
+/// As above with a per-thread workspace (the reference's N <= 8 overload,
+/// src_cuda/cuda_solver.cuh:156-162, allocates nothing per call either).
+inline void solveBatchFromCoefficients(const BatchConfig& config, const RawDeviceData& data,
+                                       hipStream_t stream = 0) {
+  thread_local SolverWorkspaceGPU w;
+  solveBatchFromCoefficients(config, data, w, stream);
+}
+
 struct HostResult {
   std::vector<double> flux_up;
   std::vector<double> flux_down;

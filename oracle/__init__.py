@@ -308,3 +308,64 @@ def solve_batch(delta_tau, single_scat_albedo, phase_moments, num_quadrature,
     if rc:
         raise OracleError(rc, err.value.decode())
     return out
+
+
+# ---- raw retrieval coefficients (numpy restatement) -------------------------
+
+PLANCK_SINGLE_C1 = 1.19105e-08   # 2 h c^2, scaled to W m^-2 cm (src_cuda/cuda_planck_single.cuh:27)
+PLANCK_SINGLE_C2 = 1.43879       # h c / k in cm K (src_cuda/cuda_planck_single.cuh:28)
+
+
+def planck_single(temperature, wavenumber):
+    """Single-wavenumber Planck function B(T, nu) in W m^-2 cm
+    (src_cuda/cuda_planck_single.cuh:18-34): c1 nu^3 / (exp(c2 nu / T) - 1),
+    0 for T < 1e-5 K.  Broadcasts."""
+    T = np.asarray(temperature, dtype=np.float64)
+    nu = np.asarray(wavenumber, dtype=np.float64)
+    cold = T < 1e-5
+    Ts = np.where(cold, 1.0, T)
+    with np.errstate(over="ignore"):
+        B = PLANCK_SINGLE_C1 * (nu * nu * nu) / np.expm1(PLANCK_SINGLE_C2 * nu / Ts)
+    return np.where(cold, 0.0, B)
+
+
+def raw_optics(absorption_coeff, scattering_coeff, altitude, temperature, wavenumber,
+               cloud_optical_depth=None, surface_temperature=-1.0, top_temperature=-1.0):
+    """Solver inputs from raw retrieval coefficients, as the reference's
+    preprocessRawKernel (src_cuda/cuda_solver.cu:1131-1182) with the boundary
+    rules of its inline raw path (src_cuda/cuda_solver.cu:220-241,428-441).
+
+    absorption_coeff / scattering_coeff ``[L+1][W]`` and altitude / temperature
+    ``[L+1]`` are bottom of the atmosphere first; cloud_optical_depth ``[L][W]``
+    (BOA first) or None.  Layer b (BOA first) spans levels b and b+1:
+    tau = dz (ext_top + ext_bot) / 2 (+ cloud), clamped at 0, ext = abs + sca;
+    omega = dz (sca_top + sca_bot) / 2 / tau (0 where tau = 0).  Returns
+    TOA-first ``delta_tau`` / ``single_scat_albedo`` ``[W][L]``,
+    ``planck_levels`` ``[W][L+1]`` and ``surface_emission`` / ``top_emission``
+    ``[W]`` (B at surface_temperature / top_temperature when >= 0, else at the
+    bottom / top level).
+    """
+    a = np.asarray(absorption_coeff, dtype=np.float64)
+    s = np.asarray(scattering_coeff, dtype=np.float64)
+    z = np.asarray(altitude, dtype=np.float64)
+    T = np.asarray(temperature, dtype=np.float64)
+    nu = np.asarray(wavenumber, dtype=np.float64)
+    dz = (z[1:] - z[:-1])[:, None]
+    ext = a + s
+    tau = dz * (ext[1:] + ext[:-1]) * 0.5
+    sdepth = dz * (s[1:] + s[:-1]) * 0.5
+    if cloud_optical_depth is not None:
+        tau = tau + np.asarray(cloud_optical_depth, dtype=np.float64)
+    tau = np.maximum(tau, 0.0)
+    ssa = np.zeros_like(tau)
+    np.divide(sdepth, tau, out=ssa, where=tau > 0.0)
+    L = len(z) - 1
+    Ts = surface_temperature if surface_temperature >= 0.0 else T[0]
+    Tt = top_temperature if top_temperature >= 0.0 else T[L]
+    return {
+        "delta_tau": np.ascontiguousarray(tau[::-1].T),
+        "single_scat_albedo": np.ascontiguousarray(ssa[::-1].T),
+        "planck_levels": np.ascontiguousarray(planck_single(T[::-1][None, :], nu[:, None])),
+        "surface_emission": planck_single(Ts, nu),
+        "top_emission": planck_single(Tt, nu),
+    }

@@ -1,7 +1,8 @@
 // Test driver of the C++ host layer (include/adrt_hip.hpp): reads one case
-// from stdin, solves it on the GPU through adrt::solve or
-// adrt::hip::solveBatch, prints the outputs as "name v0 v1 ..." lines, or
-// "error <message>" when the configuration is rejected.
+// from stdin, solves it on the GPU through adrt::solve, adrt::hip::solveBatch
+// or adrt::hip::solveBatchFromCoefficients, prints the outputs as
+// "name v0 v1 ..." lines, or "error <message>" when the configuration is
+// rejected.
 //
 //   column: L N thermal delta_m diffusion from_bottom jacobian
 //           albedo Bs Ts Bt Tt F0 mu0 wn_lo wn_hi
@@ -9,6 +10,9 @@
 //   batch:  W L N M shared thermal delta_m diffusion from_bottom all_interfaces
 //           albedo Bs Ts Bt Tt F0 mu0 wn_lo wn_hi scaling
 //           tau[W][L] ssa[W][L] chi[(W)][L][M] nT T[nT] nB B[nB]
+//   raw:    W L N M shared delta_m diffusion all_interfaces cloud
+//           albedo Ts Tt F0 mu0
+//           abs[L+1][W] sca[L+1][W] z[L+1] T[L+1] nu[W] chi[(W)][L][M] (cloud[L][W])
 #include "adrt_hip.hpp"
 
 #include <cstdio>
@@ -150,6 +154,62 @@ int batch() {
   return 0;
 }
 
+int raw() {
+  adrt::hip::BatchConfig c;
+  int shared, dm, diff, all, cloud;
+  std::cin >> c.num_wavenumbers >> c.num_layers >> c.num_quadrature >> c.num_moments_max >>
+      shared >> dm >> diff >> all >> cloud;
+  c.use_delta_m = dm;
+  c.use_diffusion_lower_bc = diff;
+  std::cin >> c.surface_albedo >> c.surface_temperature >> c.top_temperature >> c.solar_flux >>
+      c.solar_mu;
+  const std::size_t W = c.num_wavenumbers, L = c.num_layers, M = c.num_moments_max;
+  const auto a = read_n((L + 1) * W), s = read_n((L + 1) * W);
+  const auto z = read_n(L + 1), T = read_n(L + 1), nu = read_n(W);
+  const auto mom = read_n((shared ? 1 : W) * L * M);
+  const auto cl = cloud ? read_n(L * W) : std::vector<double>();
+  adrt::hip::RawDeviceData d;
+  d.absorption_coeff = to_device(a);
+  d.scattering_coeff = to_device(s);
+  d.altitude = to_device(z);
+  d.temperature = to_device(T);
+  d.wavenumber = to_device(nu);
+  d.phase_moments = to_device(mom);
+  d.phase_moments_shared = shared;
+  d.cloud_optical_depth = to_device(cl);
+  d.all_interfaces = all;
+  const std::size_t n = all ? W * (L + 1) : W;
+  std::vector<double> zero(n, 0.0);
+  d.flux_up = to_device(zero);
+  d.flux_down = to_device(zero);
+  d.flux_direct = to_device(zero);
+  if (all) {
+    d.mean_intensity = to_device(zero);
+    d.flux_divergence = to_device(zero);
+  }
+  try {
+    adrt::hip::solveBatchFromCoefficients(c, d, 0);
+  } catch (const std::invalid_argument& e) {
+    std::printf("error %s\n", e.what());
+    return 0;
+  }
+  adrt::detail::hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+  auto get = [&](const double* p) {
+    std::vector<double> v(n);
+    adrt::detail::hip_check(hipMemcpy(v.data(), p, n * sizeof(double), hipMemcpyDeviceToHost),
+                            "hipMemcpy");
+    return v;
+  };
+  print("flux_up", get(d.flux_up));
+  print("flux_down", get(d.flux_down));
+  print("flux_direct", get(d.flux_direct));
+  if (all) {
+    print("mean_intensity", get(d.mean_intensity));
+    print("flux_divergence", get(d.flux_divergence));
+  }
+  return 0;
+}
+
 }  // namespace
 
 int main() {
@@ -158,6 +218,7 @@ int main() {
   try {
     if (mode == "column") return column();
     if (mode == "batch") return batch();
+    if (mode == "raw") return raw();
   } catch (const std::exception& e) {
     std::printf("failure %s\n", e.what());
     return 2;

@@ -11,7 +11,9 @@ tests/cpp/adrt_cpp_check, against the CPU oracle.
   the oracle's forward-mode computeTemperatureJacobian (src/solver.cpp:117-341)
   to 1e-8 relative, for temperature-driven and Planck-level-driven columns;
 * adrt::hip::solveBatch with temperatures on the device (band Planck in the
-  library) in TOA mode and all-interfaces mode against the oracle per column.
+  library) in TOA mode and all-interfaces mode against the oracle per column;
+* adrt::hip::solveBatchFromCoefficients (raw coefficients) in both modes
+  against oracle.raw_optics + the oracle solve.
 """
 import os
 import subprocess
@@ -184,3 +186,42 @@ def test_batch_with_device_planck(all_interfaces):
                 atol = 1e-10 * 4 * np.pi * (np.abs(ref["mean_intensity"]) + Bl)
             bad = np.abs(g - r) > 1e-10 * np.abs(r) + atol
             assert not bad.any(), (w, k, g[bad], r[bad])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("all_interfaces", [False, True])
+def test_batch_from_coefficients(all_interfaces):
+    rng = np.random.default_rng(13)
+    W, L, N, M = 7, 5, 8, 16
+    z = np.concatenate([[0.0], np.cumsum(rng.uniform(1e5, 6e5, L))])
+    # omega >= 0.02 everywhere: no unstable doubling start (the C-ABI raw path
+    # is held to the oracle in that regime by tests/test_gpu_raw.py)
+    a = 10 ** rng.uniform(-9, -5.5, (L + 1, W))
+    s = a * rng.uniform(0.05, 1.0, (L + 1, W))
+    T = np.linspace(295.0, 200.0, L + 1)
+    nu = np.linspace(500.0, 1800.0, W)
+    mom = 0.6 ** np.arange(M)[None, :].repeat(L, 0)
+    cloud = np.zeros((L, W))
+    cloud[1] = rng.uniform(0, 1e-3, W)
+    Ts, Tt = 301.0, -1.0
+    text = (f"raw\n{W} {L} {N} {M} 1 0 0 {int(all_interfaces)} 1\n"
+            f"0.2 {Ts!r} {Tt!r} 1.0 0.6\n{_fmt(a)}\n{_fmt(s)}\n{_fmt(z)}\n{_fmt(T)}\n"
+            f"{_fmt(nu)}\n{_fmt(mom)}\n{_fmt(cloud)}\n")
+    got = run_driver(text)
+    assert "error" not in got, got
+    r = oracle.raw_optics(a, s, z, T, nu, cloud_optical_depth=cloud, surface_temperature=Ts,
+                          top_temperature=Tt)
+    ref = oracle.solve_batch(r["delta_tau"], r["single_scat_albedo"], mom, N,
+                             planck_levels=r["planck_levels"],
+                             surface_emission=r["surface_emission"],
+                             top_emission=r["top_emission"], surface_albedo=0.2, solar_flux=1.0,
+                             solar_mu=0.6)
+    for k in ("flux_up", "flux_down", "flux_direct") + (
+            ("mean_intensity", "flux_divergence") if all_interfaces else ()):
+        g = got[k].reshape(W, -1)
+        rk = ref[k] if all_interfaces else ref[k][:, [L if k == "flux_direct" else 0]]
+        atol = 1e-13 * np.abs(rk).max()
+        if k == "flux_divergence":  # 4 pi (1 - omega) (J - B): J ~ B cancels
+            atol = 1e-10 * 4 * np.pi * (np.abs(ref["mean_intensity"]) + r["planck_levels"])
+        bad = np.abs(g - rk) > 1e-10 * np.abs(rk) + atol
+        assert not bad.any(), (k, g[bad], rk[bad])
