@@ -11,6 +11,8 @@
  *                                      <- src/adding_doubling.hpp:277-296, src/planck.hpp
  *   adrt::hip::BatchConfig, DeviceData, solveBatch, solveBatchHost
  *                                      <- adrt::cuda::* of src_cuda/cuda_solver.cuh:18-98
+ *   adrt::hip::RawDeviceData, SolverWorkspaceGPU, solveBatchFromCoefficients
+ *                                      <- src_cuda/cuda_solver.cuh:104-179
  *
  * Invalid configurations throw std::invalid_argument with the reference's
  * messages; HIP failures throw std::runtime_error.  Header only: include it
@@ -19,8 +21,8 @@
  *
  * Differences from the reference, all deliberate:
  *  - solve() runs the column on the current HIP device in FP64 (a batch of
- *    one).  Temperature Jacobians come from batched reverse passes (one
- *    column per output row) and need num_quadrature <= 16.
+ *    one).  Temperature Jacobians come from one batched solve of L+3 copies
+ *    of the column with unit Planck sources.
  *  - hip::DeviceData is FP64.  Its outputs are TOA values per wavenumber
  *    (flux_direct: at the surface) as in the reference, unless
  *    all_interfaces is set, in which case they are [nwav][nlay+1] and
@@ -300,8 +302,8 @@ class SolverWorkspace {
     if (!stream_) detail::hip_check(hipStreamCreate(&stream_), "hipStreamCreate");
     return stream_;
   }
-  detail::Workspace forward, reverse;
-  detail::DeviceBuffer in[6], out[5], cot[4], grad[3];
+  detail::Workspace forward, jacobian;
+  detail::DeviceBuffer in[6], out[5];
 
  private:
   hipStream_t stream_ = nullptr;
@@ -376,64 +378,58 @@ inline Column column_of(const ADConfig& c) {
 
 }  // namespace detail
 
-/// Temperature (or Planck-level) Jacobians by reverse passes: one copy of the
-/// column per output row, each with the unit cotangent of that row
-/// (the forward-mode computeTemperatureJacobian of src/solver.cpp:117-341
-/// gives the same matrices).
+/// Temperature (or Planck-level) Jacobians, as computeTemperatureJacobian
+/// (src/solver.cpp:117-341).  R and T do not depend on temperature and every
+/// output is affine in the Planck values, so the derivative of the outputs by
+/// level value B_m is the solve with B = e_m and no beam (the reference runs
+/// the same linear recursion in forward mode); one batch holds the L+1 level
+/// columns and the surface- and top-emission columns.  Any quadrature order.
 inline void temperature_jacobians(const ADConfig& c, const detail::Column& k, RTOutput& r,
                                   SolverWorkspace& ws) {
-  if (c.num_quadrature > 16)
-    throw std::invalid_argument(
-        "compute_temperature_jacobian: the reverse pass supports num_quadrature <= 16");
   const int L = c.num_layers, nif = L + 1, ndof = L + 2;
   const int top = c.index_from_bottom ? L : 0, bottom = c.index_from_bottom ? 0 : L;
   adrt_batch_config cfg = k.cfg;
-  const int W = 4 * nif;  // output a, interface i -> column a * nif + i
+  const int W = nif + 2;  // column m <= L: B = e_m; L+1: surface emission 1; L+2: top 1
   cfg.num_wavenumbers = W;
+  cfg.solar_flux = 0.0;
+  cfg.surface_emission = 0.0;
+  cfg.top_emission = 0.0;
+  cfg.spectrum_scaling = 1.0;
   hipStream_t s = ws.stream();
-  adrt_workspace* aw = ws.reverse.get(cfg, ADRT_WORKSPACE_ADJOINT);
+  adrt_workspace* jw = ws.jacobian.get(cfg, 0);
   std::vector<double> rep;
   auto replicate = [&](const std::vector<double>& v) {
     rep.resize(v.size() * W);
     for (int w = 0; w < W; ++w) std::copy(v.begin(), v.end(), rep.begin() + (std::size_t)w * v.size());
     return rep;
   };
+  std::vector<double> unitB((std::size_t)W * nif, 0.0), unitS(W, 0.0), unitT(W, 0.0);
+  for (int m = 0; m < nif; ++m) unitB[(std::size_t)m * nif + m] = 1.0;
+  unitS[nif] = 1.0;
+  unitT[nif + 1] = 1.0;
   double* dtau = ws.in[0].get((std::size_t)W * L);
   double* dssa = ws.in[1].get((std::size_t)W * L);
   double* dmom = ws.in[2].get(k.mom.size());
-  double* dB = k.have_B ? ws.in[3].get((std::size_t)W * nif) : nullptr;
+  double* dB = ws.in[3].get((std::size_t)W * nif);
+  double* dS = ws.in[4].get(W);
+  double* dT = ws.in[5].get(W);
   detail::upload(dtau, replicate(k.tau), s);
   detail::upload(dssa, replicate(k.ssa), s);
   detail::upload(dmom, k.mom, s);
-  if (dB) detail::upload(dB, replicate(k.B), s);
-  adrt_device_inputs in{dtau, dssa, dmom, 1, dB, nullptr, nullptr};
+  detail::upload(dB, unitB, s);
+  detail::upload(dS, unitS, s);
+  detail::upload(dT, unitT, s);
+  adrt_device_inputs in{dtau, dssa, dmom, 1, dB, dS, dT};
   double* o[5];
   for (int q = 0; q < 5; ++q) o[q] = ws.out[q].get((std::size_t)W * nif);
   adrt_device_outputs out{o[0], o[1], o[2], o[3], o[4]};
-  detail::check(adrt_solve_batch(&cfg, &in, &out, ADRT_PRECISION_F64, aw, s));
-  unsigned long long gen = 0;
-  detail::check(adrt_workspace_generation(aw, &gen));
-  std::vector<double> e((std::size_t)W * nif, 0.0);
-  double* dc[4];
-  for (int a = 0; a < 4; ++a) {
-    std::fill(e.begin(), e.end(), 0.0);
-    for (int i = 0; i < nif; ++i) e[(std::size_t)(a * nif + i) * nif + i] = 1.0;
-    dc[a] = ws.cot[a].get(e.size());
-    detail::upload(dc[a], e, s);
-    detail::hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
-  }
-  adrt_device_cotangents cot{dc[0], dc[1], dc[2], nullptr, dc[3]};
-  double* gB = ws.grad[0].get((std::size_t)W * nif);
-  double* gS = ws.grad[1].get(W);
-  double* gT = ws.grad[2].get(W);
-  adrt_device_gradients g{nullptr, nullptr, nullptr, gB, gS, gT, nullptr};
-  detail::check(adrt_adjoint_batch(&cfg, &in, &out, &cot, &g, ADRT_PRECISION_F64, gen, aw, s));
-  detail::check(adrt_adjoint_status(aw, s));
-  std::vector<double> hB, hS, hT;
-  detail::download(hB, gB, (std::size_t)W * nif, s);
-  detail::download(hS, gS, W, s);
-  detail::download(hT, gT, W, s);
+  detail::check(adrt_solve_batch(&cfg, &in, &out, ADRT_PRECISION_F64, jw, s));
+  std::vector<double> h[4];
+  const int oq[4] = {0, 1, 2, 4};  // flux_up, flux_down, mean_intensity, flux_divergence
+  for (int a = 0; a < 4; ++a) detail::download(h[a], o[oq[a]], (std::size_t)W * nif, s);
   detail::hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+  // d(output a at interface i) / d(B_m): column m, interface i
+  auto d = [&](int a, int i, int col) { return h[a][(std::size_t)col * nif + i]; };
   // Planck chain rule (src/solver.cpp:307-327)
   std::vector<double> chain(ndof, 1.0);
   if (c.use_thermal_emission) {
@@ -454,15 +450,14 @@ inline void temperature_jacobians(const ADConfig& c, const detail::Column& k, RT
   for (int a = 0; a < 4; ++a) {
     J[a]->assign(nif, std::vector<double>(ndof, 0.0));
     for (int i = 0; i < nif; ++i) {
-      const int w = a * nif + i;
       std::vector<double>& row = (*J[a])[i];
       if (k.have_B)
-        for (int m = 0; m <= L; ++m) row[m] = hB[(std::size_t)w * nif + m];
-      if (top_dep) row[top] += hT[w];
+        for (int m = 0; m <= L; ++m) row[m] = d(a, i, m);
+      if (top_dep) row[top] += d(a, i, nif + 1);
       if (k.surface_independent)
-        row[L + 1] = hS[w];
+        row[L + 1] = d(a, i, nif);
       else if (c.use_thermal_emission)
-        row[bottom] += hS[w];
+        row[bottom] += d(a, i, nif);
       for (int m = 0; m < ndof; ++m) row[m] *= chain[m];
     }
   }

@@ -7,7 +7,7 @@ tests/cpp/adrt_cpp_check, against the CPU oracle.
 * adrt::solve (one column on the GPU, FP64) at the parity bar of the batched
   path: within max(1e-10, 4 x the FP64 oracle's own error) of the
   extended-precision oracle;
-* the temperature Jacobians of adrt::solve (batched reverse passes) against
+* the temperature Jacobians of adrt::solve (batched unit-source solves) against
   the oracle's forward-mode computeTemperatureJacobian (src/solver.cpp:117-341)
   to 1e-8 relative, for temperature-driven and Planck-level-driven columns;
 * adrt::hip::solveBatch with temperatures on the device (band Planck in the
@@ -123,6 +123,12 @@ def _cases():
                                  T=T, lo=700.0, hi=720.0, diffusion=True, jac=True),
         "n32_solar": dict(L=L, N=32, tau=tau, ssa=ssa, chi=_hg(0.85, 64, L), F0=1.0, mu0=0.5,
                           albedo=0.25),
+        "n32_thermal_jacobian": dict(L=L, N=32, tau=tau, ssa=ssa, chi=_hg(0.7, 64, L),
+                                     thermal=True, T=T, lo=650.0, hi=700.0, albedo=0.2,
+                                     Ts=300.0, F0=1.0, mu0=0.5, jac=True),
+        "n6_padded_jacobian": dict(L=L, N=6, tau=tau, ssa=ssa, chi=_hg(0.5, 12, L),
+                                   thermal=True, T=T, lo=800.0, hi=850.0, albedo=0.05,
+                                   from_bottom=True, jac=True),
     }
 
 
