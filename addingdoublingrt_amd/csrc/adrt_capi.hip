@@ -20,6 +20,7 @@
 #include "adrt_adj16.hpp"
 #include "adrt_mix16.hpp"
 #include "adrt_adjm16.hpp"
+#include "adrt_wave32.hpp"
 
 namespace {
 
@@ -369,7 +370,11 @@ int launch_chunk(const adrt::dev::FwdArgs& a, adrt_workspace* ws, hipStream_t st
   } else {
     if (mixed) return ADRT_ERR_PRECISION;
     e = prof_begin(ws, 1, stream);
-    layer_kernel<N><<<(unsigned)((tasks + GPB - 1) / GPB), 256, 0, stream>>>(a);
+    if constexpr (N == 32)  // one wave per problem on the matrix cores
+      w32::layer32_kernel<<<(unsigned)((tasks + w32::WAVES - 1) / w32::WAVES), 64 * w32::WAVES,
+                            0, stream>>>(a);
+    else
+      layer_kernel<N><<<(unsigned)((tasks + GPB - 1) / GPB), 256, 0, stream>>>(a);
     prof_end(ws, e, stream);
     e = prof_begin(ws, 2, stream);
     adding_kernel<N><<<(unsigned)((a.W + GPB - 1) / GPB), 256, 0, stream>>>(a);

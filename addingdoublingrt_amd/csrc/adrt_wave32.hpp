@@ -1,0 +1,579 @@
+// N = 32 doubling on the matrix cores: one wave per (wavenumber, layer).
+//
+// A 32 x 32 matrix is four 16 x 16 blocks in the register layout of the N = 16
+// path (adrt_wave16.hpp): block b = I + 2J holds rows 16I.., columns 16J..,
+// and lane l = j + 16h holds entries (16I + h + 4r, 16J + j), r = 0..3, of
+// every block.  A product C = X Y reads X from a padded (stride 33)
+// column-major LDS tile and Y from registers: each output block is eight
+// v_mfma_f64_16x16x4 (K = 32), 32 per product.  Vectors of 32 are held as the
+// pair (element j, element 16 + j) in every lane of column j.
+//
+// The layer kernel restates the doubling of src/doubling.hpp:29-230 and the
+// phase matrices of src/phase_matrix.hpp:61-181 as the column-slice layer
+// kernel does (adrt_fwd.hpp), and writes the same records, so the adding
+// kernels read them unchanged.  (I - R^2)^-1 is a Newton-Schulz inverse from
+// the Neumann start; where that start is poor, Gauss-Jordan with partial
+// pivoting solves TG (I - R^2) = T in LDS, one augmented column per lane.
+#pragma once
+
+#include "adrt_wave16.hpp"
+
+namespace adrt {
+namespace dev {
+namespace w32 {
+
+using w16::d4;
+using w16::fence_sched;
+using w16::Lane;
+using w16::mfma;
+using w16::NS_CUBIC;
+using w16::NS_DONE;
+using w16::NS_MAX_STEPS;
+using w16::NS_NEWTON;
+using w16::readlane;
+using w16::row_max;
+using w16::row_sum;
+using w16::shfl;
+using w16::wave_index;
+
+constexpr int N = 32;
+constexpr int TS = 33;          // LDS tile column stride
+constexpr int TILE = N * TS;    // doubles per tile
+constexpr int WAVES = 2;        // waves per 128-thread block
+constexpr int SLOT = 2 * TILE + 4 * N;  // LDS doubles per wave
+
+struct Mat {  // this lane's 16 entries: v[b][r] = M(16 (b & 1) + h + 4r, 16 (b >> 1) + j)
+  double v[4][4];
+};
+struct Vec {  // elements j and 16 + j
+  double lo, hi;
+};
+
+__device__ __forceinline__ int rw(int h, int r) { return h + 4 * r; }
+__device__ __forceinline__ int bi(int b) { return 16 * (b & 1); }
+__device__ __forceinline__ int bj(int b) { return 16 * (b >> 1); }
+__device__ __forceinline__ double delta(const Lane& L, int b, int r) {
+  return (bi(b) + rw(L.h, r) == bj(b) + L.j) ? 1.0 : 0.0;
+}
+
+__device__ __forceinline__ void put(double* t, const Lane& L, const Mat& m) {
+#pragma unroll
+  for (int b = 0; b < 4; ++b)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) t[(bj(b) + L.j) * TS + bi(b) + rw(L.h, r)] = m.v[b][r];
+}
+__device__ __forceinline__ void get(const double* t, const Lane& L, Mat& m) {
+#pragma unroll
+  for (int b = 0; b < 4; ++b)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) m.v[b][r] = t[(bj(b) + L.j) * TS + bi(b) + rw(L.h, r)];
+}
+// the transpose of m into the tile (column-major M^T)
+__device__ __forceinline__ void put_t(double* t, const Lane& L, const Mat& m) {
+#pragma unroll
+  for (int b = 0; b < 4; ++b)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) t[(bi(b) + rw(L.h, r)) * TS + bj(b) + L.j] = m.v[b][r];
+}
+// the transpose of the tile's matrix
+__device__ __forceinline__ void get_t(const double* t, const Lane& L, Mat& m) {
+#pragma unroll
+  for (int b = 0; b < 4; ++b)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) m.v[b][r] = t[(bi(b) + rw(L.h, r)) * TS + bj(b) + L.j];
+}
+
+// C = Z + X Y (ADD) or X Y, X in a tile, Y and Z in registers; C may alias Y or Z
+template <bool ADD>
+__device__ __forceinline__ void mul_impl(const double* X, const Mat& Y, const Mat& Z,
+                                         const Lane& L, Mat& C) {
+#pragma unroll
+  for (int J = 0; J < 2; ++J) {
+    fence_sched();
+    d4 acc[2];
+#pragma unroll
+    for (int I = 0; I < 2; ++I) {
+      const int b = I + 2 * J;
+      acc[I] = ADD ? d4{Z.v[b][0], Z.v[b][1], Z.v[b][2], Z.v[b][3]} : d4{0.0, 0.0, 0.0, 0.0};
+    }
+#pragma unroll
+    for (int K = 0; K < 2; ++K)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const double y = Y.v[K + 2 * J][r];
+#pragma unroll
+        for (int I = 0; I < 2; ++I)
+          acc[I] = mfma(X[(16 * K + rw(L.h, r)) * TS + 16 * I + L.j], y, acc[I]);
+      }
+#pragma unroll
+    for (int I = 0; I < 2; ++I)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) C.v[I + 2 * J][r] = acc[I][r];
+    fence_sched();
+  }
+}
+__device__ __forceinline__ void mul(const double* X, const Mat& Y, const Lane& L, Mat& C) {
+  mul_impl<false>(X, Y, Y, L, C);
+}
+__device__ __forceinline__ void mul_add(const double* X, const Mat& Y, const Mat& Z,
+                                        const Lane& L, Mat& C) {
+  mul_impl<true>(X, Y, Z, L, C);
+}
+
+// (X v) for NV vectors stored in LDS slots of 32 values, v + 32 q
+template <int NV>
+__device__ __forceinline__ void matvecs(const double* X, const double* v, const Lane& L,
+                                        Vec (&s)[NV]) {
+  fence_sched();
+#pragma unroll
+  for (int q = 0; q < NV; ++q) s[q] = Vec{0.0, 0.0};
+#pragma unroll 4
+  for (int k = 0; k < N; ++k) {
+    const double xl = X[k * TS + L.j], xh = X[k * TS + 16 + L.j];
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+      const double vk = v[q * N + k];
+      s[q].lo = fma_(xl, vk, s[q].lo);
+      s[q].hi = fma_(xh, vk, s[q].hi);
+    }
+  }
+  fence_sched();
+}
+
+__device__ __forceinline__ void put_vec(double* slot, const Lane& L, const Vec& x) {
+  if (L.h == 0) {
+    slot[L.j] = x.lo;
+    slot[16 + L.j] = x.hi;
+  }
+}
+
+// N max |E_ij| over the wave, wave-uniform
+__device__ __forceinline__ float residual_size(const Mat& E) {
+  float m = 0.0f;
+#pragma unroll
+  for (int b = 0; b < 4; ++b)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) m = __builtin_fmaxf(m, __builtin_fabsf((float)E.v[b][r]));
+  m = row_max<float>(m);
+  m = __builtin_fmaxf(__builtin_fmaxf(readlane(m, 0), readlane(m, 16)),
+                      __builtin_fmaxf(readlane(m, 32), readlane(m, 48)));
+  return m * (float)N;
+}
+
+// X = (I - P)^-1 by Newton-Schulz for P in registers and in the tile tP (tX:
+// scratch tile), the iteration of the N = 16 path (inverse_ns): X0 = I + P,
+// E0 = P^2, X <- X + X E (or X + X E + X E E once N max|E| <= 1e-5),
+// E = I - X + P X.  Returns true, X unset, where the start is poor
+// (N max|E0| > 1/2) or the iteration has not finished in 8 steps: the caller
+// then solves with its right-hand side.
+__device__ __forceinline__ bool inverse_ns(const Mat& P, const double* tP, double* tX, Mat& X,
+                                           const Lane& L) {
+  Mat E;
+  mul(tP, P, L, E);
+#pragma unroll
+  for (int b = 0; b < 4; ++b)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) X.v[b][r] = delta(L, b, r) + P.v[b][r];
+  float e = residual_size(E);
+  for (int it = 0; e > NS_DONE; ++it) {
+    if (!(e <= 0.5f) || it == NS_MAX_STEPS) return true;
+    put(tX, L, X);
+    wave_sync();
+    if (e <= NS_NEWTON) {
+      mul_add(tX, E, X, L, X);
+      wave_sync();
+      break;
+    }
+    Mat XE;
+    mul(tX, E, L, XE);
+    wave_sync();
+    if (e <= NS_CUBIC) {
+      put(tX, L, XE);
+      wave_sync();
+#pragma unroll
+      for (int b = 0; b < 4; ++b)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) X.v[b][r] += XE.v[b][r];
+      mul_add(tX, E, X, L, X);
+      wave_sync();
+      break;
+    }
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) X.v[b][r] += XE.v[b][r];
+    Mat PX;
+    mul(tP, X, L, PX);
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) E.v[b][r] = (delta(L, b, r) - X.v[b][r]) + PX.v[b][r];
+    e = residual_size(E);
+  }
+  wave_sync();
+  return false;
+}
+
+// Gauss-Jordan with partial pivoting on the 32 x 64 augmented system
+// [A^T | T^T], A^T column-major in tA and T^T in tB; lane c owns augmented
+// column c.  The pivot of step k is the first largest |A^T(i, k)|, i >= k (the
+// pivots of the reference's LU of A^T, src/matrix.hpp:120-124).  On return tB
+// holds X^T with X A = T.
+__device__ __forceinline__ void gauss_jordan(double* tA, double* tB, const Lane& L) {
+  double* col = (L.l < 32 ? tA : tB) + (L.l & 31) * TS;
+  for (int k = 0; k < N; ++k) {
+    double v = (L.l < 32 && L.l >= k) ? __builtin_fabs(tA[k * TS + L.l]) : -1.0;
+    int idx = L.l;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const double ov = shfl(v, L.l ^ off);
+      const int oi = __builtin_amdgcn_ds_bpermute((L.l ^ off) << 2, idx);
+      if (ov > v || (ov == v && oi < idx)) {
+        v = ov;
+        idx = oi;
+      }
+    }
+    const int p = __builtin_amdgcn_readfirstlane(idx);
+    if (p != k) {
+      const double ak = col[k], ap = col[p];
+      col[k] = ap;
+      col[p] = ak;
+    }
+    wave_sync();
+    const double d = tA[k * TS + k];
+    const double rk = col[k] / d;
+    col[k] = rk;
+    wave_sync();
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      double f[8];
+#pragma unroll
+      for (int t = 0; t < 8; ++t) f[t] = tA[k * TS + 8 * q + t];
+      wave_sync();
+#pragma unroll
+      for (int t = 0; t < 8; ++t) {
+        const int i = 8 * q + t;
+        if (i != k) col[i] = fma_(-f[t], rk, col[i]);
+      }
+      wave_sync();
+    }
+  }
+}
+
+// D = T (I - P)^-1 for P in the tile t1 (t2: scratch), by Gauss-Jordan
+__device__ __forceinline__ void right_solve_tile(double* t1, double* t2, const Mat& T, Mat& D,
+                                                 const Lane& L) {
+  Mat A;
+  get(t1, L, A);
+#pragma unroll
+  for (int b = 0; b < 4; ++b)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) A.v[b][r] = delta(L, b, r) - A.v[b][r];
+  wave_sync();
+  put_t(t1, L, A);
+  put_t(t2, L, T);
+  wave_sync();
+  gauss_jordan(t1, t2, L);
+  wave_sync();
+  get_t(t2, L, D);
+  wave_sync();
+}
+
+// ---------------------------------------------------------------------------
+// layer: phase matrices + doubling (src/phase_matrix.hpp:61-181,
+// src/doubling.hpp:29-230); one wave per (wavenumber, layer)
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(64 * WAVES) layer32_kernel(FwdArgs a) {
+  __shared__ double smem[WAVES * SLOT];
+  const int wv = wave_index();
+  const long task = (long)blockIdx.x * WAVES + wv;
+  const int Lay = a.L;
+  if (task >= (long)a.W * Lay) return;
+  const Lane L;
+  const int j = L.j, h = L.h;
+  const int w = (int)(task / Lay), l = (int)(task % Lay);
+  const long gw = (long)(a.w0 + w);
+  double* sm = smem + wv * SLOT;
+  double* tile = sm;
+  double* tile2 = sm + TILE;
+  double* v0 = sm + 2 * TILE;
+
+  constexpr int REC = 2 * N * N + 2 * N;
+  double* out = a.lay + task * REC;
+  auto store = [&](double* m, const Mat& x) {  // column-major record
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) m[(bj(b) + j) * N + bi(b) + rw(h, r)] = x.v[b][r];
+  };
+
+  const double tau = a.tau_s[task];
+  const double omega_in = a.omega_s[task];
+  const int mode = a.mode[task];
+  const double Btop = a.B ? a.B[gw * (Lay + 1) + lev(a, l)] : 0.0;
+  const double Bbot = a.B ? a.B[gw * (Lay + 1) + lev(a, l + 1)] : 0.0;
+  const double B_bar = (Bbot + Btop) / 2.0;
+  const double B_d = (tau > 0.0) ? (Bbot - Btop) / tau : 0.0;
+  const double mu_c[2] = {a.mu[j], a.mu[16 + j]};
+  const double wt_c[2] = {a.wt[j], a.wt[16 + j]};
+
+  if (tau <= 0.0) {  // transparent layer (src/doubling.hpp:52-53)
+    Mat Z, I1;
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        Z.v[b][r] = 0.0;
+        I1.v[b][r] = delta(L, b, r);
+      }
+    store(out, Z);
+    store(out + N * N, I1);
+    if (h == 0)
+#pragma unroll
+      for (int J = 0; J < 2; ++J) {
+        out[2 * N * N + 16 * J + j] = 0.0;
+        out[2 * N * N + N + 16 * J + j] = 0.0;
+      }
+    if (j == 0 && h == 0) a.lflag[task] = 0;
+    return;
+  }
+  if (omega_in <= 0.0) {  // pure absorption (src/doubling.hpp:56-77)
+    double trans[2], omt[2], slope[2];
+#pragma unroll
+    for (int J = 0; J < 2; ++J) {
+      const double tex = -tau / mu_c[J];
+      trans[J] = (tex > -200.0) ? exp(tex) : 0.0;
+      omt[J] = 1.0 - trans[J];
+      slope[J] = mu_c[J] * omt[J] - 0.5 * tau * (1.0 + trans[J]);
+    }
+    Mat Z, Tr;
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        Z.v[b][r] = 0.0;
+        Tr.v[b][r] = delta(L, b, r) * trans[b >> 1];
+      }
+    store(out, Z);
+    store(out + N * N, Tr);
+    if (h == 0)
+#pragma unroll
+      for (int J = 0; J < 2; ++J) {
+        out[2 * N * N + 16 * J + j] = B_bar * omt[J] + B_d * slope[J];
+        out[2 * N * N + N + 16 * J + j] = B_bar * omt[J] - B_d * slope[J];
+      }
+    if (j == 0 && h == 0) a.lflag[task] = 0;
+    return;
+  }
+
+  // ---- scattering layer ----
+  const double omega = omega_in > 1.0 ? 1.0 : omega_in;
+  const double con = 2.0 * omega * kPi;
+  const int li = lin(a, l);
+  const double* chi = a.shared_moments ? a.mom + (long)li * a.M : a.mom + (gw * Lay + li) * a.M;
+  const int twoM = 2 * a.n_real;
+  int Lm = a.M;
+  double f = 0.0, inv1mf = 1.0;
+  if (mode == 3) {
+    f = a.fdm[task];
+    inv1mf = 1.0 / (1.0 - f);
+    Lm = twoM;
+  } else if (mode == 2) {
+    Lm = a.M < twoM ? a.M : twoM;
+  }
+
+  // phase matrices (src/phase_matrix.hpp:73-91): P(i, c) = sum_l c_l P_l(mu_i) P_l(+-mu_c)
+  Mat Ppp, Ppm;
+#pragma unroll
+  for (int b = 0; b < 4; ++b)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      Ppp.v[b][r] = 0.0;
+      Ppm.v[b][r] = 0.0;
+    }
+  double sign = 1.0;
+  double sp[2] = {0.0, 0.0}, smv[2] = {0.0, 0.0};  // solar phase vectors, elements j, 16 + j
+  for (int ll = 0; ll < Lm; ++ll) {
+    const double chil = (mode == 3) ? (chi[ll] - f) * inv1mf : chi[ll];
+    const double cl = (2 * ll + 1) * chil;
+    const double* Pl = a.Pl + ll * N;
+    const double plc[2] = {Pl[j], Pl[16 + j]};
+#pragma unroll
+    for (int I = 0; I < 2; ++I)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const double pi = cl * Pl[16 * I + rw(h, r)];
+#pragma unroll
+        for (int J = 0; J < 2; ++J) {
+          const double term = pi * plc[J];
+          Ppp.v[I + 2 * J][r] += term;
+          Ppm.v[I + 2 * J][r] += sign * term;
+        }
+      }
+    if (a.has_solar) {
+#pragma unroll
+      for (int J = 0; J < 2; ++J) {
+        const double term = cl * plc[J] * a.Pl0[ll];
+        sp[J] += term;
+        smv[J] += sign * term;
+      }
+    }
+    sign = -sign;
+  }
+  {
+    // Hansen column normalisation (src/phase_matrix.hpp:93-110)
+    double part[2] = {0.0, 0.0};
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        Ppp.v[b][r] = Ppp.v[b][r] / (2.0 * kPi);
+        Ppm.v[b][r] = Ppm.v[b][r] / (2.0 * kPi);
+        part[b >> 1] += (Ppp.v[b][r] + Ppm.v[b][r]) * a.wt[bi(b) + rw(h, r)];
+      }
+#pragma unroll
+    for (int J = 0; J < 2; ++J) {
+      const double csum = w16::col_reduce<double>(part[J], L);
+      if (csum > 0.0) {
+        const double corr = 1.0 / (2.0 * kPi * csum);
+#pragma unroll
+        for (int I = 0; I < 2; ++I)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            Ppp.v[I + 2 * J][r] *= corr;
+            Ppm.v[I + 2 * J][r] *= corr;
+          }
+      }
+    }
+  }
+  double pplus[2] = {0.0, 0.0}, pminus[2] = {0.0, 0.0};
+  if (a.has_solar) {  // src/phase_matrix.hpp:153-180
+#pragma unroll
+    for (int J = 0; J < 2; ++J) {
+      pplus[J] = sp[J] / (2.0 * kPi);
+      pminus[J] = smv[J] / (2.0 * kPi);
+    }
+    const double s = row_sum<double>((pplus[0] + pminus[0]) * wt_c[0] +
+                                     (pplus[1] + pminus[1]) * wt_c[1]);
+    if (s > 0.0) {
+      const double corr = 1.0 / (2.0 * kPi * s);
+#pragma unroll
+      for (int J = 0; J < 2; ++J) {
+        pplus[J] *= corr;
+        pminus[J] *= corr;
+      }
+    }
+  }
+
+  // thin-layer initialisation (src/doubling.hpp:83-147)
+  const int nn0 = (int)(log(tau) / log(2.0)) + ipow0(omega);
+  const int nn = nn0 < 1 ? 1 : nn0;
+  const double tau0 = tau * ldexp(1.0, -nn);
+  Mat Rm, Tm;
+#pragma unroll
+  for (int b = 0; b < 4; ++b)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const double mui = a.mu[bi(b) + rw(h, r)];
+      const double wc = wt_c[b >> 1];
+      const double d = delta(L, b, r);
+      const double gpp = (d + (-con) * (Ppp.v[b][r] * wc)) / mui;
+      const double gpm = con * (Ppm.v[b][r] * wc) / mui;
+      Tm.v[b][r] = d - tau0 * gpp;
+      Rm.v[b][r] = tau0 * gpm;
+    }
+  Vec y{(1.0 - omega) * tau0 / mu_c[0], (1.0 - omega) * tau0 / mu_c[1]};
+  Vec z{0.0, 0.0};
+  Vec sus{0.0, 0.0}, sds{0.0, 0.0};
+  double gam = 0.0;
+  if (a.has_solar) {
+    const double Ftop = a.F0 * exp(-a.tau_cum[(long)w * (Lay + 1) + l] / a.mu0);
+    const double b0 = omega * tau0 / mu_c[0] * Ftop, b1 = omega * tau0 / mu_c[1] * Ftop;
+    sus = Vec{b0 * pminus[0], b1 * pminus[1]};
+    sds = Vec{b0 * pplus[0], b1 * pplus[1]};
+    gam = exp(-tau0 / a.mu0);
+  }
+  double gk = 0.5 * tau0;
+
+  double* v1 = v0 + N;
+  double* v2 = v1 + N;
+  double* v3 = v2 + N;
+
+  // doubling recursion (src/doubling.hpp:149-209)
+  for (int k = 0; k < nn; ++k) {
+    put(tile, L, Rm);
+    put_vec(v0, L, sds);
+    put_vec(v1, L, sus);
+    wave_sync();
+    Mat P;
+    mul(tile, Rm, L, P);  // R^2
+    Vec b12[2] = {Vec{0.0, 0.0}, Vec{0.0, 0.0}};
+    if (a.has_solar) matvecs<2>(tile, v0, L, b12);  // R s_down_sol, R s_up_sol
+    wave_sync();
+    put(tile, L, P);
+    wave_sync();
+    Mat X;
+    if (inverse_ns(P, tile, tile2, X, L)) {  // (I - R^2)^-1 ...
+      right_solve_tile(tile, tile2, Tm, X, L);  // ... or TG by a solve
+    } else {
+      put(tile2, L, Tm);
+      wave_sync();
+      mul(tile2, X, L, X);  // TG = T (I - R^2)^-1
+    }
+    wave_sync();
+    put(tile2, L, X);
+    const Vec zpgy{z.lo + gk * y.lo, z.hi + gk * y.hi};
+    put_vec(v0, L, zpgy);
+    put_vec(v1, L, y);
+    put_vec(v2, L, Vec{b12[0].lo + gam * sus.lo, b12[0].hi + gam * sus.hi});
+    put_vec(v3, L, Vec{gam * b12[1].lo + sds.lo, gam * b12[1].hi + sds.hi});
+    wave_sync();
+    mul(tile2, Rm, L, X);  // TGR = TG R
+    mul(tile2, Tm, L, P);  // T_new = TG T
+    Vec s4[4];
+    if (a.has_solar) {
+      matvecs<4>(tile2, v0, L, s4);
+    } else {
+      Vec s2[2];
+      matvecs<2>(tile2, v0, L, s2);
+      s4[0] = s2[0];
+      s4[1] = s2[1];
+      s4[2] = Vec{0.0, 0.0};
+      s4[3] = Vec{0.0, 0.0};
+    }
+    wave_sync();
+    put(tile, L, X);
+    wave_sync();
+    mul_add(tile, Tm, Rm, L, Rm);  // R_new = R + TGR T
+    Vec s2[2];
+    matvecs<2>(tile, v0, L, s2);  // TGR (z + g y), TGR y
+    wave_sync();
+    Tm = P;
+    const Vec znew{(s4[0].lo - s2[0].lo) + z.lo - gk * y.lo,
+                   (s4[0].hi - s2[0].hi) + z.hi - gk * y.hi};
+    const Vec ynew{s4[1].lo + s2[1].lo + y.lo, s4[1].hi + s2[1].hi + y.hi};
+    z = znew;
+    y = ynew;
+    if (a.has_solar) {
+      sus = Vec{s4[2].lo + sus.lo, s4[2].hi + sus.hi};
+      sds = Vec{s4[3].lo + gam * sds.lo, s4[3].hi + gam * sds.hi};
+      gam = gam * gam;
+    }
+    gk = 2.0 * gk;
+  }
+
+  // layer assembly (src/doubling.hpp:211-229), sources combined
+  store(out, Rm);
+  store(out + N * N, Tm);
+  if (h == 0) {
+    out[2 * N * N + j] = (y.lo * B_bar + z.lo * B_d) + sus.lo;
+    out[2 * N * N + 16 + j] = (y.hi * B_bar + z.hi * B_d) + sus.hi;
+    out[2 * N * N + N + j] = (y.lo * B_bar - z.lo * B_d) + sds.lo;
+    out[2 * N * N + N + 16 + j] = (y.hi * B_bar - z.hi * B_d) + sds.hi;
+  }
+  if (j == 0 && h == 0) a.lflag[task] = 1;
+}
+
+}  // namespace w32
+}  // namespace dev
+}  // namespace adrt
