@@ -377,7 +377,11 @@ int launch_chunk(const adrt::dev::FwdArgs& a, adrt_workspace* ws, hipStream_t st
       layer_kernel<N><<<(unsigned)((tasks + GPB - 1) / GPB), 256, 0, stream>>>(a);
     prof_end(ws, e, stream);
     e = prof_begin(ws, 2, stream);
-    adding_kernel<N><<<(unsigned)((a.W + GPB - 1) / GPB), 256, 0, stream>>>(a);
+    if constexpr (N == 32)
+      w32::adding32_kernel<<<(unsigned)((a.W + w32::WAVES - 1) / w32::WAVES), 64 * w32::WAVES, 0,
+                             stream>>>(a);
+    else
+      adding_kernel<N><<<(unsigned)((a.W + GPB - 1) / GPB), 256, 0, stream>>>(a);
     prof_end(ws, e, stream);
   }
   return hipGetLastError() == hipSuccess ? ADRT_OK : ADRT_ERR_HIP;

@@ -1,4 +1,5 @@
-// N = 32 doubling on the matrix cores: one wave per (wavenumber, layer).
+// N = 32 on the matrix cores: the doubling with one wave per (wavenumber,
+// layer), the adding sweeps with one wave per wavenumber.
 //
 // A 32 x 32 matrix is four 16 x 16 blocks in the register layout of the N = 16
 // path (adrt_wave16.hpp): block b = I + 2J holds rows 16I.., columns 16J..,
@@ -10,10 +11,12 @@
 //
 // The layer kernel restates the doubling of src/doubling.hpp:29-230 and the
 // phase matrices of src/phase_matrix.hpp:61-181 as the column-slice layer
-// kernel does (adrt_fwd.hpp), and writes the same records, so the adding
-// kernels read them unchanged.  (I - R^2)^-1 is a Newton-Schulz inverse from
-// the Neumann start; where that start is poor, Gauss-Jordan with partial
-// pivoting solves TG (I - R^2) = T in LDS, one augmented column per lane.
+// kernel does (adrt_fwd.hpp) and writes the same records; the adding kernel
+// restates src/adding.hpp:69-258 and src/solver.cpp:516-700 as the
+// column-slice adding kernel does.  Every (I - P)^-1 is a Newton-Schulz
+// inverse from the Neumann start; where that start is poor, Gauss-Jordan with
+// partial pivoting solves against the right-hand side in LDS, one augmented
+// column per lane.
 #pragma once
 
 #include "adrt_wave16.hpp"
@@ -572,6 +575,550 @@ __global__ void __launch_bounds__(64 * WAVES) layer32_kernel(FwdArgs a) {
     out[2 * N * N + N + 16 + j] = (y.hi * B_bar - z.hi * B_d) + sds.hi;
   }
   if (j == 0 && h == 0) a.lflag[task] = 1;
+}
+
+// ---------------------------------------------------------------------------
+// adding sweeps + interfaces (src/adding.hpp:69-258, src/solver.cpp:516-700),
+// one wave per wavenumber; the two inverses of each general add and of each
+// interface by inverse_ns, with a solve against the right-hand side where
+// Newton-Schulz does not apply
+// ---------------------------------------------------------------------------
+
+// x = (I - P)^-1 b for P in the tile t1 (t2: scratch), b in the LDS slot vb,
+// by Gauss-Jordan: x^T (I - P)^T = b^T is X A = T with A^T = I - P and b in
+// the first column of T^T
+__device__ __forceinline__ Vec left_solve_tile(double* t1, double* t2, const double* vb,
+                                               const Lane& L) {
+  Mat A;
+  get(t1, L, A);
+  const double b = (L.l < 32) ? vb[L.l] : 0.0;
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) A.v[q][r] = delta(L, q, r) - A.v[q][r];
+  wave_sync();
+  put(t1, L, A);
+  if (L.l < 32) t2[L.l] = b;  // column 0 of T^T; the other columns are not read back
+  wave_sync();
+  gauss_jordan(t1, t2, L);
+  wave_sync();
+  const Vec x{t2[L.j], t2[16 + L.j]};
+  wave_sync();
+  return x;
+}
+
+// running composite: R_ab, R_ba, T_ab, T_ba; s_up, s_down
+struct Comp {
+  Mat Rab, Rba, Tab, Tba;
+  Vec su, sd;
+  int scat;
+};
+
+struct Ws {
+  double *tile, *tile2, *v0, *v1, *v2, *v3;
+};
+
+// column-major 32 x 32 in global memory
+__device__ __forceinline__ void gload(const double* g, const Lane& L, Mat& m) {
+#pragma unroll
+  for (int b = 0; b < 4; ++b)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) m.v[b][r] = g[(bj(b) + L.j) * N + bi(b) + rw(L.h, r)];
+}
+__device__ __forceinline__ void gstore(double* g, const Lane& L, const Mat& m) {
+#pragma unroll
+  for (int b = 0; b < 4; ++b)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) g[(bj(b) + L.j) * N + bi(b) + rw(L.h, r)] = m.v[b][r];
+}
+__device__ __forceinline__ Vec gvec(const double* g, const Lane& L) {
+  return Vec{g[L.j], g[16 + L.j]};
+}
+// diagonal M(c, c) for c = j, 16 + j
+__device__ __forceinline__ Vec diag(const Mat& m, const Lane& L) {
+  w16::Mat<double> b0, b3;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    b0.v[r] = m.v[0][r];
+    b3.v[r] = m.v[3][r];
+  }
+  return Vec{w16::diag<double>(b0, L), w16::diag<double>(b3, L)};
+}
+// this lane's rows of a vector in an LDS slot: t[b][r] = v(16 (b & 1) + h + 4r)
+__device__ __forceinline__ void rows_of(const double* v, const Lane& L, double (&t)[2][4]) {
+#pragma unroll
+  for (int I = 0; I < 2; ++I)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) t[I][r] = v[16 * I + rw(L.h, r)];
+}
+__device__ __forceinline__ double col_of(const Vec& v, int b) { return (b >> 1) ? v.hi : v.lo; }
+__device__ __forceinline__ Vec vadd(const Vec& a, const Vec& b) {
+  return Vec{a.lo + b.lo, a.hi + b.hi};
+}
+__device__ __forceinline__ Vec vmul(const Vec& a, const Vec& b) {
+  return Vec{a.lo * b.lo, a.hi * b.hi};
+}
+__device__ __forceinline__ Vec matvec(const double* X, const double* v, const Lane& L) {
+  Vec s[1];
+  matvecs<1>(X, v, L, s);
+  return s[0];
+}
+
+// D = T (I - P)^-1, P in registers and in tile s.tile, T in registers
+__device__ __forceinline__ void right_inverse(const Mat& P, const Mat& T, Mat& D, const Ws& s,
+                                              const Lane& L) {
+  if (inverse_ns(P, s.tile, s.tile2, D, L)) {
+    right_solve_tile(s.tile, s.tile2, T, D, L);
+  } else {
+    put(s.tile2, L, T);
+    wave_sync();
+    mul(s.tile2, D, L, D);
+  }
+  wave_sync();
+}
+
+// (I - P)^-1 b, P in registers and in tile s.tile, b in the LDS slot vb
+__device__ __forceinline__ Vec left_inverse(const Mat& P, const double* vb, const Ws& s,
+                                            const Lane& L) {
+  Mat X;
+  Vec x;
+  if (inverse_ns(P, s.tile, s.tile2, X, L)) {
+    x = left_solve_tile(s.tile, s.tile2, vb, L);
+  } else {
+    put(s.tile2, L, X);
+    wave_sync();
+    x = matvec(s.tile2, vb, L);
+  }
+  wave_sync();
+  return x;
+}
+
+// rbase step: new = add(top = stored layer, bot = C)  (src/adding.hpp:246-258)
+__device__ __forceinline__ void add32_on_top(Comp& C, const double* rec, int lscat, const Lane& L,
+                             const Ws& s) {
+  const double* Rt = rec;
+  const double* Tt = rec + N * N;
+  const Vec tsu = gvec(rec + 2 * N * N, L), tsd = gvec(rec + 2 * N * N + N, L);
+  Mat A, B, X, D1;
+  if (!lscat) {
+    // NonscatTop (src/adding.hpp:117-174): t = diag(top.T_ab)
+    const Vec t{Tt[L.j * N + L.j], Tt[(16 + L.j) * N + 16 + L.j]};
+    put_vec(s.v0, L, t);
+    put_vec(s.v1, L, tsd);
+    put(s.tile, L, C.Rab);
+    put(s.tile2, L, C.Tab);
+    wave_sync();
+    const Vec rbc = matvec(s.tile, s.v1, L);   // bot.R_ab s1_down
+    const Vec tbc = matvec(s.tile2, s.v1, L);  // bot.T_ab s1_down
+    double tv[2][4];
+    rows_of(s.v0, L, tv);
+    wave_sync();
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const double ti = tv[b & 1][r], tc = col_of(t, b);
+        C.Rab.v[b][r] = ti * C.Rab.v[b][r] * tc;
+        C.Tab.v[b][r] = C.Tab.v[b][r] * tc;
+        C.Tba.v[b][r] = ti * C.Tba.v[b][r];
+      }
+    const Vec nsu = vadd(tsu, vmul(t, vadd(C.su, rbc)));
+    const Vec nsd = vadd(C.sd, tbc);
+    C.su = nsu;
+    C.sd = nsd;
+    return;  // scat stays bot's
+  }
+  if (!C.scat) {
+    // NonscatBot (src/adding.hpp:181-239): t = diag(bot.T_ab)
+    const Vec t = diag(C.Tab, L);
+    put_vec(s.v0, L, t);
+    put_vec(s.v1, L, C.su);
+    gload(Tt, L, B);
+    gload(Rt, L, A);
+    put(s.tile, L, B);
+    put(s.tile2, L, A);
+    wave_sync();
+    const Vec tba = matvec(s.tile, s.v1, L);   // top.T_ba s2_up
+    const Vec rba = matvec(s.tile2, s.v1, L);  // top.R_ba s2_up
+    double tv[2][4];
+    rows_of(s.v0, L, tv);
+    wave_sync();
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const double ti = tv[b & 1][r], tc = col_of(t, b);
+        C.Rab.v[b][r] = A.v[b][r];
+        C.Rba.v[b][r] = ti * A.v[b][r] * tc;
+        C.Tab.v[b][r] = ti * B.v[b][r];
+        C.Tba.v[b][r] = B.v[b][r] * tc;
+      }
+    const Vec nsu = vadd(tsu, tba);
+    const Vec nsd = vadd(C.sd, vmul(t, vadd(tsd, rba)));
+    C.su = nsu;
+    C.sd = nsd;
+    C.scat = 1;
+    return;
+  }
+  // General (src/adding.hpp:69-110)
+  gload(Rt, L, A);
+  put(s.tile, L, C.Rab);
+  put(s.tile2, L, A);
+  put_vec(s.v0, L, tsd);
+  put_vec(s.v1, L, C.su);
+  wave_sync();
+  mul(s.tile, A, L, X);                                  // P1 = bot.R_ab top.R_ba
+  const Vec rhs_up = vadd(C.su, matvec(s.tile, s.v0, L));  // s2_up + R_bc s1_down
+  const Vec rhs_dn = vadd(tsd, matvec(s.tile2, s.v1, L));  // s1_down + R_ba s2_up
+  wave_sync();
+  put(s.tile, L, X);
+  wave_sync();
+  gload(Tt, L, B);
+  right_inverse(X, B, D1, s, L);                         // D1 = top.T_ba A1^-1
+  gload(Rt, L, A);
+  put(s.tile2, L, A);
+  wave_sync();
+  mul(s.tile2, C.Rab, L, X);                             // P2 = top.R_ba bot.R_ab
+  wave_sync();
+  put(s.tile, L, X);
+  wave_sync();
+  right_inverse(X, C.Tab, B, s, L);                      // D2 = bot.T_ab A2^-1
+  put_vec(s.v2, L, rhs_up);
+  put_vec(s.v3, L, rhs_dn);
+  put(s.tile, L, B);   // D2
+  put(s.tile2, L, D1);  // D1
+  wave_sync();
+  const Vec nsd = vadd(C.sd, matvec(s.tile, s.v3, L));
+  const Vec nsu = vadd(tsu, matvec(s.tile2, s.v2, L));
+  gload(Rt, L, A);
+  mul(s.tile, A, L, X);                                  // D2 top.R_ba
+  gload(Tt, L, A);
+  mul(s.tile, A, L, C.Tab);                              // T_ab = D2 top.T_ab
+  mul(s.tile2, C.Rab, L, B);                             // D1 bot.R_ab
+  mul(s.tile2, C.Tba, L, D1);                            // new T_ba = D1 bot.T_ba
+  wave_sync();
+  put(s.tile, L, X);
+  put(s.tile2, L, B);
+  wave_sync();
+  mul_add(s.tile, C.Tba, C.Rba, L, C.Rba);               // R_ba = bot.R_ba + D2 R_ba T_ba
+  gload(Rt, L, B);
+  mul_add(s.tile2, A, B, L, C.Rab);                      // R_ab = top.R_ab + D1 R_ab T_ab
+  wave_sync();
+  C.Tba = D1;
+  C.su = nsu;
+  C.sd = nsd;
+  C.scat = 1;
+}
+
+// rtop step: new = add(top = C, bot = stored layer)
+__device__ __forceinline__ void add32_below(Comp& C, const double* rec, int lscat, const Lane& L,
+                            const Ws& s) {
+  const double* Rb = rec;
+  const double* Tb = rec + N * N;
+  const Vec bsu = gvec(rec + 2 * N * N, L), bsd = gvec(rec + 2 * N * N + N, L);
+  Mat A, B, X, D1;
+  if (!C.scat) {
+    // NonscatTop with top = C: t = diag(C.T_ab)
+    const Vec t = diag(C.Tab, L);
+    put_vec(s.v0, L, t);
+    put_vec(s.v1, L, C.sd);
+    gload(Rb, L, A);
+    gload(Tb, L, B);
+    put(s.tile, L, A);
+    put(s.tile2, L, B);
+    wave_sync();
+    const Vec rbc = matvec(s.tile, s.v1, L);   // bot.R_ab s1_down
+    const Vec tbc = matvec(s.tile2, s.v1, L);  // bot.T_ab s1_down
+    double tv[2][4];
+    rows_of(s.v0, L, tv);
+    wave_sync();
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const double ti = tv[b & 1][r], tc = col_of(t, b);
+        C.Rab.v[b][r] = ti * A.v[b][r] * tc;
+        C.Rba.v[b][r] = A.v[b][r];
+        C.Tab.v[b][r] = B.v[b][r] * tc;
+        C.Tba.v[b][r] = ti * B.v[b][r];
+      }
+    const Vec nsu = vadd(C.su, vmul(t, vadd(bsu, rbc)));
+    const Vec nsd = vadd(bsd, tbc);
+    C.su = nsu;
+    C.sd = nsd;
+    C.scat = lscat;
+    return;
+  }
+  if (!lscat) {
+    // NonscatBot with bot = layer: t = diag(layer T)
+    const Vec t{Tb[L.j * N + L.j], Tb[(16 + L.j) * N + 16 + L.j]};
+    put_vec(s.v0, L, t);
+    put_vec(s.v1, L, bsu);
+    put(s.tile, L, C.Tba);
+    put(s.tile2, L, C.Rba);
+    wave_sync();
+    const Vec tba = matvec(s.tile, s.v1, L);   // top.T_ba s2_up
+    const Vec rba = matvec(s.tile2, s.v1, L);  // top.R_ba s2_up
+    double tv[2][4];
+    rows_of(s.v0, L, tv);
+    wave_sync();
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const double ti = tv[b & 1][r], tc = col_of(t, b);
+        C.Rba.v[b][r] = ti * C.Rba.v[b][r] * tc;
+        C.Tab.v[b][r] = ti * C.Tab.v[b][r];
+        C.Tba.v[b][r] = C.Tba.v[b][r] * tc;
+      }
+    const Vec nsu = vadd(C.su, tba);
+    const Vec nsd = vadd(bsd, vmul(t, vadd(C.sd, rba)));
+    C.su = nsu;
+    C.sd = nsd;
+    return;
+  }
+  // General with top = C, bot = layer
+  gload(Rb, L, A);
+  put(s.tile, L, A);
+  put(s.tile2, L, C.Rba);
+  put_vec(s.v0, L, C.sd);
+  put_vec(s.v1, L, bsu);
+  wave_sync();
+  mul(s.tile, C.Rba, L, X);                               // P1 = bot.R_ab top.R_ba
+  const Vec rhs_up = vadd(bsu, matvec(s.tile, s.v0, L));   // s2_up + R_bc s1_down
+  const Vec rhs_dn = vadd(C.sd, matvec(s.tile2, s.v1, L));  // s1_down + R_ba s2_up
+  wave_sync();
+  put(s.tile, L, X);
+  wave_sync();
+  right_inverse(X, C.Tba, D1, s, L);                      // D1 = top.T_ba A1^-1
+  put(s.tile2, L, C.Rba);
+  gload(Rb, L, A);
+  wave_sync();
+  mul(s.tile2, A, L, X);                                  // P2 = top.R_ba bot.R_ab
+  wave_sync();
+  put(s.tile, L, X);
+  wave_sync();
+  gload(Tb, L, B);
+  right_inverse(X, B, A, s, L);                           // D2 = bot.T_ab A2^-1
+  put_vec(s.v2, L, rhs_up);
+  put_vec(s.v3, L, rhs_dn);
+  put(s.tile, L, D1);
+  put(s.tile2, L, A);  // D2
+  wave_sync();
+  const Vec nsu = vadd(C.su, matvec(s.tile, s.v2, L));
+  const Vec nsd = vadd(bsd, matvec(s.tile2, s.v3, L));
+  gload(Rb, L, B);
+  mul(s.tile, B, L, X);                                   // D1 bot.R_ab
+  gload(Tb, L, B);
+  mul(s.tile, B, L, D1);                                  // new T_ba = D1 bot.T_ba
+  mul(s.tile2, C.Rba, L, A);                              // D2 top.R_ba
+  wave_sync();
+  put(s.tile, L, X);
+  wave_sync();
+  mul_add(s.tile, C.Tab, C.Rab, L, C.Rab);                // R_ab = top.R_ab + D1 R_ab T_ab
+  mul(s.tile2, C.Tab, L, X);                              // new T_ab = D2 top.T_ab
+  wave_sync();
+  put(s.tile2, L, A);
+  wave_sync();
+  gload(Rb, L, A);
+  mul_add(s.tile2, B, A, L, C.Rba);                       // R_ba = bot.R_ba + D2 R_ba T_ba
+  wave_sync();
+  C.Tab = X;
+  C.Tba = D1;
+  C.su = nsu;
+  C.sd = nsd;
+  C.scat = 1;
+}
+
+__global__ void __launch_bounds__(64 * WAVES) adding32_kernel(FwdArgs a) {
+  __shared__ double smem[WAVES * SLOT];
+  const int wv = wave_index();
+  const int w = blockIdx.x * WAVES + wv;
+  if (w >= a.W) return;
+  const Lane L;
+  const int j = L.j;
+  const long gw = (long)(a.w0 + w);
+  const int Lay = a.L;
+  double* sm = smem + wv * SLOT;
+  const Ws s{sm, sm + TILE, sm + 2 * TILE, sm + 2 * TILE + N, sm + 2 * TILE + 2 * N,
+             sm + 2 * TILE + 3 * N};
+  constexpr int REC = 2 * N * N + 2 * N;
+  constexpr int RBR = 2 * N * N + N;
+  const double* lay = a.lay + (long)w * Lay * REC;
+  const int* lflag = a.lflag + (long)w * Lay;
+  double* rbw = a.rb + (long)w * Lay * RBR;
+  const double mu_c[2] = {a.mu[j], a.mu[16 + j]};
+  const double wt_c[2] = {a.wt[j], a.wt[16 + j]};
+  const double* Bw = a.B ? a.B + gw * (Lay + 1) : nullptr;
+  auto Blev = [&](int l) -> double { return Bw ? Bw[lev(a, l)] : 0.0; };
+
+  // boundary conditions (src/solver.cpp:385-419, 516-551, 585-603)
+  const double Bs = a.surf_w ? a.surf_w[gw] : a.surf_em;
+  const double Btop = a.top_w ? a.top_w[gw] : a.top_em;
+  const double A = a.albedo;
+  const int has_surface = !a.diffusion && (A > 0.0 || Bs > 0.0);
+  const int ltot = Lay + (has_surface ? 1 : 0);
+  const double tau_tot = a.tau_cum[(long)w * (Lay + 1) + Lay];
+  const Vec Itd{Btop, Btop};
+  Vec Ibu{0.0, 0.0};
+  if (a.diffusion) {
+    const double Bb = Blev(Lay);
+    const double dt = a.tau_s[(long)w * Lay + Lay - 1];
+    const double dBdt = (dt > 0.0) ? (Bb - Blev(Lay - 1)) / dt : 0.0;
+    Ibu = Vec{Bb + mu_c[0] * dBdt, Bb + mu_c[1] * dBdt};
+  } else if (!has_surface) {
+    Ibu = Vec{Bs, Bs};
+  }
+
+  Comp C;
+  // ---- bottom-up sweep (src/solver.cpp:560-572) ----
+  if (has_surface) {
+    const double rr[2] = {2.0 * A * mu_c[0] * wt_c[0] * a.xfac, 2.0 * A * mu_c[1] * wt_c[1] * a.xfac};
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        C.Rab.v[b][r] = (bi(b) + rw(L.h, r) < a.n_real) ? rr[b >> 1] : 0.0;
+        C.Rba.v[b][r] = C.Rab.v[b][r];
+        C.Tab.v[b][r] = 0.0;
+        C.Tba.v[b][r] = 0.0;
+      }
+    double ssol = 0.0;
+    if (a.has_solar && A > 0.0) ssol = (A / kPi) * a.F0 * a.mu0 * exp(-tau_tot / a.mu0);
+    const double su = (1.0 - A) * Bs + ssol;
+    C.su = Vec{su, su};
+    C.sd = Vec{0.0, 0.0};
+    C.scat = A > 0.0;
+  } else {
+    const double* rec = lay + (long)(Lay - 1) * REC;
+    gload(rec, L, C.Rab);
+    gload(rec + N * N, L, C.Tab);
+    C.Rba = C.Rab;
+    C.Tba = C.Tab;
+    C.su = gvec(rec + 2 * N * N, L);
+    C.sd = gvec(rec + 2 * N * N + N, L);
+    C.scat = lflag[Lay - 1];
+  }
+  for (int n = 1;; ++n) {  // C == rbase[n]
+    if (n < ltot) {        // keep base composites n = 1..ltot-1 for the interfaces
+      double* o = rbw + (long)(n - 1) * RBR;
+      gstore(o, L, C.Rab);
+      gstore(o + N * N, L, C.Tba);
+      if (L.h == 0) {
+        o[2 * N * N + j] = C.su.lo;
+        o[2 * N * N + 16 + j] = C.su.hi;
+      }
+    }
+    if (n == ltot) break;
+    const int k = ltot - 1 - n;
+    add32_on_top(C, lay + (long)k * REC, lflag[k], L, s);
+  }
+
+  auto emit = [&](int l, const Vec& Iup, const Vec& Idn) {
+    const double c0 = 2.0 * kPi * wt_c[0] * mu_c[0], c1 = 2.0 * kPi * wt_c[1] * mu_c[1];
+    const double fu = row_sum<double>(c0 * Iup.lo + c1 * Iup.hi);
+    const double fdn = row_sum<double>(c0 * Idn.lo + c1 * Idn.hi);
+    double J = row_sum<double>(0.5 * wt_c[0] * (Iup.lo + Idn.lo) +
+                               0.5 * wt_c[1] * (Iup.hi + Idn.hi));
+    if (j == 0 && L.h == 0) {
+      const long o = gw * (Lay + 1) + lev(a, l);
+      double fdir = 0.0;
+      if (a.has_solar) {
+        fdir = (l == 0) ? a.F0 * a.mu0
+                        : a.F0 * a.mu0 * exp(-a.tau_cum[(long)w * (Lay + 1) + l] / a.mu0);
+        J += fdir * (1.0 / (4.0 * kPi * a.mu0));
+      }
+      a.fu[o] = fu * a.scaling;
+      a.fd[o] = fdn;
+      a.mi[o] = J;
+      a.fdir[o] = fdir;
+      if (a.fdiv) {
+        const int lyr = (l == 0) ? 0 : l - 1;
+        const double om = a.ssa[gw * Lay + lin(a, lyr)];
+        a.fdiv[o] = 4.0 * kPi * (1.0 - om) * (J - Blev(l));
+      }
+    }
+  };
+
+  // ---- TOA (src/solver.cpp:628-642) ----
+  {
+    put(s.tile, L, C.Rab);
+    put(s.tile2, L, C.Tba);
+    put_vec(s.v0, L, Itd);
+    put_vec(s.v1, L, Ibu);
+    wave_sync();
+    const Vec rI = matvec(s.tile, s.v0, L);
+    const Vec tI = matvec(s.tile2, s.v1, L);
+    wave_sync();
+    emit(0, vadd(rI, vadd(tI, C.su)), Itd);
+  }
+
+  // ---- top-down sweep with interfaces (src/solver.cpp:574-583, 644-678) ----
+  gload(lay, L, C.Rab);
+  gload(lay + N * N, L, C.Tab);
+  C.Rba = C.Rab;
+  C.Tba = C.Tab;
+  C.su = gvec(lay + 2 * N * N, L);
+  C.sd = gvec(lay + 2 * N * N + N, L);
+  C.scat = lflag[0];
+  for (int l = 1; l <= Lay; ++l) {  // C == rtop[l]
+    const int n_base = ltot - l;
+    Vec Iup, Idn;
+    if (n_base > 0) {
+      const double* o = rbw + (long)(n_base - 1) * RBR;
+      const Vec sbu = gvec(o + 2 * N * N, L);
+      Mat Bt, P;
+      gload(o + N * N, L, Bt);
+      put(s.tile, L, C.Tab);
+      put(s.tile2, L, Bt);
+      put_vec(s.v0, L, Itd);
+      put_vec(s.v1, L, Ibu);
+      wave_sync();
+      const Vec tti = matvec(s.tile, s.v0, L);   // top.T_ab I_top_down
+      const Vec tbi = matvec(s.tile2, s.v1, L);  // base.T_ba I_bot_up
+      wave_sync();
+      // M_up = I - base.R_ab top.R_ba; terms 2 and 3 (src/solver.cpp:24-46)
+      gload(o, L, Bt);  // base.R_ab
+      put(s.tile, L, Bt);
+      put_vec(s.v0, L, tti);
+      put_vec(s.v1, L, C.sd);
+      wave_sync();
+      mul(s.tile, C.Rba, L, P);
+      Vec t23[2];
+      matvecs<2>(s.tile, s.v0, L, t23);
+      wave_sync();
+      put(s.tile, L, P);
+      put_vec(s.v2, L, vadd(vadd(vadd(tbi, t23[0]), t23[1]), sbu));
+      wave_sync();
+      Iup = left_inverse(P, s.v2, s, L);
+      // M_dn = I - top.R_ba base.R_ab (src/solver.cpp:51-73)
+      put(s.tile, L, C.Rba);
+      put_vec(s.v0, L, tbi);
+      put_vec(s.v1, L, sbu);
+      wave_sync();
+      mul(s.tile, Bt, L, P);
+      Vec u23[2];
+      matvecs<2>(s.tile, s.v0, L, u23);
+      wave_sync();
+      put(s.tile, L, P);
+      put_vec(s.v2, L, vadd(vadd(vadd(tti, u23[0]), u23[1]), C.sd));
+      wave_sync();
+      Idn = left_inverse(P, s.v2, s, L);
+    } else {
+      put(s.tile, L, C.Tab);
+      put(s.tile2, L, C.Rba);
+      put_vec(s.v0, L, Itd);
+      put_vec(s.v1, L, Ibu);
+      wave_sync();
+      const Vec tti = matvec(s.tile, s.v0, L);
+      const Vec rbi = matvec(s.tile2, s.v1, L);
+      wave_sync();
+      Idn = vadd(tti, vadd(rbi, C.sd));
+      Iup = Ibu;
+    }
+    emit(l, Iup, Idn);
+    if (l < Lay) add32_below(C, lay + (long)l * REC, lflag[l], L, s);
+  }
 }
 
 }  // namespace w32

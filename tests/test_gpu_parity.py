@@ -166,8 +166,8 @@ def test_parity_thick_conservative(n):
 
 @pytest.mark.parametrize("opt", ["delta_m", "diffusion", "reversed", "no_surface", "black_cold",
                                  "shared_moments", "scaling", "albedo_only", "solar_only"])
-def test_parity_options(opt):
-    n = 8
+@pytest.mark.parametrize("n", [8, 32])
+def test_parity_options(opt, n):
     tau, ssa, mom, B = _edge_batch(n, delta_m=(opt == "delta_m"), seed=3)
     W = tau.shape[0]
     cfg = dict(num_quadrature=n, surface_albedo=0.2, solar_flux=1.0, solar_mu=0.45)
@@ -196,7 +196,7 @@ def test_parity_options(opt):
         cfg["top_emission"] = 0.25
     elif opt == "solar_only":
         B, Bs, Bt = None, None, None
-    run_both(tau, ssa, mom, BatchConfig(**cfg), B, Bs, Bt, label=f"option {opt}")
+    run_both(tau, ssa, mom, BatchConfig(**cfg), B, Bs, Bt, label=f"option {opt} N {n}")
 
 
 def test_chunked_workspace_matches_single_chunk():
