@@ -66,6 +66,9 @@ def parse():
     p.add_argument("--form", choices=("f64", "mixed"), default="f64")
     p.add_argument("--cpu-seconds", type=float, default=12.0)
     p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--forward-only", action="store_true",
+                   help="time the forward solve alone (any quadrature order; the default "
+                        "line is forward + reverse pass)")
     p.add_argument("--workspace-gb", type=float, default=96.0)
     p.add_argument("--copy-mib", type=int, default=0,
                    help="only run a plain device copy of this many MiB, three times (the "
@@ -140,18 +143,21 @@ def main():
               top_emission=d["top_emission"])
     grads = {}
 
+    adjoint_ws = not a.forward_only or a.form == "mixed"
+
     def step():
         o = solve_batch(*inputs, **kw, out=out, max_workspace_bytes=ws_bytes,
-                        adjoint_workspace=True, precision=a.form)
-        gen = o.pop("_generation")
-        grads.update(adjoint_batch(*inputs, out, cot, **kw, forward_generation=gen,
-                                   max_workspace_bytes=ws_bytes, check_status=False,
-                                   precision=a.form))
+                        adjoint_workspace=adjoint_ws, precision=a.form)
+        gen = o.pop("_generation", None)
+        if not a.forward_only:
+            grads.update(adjoint_batch(*inputs, out, cot, **kw, forward_generation=gen,
+                                       max_workspace_bytes=ws_bytes, check_status=False,
+                                       precision=a.form))
 
     for _ in range(a.warmup):
         step()
     torch.cuda.synchronize()
-    ws = get_workspace(W, L, cols.N, cols.phase_moments.shape[-1], ws_bytes, True)
+    ws = get_workspace(W, L, cols.N, cols.phase_moments.shape[-1], ws_bytes, adjoint_ws)
     lib = _lib.lib()
     _lib.check(lib.adrt_profile(ws.handle, 1))
     torch.cuda.synchronize()
@@ -165,7 +171,8 @@ def main():
     _lib.check(lib.adrt_profile_read(ws.handle, ms, nl))
     _lib.check(lib.adrt_profile(ws.handle, 0))
     stream = C.c_void_p(torch.cuda.current_stream().cuda_stream)
-    _lib.check(lib.adrt_adjoint_status(ws.handle, stream))
+    if not a.forward_only:
+        _lib.check(lib.adrt_adjoint_status(ws.handle, stream))
 
     ms_per_step = elapsed * 1e3 / a.steps
     Bs_host = d["surface_emission"].cpu().numpy() if d["surface_emission"] is not None else None
@@ -183,6 +190,17 @@ def main():
                  4: ("layer16m_adjoint_kernel",
                      "mixed recomputed doubling + reverse, FP64 flop count",
                      work["doubling_reverse"])}
+    if a.forward_only:
+        n16 = cols.N > 8 and cols.N <= 16 or adjoint_ws
+        names = ({1: "layer16_kernel<false>", 2: "adding16_kernel<false, false>"} if n16 else
+                 {1: "layer32_kernel", 2: "adding32_kernel"} if cols.N == 32 else
+                 {1: f"layer_kernel<{cols.N}>", 2: f"adding_kernel<{cols.N}>"})
+        if a.form == "mixed":
+            names = {1: "layer16m_kernel + layer16_kernel<true>",
+                     2: "adding16_kernel<false, true>"}
+        model = {1: (names[1], "phase setup + doubling (forward)", work["doubling"]),
+                 2: (names[2], "adding sweeps + interfaces (forward)",
+                     work["adding"] + work["interfaces"])}
     dom = max(model, key=lambda k: ms[k])
     kname, name, fl_step = model[dom]
     launches = nl[dom] / a.steps
@@ -190,7 +208,7 @@ def main():
     achieved = (fl_step / launches) / (avg_ms * 1e-3) / 1e12
     peak = FP64_PEAK_TFLOPS if a.form == "f64" else FP32_MFMA_PEAK_TFLOPS
     res = {
-        "metric": METRIC,
+        "metric": METRIC if not a.forward_only else "column-wavenumbers/s forward",
         "value": W / (ms_per_step * 1e-3),
         "unit": "column-wavenumbers/s",
         "n_gpus": 1,
@@ -205,11 +223,12 @@ def main():
         "config": {
             "workload": f"config {a.config}: {cols.name}, {W} wavenumbers x {L} layers, "
                         f"N={cols.N}",
-            "form": ("plain FP64 forward + reverse pass" if a.form == "f64" else
-                     "mixed forward (FP32 doubling with FP64-refined solves, FP64 adding) "
-                     "+ reverse pass"),
+            "form": ("plain FP64 forward" if a.form == "f64" else
+                     "mixed forward (FP32 doubling with FP64-refined solves, FP64 adding)")
+                    + ("" if a.forward_only else " + reverse pass"),
             "num_wavenumbers": W, "num_layers": L, "num_quadrature": cols.N,
-            "algorithmic_gflop_per_step": (2 * work["total"] + work["doubling_reverse"]
+            "algorithmic_gflop_per_step": (work["total"] if a.forward_only else
+                                           2 * work["total"] + work["doubling_reverse"]
                                            - work["doubling"]) / 1e9,
             "doubling_steps": work["doubling_steps"], "general_adds": work["general_adds"],
             "kernel_ms_per_step": per_kind,
