@@ -236,7 +236,8 @@ __device__ __forceinline__ void gauss_jordan(double* tA, double* tB, const Lane&
         idx = oi;
       }
     }
-    const int p = __builtin_amdgcn_readfirstlane(idx);
+    // (NaN entries can leave the lanes disagreeing: keep p a row k..31 anyway)
+    const int p = __builtin_amdgcn_readfirstlane(min(max(idx, k), N - 1));
     if (p != k) {
       const double ak = col[k], ap = col[p];
       col[k] = ap;
