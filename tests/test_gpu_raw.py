@@ -148,11 +148,12 @@ def test_raw_coefficients_match_oracle(name):
     _check(CASES[name], len(name), f"raw {name}")
 
 
-@pytest.mark.parametrize("seed", [26, 27])
-def test_raw_unstable_cloud_deck(seed):
-    print(f"cloud deck seed {seed}", flush=True)
-    _check(DECK, seed, f"raw cloud deck {seed}", spread=True, class_factor=SPREAD_FACTOR,
-           normwise=True)
+@pytest.mark.parametrize("n,seed", [(16, 26), (16, 27), (32, 26)])
+def test_raw_unstable_cloud_deck(n, seed):
+    print(f"cloud deck N {n} seed {seed}", flush=True)
+    c = dict(DECK, N=n, M=2 * n + 8, W=17 if n == 32 else DECK["W"])
+    _check(c, seed, f"raw cloud deck N {n} seed {seed}", spread=True,
+           class_factor=SPREAD_FACTOR, normwise=True)
 
 
 def test_raw_chunked_matches_single_chunk():
