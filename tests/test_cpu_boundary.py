@@ -78,6 +78,24 @@ def test_workspace_rejects_bad_config_without_gpu():
     assert L.adrt_workspace_create(C.byref(cfg), 0, C.byref(h)) == 9
 
 
+def test_raw_entry_rejects_bad_arguments_without_gpu():
+    L = _lib.lib()
+    cfg = _lib.BatchConfig(num_wavenumbers=4, num_layers=3, num_quadrature=8, num_moments=16,
+                           solar_mu=1.0, spectrum_scaling=1.0)
+    coef = _lib.DeviceCoefficients()
+    out = _lib.DeviceOutputs()
+    null = C.c_void_p()
+    f = L.adrt_solve_batch_from_coefficients
+    assert f(None, C.byref(coef), -1.0, -1.0, C.byref(out), 0, null, null) == 23
+    assert f(C.byref(cfg), None, -1.0, -1.0, C.byref(out), 0, null, null) == 23
+    assert f(C.byref(cfg), C.byref(coef), -1.0, -1.0, C.byref(out), 0, null, null) == 23
+    cfg.num_quadrature = 1
+    assert f(C.byref(cfg), C.byref(coef), -1.0, -1.0, C.byref(out), 0, null, null) == 2
+    cfg.num_quadrature = 8
+    cfg.num_wavenumbers = 0  # nothing to do: no pointer is touched
+    assert f(C.byref(cfg), C.byref(coef), -1.0, -1.0, C.byref(out), 0, null, null) == 0
+
+
 @pytest.mark.parametrize("k,W,L,N", [(1, 1000, 50, 8), (2, 16384, 100, 16), (3, 8192, 80, 32),
                                      (4, 4096, 60, 8), (5, 65536, 100, 16)])
 def test_workload_shapes(k, W, L, N):
