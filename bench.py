@@ -191,10 +191,11 @@ def main():
                      "mixed recomputed doubling + reverse, FP64 flop count",
                      work["doubling_reverse"])}
     if a.forward_only:
-        n16 = cols.N > 8 and cols.N <= 16 or adjoint_ws
-        names = ({1: "layer16_kernel<false>", 2: "adding16_kernel<false, false>"} if n16 else
-                 {1: "layer32_kernel", 2: "adding32_kernel"} if cols.N == 32 else
-                 {1: f"layer_kernel<{cols.N}>", 2: f"adding_kernel<{cols.N}>"})
+        npad = 1 << max(cols.N - 1, 1).bit_length()  # the library's padded order
+        names = ({1: "layer16_kernel<false>", 2: "adding16_kernel<false, false>"}
+                 if npad == 16 or adjoint_ws else
+                 {1: "layer32_kernel", 2: "adding32_kernel"} if npad == 32 else
+                 {1: f"layer_kernel<{npad}>", 2: f"adding_kernel<{npad}>"})
         if a.form == "mixed":
             names = {1: "layer16m_kernel + layer16_kernel<true>",
                      2: "adding16_kernel<false, true>"}
