@@ -1,0 +1,122 @@
+"""The reference's own test cases, run on the GPU batched path.
+
+* Published-table cases (CDISORT, Van de Hulst, Sweigart, Garcia-Siewert,
+  Ozisik-Shouman) that tests/test_ad_solver.cpp checks on the CPU solver and
+  tests/test_cuda_solver.cu:194-1450 runs through the reference's batched CUDA
+  solver, solved here by adrt_solve_batch: the published values at the
+  reference's own tolerances, and the FP64 oracle at the parity bar (1e-10
+  relative to the column maximum).
+* Batch consistency (tests/test_cuda_solver.cu:1248-1296): 100 identical
+  columns give identical results; here bit for bit (no reduction in the
+  kernels depends on the batch).
+* The raw-coefficient case of tests/test_cuda_solver.cu:1599-1760 (4 wavenumbers,
+  5 layers, N = 4, hot profile 1500 -> 500 K): the raw entry against the batched
+  solve on optics prepared on the host, and against the oracle.
+"""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+import oracle  # noqa: E402
+from addingdoublingrt_amd import BatchConfig, solve_batch, solve_batch_from_coefficients  # noqa: E402
+from golden.reference_tables import CASES, THERMAL_CASES  # noqa: E402
+from test_oracle_golden import _build  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+OUTS = ("flux_up", "flux_down", "mean_intensity", "flux_direct")
+
+
+def _t(x):
+    return torch.as_tensor(np.ascontiguousarray(x, dtype=np.float64), device="cuda")
+
+
+def _gpu_column(cfg):
+    """One column of an oracle.solve configuration through the batched path."""
+    L = cfg["num_layers"]
+    M = max(len(m) for m in cfg["phase_function_moments"])
+    mom = np.zeros((L, M))
+    for l, m in enumerate(cfg["phase_function_moments"]):
+        mom[l, :len(m)] = m
+    bc = BatchConfig(num_quadrature=cfg["num_quadrature"], use_delta_m=cfg.get("use_delta_m", False),
+                     surface_albedo=cfg.get("surface_albedo", 0.0),
+                     surface_emission=cfg.get("surface_emission", 0.0),
+                     top_emission=cfg.get("top_emission", 0.0),
+                     solar_flux=cfg.get("solar_flux", 0.0), solar_mu=cfg.get("solar_mu", 1.0))
+    B = cfg.get("planck_levels")
+    out = solve_batch(_t(np.array(cfg["delta_tau"])[None]), _t(np.array(cfg["single_scat_albedo"])[None]),
+                      _t(mom), bc, planck_levels=None if B is None else _t(np.array(B)[None]))
+    torch.cuda.synchronize()
+    return {k: v[0].cpu().numpy() for k, v in out.items()}
+
+
+def _check(name, cfg, checks):
+    g = _gpu_column(cfg)
+    ref = oracle.solve(**cfg)
+    for out, idx, expected, tol in checks:
+        assert abs(g[out][idx] - expected) <= tol, (name, out, idx, g[out][idx], expected)
+    for k in OUTS:
+        scale = max(np.abs(ref[k]).max(), 1e-300)
+        err = np.abs(g[k] - ref[k]).max() / scale
+        assert err <= 1e-10, (name, k, err)
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c["name"] for c in CASES])
+def test_published_tables_on_gpu(case):
+    _check(case["name"], _build(case), case["checks"])
+
+
+@pytest.mark.parametrize("case", THERMAL_CASES, ids=[c["name"] for c in THERMAL_CASES])
+def test_published_thermal_tables_on_gpu(case):
+    cfg = _build(case)
+    cfg["planck_levels"] = [oracle.planck(*p) for p in case["planck"]]
+    if case["top"] is not None:
+        iso, p = case["top"]
+        cfg["top_emission"] = iso + oracle.planck(*p)
+    if case["surface"] is not None:
+        cfg["surface_emission"] = oracle.planck(*case["surface"])
+    _check(case["name"], cfg, case["checks"])
+
+
+def test_batch_consistency():
+    W, L, M = 100, 5, 3
+    mom = np.zeros((L, M))
+    mom[:, 0] = 1.0
+    tau = np.full((W, L), 0.5)
+    ssa = np.full((W, L), 0.8)
+    B = np.tile(100.0 + 20.0 * np.arange(L + 1), (W, 1))
+    out = solve_batch(_t(tau), _t(ssa), _t(mom), BatchConfig(num_quadrature=8, surface_albedo=0.1),
+                      planck_levels=_t(B))
+    torch.cuda.synchronize()
+    for k in OUTS:
+        v = out[k].cpu().numpy()
+        assert np.array_equal(v, np.broadcast_to(v[0], v.shape)), k
+
+
+def test_raw_entry_reference_case():
+    nwav, nlay, N, nmom = 4, 5, 4, 8
+    nu = np.array([500.0, 1000.0, 1500.0, 2000.0])
+    z = np.array([0.0, 1e5, 3e5, 6e5, 1e6, 2e6])
+    T = np.array([1500.0, 1300.0, 1100.0, 900.0, 700.0, 500.0])
+    lev, w = np.meshgrid(np.arange(nlay + 1), np.arange(nwav), indexing="ij")
+    a = 1e-6 * (1.0 + 0.5 * lev) * (1.0 + 0.3 * w)
+    s = 0.3 * a
+    mom = np.zeros((nlay, nmom))
+    mom[:, 0] = 1.0
+    cfg = BatchConfig(num_quadrature=N)
+    raw = solve_batch_from_coefficients(_t(a), _t(s), _t(z), _t(T), _t(nu), _t(mom), cfg)
+    r = oracle.raw_optics(a, s, z, T, nu)
+    pre = solve_batch(_t(r["delta_tau"]), _t(r["single_scat_albedo"]), _t(mom), cfg,
+                      planck_levels=_t(r["planck_levels"]),
+                      surface_emission=_t(r["surface_emission"]),
+                      top_emission=_t(r["top_emission"]))
+    torch.cuda.synchronize()
+    ref = oracle.solve_batch(r["delta_tau"], r["single_scat_albedo"], mom, N,
+                             planck_levels=r["planck_levels"],
+                             surface_emission=r["surface_emission"],
+                             top_emission=r["top_emission"])
+    for k in OUTS:
+        g, p, o = raw[k].cpu().numpy(), pre[k].cpu().numpy(), ref[k]
+        scale = np.abs(o).max(axis=1, keepdims=True)
+        assert np.all(np.abs(g - p) <= 1e-13 * scale), k     # same solve, Planck to an ulp
+        assert np.all(np.abs(g - o) <= 1e-10 * scale), k
