@@ -12,6 +12,10 @@
 * The raw-coefficient case of tests/test_cuda_solver.cu:1599-1760 (4 wavenumbers,
   5 layers, N = 4, hot profile 1500 -> 500 K): the raw entry against the batched
   solve on optics prepared on the host, and against the oracle.
+* Closed forms and invariants of those test files (Beer's law with and without
+  a Lambertian surface, the beam term of the mean intensity, the linear-in-tau
+  pure-absorption source, flux conservation at omega = 1, delta-M leaving
+  Rayleigh scattering unchanged) at N = 8, 16 and 32.
 """
 import numpy as np
 import pytest
@@ -120,3 +124,71 @@ def test_raw_entry_reference_case():
         scale = np.abs(o).max(axis=1, keepdims=True)
         assert np.all(np.abs(g - p) <= 1e-13 * scale), k     # same solve, Planck to an ulp
         assert np.all(np.abs(g - o) <= 1e-10 * scale), k
+
+
+# ---- closed forms and invariants the reference tests (tests/test_ad_solver.cpp,
+# tests/test_cuda_solver.cu), on the GPU path at N = 8 (column slices), 16 and 32
+# (matrix cores) ----
+
+def _iso_cfg(nq, tau, ssa, **kw):
+    n = len(tau)
+    mom = [[1.0] + [0.0] * (2 * nq - 1)] * n
+    return dict(num_layers=n, num_quadrature=nq, delta_tau=tau, single_scat_albedo=ssa,
+                phase_function_moments=mom, **kw)
+
+
+@pytest.mark.parametrize("nq", [8, 16, 32])
+def test_beers_law_on_gpu(nq):             # test_ad_solver.cpp:144-195
+    g = _gpu_column(_iso_cfg(nq, [1.0], [0.0], solar_flux=200.0, solar_mu=0.5))
+    assert abs(g["flux_direct"][0] - 100.0) < 1e-6
+    assert abs(g["flux_direct"][1] - 100.0 * np.exp(-2.0)) < 1e-10
+    assert np.all(np.abs(g["flux_up"]) < 1e-12) and np.all(np.abs(g["flux_down"]) < 1e-12)
+    g = _gpu_column(_iso_cfg(nq, [1.0], [0.0], solar_flux=200.0, solar_mu=0.5,
+                             surface_albedo=0.5))
+    rfldir = 100.0 * np.exp(-2.0)
+    assert abs(g["flux_up"][1] - 0.5 * rfldir) <= 1e-10 * rfldir
+    assert 0.0 < g["flux_up"][0] < g["flux_up"][1]
+
+
+@pytest.mark.parametrize("nq", [8, 16, 32])
+def test_mean_intensity_is_beam_term_on_gpu(nq):   # test_ad_solver.cpp:201-227
+    g = _gpu_column(_iso_cfg(nq, [0.4, 0.6], [0.0, 0.0], solar_flux=100.0, solar_mu=0.6))
+    tc = np.array([0.0, 0.4, 1.0])
+    assert np.allclose(g["mean_intensity"], 100.0 * np.exp(-tc / 0.6) / (4 * np.pi),
+                       rtol=1e-13, atol=0)
+
+
+@pytest.mark.parametrize("nq", [8, 16, 32])
+def test_linear_source_pure_absorption_on_gpu(nq):  # test_ad_solver.cpp:1472-1510
+    g = _gpu_column(_iso_cfg(nq, [1.0], [0.0], planck_levels=[1.0, 3.0]))
+    mu, wt = oracle.gauss_legendre(nq)
+    t = np.exp(-1.0 / mu)
+    slope = mu * (1 - t) - 0.5 * (1 + t)
+    Fu = np.sum(2 * np.pi * wt * mu * (2.0 * (1 - t) + 2.0 * slope))
+    Fd = np.sum(2 * np.pi * wt * mu * (2.0 * (1 - t) - 2.0 * slope))
+    assert abs(g["flux_up"][0] - Fu) < 1e-12 * Fu
+    assert abs(g["flux_down"][1] - Fd) < 1e-12 * Fd
+
+
+@pytest.mark.parametrize("nq", [8, 16, 32])
+@pytest.mark.parametrize("delta_m,gg,mu0,tau", [(False, 0.7, 0.5, 0.3), (True, 0.85, 0.6, 0.5)])
+def test_energy_conservation_on_gpu(nq, delta_m, gg, mu0, tau):  # test_ad_solver.cpp:1155-1203
+    n = 5
+    nm = 2 * nq + (1 if delta_m else 0)
+    cfg = dict(num_layers=n, num_quadrature=nq, delta_tau=[tau] * n, single_scat_albedo=[1.0] * n,
+               phase_function_moments=[list(gg ** np.arange(nm))] * n, use_delta_m=delta_m,
+               solar_flux=1.0, solar_mu=mu0)
+    g = _gpu_column(cfg)
+    fnet = g["flux_up"] - g["flux_down"] - g["flux_direct"]
+    assert np.all(np.abs(fnet - fnet[0]) < 1e-5)
+
+
+@pytest.mark.parametrize("nq", [8, 16, 32])
+def test_delta_m_rayleigh_no_effect_on_gpu(nq):   # test_ad_solver.cpp:1210-1238
+    ray = lambda nm: [1.0, 0.0, 0.1] + [0.0] * (nm - 3)
+    base = dict(num_layers=1, num_quadrature=nq, delta_tau=[0.5], single_scat_albedo=[1.0],
+                solar_flux=1.0, solar_mu=1.0)
+    a = _gpu_column(dict(base, phase_function_moments=[ray(2 * nq)]))
+    b = _gpu_column(dict(base, phase_function_moments=[ray(2 * nq + 1)], use_delta_m=True))
+    for k in ("flux_up", "flux_down"):
+        assert np.allclose(a[k], b[k], rtol=1e-12, atol=0)
