@@ -24,7 +24,7 @@ constexpr double kPi = 3.14159265358979323846;
 
 template <int N>
 struct Geo {
-  static constexpr int H = N >= 16 ? 2 : 1;  // lanes per column
+  static constexpr int H = N >= 8 ? 2 : 1;  // lanes per column
   static constexpr int RP = N / H;           // rows per lane
   static constexpr int G = N * H;            // lanes per group
   static constexpr int GPB = 256 / G;        // groups per 256-thread block
