@@ -462,7 +462,7 @@ __device__ void layer16_adjoint_task(const FwdArgs& a, const AdjArgs& b, long ta
   R* s5 = vs + 5 * N;
   R* s6 = vs + 6 * N;
 
-  // ---- doubling recomputed with checkpoints; (I-R^2)^-1 by Gauss-Jordan ----
+  // ---- doubling recomputed with checkpoints; (I-R^2)^-1 by Newton-Schulz ----
   for (int k = 0; k < nn; ++k) {
     R* st = slab + (long)k * SLAB;
     gstore<R>(st, L, Rm);

@@ -219,29 +219,42 @@ __device__ __forceinline__ R div_by(R x, R f, R inv) {
 // X * A = T for X (the reference's rightSolveMatrix, src/matrix.hpp:120-124,
 // which factors A^T with row partial pivoting).  Lane (j, h) holds its slice of
 // column j of A (= row j of A^T) in a[] and of column j of T (= row j of T^T)
-// in t[].  Gauss-Jordan row operations on [A^T | T^T] with partial pivoting
-// over the rows not used yet: at step k the pivot is the first largest
-// |A^T(i,k)|, the same choice an LU of A^T makes.  Every row's multiplier is
-// its own A^T(i,k) (static register index, fetched from the neighbour lane
-// when it sits in the other half), and the pivot row travels through LDS.
-// Entries left of the pivot column are never read again, so they are updated
-// unconditionally.  On return t[] holds this lane's slice of column j of X.
+// in t[].  LU of [A^T | T^T] with partial pivoting over the rows not used yet:
+// at step k the pivot is the first largest |A^T(i,k)|, the pivot row is scaled
+// to a unit diagonal and eliminated from the rows not yet used; back
+// substitution in reverse pivot order then clears the rows pivoted earlier.
+// With LU = false the pivot row is eliminated from every other row at once
+// (Gauss-Jordan): the same forward error bound, a residual larger by up to
+// cond(A), no back-substitution pass; the doubling uses it, where it runs
+// every step and A = I - R^2 is far from singular outside the reference's
+// unstable starts.
+// Every row's multiplier is its own A^T(i,k) (static register index, fetched
+// from the neighbour lane when it sits in the other half), and the pivot row
+// travels through LDS.  On return t[] holds this lane's slice of column j of X.
 //
 // LDS: prow (2N values), pval (N values), perm (N*N values).
 template <int N, typename R>
+__device__ __forceinline__ R column_entry(const R (&a)[Geo<N>::RP], int k, int h) {
+  constexpr int H = Geo<N>::H, RP = Geo<N>::RP;
+  const int kh = k / RP, kr = k % RP;
+  R f = a[kr];
+  if constexpr (H == 2) {
+    const R other = swap_pair(a[kr]);
+    f = (h == kh) ? a[kr] : other;
+  }
+  return f;
+}
+
+template <int N, typename R, bool LU = true>
 __device__ __forceinline__ void right_solve(R (&a)[Geo<N>::RP], R (&t)[Geo<N>::RP], int j,
                                             int h, R* prow, R* pval, R* perm) {
-  constexpr int H = Geo<N>::H, RP = Geo<N>::RP;
+  constexpr int RP = Geo<N>::RP;
   bool used = false;
   int mystep = 0;
+  int piv_of[N];
 #pragma unroll
   for (int k = 0; k < N; ++k) {
-    const int kh = k / RP, kr = k % RP;
-    R f = a[kr];
-    if constexpr (H == 2) {
-      const R other = swap_pair(a[kr]);
-      f = (h == kh) ? a[kr] : other;
-    }
+    const R f = column_entry<N, R>(a, k, h);
     if (h == 0) pval[j] = used ? R(-1) : __builtin_fabs(f);
     wave_sync();
     int p = 0;
@@ -251,6 +264,8 @@ __device__ __forceinline__ void right_solve(R (&a)[Geo<N>::RP], R (&t)[Geo<N>::R
       const R v = pval[i];
       if (v > best) { best = v; p = i; }
     }
+    piv_of[k] = p;
+    const bool was_used = used;
     if (j == p) {
       const R inv = R(1) / f;
 #pragma unroll
@@ -264,12 +279,28 @@ __device__ __forceinline__ void right_solve(R (&a)[Geo<N>::RP], R (&t)[Geo<N>::R
       mystep = k;
     }
     wave_sync();
-    if (j != p) {
+    if (j != p && (!LU || !was_used)) {
 #pragma unroll
       for (int r = 0; r < RP; ++r) {
         a[r] = fma_(-f, prow[h * RP + r], a[r]);
         t[r] = fma_(-f, prow[N + h * RP + r], t[r]);
       }
+    }
+    wave_sync();
+  }
+  // back substitution: the row pivoted at step k is final once the later
+  // steps are done; it is then cleared from the rows pivoted before it
+#pragma unroll
+  for (int k = LU ? N - 1 : 0; k > 0; --k) {
+    const R f = column_entry<N, R>(a, k, h);
+    if (j == piv_of[k]) {
+#pragma unroll
+      for (int r = 0; r < RP; ++r) prow[N + h * RP + r] = t[r];
+    }
+    wave_sync();
+    if (mystep < k) {
+#pragma unroll
+      for (int r = 0; r < RP; ++r) t[r] = fma_(-f, prow[N + h * RP + r], t[r]);
     }
     wave_sync();
   }
@@ -289,17 +320,13 @@ __device__ __forceinline__ void right_solve(R (&a)[Geo<N>::RP], R (&t)[Geo<N>::R
 template <int N, typename R>
 __device__ __forceinline__ R right_solve_vec(R (&a)[Geo<N>::RP], R t0, int j, int h, R* prow,
                                              R* pval, R* perm) {
-  constexpr int H = Geo<N>::H, RP = Geo<N>::RP;
+  constexpr int RP = Geo<N>::RP;
   bool used = false;
   int mystep = 0;
+  int piv_of[N];
 #pragma unroll
   for (int k = 0; k < N; ++k) {
-    const int kh = k / RP, kr = k % RP;
-    R f = a[kr];
-    if constexpr (H == 2) {
-      const R other = swap_pair(a[kr]);
-      f = (h == kh) ? a[kr] : other;
-    }
+    const R f = column_entry<N, R>(a, k, h);
     if (h == 0) pval[j] = used ? R(-1) : __builtin_fabs(f);
     wave_sync();
     int p = 0;
@@ -309,6 +336,8 @@ __device__ __forceinline__ R right_solve_vec(R (&a)[Geo<N>::RP], R t0, int j, in
       const R v = pval[i];
       if (v > best) { best = v; p = i; }
     }
+    piv_of[k] = p;
+    const bool was_used = used;
     if (j == p) {
       const R inv = R(1) / f;
 #pragma unroll
@@ -322,11 +351,19 @@ __device__ __forceinline__ R right_solve_vec(R (&a)[Geo<N>::RP], R t0, int j, in
       mystep = k;
     }
     wave_sync();
-    if (j != p) {
+    if (j != p && !was_used) {
 #pragma unroll
       for (int r = 0; r < RP; ++r) a[r] = fma_(-f, prow[h * RP + r], a[r]);
       t0 = fma_(-f, prow[N], t0);
     }
+    wave_sync();
+  }
+#pragma unroll
+  for (int k = N - 1; k > 0; --k) {
+    const R f = column_entry<N, R>(a, k, h);
+    if (j == piv_of[k] && h == 0) prow[N] = t0;
+    wave_sync();
+    if (mystep < k) t0 = fma_(-f, prow[N], t0);
     wave_sync();
   }
   if (h == 0) perm[mystep] = t0;

@@ -336,7 +336,7 @@ __global__ void __launch_bounds__(256) layer_kernel(FwdArgs a) {
     wave_sync();
 #pragma unroll
     for (int r = 0; r < RP; ++r) X[r] = Tm[r];
-    right_solve<N, R>(A, X, j, h, prow, pval, tile);  // TG = T (I - R^2)^-1
+    right_solve<N, R, false>(A, X, j, h, prow, pval, tile);  // TG = T (I - R^2)^-1
     put_col<N, R>(tile, j, h, X);
     const double zpgy = z + gk * y;
     if (h == 0) {
@@ -686,7 +686,7 @@ __device__ __forceinline__ double left_solve(const double (&Mc)[Geo<N>::RP], dou
 }
 
 template <int N>
-__global__ void __launch_bounds__(256) adding_kernel(FwdArgs a) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 8))) adding_kernel(FwdArgs a) {
   using R = double;
   constexpr int RP = Geo<N>::RP, G = Geo<N>::G, GPB = Geo<N>::GPB, H = Geo<N>::H;
   constexpr int S = Lds<N>::stride;

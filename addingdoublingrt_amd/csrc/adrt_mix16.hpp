@@ -6,7 +6,7 @@
 // FP32 T would keep only a few digits of what matters; the recursion therefore
 // carries S = I - T, and every FP32 quantity is a small matrix held to full
 // relative precision:
-//   Q  = R^2 (I - R^2)^-1         FP32 Gauss-Jordan on [A^T | (R^2)^T], A = I - R^2,
+//   Q  = R^2 (I - R^2)^-1         FP32 Newton-Schulz (LU fallback), A = I - R^2,
 //                                  plus one refinement step with an FP64
 //                                  residual E = R R - Q + Q (R R) (FP64 matrix
 //                                  cores, R and Q exact in FP64): Q += E + E Q

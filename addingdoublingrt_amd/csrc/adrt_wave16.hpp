@@ -244,7 +244,7 @@ __device__ __forceinline__ R diag(const Mat<R>& m, const Lane& L) {
   return shfl(sel, L.j + 16 * oh);
 }
 
-// One Gauss-Jordan step k on [A^T | T^T] (row partial pivoting as an LU of
+// Pivot choice of elimination step k on [A^T | T^T] (row partial pivoting as an LU of
 // A^T; the pivot is the first row of largest magnitude among those not yet
 // used, and the pivot row is scaled with the corrected reciprocal).  Lane
 // (j, h) holds row j of A^T (= column j of A) at columns row(h, .).
@@ -262,18 +262,27 @@ __device__ __forceinline__ int gj_pivot(const Mat<R>& A, const Lane& L, bool use
   return p;
 }
 
-// X A = T for X, in place on T.  No LDS: pivots by DPP row maxima and a
-// ballot, the pivot row by lane permutes, and the final row permutation by
-// one permute per register.
+// X A = T for X, in place on T, as the reference's LU of A^T with partial
+// pivoting (src/matrix.hpp:120-124): forward elimination of the rows not yet
+// pivoted (pivot rows scaled to a unit diagonal), then back substitution in
+// reverse pivot order.  (Eliminating above the pivot as well, Gauss-Jordan,
+// gives the same forward error bound but a residual larger by up to
+// cond(A); on the near-singular A of the reference's unstable doubling
+// starts that cost two digits.)  No LDS: pivots by DPP row maxima and a
+// ballot, pivot rows by lane permutes, the final row permutation by one
+// permute per register.
 template <typename R>
 __device__ __forceinline__ void right_solve(Mat<R>& A, Mat<R>& T, const Lane& L) {
   fence_sched();
   bool used = false;
-  int src = 0;  // lane column holding row j of the result
+  int src = 0;       // lane column holding row j of the result
+  int mystep = N;    // step at which this lane column's row was the pivot
+  int piv_of[N];     // pivot row of each step (wave-uniform)
   auto step = [&](auto kc) {
     constexpr int k = decltype(kc)::value;
     R f, fp;
     const int p = gj_pivot<R, k>(A, L, used, f, fp);
+    piv_of[k] = p;
     const R inv = R(1) / fp;
     const bool piv = (L.j == p);
     const int pl = p + 16 * L.h;
@@ -281,15 +290,32 @@ __device__ __forceinline__ void right_solve(Mat<R>& A, Mat<R>& T, const Lane& L)
     for (int r = 0; r < 4; ++r) {
       const R pa = div_by(shfl(A.v[r], pl), fp, inv);
       const R pt = div_by(shfl(T.v[r], pl), fp, inv);
-      A.v[r] = piv ? pa : fma_(-f, pa, A.v[r]);
-      T.v[r] = piv ? pt : fma_(-f, pt, T.v[r]);
+      A.v[r] = piv ? pa : (used ? A.v[r] : fma_(-f, pa, A.v[r]));
+      T.v[r] = piv ? pt : (used ? T.v[r] : fma_(-f, pt, T.v[r]));
     }
+    mystep = piv ? k : mystep;
     used = used || piv;
     src = (L.j == k) ? p : src;
+  };
+  auto back = [&](auto kc) {
+    constexpr int k = N - 1 - decltype(kc)::value;  // N-1 .. 1
+    constexpr int kh = (sizeof(R) == 8) ? (k & 3) : (k >> 2);
+    constexpr int kr = (sizeof(R) == 8) ? (k >> 2) : (k & 3);
+    const R f = shfl(A.v[kr], L.j + 16 * kh);  // U(row j, k)
+    const int pl = piv_of[k] + 16 * L.h;
+    const bool upd = mystep < k;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const R pt = shfl(T.v[r], pl);
+      T.v[r] = upd ? fma_(-f, pt, T.v[r]) : T.v[r];
+    }
   };
   [&]<int... K>(std::integer_sequence<int, K...>) {
     (step(std::integral_constant<int, K>{}), ...);
   }(std::make_integer_sequence<int, N>{});
+  [&]<int... K>(std::integer_sequence<int, K...>) {
+    (back(std::integral_constant<int, K>{}), ...);
+  }(std::make_integer_sequence<int, N - 1>{});
   // row mystep of the reduced system is row mystep of X^T: lane (j, h) takes
   // column j of X from the lane column pivoted at step j
   const int sl = src + 16 * L.h;
@@ -324,14 +350,14 @@ constexpr int NS_MAX_STEPS = 8;
 // with E = I - X + P X recomputed from each iterate, so rounding does not
 // accumulate.  The doubling's thin layers give e ~ (tau0 |G|)^4 and stop at
 // once; where the start is poor (e > 1/2: (I - P) close to singular) or the
-// iteration has not finished in NS_MAX_STEPS steps, Gauss-Jordan with partial
+// iteration has not finished in NS_MAX_STEPS steps, LU with partial
 // pivoting (right_solve) solves X (I - P) = I instead.  Every branch is
-// wave-uniform.  Returns true if Gauss-Jordan ran.  With GJ = false the
+// wave-uniform.  Returns true if the LU solve ran.  With FALLBACK = false the
 // fallback is left to the caller: the function returns true at once, X unset,
 // so that a caller with a right-hand side can solve for it directly (a solve
 // keeps the error of an ill-conditioned I - P at that of an LU solve; the
 // product with a computed inverse loses up to cond(I - P) more).
-template <typename R, bool GJ = true>
+template <typename R, bool FALLBACK = true>
 __device__ __forceinline__ bool inverse_ns(const Mat<R>& P, const R* tP, R* tX, Mat<R>& X,
                                            const Lane& L) {
   Mat<R> E;
@@ -341,7 +367,7 @@ __device__ __forceinline__ bool inverse_ns(const Mat<R>& P, const R* tP, R* tX, 
   float e = residual_size<R>(E);
   for (int it = 0; e > NS_DONE; ++it) {
     if (!(e <= 0.5f) || it == NS_MAX_STEPS) {
-      if constexpr (!GJ) return true;
+      if constexpr (!FALLBACK) return true;
       // P again from its tile, so no register holds it through the iteration
       Mat<R> A;
       get<R>(tP, L, A);
@@ -387,7 +413,7 @@ __device__ __forceinline__ bool inverse_ns(const Mat<R>& P, const R* tP, R* tX, 
   return false;
 }
 
-// D = T (I - P)^-1 for P in the tile tP, by Gauss-Jordan with partial pivoting
+// D = T (I - P)^-1 for P in the tile tP, by LU with partial pivoting
 // on D (I - P) = T (the fallback of inverse_ns<R, false>).
 template <typename R>
 __device__ __forceinline__ void right_solve_tile(const R* tP, const Mat<R>& T, Mat<R>& D,
@@ -409,7 +435,7 @@ __device__ __forceinline__ void right_solve_tile(const R* tP, const Mat<R>& T, M
 //   E = P - Q + P Q  (= I - (I - P) X),   Q <- Q + E + Q E  (residual E^2),
 // until N max|E| <= 1e-5 N max|P| -- a relative accuracy that one refinement
 // step with an FP64 residual then carries to FP32 rounding.  E0 = P^2; an
-// E0 past 1/2 (I - P close to singular) or 8 steps fall back to Gauss-Jordan.
+// E0 past 1/2 (I - P close to singular) or 8 steps fall back to the LU solve.
 template <typename R>
 __device__ __forceinline__ void q_form_ns(Mat<R>& Q, const R* tP, R* tX, const Lane& L) {
   const float pmax = residual_size<R>(Q);
@@ -449,32 +475,48 @@ __device__ __forceinline__ void q_form_ns(Mat<R>& Q, const R* tP, R* tX, const L
 }
 
 // x = M^-1 b for A = M^T in registers (destroyed), b element j (all lanes of
-// column j); returns x_j.  x^T M^T = b^T is a right solve with A = M^T.
+// column j); returns x_j.  x^T M^T = b^T is a right solve with A = M^T, by
+// the forward elimination and back substitution of right_solve.
 template <typename R>
 __device__ __forceinline__ R left_solve_t(Mat<R>& A, R b, const Lane& L) {
   bool used = false;
   int src = 0;
+  int mystep = N;
+  int piv_of[N];
   R t0 = b;
   auto step = [&](auto kc) {
     constexpr int k = decltype(kc)::value;
     R f, fp;
     const int p = gj_pivot<R, k>(A, L, used, f, fp);
+    piv_of[k] = p;
     const R inv = R(1) / fp;
     const bool piv = (L.j == p);
     const int pl = p + 16 * L.h;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const R pa = div_by(shfl(A.v[r], pl), fp, inv);
-      A.v[r] = piv ? pa : fma_(-f, pa, A.v[r]);
+      A.v[r] = piv ? pa : (used ? A.v[r] : fma_(-f, pa, A.v[r]));
     }
     const R ptv = div_by(readlane(t0, p), fp, inv);
-    t0 = piv ? ptv : fma_(-f, ptv, t0);
+    t0 = piv ? ptv : (used ? t0 : fma_(-f, ptv, t0));
+    mystep = piv ? k : mystep;
     used = used || piv;
     src = (L.j == k) ? p : src;
+  };
+  auto back = [&](auto kc) {
+    constexpr int k = N - 1 - decltype(kc)::value;  // N-1 .. 1
+    constexpr int kh = (sizeof(R) == 8) ? (k & 3) : (k >> 2);
+    constexpr int kr = (sizeof(R) == 8) ? (k >> 2) : (k & 3);
+    const R f = shfl(A.v[kr], L.j + 16 * kh);
+    const R ptv = readlane(t0, piv_of[k]);
+    t0 = (mystep < k) ? fma_(-f, ptv, t0) : t0;
   };
   [&]<int... K>(std::integer_sequence<int, K...>) {
     (step(std::integral_constant<int, K>{}), ...);
   }(std::make_integer_sequence<int, N>{});
+  [&]<int... K>(std::integer_sequence<int, K...>) {
+    (back(std::integral_constant<int, K>{}), ...);
+  }(std::make_integer_sequence<int, N - 1>{});
   return shfl(t0, src + 16 * L.h);
 }
 
@@ -490,7 +532,7 @@ __device__ __forceinline__ R left_solve(const Mat<R>& M, R b, const Lane& L, R* 
   return left_solve_t<R>(A, b, L);
 }
 
-// x = (I - P)^-1 b for P in the tile tP, by Gauss-Jordan (the fallback of
+// x = (I - P)^-1 b for P in the tile tP, by LU (the fallback of
 // inverse_ns<R, false> for one right-hand side)
 template <typename R>
 __device__ __forceinline__ R left_solve_tile(const R* tP, R b, const Lane& L) {

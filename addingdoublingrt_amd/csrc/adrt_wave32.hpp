@@ -14,7 +14,7 @@
 // kernel does (adrt_fwd.hpp) and writes the same records; the adding kernel
 // restates src/adding.hpp:69-258 and src/solver.cpp:516-700 as the
 // column-slice adding kernel does.  Every (I - P)^-1 is a Newton-Schulz
-// inverse from the Neumann start; where that start is poor, Gauss-Jordan with
+// inverse from the Neumann start; where that start is poor, LU with
 // partial pivoting solves against the right-hand side in LDS, one augmented
 // column per lane.
 #pragma once
@@ -217,11 +217,15 @@ __device__ __forceinline__ bool inverse_ns(const Mat& P, const double* tP, doubl
   return false;
 }
 
-// Gauss-Jordan with partial pivoting on the 32 x 64 augmented system
-// [A^T | T^T], A^T column-major in tA and T^T in tB; lane c owns augmented
-// column c.  The pivot of step k is the first largest |A^T(i, k)|, i >= k (the
-// pivots of the reference's LU of A^T, src/matrix.hpp:120-124).  On return tB
-// holds X^T with X A = T.
+// LU with partial pivoting on the 32 x 64 augmented system [A^T | T^T], A^T
+// column-major in tA and T^T in tB; lane c owns augmented column c.  The pivot
+// of step k is the first largest |A^T(i, k)|, i >= k (the pivots of the
+// reference's LU of A^T, src/matrix.hpp:120-124); the pivot row is scaled to a
+// unit diagonal and eliminated from the rows below, then back substitution
+// clears the rows above (eliminating above during the forward sweep,
+// Gauss-Jordan, leaves a residual larger by up to cond(A)).  Every read of a
+// column that its owner rewrites in the same pass is separated from the write
+// by a wave barrier, chunk by chunk.  On return tB holds X^T with X A = T.
 __device__ __forceinline__ void gauss_jordan(double* tA, double* tB, const Lane& L) {
   double* col = (L.l < 32 ? tA : tB) + (L.l & 31) * TS;
   for (int k = 0; k < N; ++k) {
@@ -250,6 +254,7 @@ __device__ __forceinline__ void gauss_jordan(double* tA, double* tB, const Lane&
     wave_sync();
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
+      if (8 * q + 7 <= k) continue;  // wave-uniform: no row of this chunk is below k
       double f[8];
 #pragma unroll
       for (int t = 0; t < 8; ++t) f[t] = tA[k * TS + 8 * q + t];
@@ -257,14 +262,32 @@ __device__ __forceinline__ void gauss_jordan(double* tA, double* tB, const Lane&
 #pragma unroll
       for (int t = 0; t < 8; ++t) {
         const int i = 8 * q + t;
-        if (i != k) col[i] = fma_(-f[t], rk, col[i]);
+        if (i > k) col[i] = fma_(-f[t], rk, col[i]);
+      }
+      wave_sync();
+    }
+  }
+  // back substitution: row k of X^T is final once the columns above k are done
+  for (int k = N - 1; k > 0; --k) {
+    const double xk = col[k];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      if (8 * q >= k) continue;  // wave-uniform: no row of this chunk is above k
+      double f[8];
+#pragma unroll
+      for (int t = 0; t < 8; ++t) f[t] = tA[k * TS + 8 * q + t];
+      wave_sync();
+#pragma unroll
+      for (int t = 0; t < 8; ++t) {
+        const int i = 8 * q + t;
+        if (i < k) col[i] = fma_(-f[t], xk, col[i]);
       }
       wave_sync();
     }
   }
 }
 
-// D = T (I - P)^-1 for P in the tile t1 (t2: scratch), by Gauss-Jordan
+// D = T (I - P)^-1 for P in the tile t1 (t2: scratch), by LU
 __device__ __forceinline__ void right_solve_tile(double* t1, double* t2, const Mat& T, Mat& D,
                                                  const Lane& L) {
   Mat A;
@@ -589,7 +612,7 @@ layer32_kernel(FwdArgs a) {
 // ---------------------------------------------------------------------------
 
 // x = (I - P)^-1 b for P in the tile t1 (t2: scratch), b in the LDS slot vb,
-// by Gauss-Jordan: x^T (I - P)^T = b^T is X A = T with A^T = I - P and b in
+// by LU: x^T (I - P)^T = b^T is X A = T with A^T = I - P and b in
 // the first column of T^T
 __device__ __forceinline__ Vec left_solve_tile(double* t1, double* t2, const double* vb,
                                                const Lane& L) {

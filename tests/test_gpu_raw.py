@@ -19,11 +19,9 @@ extended solution, 1e-10 to 2.5e-10 for the same algorithm on inputs moved by
 1-2 ulp, which move the extended solution by only 2e-13).  There the class is
 set by the FP64 oracle's largest error over the inputs and three ulp-perturbed
 copies, errors are taken against the column maximum (elementwise errors at the
-zero crossings of a negative flux profile say nothing), and, the GPU's
-operation order being another (matrix-core products, Newton-Schulz inverses),
-the bar is 100 x that FP64 error.  Measured: 27 x on the worst column (a
-deck of tau 3.5 at omega 6e-5 under a layer of tau 0.3 at omega 4e-4:
-1.2e-5 of the column maximum against 4.2e-7), within 4 x on the others.
+zero crossings of a negative flux profile say nothing), and the bar is the
+usual 4 x.  Measured: at most 2.8 x (N = 16); the decks run at N = 8, 16
+and 32.
 """
 import numpy as np
 import pytest
@@ -35,7 +33,7 @@ from addingdoublingrt_amd import BatchConfig, solve_batch_from_coefficients  # n
 from test_gpu_parity import OUTS, compare  # noqa: E402
 
 pytestmark = pytest.mark.gpu
-SPREAD_FACTOR = 100.0
+SPREAD_FACTOR = 4.0
 
 
 def _atmosphere(W, L, seed, cloud=None):
@@ -148,7 +146,7 @@ def test_raw_coefficients_match_oracle(name):
     _check(CASES[name], len(name), f"raw {name}")
 
 
-@pytest.mark.parametrize("n,seed", [(16, 26), (16, 27), (32, 26)])
+@pytest.mark.parametrize("n,seed", [(8, 26), (16, 26), (16, 27), (32, 26)])
 def test_raw_unstable_cloud_deck(n, seed):
     print(f"cloud deck N {n} seed {seed}", flush=True)
     c = dict(DECK, N=n, M=2 * n + 8, W=17 if n == 32 else DECK["W"])
