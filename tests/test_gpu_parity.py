@@ -152,7 +152,7 @@ def test_parity_edge_layers(n):
 def test_parity_thick_conservative(n):
     """Thick, (near-)conservative layers: the doubling's I - R^2 approaches
     singularity in the last steps, where the N = 16 kernels leave the
-    Newton-Schulz inverse for pivoted Gauss-Jordan."""
+    Newton-Schulz inverse for the pivoted LU solve."""
     W, L = 6, 5
     tau = np.array([[50.0, 200.0, 1000.0, 3.0, 0.7]] * W)
     tau *= np.linspace(1.0, 1.3, W)[:, None]
