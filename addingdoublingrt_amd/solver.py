@@ -256,14 +256,82 @@ def adjoint_batch(delta_tau, single_scat_albedo, phase_moments, config: BatchCon
     return grads
 
 
+REVERSE_MAX_QUADRATURE = 16
+
+
+def planck_source_vjp(delta_tau, single_scat_albedo, phase_moments, config: BatchConfig,
+                      cotangents, max_columns=1 << 16, max_workspace_bytes=0):
+    """Gradients of ``sum(cotangent * output)`` w.r.t. the Planck levels and the
+    surface and top emission, from forward solves alone (any quadrature order).
+
+    R and T do not depend on the thermal sources and every output is affine in
+    them, so d(output)/d(B_m) is the solve with B = e_m, no beam and no other
+    emission (the linear recursion the reference's computeTemperatureJacobian
+    runs in forward mode, src/solver.cpp:117-341).  Each column is solved L + 3
+    times (unit level, surface and top sources), in batches of at most
+    ``max_columns`` columns.  Returns ``[W][L+1]``, ``[W]``, ``[W]``.
+    """
+    W, L = delta_tau.shape
+    dev = delta_tau.device
+    nu = L + 3
+    cfg = BatchConfig(**{**config.__dict__, "solar_flux": 0.0, "surface_emission": 0.0,
+                         "top_emission": 0.0, "spectrum_scaling": config.spectrum_scaling})
+    eye = torch.eye(L + 1, dtype=torch.float64, device=dev)
+    unitB = torch.cat([eye, torch.zeros((2, L + 1), dtype=torch.float64, device=dev)])
+    unitS = torch.zeros(nu, dtype=torch.float64, device=dev)
+    unitS[L + 1] = 1.0
+    unitT = torch.zeros(nu, dtype=torch.float64, device=dev)
+    unitT[L + 2] = 1.0
+    gB = torch.zeros((W, L + 1), dtype=torch.float64, device=dev)
+    gS = torch.zeros(W, dtype=torch.float64, device=dev)
+    gT = torch.zeros(W, dtype=torch.float64, device=dev)
+    names = ("flux_up", "flux_down", "mean_intensity", "flux_divergence")
+    step = max(1, max_columns // nu)
+    for w0 in range(0, W, step):
+        w1 = min(W, w0 + step)
+        n = w1 - w0
+        rep = lambda t: t[w0:w1].repeat_interleave(nu, dim=0).contiguous()
+        mom = phase_moments if phase_moments.dim() == 2 else rep(phase_moments)
+        out = solve_batch(rep(delta_tau), rep(single_scat_albedo), mom, cfg,
+                          planck_levels=unitB.repeat(n, 1).contiguous(),
+                          surface_emission=unitS.repeat(n).contiguous(),
+                          top_emission=unitT.repeat(n).contiguous(),
+                          max_workspace_bytes=max_workspace_bytes)
+        acc = torch.zeros((n, nu), dtype=torch.float64, device=dev)
+        for k in names:
+            c = cotangents.get(k)
+            if c is None:
+                continue
+            J = out[k].view(n, nu, L + 1)                       # [w][source][interface]
+            acc += torch.einsum("wsi,wi->ws", J, c[w0:w1].to(torch.float64))
+        gB[w0:w1] = acc[:, :L + 1]
+        gS[w0:w1] = acc[:, L + 1]
+        gT[w0:w1] = acc[:, L + 2]
+    return gB, gS, gT
+
+
 class _SolveFunction(torch.autograd.Function):
-    """Forward solve with a hand-written reverse pass (adrt_adjoint_batch)."""
+    """Forward solve with a hand-written reverse pass (adrt_adjoint_batch); for
+    quadrature orders above 16 the reverse pass covers the thermal sources
+    only (planck_source_vjp)."""
 
     @staticmethod
     def forward(ctx, config, precision, max_ws, delta_tau, single_scat_albedo, phase_moments,
                 planck_levels, surface_emission, top_emission, surface_albedo):
         if surface_albedo is not None:
             config = BatchConfig(**{**config.__dict__, "surface_albedo": float(surface_albedo)})
+        ctx.sources_only = config.num_quadrature > REVERSE_MAX_QUADRATURE
+        if ctx.sources_only:
+            out = solve_batch(delta_tau, single_scat_albedo, phase_moments, config,
+                              planck_levels, surface_emission, top_emission,
+                              precision=precision, max_workspace_bytes=max_ws)
+            ctx.config, ctx.precision, ctx.max_ws = config, precision, max_ws
+            ctx.has = (planck_levels is not None, surface_emission is not None,
+                       top_emission is not None, surface_albedo is not None)
+            outs = tuple(out[k] for k in OUTPUT_NAMES)
+            ctx.save_for_backward(delta_tau, single_scat_albedo, phase_moments)
+            ctx.mark_non_differentiable(outs[3])
+            return outs
         out = solve_batch(delta_tau, single_scat_albedo, phase_moments, config, planck_levels,
                           surface_emission, top_emission, precision=precision,
                           max_workspace_bytes=max_ws, adjoint_workspace=True)
@@ -279,6 +347,19 @@ class _SolveFunction(torch.autograd.Function):
 
     @staticmethod
     def backward(ctx, *gouts):
+        if ctx.sources_only:
+            if any(ctx.needs_input_grad[i] for i in (3, 4, 5, 9)):
+                raise NotImplementedError(
+                    f"gradients w.r.t. delta_tau, single_scat_albedo, phase_moments and "
+                    f"surface_albedo need num_quadrature <= {REVERSE_MAX_QUADRATURE}; above it "
+                    "the reverse pass covers planck_levels, surface_emission and top_emission")
+            tau, ssa, mom = ctx.saved_tensors
+            cot = {k: g for k, g in zip(OUTPUT_NAMES, gouts) if g is not None}
+            gB, gS, gT = planck_source_vjp(tau, ssa, mom, ctx.config, cot,
+                                           max_workspace_bytes=ctx.max_ws)
+            hasB, hasBs, hasBt, _ = ctx.has
+            return (None, None, None, None, None, None, gB if hasB else None,
+                    gS if hasBs else None, gT if hasBt else None, None)
         tau, ssa, mom, B, Bs, Bt, *outs = ctx.saved_tensors
         out = dict(zip(OUTPUT_NAMES, outs))
         cot = {k: g for k, g in zip(OUTPUT_NAMES, gouts) if g is not None}
@@ -308,7 +389,9 @@ def solve(delta_tau, single_scat_albedo, phase_moments, config: BatchConfig, pla
     Gradients flow to delta_tau, single_scat_albedo, phase_moments,
     planck_levels, surface_emission, top_emission and ``surface_albedo`` (a
     0-dim tensor that replaces ``config.surface_albedo``) through the
-    hand-written reverse pass.  flux_direct is not differentiable here (it is
+    hand-written reverse pass (num_quadrature <= 16); above 16 to
+    planck_levels, surface_emission and top_emission (so to temperatures
+    through ``planck_levels``) from forward solves with unit sources.  flux_direct is not differentiable here (it is
     the closed form F0 mu0 exp(-tau_cum / mu0)).
     """
     outs = _SolveFunction.apply(config, precision, max_workspace_bytes, delta_tau,

@@ -369,3 +369,51 @@ def test_temperature_gradient_through_device_planck():
             vals.append(r["flux_up"][:, 0].sum())
         g[m] = (vals[0] - vals[1]) / (2 * h)
     np.testing.assert_allclose(T.grad.cpu().numpy(), g, rtol=1e-4, atol=1e-6 * np.abs(g).max())
+
+
+@pytest.mark.parametrize("n,diffusion", [(32, False), (32, True)])
+def test_source_gradients_above_order_16(n, diffusion):
+    """Above order 16 solve() differentiates the thermal sources from forward
+    solves with unit sources.  Every output is affine in the sources, so central
+    differences of the GPU forward with a unit step are exact up to rounding;
+    the other inputs refuse gradients there."""
+    W, L = 3, 5
+    tau, ssa, mom, B = smooth_batch(n, W=W, L=L, seed=31)
+    rng = np.random.default_rng(4)
+    Bs, Bt = rng.uniform(0.5, 1.5, W), rng.uniform(0.1, 0.4, W)
+    cfg = BatchConfig(num_quadrature=n, surface_albedo=0.0 if diffusion else 0.25,
+                      use_diffusion_lower_bc=diffusion, solar_flux=1.0, solar_mu=0.6)
+    xB, xS, xT = (t(v).requires_grad_(True) for v in (B, Bs, Bt))
+    out = solve(t(tau), t(ssa), t(mom), cfg, planck_levels=xB, surface_emission=xS,
+                top_emission=xT)
+    names = ("flux_up", "flux_down", "mean_intensity", "flux_divergence")
+    cot = {k: rng.uniform(-1, 1, (W, L + 1)) for k in names}
+    (sum((out[k] * t(v)).sum() for k, v in cot.items())).backward()
+
+    def loss(Bv, Sv, Tv):
+        o = solve_batch(t(tau), t(ssa), t(mom), cfg, planck_levels=t(Bv),
+                        surface_emission=t(Sv), top_emission=t(Tv))
+        return sum(float((o[k].cpu().numpy() * v).sum()) for k, v in cot.items())
+
+    def fd(which, shape_idx):
+        args = [B.copy(), Bs.copy(), Bt.copy()]
+        vals = []
+        for s in (1.0, -1.0):
+            a = [x.copy() for x in args]
+            a[which][shape_idx] += s
+            vals.append(loss(*a))
+        return (vals[0] - vals[1]) / 2.0
+
+    gB = xB.grad.cpu().numpy()
+    for w in range(W):
+        for m in range(L + 1):
+            ref = fd(0, (w, m))
+            assert abs(gB[w, m] - ref) <= 1e-10 * max(abs(ref), 1.0), (w, m, gB[w, m], ref)
+        for which, g in ((1, xS.grad), (2, xT.grad)):
+            ref = fd(which, w)
+            got = g.cpu().numpy()[w]
+            assert abs(got - ref) <= 1e-10 * max(abs(ref), 1.0), (which, w, got, ref)
+    xtau = t(tau).requires_grad_(True)
+    out = solve(xtau, t(ssa), t(mom), cfg, planck_levels=t(B))
+    with pytest.raises(NotImplementedError):
+        out["flux_up"].sum().backward()
