@@ -8,14 +8,16 @@ package is its host-side face:
                                        coefficients, altitudes and temperatures
 * ``adjoint_batch``                 -- hand-written reverse pass (gradients)
 * ``solve``                         -- differentiable solve (torch.autograd)
+* ``raw_optics`` / ``solve_from_coefficients``
+                                    -- differentiable raw-coefficient optics and solve
 * ``planck_levels``                 -- device band Planck, differentiable in T
 * ``workloads``                     -- synthetic inputs of the benchmark configs
 """
 from ._lib import AdrtError, AdrtLibraryError, lib  # noqa: F401
 from .solver import (BatchConfig, Workspace, adjoint_batch, gauss_legendre, planck,  # noqa: F401
-                     planck_deriv, planck_levels, solve, solve_batch,
-                     solve_batch_from_coefficients)
+                     planck_deriv, planck_levels, raw_optics, solve, solve_batch,
+                     solve_batch_from_coefficients, solve_from_coefficients)
 
 __all__ = ["AdrtError", "AdrtLibraryError", "BatchConfig", "Workspace", "adjoint_batch",
-           "gauss_legendre", "planck", "planck_deriv", "planck_levels", "solve", "solve_batch",
-           "solve_batch_from_coefficients", "lib"]
+           "gauss_legendre", "planck", "planck_deriv", "planck_levels", "raw_optics", "solve",
+           "solve_batch", "solve_batch_from_coefficients", "solve_from_coefficients", "lib"]

@@ -85,6 +85,22 @@ class DeviceCoefficients(C.Structure):
     ]
 
 
+class RawOpticsCotangents(C.Structure):
+    _fields_ = [
+        ("delta_tau", C.c_void_p), ("single_scat_albedo", C.c_void_p),
+        ("planck_levels", C.c_void_p), ("surface_emission", C.c_void_p),
+        ("top_emission", C.c_void_p),
+    ]
+
+
+class RawGradients(C.Structure):
+    _fields_ = [
+        ("absorption_coeff", C.c_void_p), ("scattering_coeff", C.c_void_p),
+        ("cloud_optical_depth", C.c_void_p), ("temperature", C.c_void_p),
+        ("surface_temperature", C.c_void_p), ("top_temperature", C.c_void_p),
+    ]
+
+
 # Every symbol include/adrt_hip.h declares, with its ctypes signature.
 SIGNATURES = {
     "adrt_workspace_create": (C.c_int, [C.POINTER(BatchConfig), C.c_size_t, C.POINTER(C.c_void_p)]),
@@ -104,6 +120,12 @@ SIGNATURES = {
                                                      C.POINTER(DeviceCoefficients), C.c_double,
                                                      C.c_double, C.POINTER(DeviceOutputs), C.c_int,
                                                      C.c_void_p, C.c_void_p]),
+    "adrt_raw_optics": (C.c_int, [C.c_int, C.c_int, C.POINTER(DeviceCoefficients), C.c_double,
+                                  C.c_double, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                  C.c_void_p, C.c_void_p]),
+    "adrt_raw_optics_adjoint": (C.c_int, [C.c_int, C.c_int, C.POINTER(DeviceCoefficients),
+                                          C.c_double, C.c_double, C.POINTER(RawOpticsCotangents),
+                                          C.POINTER(RawGradients), C.c_void_p]),
     "adrt_validate_inputs": (C.c_int, [C.POINTER(BatchConfig), C.POINTER(DeviceInputs),
                                        C.c_void_p, C.c_void_p]),
     "adrt_planck_levels": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int,

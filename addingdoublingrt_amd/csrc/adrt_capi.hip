@@ -230,6 +230,95 @@ __global__ void raw_optics_kernel(int W, int w0, int nwav, int L, const double* 
   Bt_o[w] = planck_single(Tt >= 0.0 ? Tt : T[L], nu);
 }
 
+// dB/dT of planck_single: B (x / T) e^x / (e^x - 1), x = c2 nu / T
+__device__ __forceinline__ double planck_single_dT(double T, double nu) {
+  constexpr double c1 = 1.19105e-08, c2 = 1.43879;
+  if (T < 1e-5) return 0.0;
+  const double x = c2 * nu / T;
+  const double em = expm1(x);
+  return c1 * nu * nu * nu / em * (x / T) * (1.0 + 1.0 / em);
+}
+
+// reverse of raw_optics_kernel for tau and omega: cotangents of the TOA-first
+// [W][L] optics to the level-major absorption / scattering coefficients and
+// the cloud depths (thread per wavenumber; each level collects its two layers
+// in order, no atomics).  A clamped layer (tau <= 0) passes nothing back.
+__global__ void raw_optics_adjoint_kernel(int W, int L, const double* abs_c, const double* sca_c,
+                                          const double* alt, const double* cloud,
+                                          const double* g_tau, const double* g_ssa,
+                                          double* g_abs, double* g_sca, double* g_cloud) {
+  const int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= W) return;
+  double ga_lo = 0.0, gs_lo = 0.0;  // running cotangent of level b (from layer b - 1)
+  for (int b = 0; b < L; ++b) {
+    const int l = L - 1 - b;
+    const double dz = alt[b + 1] - alt[b];
+    const double ab = abs_c[(long)b * W + w], at = abs_c[(long)(b + 1) * W + w];
+    const double sb = sca_c[(long)b * W + w], st = sca_c[(long)(b + 1) * W + w];
+    double tau = dz * ((at + st) + (ab + sb)) * 0.5;
+    const double sdepth = dz * (st + sb) * 0.5;
+    if (cloud) tau += cloud[(long)b * W + w];
+    double gt = 0.0, gsd = 0.0;
+    if (tau > 0.0) {
+      const double gtau = g_tau ? g_tau[(long)w * L + l] : 0.0;
+      const double gom = g_ssa ? g_ssa[(long)w * L + l] : 0.0;
+      gt = gtau - gom * sdepth / (tau * tau);  // omega = sdepth / tau
+      gsd = gom / tau;
+    }
+    if (g_cloud) g_cloud[(long)b * W + w] = gt;
+    const double ge = gt * dz * 0.5, gs = (gt + gsd) * dz * 0.5;  // per level of the pair
+    if (g_abs) g_abs[(long)b * W + w] = ga_lo + ge;
+    if (g_sca) g_sca[(long)b * W + w] = gs_lo + gs;
+    ga_lo = ge;
+    gs_lo = gs;
+  }
+  if (g_abs) g_abs[(long)L * W + w] = ga_lo;
+  if (g_sca) g_sca[(long)L * W + w] = gs_lo;
+}
+
+// reverse of the Planck values: block k < L+1 sums over the wavenumbers the
+// cotangent of level k (BOA first; TOA-first level L - k) times dB/dT, plus
+// the surface / top emission where they follow the bottom / top level; blocks
+// L+1 and L+2 give d/d surface_temperature and d/d top_temperature when those
+// are set.  Fixed-order sums: deterministic.
+__global__ void raw_temperature_adjoint_kernel(int W, int L, const double* T, const double* wn,
+                                               double Ts, double Tt, const double* g_B,
+                                               const double* g_Bs, const double* g_Bt,
+                                               double* g_T, double* g_Ts, double* g_Tt) {
+  __shared__ double part[256];
+  const int k = blockIdx.x;
+  double acc = 0.0;
+  for (int w = threadIdx.x; w < W; w += blockDim.x) {
+    const double nu = wn[w];
+    if (k <= L) {
+      const double d = planck_single_dT(T[k], nu);
+      double g = g_B ? g_B[(long)w * (L + 1) + (L - k)] : 0.0;
+      if (k == 0 && Ts < 0.0 && g_Bs) g += g_Bs[w];
+      if (k == L && Tt < 0.0 && g_Bt) g += g_Bt[w];
+      acc += g * d;
+    } else if (k == L + 1) {
+      if (g_Bs) acc += g_Bs[w] * planck_single_dT(Ts, nu);
+    } else {
+      if (g_Bt) acc += g_Bt[w] * planck_single_dT(Tt, nu);
+    }
+  }
+  part[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) part[threadIdx.x] += part[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    if (k <= L) {
+      if (g_T) g_T[k] = part[0];
+    } else if (k == L + 1) {
+      if (g_Ts) g_Ts[0] = Ts >= 0.0 ? part[0] : 0.0;
+    } else {
+      if (g_Tt) g_Tt[0] = Tt >= 0.0 ? part[0] : 0.0;
+    }
+  }
+}
+
 __global__ void validate_kernel(const double* tau, const double* ssa, long n, int* flag) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -697,6 +786,56 @@ int adrt_solve_batch_from_coefficients(const adrt_batch_config* cfg_in,
                            out->flux_divergence ? out->flux_divergence + o : nullptr};
     rc = adrt_solve_batch(&cc, &in, &oc, precision, ws, stream);
     if (rc) return rc;
+  }
+  return ADRT_OK;
+}
+
+int adrt_raw_optics(int num_wavenumbers, int num_layers, const adrt_device_coefficients* coef,
+                    double surface_temperature, double top_temperature, double* delta_tau,
+                    double* single_scat_albedo, double* planck_levels, double* surface_emission,
+                    double* top_emission, hipStream_t stream) {
+  if (!coef) return ADRT_ERR_NULL;
+  if (num_wavenumbers < 0) return ADRT_ERR_NUM_WAVENUMBERS;
+  if (num_layers <= 0) return ADRT_ERR_NUM_LAYERS;
+  if (num_wavenumbers == 0) return ADRT_OK;
+  if (!coef->absorption_coeff || !coef->scattering_coeff || !coef->altitude ||
+      !coef->temperature || !coef->wavenumber || !delta_tau || !single_scat_albedo ||
+      !planck_levels || !surface_emission || !top_emission)
+    return ADRT_ERR_NULL;
+  const int W = num_wavenumbers;
+  raw_optics_kernel<<<(unsigned)((W + 127) / 128), 128, 0, stream>>>(
+      W, 0, W, num_layers, coef->absorption_coeff, coef->scattering_coeff, coef->altitude,
+      coef->temperature, coef->wavenumber, coef->cloud_optical_depth, surface_temperature,
+      top_temperature, delta_tau, single_scat_albedo, planck_levels, surface_emission,
+      top_emission);
+  return hipGetLastError() == hipSuccess ? ADRT_OK : ADRT_ERR_HIP;
+}
+
+int adrt_raw_optics_adjoint(int num_wavenumbers, int num_layers,
+                            const adrt_device_coefficients* coef, double surface_temperature,
+                            double top_temperature, const adrt_raw_optics_cotangents* cot,
+                            const adrt_raw_gradients* grad, hipStream_t stream) {
+  if (!coef || !cot || !grad) return ADRT_ERR_NULL;
+  if (num_wavenumbers < 0) return ADRT_ERR_NUM_WAVENUMBERS;
+  if (num_layers <= 0) return ADRT_ERR_NUM_LAYERS;
+  if (!coef->absorption_coeff || !coef->scattering_coeff || !coef->altitude ||
+      !coef->temperature || !coef->wavenumber)
+    return ADRT_ERR_NULL;
+  const int W = num_wavenumbers, L = num_layers;
+  if (W > 0 && (grad->absorption_coeff || grad->scattering_coeff || grad->cloud_optical_depth)) {
+    raw_optics_adjoint_kernel<<<(unsigned)((W + 127) / 128), 128, 0, stream>>>(
+        W, L, coef->absorption_coeff, coef->scattering_coeff, coef->altitude,
+        coef->cloud_optical_depth, cot->delta_tau, cot->single_scat_albedo,
+        grad->absorption_coeff, grad->scattering_coeff,
+        coef->cloud_optical_depth ? grad->cloud_optical_depth : nullptr);
+    if (hipGetLastError() != hipSuccess) return ADRT_ERR_HIP;
+  }
+  if (grad->temperature || grad->surface_temperature || grad->top_temperature) {
+    raw_temperature_adjoint_kernel<<<(unsigned)(L + 3), 256, 0, stream>>>(
+        W, L, coef->temperature, coef->wavenumber, surface_temperature, top_temperature,
+        cot->planck_levels, cot->surface_emission, cot->top_emission, grad->temperature,
+        grad->surface_temperature, grad->top_temperature);
+    if (hipGetLastError() != hipSuccess) return ADRT_ERR_HIP;
   }
   return ADRT_OK;
 }

@@ -400,6 +400,125 @@ def solve(delta_tau, single_scat_albedo, phase_moments, config: BatchConfig, pla
     return dict(zip(OUTPUT_NAMES, outs))
 
 
+def _temperature_value(t):
+    if isinstance(t, torch.Tensor):
+        if t.numel() != 1:
+            raise ValueError("a boundary temperature tensor must hold one value")
+        return float(t.detach().item())
+    return float(t)
+
+
+def _coefficients_struct(a, s, z, T, nu, cloud, L, W):
+    return _lib.DeviceCoefficients(
+        absorption_coeff=_dev_ptr(a, "absorption_coeff", (L + 1, W)),
+        scattering_coeff=_dev_ptr(s, "scattering_coeff", (L + 1, W)),
+        altitude=_dev_ptr(z, "altitude", (L + 1,)),
+        temperature=_dev_ptr(T, "temperature", (L + 1,)),
+        wavenumber=_dev_ptr(nu, "wavenumber", (W,)),
+        cloud_optical_depth=_dev_ptr(cloud, "cloud_optical_depth", (L, W)))
+
+
+RAW_OPTICS_NAMES = ("delta_tau", "single_scat_albedo", "planck_levels", "surface_emission",
+                    "top_emission")
+
+
+class _RawOpticsFunction(torch.autograd.Function):
+    """adrt_raw_optics forward, adrt_raw_optics_adjoint backward."""
+
+    @staticmethod
+    def forward(ctx, a, s, z, T, nu, cloud, Ts, Tt):
+        W, L = nu.shape[0], z.shape[0] - 1
+        Tsv, Ttv = _temperature_value(Ts), _temperature_value(Tt)
+        coef = _coefficients_struct(a, s, z, T, nu, cloud, L, W)
+        dev = nu.device
+        e = lambda *shape: torch.empty(shape, dtype=torch.float64, device=dev)
+        outs = (e(W, L), e(W, L), e(W, L + 1), e(W), e(W))
+        stream = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+        _lib.check(_lib.lib().adrt_raw_optics(W, L, C.byref(coef), Tsv, Ttv,
+                                              *(C.c_void_p(o.data_ptr()) for o in outs),
+                                              stream))
+        ctx.save_for_backward(a, s, z, T, nu, cloud)
+        ctx.temps = (Tsv, Ttv, Ts.shape if isinstance(Ts, torch.Tensor) else None,
+                     Tt.shape if isinstance(Tt, torch.Tensor) else None)
+        return outs
+
+    @staticmethod
+    def backward(ctx, *gouts):
+        a, s, z, T, nu, cloud = ctx.saved_tensors
+        W, L = nu.shape[0], z.shape[0] - 1
+        Tsv, Ttv, ts_shape, tt_shape = ctx.temps
+        need = ctx.needs_input_grad
+        dev = nu.device
+        e = lambda *shape: torch.zeros(shape, dtype=torch.float64, device=dev)
+        gr = {"absorption_coeff": e(L + 1, W) if need[0] else None,
+              "scattering_coeff": e(L + 1, W) if need[1] else None,
+              "cloud_optical_depth": e(L, W) if need[5] and cloud is not None else None,
+              "temperature": e(L + 1) if need[3] else None,
+              "surface_temperature": e(1) if need[6] and ts_shape is not None else None,
+              "top_temperature": e(1) if need[7] and tt_shape is not None else None}
+        cot = [None if g is None else g.to(torch.float64).contiguous() for g in gouts]
+        ct = _lib.RawOpticsCotangents(**{k: None if g is None else C.c_void_p(g.data_ptr())
+                                         for k, g in zip(RAW_OPTICS_NAMES, cot)})
+        g = _lib.RawGradients(**{k: None if v is None else C.c_void_p(v.data_ptr())
+                                 for k, v in gr.items()})
+        coef = _coefficients_struct(a, s, z, T, nu, cloud, L, W)
+        stream = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+        _lib.check(_lib.lib().adrt_raw_optics_adjoint(W, L, C.byref(coef), Tsv, Ttv,
+                                                      C.byref(ct), C.byref(g), stream))
+        ts = gr["surface_temperature"]
+        tt = gr["top_temperature"]
+        return (gr["absorption_coeff"], gr["scattering_coeff"], None, gr["temperature"], None,
+                gr["cloud_optical_depth"], None if ts is None else ts.reshape(ts_shape),
+                None if tt is None else tt.reshape(tt_shape))
+
+
+def raw_optics(absorption_coeff, scattering_coeff, altitude, temperature, wavenumber,
+               cloud_optical_depth=None, surface_temperature=-1.0, top_temperature=-1.0):
+    """Optics of the raw-coefficient entry (``adrt_raw_optics``), differentiable.
+
+    Inputs as ``solve_batch_from_coefficients``; surface_temperature and
+    top_temperature may be 0-dim tensors to get their gradients (a negative
+    value means "use the bottom / top level", as there).  Returns a dict of
+    delta_tau and single_scat_albedo ``[W][L]``, planck_levels ``[W][L+1]``,
+    surface_emission and top_emission ``[W]``, TOA first: the arrays the
+    reference's preprocessRawKernel (src_cuda/cuda_solver.cu:1131-1182)
+    prepares.  Gradients flow to the coefficients, the cloud depths and the
+    temperatures; altitudes and wavenumbers get none, and a layer whose optical
+    depth is clamped at zero passes nothing back.
+    """
+    outs = _RawOpticsFunction.apply(absorption_coeff, scattering_coeff, altitude, temperature,
+                                    wavenumber, cloud_optical_depth, surface_temperature,
+                                    top_temperature)
+    r = dict(zip(RAW_OPTICS_NAMES, outs))
+    grad = lambda *xs: any(isinstance(x, torch.Tensor) and x.requires_grad for x in xs)
+    # each output depends on one group of inputs only: keep the others off the graph
+    if not grad(absorption_coeff, scattering_coeff, cloud_optical_depth):
+        for k in RAW_OPTICS_NAMES[:2]:
+            r[k] = r[k].detach()
+    if not grad(temperature, surface_temperature, top_temperature):
+        for k in RAW_OPTICS_NAMES[2:]:
+            r[k] = r[k].detach()
+    return r
+
+
+def solve_from_coefficients(absorption_coeff, scattering_coeff, altitude, temperature,
+                            wavenumber, phase_moments, config: BatchConfig,
+                            cloud_optical_depth=None, surface_temperature=-1.0,
+                            top_temperature=-1.0, surface_albedo=None, precision="f64",
+                            max_workspace_bytes=0):
+    """Differentiable solve from raw retrieval coefficients: ``raw_optics``
+    chained with ``solve``.  The forward result equals
+    ``solve_batch_from_coefficients`` on the same inputs; gradients flow to the
+    coefficients, cloud depths, temperatures, phase moments and surface_albedo
+    (above quadrature order 16: to the temperatures only, as ``solve``)."""
+    r = raw_optics(absorption_coeff, scattering_coeff, altitude, temperature, wavenumber,
+                   cloud_optical_depth, surface_temperature, top_temperature)
+    return solve(r["delta_tau"], r["single_scat_albedo"], phase_moments, config,
+                 planck_levels=r["planck_levels"], surface_emission=r["surface_emission"],
+                 top_emission=r["top_emission"], surface_albedo=surface_albedo,
+                 precision=precision, max_workspace_bytes=max_workspace_bytes)
+
+
 def gauss_legendre(n: int):
     import numpy as np
     mu = np.zeros(n)

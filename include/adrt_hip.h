@@ -212,6 +212,50 @@ int adrt_solve_batch_from_coefficients(const adrt_batch_config* cfg,
                                        const adrt_device_outputs* out, adrt_precision precision,
                                        adrt_workspace* ws, hipStream_t stream);
 
+/* The optics the raw entry prepares, for all num_wavenumbers columns at once
+ * into caller buffers: delta_tau and single_scat_albedo [W][L], planck_levels
+ * [W][L+1], surface_emission and top_emission [W], TOA first; the rules of
+ * adrt_solve_batch_from_coefficients.  Feeding them to adrt_solve_batch gives
+ * that entry's result.  No counterpart of its own in the reference (its
+ * preprocessRawKernel, src_cuda/cuda_solver.cu:1131-1182, writes the same
+ * arrays into its workspace).  Asynchronous on stream. */
+int adrt_raw_optics(int num_wavenumbers, int num_layers, const adrt_device_coefficients* coef,
+                    double surface_temperature, double top_temperature, double* delta_tau,
+                    double* single_scat_albedo, double* planck_levels, double* surface_emission,
+                    double* top_emission, hipStream_t stream);
+
+/* Cotangents of adrt_raw_optics' outputs (layouts as there); NULL = zero. */
+typedef struct adrt_raw_optics_cotangents {
+  const double* delta_tau;
+  const double* single_scat_albedo;
+  const double* planck_levels;
+  const double* surface_emission;
+  const double* top_emission;
+} adrt_raw_optics_cotangents;
+
+/* Gradients written by adrt_raw_optics_adjoint, in the layouts of
+ * adrt_device_coefficients; NULL = not wanted.  temperature [L+1] and the two
+ * boundary temperatures (one double each, 0 when the temperature is not set)
+ * are sums over the wavenumbers. */
+typedef struct adrt_raw_gradients {
+  double* absorption_coeff;      /* [L+1][W] */
+  double* scattering_coeff;      /* [L+1][W] */
+  double* cloud_optical_depth;   /* [L][W] (written only if the coefficients have clouds) */
+  double* temperature;           /* [L+1] */
+  double* surface_temperature;   /* [1] */
+  double* top_temperature;       /* [1] */
+} adrt_raw_gradients;
+
+/* Reverse of adrt_raw_optics: with the cotangents of the optics (from
+ * adrt_adjoint_batch run on them), the gradients w.r.t. the raw coefficients
+ * and temperatures.  Clamped layers (tau <= 0) pass nothing back; altitudes and
+ * wavenumbers get no gradient.  No counterpart in the reference (it has no
+ * derivatives of the raw path).  Asynchronous on stream; deterministic. */
+int adrt_raw_optics_adjoint(int num_wavenumbers, int num_layers,
+                            const adrt_device_coefficients* coef, double surface_temperature,
+                            double top_temperature, const adrt_raw_optics_cotangents* cot,
+                            const adrt_raw_gradients* grad, hipStream_t stream);
+
 /* Checks delta_tau >= 0 and single_scat_albedo in [0,1] on the device and
  * returns ADRT_ERR_NEGATIVE_TAU / ADRT_ERR_SSA_RANGE (synchronises the stream). */
 int adrt_validate_inputs(const adrt_batch_config* cfg, const adrt_device_inputs* in,

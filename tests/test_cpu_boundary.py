@@ -96,6 +96,27 @@ def test_raw_entry_rejects_bad_arguments_without_gpu():
     assert f(C.byref(cfg), C.byref(coef), -1.0, -1.0, C.byref(out), 0, null, null) == 0
 
 
+def test_raw_optics_entries_reject_bad_arguments_without_gpu():
+    L = _lib.lib()
+    coef = _lib.DeviceCoefficients()
+    null = C.c_void_p()
+    outs = [null] * 5
+    f = L.adrt_raw_optics
+    assert f(4, 3, None, -1.0, -1.0, *outs, null) == 23
+    assert f(-1, 3, C.byref(coef), -1.0, -1.0, *outs, null) == 18
+    assert f(4, 0, C.byref(coef), -1.0, -1.0, *outs, null) == 1
+    assert f(4, 3, C.byref(coef), -1.0, -1.0, *outs, null) == 23
+    assert f(0, 3, C.byref(coef), -1.0, -1.0, *outs, null) == 0  # empty: nothing launched
+    g = L.adrt_raw_optics_adjoint
+    cot, grad = _lib.RawOpticsCotangents(), _lib.RawGradients()
+    assert g(4, 3, None, -1.0, -1.0, C.byref(cot), C.byref(grad), null) == 23
+    assert g(4, 3, C.byref(coef), -1.0, -1.0, None, C.byref(grad), null) == 23
+    assert g(4, 3, C.byref(coef), -1.0, -1.0, C.byref(cot), None, null) == 23
+    assert g(-2, 3, C.byref(coef), -1.0, -1.0, C.byref(cot), C.byref(grad), null) == 18
+    assert g(4, 0, C.byref(coef), -1.0, -1.0, C.byref(cot), C.byref(grad), null) == 1
+    assert g(4, 3, C.byref(coef), -1.0, -1.0, C.byref(cot), C.byref(grad), null) == 23
+
+
 @pytest.mark.parametrize("k,W,L,N", [(1, 1000, 50, 8), (2, 16384, 100, 16), (3, 8192, 80, 32),
                                      (4, 4096, 60, 8), (5, 65536, 100, 16)])
 def test_workload_shapes(k, W, L, N):
