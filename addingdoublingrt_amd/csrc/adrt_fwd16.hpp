@@ -375,31 +375,16 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))
   }
 
   // ---- layer assembly (src/doubling.hpp:211-229), sources combined ----
-  if constexpr (REDO) {
-    // to the FP32 register layout through LDS: R, then S = I - T
-    float* om = (float*)out;
-    put<R>(tile, L, Rm);
-    put<R>(tile2, L, Tm);
-    wave_sync();
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int i = row<float>(L.h, r);
-      om[L.l * 4 + r] = (float)tile[j * TS + i];
-      om[N * N + L.l * 4 + r] = (float)(((i == j) ? 1.0 : 0.0) - tile2[j * TS + i]);
-    }
-    if (L.h == 0) {
-      out[N * N + j] = (y * B_bar + z * B_d) + sus;
-      out[N * N + N + j] = (y * B_bar - z * B_d) + sds;
-    }
-  } else {
-    gstore<R>(out, L, Rm);
-    gstore<R>(out + N * N, L, Tm);
-    if (L.h == 0) {
-      out[2 * N * N + j] = (y * B_bar + z * B_d) + sus;
-      out[2 * N * N + N + j] = (y * B_bar - z * B_d) + sds;
-    }
+  // REDO: the FP64 record stays FP64 (flag 2) in a mixed chunk: an unstable
+  // start leaves entries far above one in R and T whose FP32 rounding the
+  // adding would amplify
+  gstore<R>(out, L, Rm);
+  gstore<R>(out + N * N, L, Tm);
+  if (L.h == 0) {
+    out[2 * N * N + j] = (y * B_bar + z * B_d) + sus;
+    out[2 * N * N + N + j] = (y * B_bar - z * B_d) + sds;
   }
-  if (L.l == 0) a.lflag[task] = 1;
+  if (L.l == 0) a.lflag[task] = REDO ? 2 : 1;
 }
 
 // ---------------------------------------------------------------------------
@@ -446,7 +431,8 @@ __device__ __forceinline__ void ibar16(const FwdArgs& a, long gw, int l, double 
 // The matrices are read on demand so that they are live only where used.
 // F32: records of the mixed-precision layer kernel (R and S = I - T in FP32
 // register order of the FP32 matrix instruction, rows 4h + r; sources FP64
-// after them); T is rebuilt as I - S in FP64.
+// after them); T is rebuilt as I - S in FP64.  Layers the FP64 kernel redid
+// (flag 2) keep the FP64 record.
 template <bool ADJ, bool F32>
 struct LayerRef {
   const FwdArgs& a;
@@ -454,6 +440,7 @@ struct LayerRef {
   long gw;
   double mu_j, wt_j;
   int scat;
+  __device__ bool narrow() const { return F32 && scat != 2; }
   // element (row<double>(h, r), j) of a stored N x N matrix, or of its
   // transpose; eye_minus: the record holds I - M
   __device__ static void load(const double* m, const Lane& L, bool trans, bool eye_minus,
@@ -473,14 +460,26 @@ struct LayerRef {
       else gload<double>(m, L, M);
     }
   }
+  __device__ static void load64(const double* m, const Lane& L, bool trans, Mat<double>& M) {
+    if (trans) gload_t<double>(m, L, M);
+    else gload<double>(m, L, M);
+  }
   __device__ const double* tmat() const {
     return F32 ? (const double*)((const float*)rec + N * N) : rec + N * N;
   }
   __device__ void mats(const Lane& L, Mat<double>& R, Mat<double>& T) const {
-    load(rec, L, ADJ, false, R);
-    load(tmat(), L, ADJ, true, T);
+    if (narrow()) {
+      load(rec, L, ADJ, false, R);
+      load(tmat(), L, ADJ, true, T);
+    } else {
+      load64(rec, L, ADJ, R);
+      load64(rec + N * N, L, ADJ, T);
+    }
   }
-  __device__ void trans(const Lane& L, Mat<double>& T) const { load(tmat(), L, ADJ, true, T); }
+  __device__ void trans(const Lane& L, Mat<double>& T) const {
+    if (narrow()) load(tmat(), L, ADJ, true, T);
+    else load64(rec + N * N, L, ADJ, T);
+  }
   __device__ void sources(const Lane& L, int l, double& su, double& sd) const {
     if constexpr (ADJ) {
       double u, d;
@@ -489,7 +488,7 @@ struct LayerRef {
       ibar16(a, gw, l + 1, mu_j, wt_j, u, d);
       sd = u;
     } else {
-      const double* src = rec + (F32 ? N * N : 2 * N * N);
+      const double* src = rec + (narrow() ? N * N : 2 * N * N);
       su = src[L.j];
       sd = src[N + L.j];
     }

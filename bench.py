@@ -18,6 +18,9 @@ per step, reverse doubling 20 N^3 per step, counted from the inputs) over its
 time from HIP events the library records around each launch on the solve
 stream; peak = MI355X FP64 78.6 TFLOP/s (FP32 matrix cores, 157.3, for the
 mixed form).
+mixed_vs_f64 (forward + reverse FP64 line at N <= 16): the mixed form run
+on the same inputs after the timed region, with its time per step and its
+largest differences from the FP64 results.
 cpu_baseline: the FP64 CPU oracle (a restatement of the reference solver,
 forward only: the reference has no adjoint) on a bounded sample of the same
 columns, on this host's cores.
@@ -69,6 +72,9 @@ def parse():
     p.add_argument("--forward-only", action="store_true",
                    help="time the forward solve alone (any quadrature order; the default "
                         "line is forward + reverse pass)")
+    p.add_argument("--no-compare", action="store_true",
+                   help="skip the untimed mixed-form run that the forward + reverse FP64 line "
+                        "reports beside its value (speed and largest differences)")
     p.add_argument("--workspace-gb", type=float, default=96.0)
     p.add_argument("--copy-mib", type=int, default=0,
                    help="only run a plain device copy of this many MiB, three times (the "
@@ -104,6 +110,31 @@ def cpu_baseline(cols, planck_host, seconds):
     return dict(value=n2 / dt2, unit="column-wavenumbers/s", cores=threads, kind="port",
                 sample=f"first {n2} of the {cols.W} columns, FP64 forward only "
                        f"(the reference has no adjoint), {dt2:.1f} s on {threads} threads")
+
+
+def compare_mixed(a, step_with, out, grads, W):
+    """The mixed form on the same inputs, after the timed region: its time per
+    step (same protocol) and its largest differences from the FP64 step just
+    measured -- outputs relative to each column's largest magnitude, gradients
+    relative to each array's largest magnitude."""
+    import torch
+
+    mout = {k: torch.empty_like(v) for k, v in out.items()}
+    mgrads = {}
+    step_with("mixed", mout, mgrads)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        step_with("mixed", mout, mgrads)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / a.steps
+    err = {}
+    for k in ("flux_up", "flux_down", "mean_intensity", "flux_direct"):
+        scale = out[k].abs().amax(dim=1, keepdim=True).clamp_min(1e-300)
+        err[k] = float(((mout[k] - out[k]).abs() / scale).max())
+    for k, v in grads.items():
+        err["d_" + k] = float((mgrads[k] - v).abs().max() / v.abs().max().clamp_min(1e-300))
+    return {"ms_per_step": ms, "value": W / (ms * 1e-3), "max_rel_diff": err}
 
 
 def main():
@@ -145,14 +176,17 @@ def main():
 
     adjoint_ws = not a.forward_only or a.form == "mixed"
 
-    def step():
+    def step_with(form, out, grads):
         o = solve_batch(*inputs, **kw, out=out, max_workspace_bytes=ws_bytes,
-                        adjoint_workspace=adjoint_ws, precision=a.form)
+                        adjoint_workspace=adjoint_ws, precision=form)
         gen = o.pop("_generation", None)
         if not a.forward_only:
             grads.update(adjoint_batch(*inputs, out, cot, **kw, forward_generation=gen,
                                        max_workspace_bytes=ws_bytes, check_status=False,
-                                       precision=a.form))
+                                       precision=form))
+
+    def step():
+        step_with(a.form, out, grads)
 
     for _ in range(a.warmup):
         step()
@@ -242,6 +276,8 @@ def main():
             "launches_per_step": launches, "avg_launch_ms": avg_ms,
         },
     }
+    if a.form == "f64" and not a.forward_only and not a.no_compare and cols.N <= 16:
+        res["mixed_vs_f64"] = compare_mixed(a, step_with, out, grads, W)
     if not a.no_cpu:
         ph = (None, None, None)
         if cols.thermal:
