@@ -135,3 +135,13 @@ def test_workload_is_deterministic():
     b = workloads.make(2, W=5)
     assert np.array_equal(a.delta_tau, b.delta_tau) and np.array_equal(a.phase_moments,
                                                                        b.phase_moments)
+
+
+def test_raw_optics_rejects_host_tensors():
+    torch = pytest.importorskip("torch")
+    from addingdoublingrt_amd import raw_optics
+
+    L, W = 3, 2
+    t = lambda *shape: torch.ones(shape, dtype=torch.float64)
+    with pytest.raises(TypeError, match="device tensor"):
+        raw_optics(t(L + 1, W), t(L + 1, W), t(L + 1), t(L + 1), t(W))
