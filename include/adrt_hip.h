@@ -22,6 +22,14 @@
  *   adrt_planck / _deriv    <- adrt::planckFunction(Deriv)   src/planck.hpp, src/planck.cpp:35-177
  *   status codes 1..17      <- the std::invalid_argument cases of
  *                              adrt::ADConfig::validate     src/adding_doubling.hpp:123-174
+ *
+ *   adrt_planck_levels      <- the device Planck             src_cuda/cuda_planck.cuh:33-122
+ *   adrt_validate_inputs    <- the tau / omega range checks of adrt::ADConfig::validate,
+ *                              on the device
+ *
+ * Without a counterpart (the reference has no derivatives): adrt_adjoint_batch
+ * and its workspace flag, adrt_raw_optics / adrt_raw_optics_adjoint (the
+ * raw-coefficient optics and their reverse), adrt_profile / adrt_profile_read.
  */
 #ifndef ADRT_HIP_H
 #define ADRT_HIP_H
