@@ -15,7 +15,9 @@
 * Closed forms and invariants of those test files (Beer's law with and without
   a Lambertian surface, the beam term of the mean intensity, the linear-in-tau
   pure-absorption source, flux conservation at omega = 1, delta-M leaving
-  Rayleigh scattering unchanged) at N = 8, 16 and 32.
+  Rayleigh scattering unchanged, TOA downwelling pi B_top, zero divergence in
+  an isothermal enclosure, the diffusion lower boundary's pi B + (2 pi / 3)
+  dB/dtau and its independence of the albedo) at N = 8, 16 and 32.
 """
 import numpy as np
 import pytest
@@ -192,3 +194,52 @@ def test_delta_m_rayleigh_no_effect_on_gpu(nq):   # test_ad_solver.cpp:1210-1238
     b = _gpu_column(dict(base, phase_function_moments=[ray(2 * nq + 1)], use_delta_m=True))
     for k in ("flux_up", "flux_down"):
         assert np.allclose(a[k], b[k], rtol=1e-12, atol=0)
+
+
+def _thermal_column(nq, tau, ssa, B, surface_emission, top_emission, **bc):
+    """One thermal column on the batched path, isotropic scattering."""
+    L = len(tau)
+    mom = np.zeros((L, 2 * nq))
+    mom[:, 0] = 1.0
+    cfg = BatchConfig(num_quadrature=nq, surface_emission=surface_emission,
+                      top_emission=top_emission, **bc)
+    out = solve_batch(_t(np.array(tau)[None]), _t(np.array(ssa)[None]), _t(mom), cfg,
+                      planck_levels=_t(np.array(B)[None]))
+    torch.cuda.synchronize()
+    return {k: v[0].cpu().numpy() for k, v in out.items()}
+
+
+_T4 = [200.0, 230.0, 260.0, 290.0]
+
+
+@pytest.mark.parametrize("nq", [8, 16, 32])
+def test_top_emission_gives_pi_B_on_gpu(nq):   # test_ad_solver.cpp:1043-1084
+    B = [oracle.planck(500.0, 600.0, T) for T in _T4]
+    g = _thermal_column(nq, [0.5] * 3, [0.3] * 3, B, B[-1], B[0])
+    assert abs(g["flux_down"][0] - np.pi * B[0]) <= 1e-12 * np.pi * B[0]
+    g = _thermal_column(nq, [0.5] * 3, [0.3] * 3, B, B[-1], 0.0)   # cold space
+    assert abs(g["flux_down"][0]) < 1e-12 * np.pi * B[0]
+
+
+@pytest.mark.parametrize("nq", [8, 16, 32])
+def test_isothermal_divergence_vanishes_on_gpu(nq):   # test_ad_solver.cpp:1819-1840
+    Bv = oracle.planck(300.0, 800.0, 255.0)
+    g = _thermal_column(nq, [0.4] * 5, [0.5] * 5, [Bv] * 6, Bv, Bv)
+    # the thin-layer start makes I = B a fixed point only to rounding amplified
+    # by the doubling (oracle: |J / B - 1| <= 8.4e-12 at N = 32)
+    assert np.all(np.abs(g["flux_divergence"]) <= 1e-10 * 4 * np.pi * Bv)
+    assert np.allclose(g["mean_intensity"], Bv, rtol=1e-10, atol=0)
+
+
+@pytest.mark.parametrize("nq", [8, 16, 32])
+def test_diffusion_bc_on_gpu(nq):   # test_ad_solver.cpp:1517-1558, 1589-1624
+    B = [oracle.planck(500.0, 600.0, T) for T in _T4]
+    g = _thermal_column(nq, [0.5] * 3, [0.0] * 3, B, 0.0, B[0], use_diffusion_lower_bc=True)
+    # I_up(L) = B_L + mu dB/dtau: 2 pi sum w mu (B + mu dB) = pi B + (2 pi / 3) dB
+    exp = np.pi * B[3] + (2 * np.pi / 3) * (B[3] - B[2]) / 0.5
+    assert abs(g["flux_up"][3] - exp) <= 1e-12 * exp
+    a = _thermal_column(nq, [0.5] * 3, [0.5] * 3, B, 0.0, B[0], use_diffusion_lower_bc=True)
+    b = _thermal_column(nq, [0.5] * 3, [0.5] * 3, B, 0.0, B[0], use_diffusion_lower_bc=True,
+                        surface_albedo=0.7)
+    for k in OUTS:
+        assert np.array_equal(a[k], b[k]), k
