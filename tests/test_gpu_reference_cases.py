@@ -17,7 +17,8 @@
   pure-absorption source, flux conservation at omega = 1, delta-M leaving
   Rayleigh scattering unchanged, TOA downwelling pi B_top, zero divergence in
   an isothermal enclosure, the diffusion lower boundary's pi B + (2 pi / 3)
-  dB/dtau and its independence of the albedo) at N = 8, 16 and 32.
+  dB/dtau and its independence of the albedo, the divergence integral against
+  the net-flux change, thermal and solar) at N = 8, 16 and 32.
 """
 import numpy as np
 import pytest
@@ -243,3 +244,29 @@ def test_diffusion_bc_on_gpu(nq):   # test_ad_solver.cpp:1517-1558, 1589-1624
                         surface_albedo=0.7)
     for k in OUTS:
         assert np.array_equal(a[k], b[k]), k
+
+
+def _divergence_balance(g, dtau):
+    """Trapezoid integral of the divergence over tau against the change of the
+    net flux across the column (test_ad_solver.cpp:1842-1907)."""
+    d = g["flux_divergence"]
+    integral = np.sum(0.5 * (d[:-1] + d[1:]) * dtau)
+    fnet = g["flux_up"] - g["flux_down"] - g["flux_direct"]
+    return integral, fnet[-1] - fnet[0]
+
+
+@pytest.mark.parametrize("nq", [8, 16, 32])
+def test_divergence_integral_thermal_on_gpu(nq):   # test_ad_solver.cpp:1842-1876
+    n = 60
+    B = [oracle.planck(300.0, 800.0, 200.0 + 1.5 * l) for l in range(n + 1)]
+    g = _thermal_column(nq, [0.05] * n, [0.5] * n, B, B[-1], B[0])
+    integral, dF = _divergence_balance(g, 0.05)
+    assert abs(integral - dF) <= 0.01 * abs(dF)
+
+
+@pytest.mark.parametrize("nq", [8, 16, 32])
+def test_divergence_integral_solar_on_gpu(nq):     # test_ad_solver.cpp:1878-1907
+    n = 40
+    g = _gpu_column(_iso_cfg(nq, [0.05] * n, [0.0] * n, solar_flux=200.0, solar_mu=0.5))
+    integral, dF = _divergence_balance(g, 0.05)
+    assert abs(integral - dF) <= 0.01 * abs(dF)
