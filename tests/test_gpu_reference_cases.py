@@ -18,7 +18,8 @@
   Rayleigh scattering unchanged, TOA downwelling pi B_top, zero divergence in
   an isothermal enclosure, the diffusion lower boundary's pi B + (2 pi / 3)
   dB/dtau and its independence of the albedo, the divergence integral against
-  the net-flux change, thermal and solar) at N = 8, 16 and 32.
+  the net-flux change, thermal and solar, index reversal, workspace reuse)
+  at N = 8, 16 and 32.
 """
 import numpy as np
 import pytest
@@ -270,3 +271,39 @@ def test_divergence_integral_solar_on_gpu(nq):     # test_ad_solver.cpp:1878-190
     g = _gpu_column(_iso_cfg(nq, [0.05] * n, [0.0] * n, solar_flux=200.0, solar_mu=0.5))
     integral, dF = _divergence_balance(g, 0.05)
     assert abs(integral - dF) <= 0.01 * abs(dF)
+
+
+def _mixed_column_batch(nq, W=16, L=9, seed=3):
+    rng = np.random.default_rng(seed)
+    tau = 10 ** rng.uniform(-2, 0.7, (W, L))
+    ssa = rng.uniform(0.05, 0.99, (W, L))
+    mom = rng.uniform(0.3, 0.8, L)[:, None] ** np.arange(2 * nq)
+    B = np.sort(rng.uniform(1.0, 3.0, (W, L + 1)), axis=1)
+    return tau, ssa, mom, B
+
+
+@pytest.mark.parametrize("nq", [8, 16, 32])
+def test_index_reversal_on_gpu(nq):   # test_ad_solver.cpp:1773-1812
+    tau, ssa, mom, B = _mixed_column_batch(nq)
+    cfg = dict(num_quadrature=nq, surface_albedo=0.2, solar_flux=1.0, solar_mu=0.6)
+    a = solve_batch(_t(tau), _t(ssa), _t(mom), BatchConfig(**cfg), planck_levels=_t(B))
+    b = solve_batch(_t(tau[:, ::-1]), _t(ssa[:, ::-1]), _t(mom[::-1]),
+                    BatchConfig(index_from_bottom=True, **cfg), planck_levels=_t(B[:, ::-1]))
+    torch.cuda.synchronize()
+    for k in OUTS:
+        x, y = a[k].cpu().numpy(), b[k].cpu().numpy()[:, ::-1]
+        assert np.allclose(x, y, rtol=1e-12, atol=1e-14 * np.abs(x).max()), k
+
+
+@pytest.mark.parametrize("nq", [8, 16, 32])
+def test_workspace_reuse_on_gpu(nq):   # test_ad_solver.cpp:1733-1766
+    tau, ssa, mom, B = _mixed_column_batch(nq)
+    cfg = BatchConfig(num_quadrature=nq, surface_albedo=0.2, solar_flux=1.0, solar_mu=0.6)
+    first = solve_batch(_t(tau), _t(ssa), _t(mom), cfg, planck_levels=_t(B))
+    first = {k: v.cpu().numpy() for k, v in first.items()}
+    other = solve_batch(_t(tau * 3.0), _t(ssa * 0.5), _t(mom), cfg, planck_levels=_t(B * 2.0))
+    again = solve_batch(_t(tau), _t(ssa), _t(mom), cfg, planck_levels=_t(B))
+    torch.cuda.synchronize()
+    for k in OUTS:
+        assert np.array_equal(first[k], again[k].cpu().numpy()), k
+        assert not np.array_equal(first[k], other[k].cpu().numpy()) or k == "flux_direct", k
