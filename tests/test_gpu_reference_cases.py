@@ -18,8 +18,9 @@
   Rayleigh scattering unchanged, TOA downwelling pi B_top, zero divergence in
   an isothermal enclosure, the diffusion lower boundary's pi B + (2 pi / 3)
   dB/dtau and its independence of the albedo, the divergence integral against
-  the net-flux change, thermal and solar, index reversal, workspace reuse)
-  at N = 8, 16 and 32.
+  the net-flux change, thermal and solar, index reversal, workspace reuse,
+  layer-split invariance) at N = 8, 16 and 32, and the convergence of orders
+  6, 8, 12, 16 (6 and 12 padded with zero-weight nodes).
 """
 import numpy as np
 import pytest
@@ -307,3 +308,31 @@ def test_workspace_reuse_on_gpu(nq):   # test_ad_solver.cpp:1733-1766
     for k in OUTS:
         assert np.array_equal(first[k], again[k].cpu().numpy()), k
         assert not np.array_equal(first[k], other[k].cpu().numpy()) or k == "flux_direct", k
+
+
+@pytest.mark.parametrize("nq", [8, 16, 32])
+def test_layer_split_invariance_on_gpu(nq):   # test_ad_solver.cpp:859-886
+    kw = dict(solar_flux=1.0, solar_mu=0.5, top_emission=0.5 / np.pi, surface_albedo=0.5)
+    a = _gpu_column(_iso_cfg(nq, [1.0], [0.9], **kw))
+    b = _gpu_column(_iso_cfg(nq, [0.5, 0.5], [0.9, 0.9], **kw))
+    for k in ("flux_up", "flux_down", "mean_intensity"):
+        assert abs(a[k][0] - b[k][0]) < 1e-6 * abs(a[k][0]) + 1e-12, k
+        assert abs(a[k][1] - b[k][2]) < 1e-6 * abs(a[k][1]) + 1e-12, k
+
+
+def test_dynamic_quadrature_orders_converge_on_gpu():   # test_ad_solver.cpp:1959-1983, 2047-2075
+    """Orders 6 and 12 run padded to 8 and 16 with zero-weight nodes."""
+    vals = []
+    for nq in (6, 8, 12, 16):
+        g = _gpu_column(dict(num_layers=1, num_quadrature=nq, delta_tau=[0.5],
+                             single_scat_albedo=[0.9],
+                             phase_function_moments=[list(0.5 ** np.arange(2 * nq))],
+                             solar_flux=1.0, solar_mu=0.5))
+        ref = oracle.solve(num_layers=1, num_quadrature=nq, delta_tau=[0.5],
+                           single_scat_albedo=[0.9],
+                           phase_function_moments=[list(0.5 ** np.arange(2 * nq))],
+                           solar_flux=1.0, solar_mu=0.5)
+        assert abs(g["flux_up"][0] - ref["flux_up"][0]) <= 1e-10 * abs(ref["flux_up"][0]), nq
+        vals.append(g["flux_up"][0])
+    assert abs(vals[0] - vals[-1]) < 0.01 * abs(vals[-1])
+    assert abs(vals[2] - vals[3]) < abs(vals[0] - vals[3])
