@@ -505,8 +505,9 @@ __device__ void layer16_adjoint_task(const FwdArgs& a, const AdjArgs& b, long ta
       s3[j] = gam * b2 + sds;
     }
     wave_sync();
+    const bool last = k == nn - 1;  // the last R_new, T_new are never read back
     mul<R>(t2, Rm, L, Y);  // TGR
-    mul<R>(t2, Tm, L, X);  // T_new = TG T
+    if (!last) mul<R>(t2, Tm, L, X);  // T_new = TG T
     double a1, a3, c1 = 0.0, c2 = 0.0;
     if (solar) {
       double s4v[4];
@@ -523,11 +524,11 @@ __device__ void layer16_adjoint_task(const FwdArgs& a, const AdjArgs& b, long ta
     }
     put<R>(t0, L, Y);
     wave_sync();
-    mul_add<R>(t0, Tm, Rm, L, Rm);  // R_new = R + TGR T
+    if (!last) mul_add<R>(t0, Tm, Rm, L, Rm);  // R_new = R + TGR T
     double s2v[2];
     matvecs<R, 2>(t0, s0, L, s2v);
     wave_sync();
-    Tm = X;
+    if (!last) Tm = X;
     const double znew = (a1 - s2v[0]) + z - gk * y;
     const double ynew = a3 + s2v[1] + y;
     z = znew;
