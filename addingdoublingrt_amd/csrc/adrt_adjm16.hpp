@@ -214,21 +214,26 @@ __device__ void layer16m_adjoint_task(const FwdArgs& a, const AdjArgs& b, long t
       gu = s2[0];
       gy = s2[1];
     }
-    Mat<R> GS;
-    mul<R>(tB, Sm, L, GS);
+    const bool last = k == nn - 1;  // the last R', S' are never read back
+    if (!last) {
+      Mat<R> GS;
+      mul<R>(tB, Sm, L, GS);
 #pragma unroll
-    for (int r = 0; r < 4; ++r) X.v[r] = (G.v[r] + Sm.v[r]) - GS.v[r];  // S'
+      for (int r = 0; r < 4; ++r) X.v[r] = (G.v[r] + Sm.v[r]) - GS.v[r];  // S'
+    }
     wave_sync();
     put<R>(tA, L, TGR);
     wave_sync();
-    Mat<R> TS_;
-    mul<R>(tA, Sm, L, TS_);
+    if (!last) {
+      Mat<R> TS_;
+      mul<R>(tA, Sm, L, TS_);
 #pragma unroll
-    for (int r = 0; r < 4; ++r) Rm.v[r] = (Rm.v[r] + TGR.v[r]) - TS_.v[r];
+      for (int r = 0; r < 4; ++r) Rm.v[r] = (Rm.v[r] + TGR.v[r]) - TS_.v[r];
+    }
     float s2[2];
     matvecs<R, 2>(tA, fs, L, s2);
     wave_sync();
-    Sm = X;
+    if (!last) Sm = X;
     const double znew = ((u - gu) - (double)s2[0]) + z - gk * y;
     const double ynew = ((y - gy) + (double)s2[1]) + y;
     z = znew;
