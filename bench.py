@@ -42,15 +42,18 @@ FP64_PEAK_TFLOPS = 78.6
 FP32_MFMA_PEAK_TFLOPS = 157.3  # the mixed form's doubling runs on FP32 matrix cores
 KINDS = ("prep", "layer_doubling", "adding_interfaces", "reverse_adding_interfaces",
          "reverse_layer_doubling", "reverse_column_terms", "reverse_start_terms")
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "fwd_adj_f64_n16_65536x100_hbm_traffic.md")
+TRAFFIC_FILES = {form: os.path.join(ROOT, "profiles",
+                                    f"fwd_adj_{form}_n16_65536x100_hbm_traffic.md")
+                 for form in ("f64", "mixed")}
 
 
 def profiled_traffic(kernel, W, config, form):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE /
     WRITE_SIZE summary, for the workload it was measured on; None otherwise."""
-    if (W, config, form) != (65536, 5, "f64") or not os.path.exists(TRAFFIC_FILE):
+    path = TRAFFIC_FILES.get(form)
+    if (W, config) != (65536, 5) or path is None or not os.path.exists(path):
         return None
-    with open(TRAFFIC_FILE) as f:
+    with open(path) as f:
         for line in f:
             if line.startswith("bench-traffic:"):
                 name, val = line[len("bench-traffic:"):].strip().rsplit(None, 1)
