@@ -23,6 +23,20 @@ HIPCC_FLAGS = [
 SOURCES = ["adrt_capi.hip"]
 
 
+def source_digest() -> str:
+    """Short SHA-256 of the HIP sources (csrc/, sorted by name): stamps a
+    measurement with the kernels it was taken on."""
+    import hashlib
+
+    h = hashlib.sha256()
+    for name in sorted(os.listdir(CSRC)):
+        if name.endswith((".hip", ".hpp")):
+            h.update(name.encode())
+            with open(os.path.join(CSRC, name), "rb") as f:
+                h.update(f.read())
+    return h.hexdigest()[:16]
+
+
 def build(verbose: bool = False) -> str:
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
     tmp = OUT + ".tmp"

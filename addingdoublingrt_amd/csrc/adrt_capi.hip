@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cfloat>
 #include <cmath>
 #include <cstdint>
@@ -30,6 +31,11 @@ constexpr double kPi = 3.14159265358979323846;
   do {                                        \
     if ((expr) != hipSuccess) return ADRT_ERR_HIP; \
   } while (0)
+
+// Tokens of forward solves, unique across every workspace of the process, so
+// that a token kept from one workspace can never match a solve on another
+// (for instance on a workspace rebuilt at the same address).
+std::atomic<unsigned long long> g_forward_tokens{0};
 
 int padded_order(int n) {
   int p = 2;
@@ -346,7 +352,7 @@ struct adrt_workspace {
          *mom_part = nullptr, *phase = nullptr, *raw = nullptr;
   int* status = nullptr;
   int nwaves = 0, nn_cap = 0;
-  unsigned long long generation = 0;  // forward solves so far
+  unsigned long long generation = 0;  // token of the last forward solve (unique in the process)
   bool kept_valid = false;            // the last forward's state (one chunk) is still held
   bool kept_mixed = false;            // ... and was computed by the mixed-precision form
   double xfac = 0.0;
@@ -735,7 +741,7 @@ int adrt_solve_batch(const adrt_batch_config* cfg, const adrt_device_inputs* in,
     rc = forward_chunk(a, ws, stream, precision == ADRT_PRECISION_MIXED);
     if (rc) return rc;
   }
-  ws->generation += 1;
+  ws->generation = ++g_forward_tokens;
   ws->kept_valid = ws->Iv != nullptr && cfg->num_wavenumbers <= ws->chunk;
   ws->kept_mixed = precision == ADRT_PRECISION_MIXED;
   return ADRT_OK;

@@ -55,6 +55,9 @@ class Workspace:
         self.bytes = b.value
         self.chunk = ch.value
         self.created_for = num_wavenumbers
+        # held around every call that enqueues work on this workspace, so that
+        # two host threads never interleave their use of its scratch
+        self.lock = threading.Lock()
 
     def generation(self) -> int:
         g = C.c_ulonglong()
@@ -77,8 +80,11 @@ _ws_lock = threading.Lock()
 
 def get_workspace(num_wavenumbers, num_layers, num_quadrature, num_moments, max_bytes=0,
                   adjoint=False):
-    key = (torch.cuda.current_device(), num_layers, num_quadrature, num_moments, int(max_bytes),
-           bool(adjoint))
+    """The cached workspace for this column shape on the current device and
+    stream: work on one stream runs in order, so solves on one stream may share
+    scratch; another stream gets a workspace of its own."""
+    key = (torch.cuda.current_device(), torch.cuda.current_stream().cuda_stream, num_layers,
+           num_quadrature, num_moments, int(max_bytes), bool(adjoint))
     with _ws_lock:
         ws = _ws_cache.get(key)
         # a workspace serves any batch size (the library chunks); rebuild it
@@ -145,6 +151,16 @@ def solve_batch(delta_tau, single_scat_albedo, phase_moments, config: BatchConfi
     workspace that keeps what ``adjoint_batch`` needs; the returned dict then
     also holds ``"_generation"``, the token that lets the reverse pass reuse it.
     """
+    return _solve_batch_ws(delta_tau, single_scat_albedo, phase_moments, config, planck_levels,
+                           surface_emission, top_emission, precision, out, validate,
+                           max_workspace_bytes, adjoint_workspace)[0]
+
+
+def _solve_batch_ws(delta_tau, single_scat_albedo, phase_moments, config: BatchConfig,
+                    planck_levels=None, surface_emission=None, top_emission=None,
+                    precision="f64", out=None, validate=False, max_workspace_bytes=0,
+                    adjoint_workspace=False):
+    """solve_batch, also returning the workspace the solve ran on."""
     W, L, M, shared, cfg, inp = _prepare(delta_tau, single_scat_albedo, phase_moments, config,
                                          planck_levels, surface_emission, top_emission)
     if out is None:
@@ -153,13 +169,15 @@ def solve_batch(delta_tau, single_scat_albedo, phase_moments, config: BatchConfi
     o = _outputs_struct(out, W, L)
     ws = get_workspace(W, L, config.num_quadrature, M, max_workspace_bytes, adjoint_workspace)
     stream = C.c_void_p(torch.cuda.current_stream().cuda_stream)
-    if validate:
-        _lib.check(_lib.lib().adrt_validate_inputs(C.byref(cfg), C.byref(inp), ws.handle, stream))
-    _lib.check(_lib.lib().adrt_solve_batch(C.byref(cfg), C.byref(inp), C.byref(o),
-                                           _PREC[precision], ws.handle, stream))
-    if adjoint_workspace:
-        out["_generation"] = ws.generation()
-    return out
+    with ws.lock:
+        if validate:
+            _lib.check(_lib.lib().adrt_validate_inputs(C.byref(cfg), C.byref(inp), ws.handle,
+                                                       stream))
+        _lib.check(_lib.lib().adrt_solve_batch(C.byref(cfg), C.byref(inp), C.byref(o),
+                                               _PREC[precision], ws.handle, stream))
+        if adjoint_workspace:
+            out["_generation"] = ws.generation()
+    return out, ws
 
 
 def solve_batch_from_coefficients(absorption_coeff, scattering_coeff, altitude, temperature,
@@ -207,9 +225,10 @@ def solve_batch_from_coefficients(absorption_coeff, scattering_coeff, altitude, 
     ws = get_workspace(W, L, config.num_quadrature, M, max_workspace_bytes,
                        precision == "mixed")
     stream = C.c_void_p(torch.cuda.current_stream().cuda_stream)
-    _lib.check(_lib.lib().adrt_solve_batch_from_coefficients(
-        C.byref(cfg), C.byref(coef), float(surface_temperature), float(top_temperature),
-        C.byref(o), _PREC[precision], ws.handle, stream))
+    with ws.lock:
+        _lib.check(_lib.lib().adrt_solve_batch_from_coefficients(
+            C.byref(cfg), C.byref(coef), float(surface_temperature), float(top_temperature),
+            C.byref(o), _PREC[precision], ws.handle, stream))
     return out
 
 
@@ -220,7 +239,7 @@ GRADIENT_NAMES = ("delta_tau", "single_scat_albedo", "phase_moments", "planck_le
 def adjoint_batch(delta_tau, single_scat_albedo, phase_moments, config: BatchConfig, outputs,
                   cotangents, planck_levels=None, surface_emission=None, top_emission=None,
                   precision="f64", forward_generation=0, max_workspace_bytes=0,
-                  want=GRADIENT_NAMES, check_status=True):
+                  want=GRADIENT_NAMES, check_status=True, workspace=None):
     """Gradients of ``sum(cotangent * output)`` w.r.t. the inputs (``adrt_adjoint_batch``).
 
     outputs: the dict ``solve_batch`` returned for the same inputs;
@@ -228,6 +247,10 @@ def adjoint_batch(delta_tau, single_scat_albedo, phase_moments, config: BatchCon
     zero).  Returns a dict keyed by ``GRADIENT_NAMES`` (the ones in ``want``):
     ``[W][L]``, moments in their input shape, ``[W][L+1]``, and ``[W]`` for the
     two emissions and the per-column parts of the surface-albedo gradient.
+    ``workspace``: the reverse-pass ``Workspace`` of the forward solve (default:
+    the cached one of this shape and stream).
+    The forward state is reused only if ``forward_generation`` is the token of
+    the last solve on that workspace; otherwise it is recomputed.
     """
     W, L, M, shared, cfg, inp = _prepare(delta_tau, single_scat_albedo, phase_moments, config,
                                          planck_levels, surface_emission, top_emission)
@@ -246,13 +269,17 @@ def adjoint_batch(delta_tau, single_scat_albedo, phase_moments, config: BatchCon
     ct = _lib.DeviceCotangents(**{k: _dev_ptr(v, "cotangent " + k, (W, L + 1))
                                   for k, v in cot.items()})
     o = _outputs_struct(outputs, W, L)
-    ws = get_workspace(W, L, config.num_quadrature, M, max_workspace_bytes, True)
+    ws = workspace if workspace is not None else get_workspace(
+        W, L, config.num_quadrature, M, max_workspace_bytes, True)
+    if not ws.adjoint or ws.shape != (L, config.num_quadrature, M):
+        raise ValueError("workspace is not a reverse-pass workspace of this column shape")
     stream = C.c_void_p(torch.cuda.current_stream().cuda_stream)
-    _lib.check(_lib.lib().adrt_adjoint_batch(C.byref(cfg), C.byref(inp), C.byref(o), C.byref(ct),
-                                             C.byref(g), _PREC[precision],
-                                             int(forward_generation), ws.handle, stream))
-    if check_status:
-        _lib.check(_lib.lib().adrt_adjoint_status(ws.handle, stream))
+    with ws.lock:
+        _lib.check(_lib.lib().adrt_adjoint_batch(C.byref(cfg), C.byref(inp), C.byref(o),
+                                                 C.byref(ct), C.byref(g), _PREC[precision],
+                                                 int(forward_generation), ws.handle, stream))
+        if check_status:
+            _lib.check(_lib.lib().adrt_adjoint_status(ws.handle, stream))
     return grads
 
 
@@ -330,19 +357,21 @@ class _SolveFunction(torch.autograd.Function):
                        top_emission is not None, surface_albedo is not None)
             outs = tuple(out[k] for k in OUTPUT_NAMES)
             ctx.save_for_backward(delta_tau, single_scat_albedo, phase_moments)
-            ctx.mark_non_differentiable(outs[3])
             return outs
-        out = solve_batch(delta_tau, single_scat_albedo, phase_moments, config, planck_levels,
-                          surface_emission, top_emission, precision=precision,
-                          max_workspace_bytes=max_ws, adjoint_workspace=True)
+        out, ws = _solve_batch_ws(delta_tau, single_scat_albedo, phase_moments, config,
+                                  planck_levels, surface_emission, top_emission,
+                                  precision=precision, max_workspace_bytes=max_ws,
+                                  adjoint_workspace=True)
         gen = out.pop("_generation")
+        # the workspace that holds this forward's state: a later solve of
+        # another shape or size may replace the cached one before backward runs
+        ctx.ws = ws
         ctx.config, ctx.precision, ctx.max_ws, ctx.generation = config, precision, max_ws, gen
         ctx.has = (planck_levels is not None, surface_emission is not None,
                    top_emission is not None, surface_albedo is not None)
         outs = tuple(out[k] for k in OUTPUT_NAMES)
         ctx.save_for_backward(delta_tau, single_scat_albedo, phase_moments, planck_levels,
                               surface_emission, top_emission, *outs)
-        ctx.mark_non_differentiable(outs[3])  # flux_direct depends on tau only through the beam
         return outs
 
     @staticmethod
@@ -375,7 +404,8 @@ class _SolveFunction(torch.autograd.Function):
             want.append("surface_albedo")
         g = adjoint_batch(tau, ssa, mom, ctx.config, out, cot, B, Bs, Bt,
                           precision=ctx.precision, forward_generation=ctx.generation,
-                          max_workspace_bytes=ctx.max_ws, want=tuple(want))
+                          max_workspace_bytes=ctx.max_ws, want=tuple(want), workspace=ctx.ws)
+        ctx.ws = None
         gA = g["surface_albedo"].sum() if hasA else None
         return (None, None, None, g["delta_tau"], g["single_scat_albedo"], g["phase_moments"],
                 g.get("planck_levels"), g.get("surface_emission"), g.get("top_emission"), gA)
@@ -391,8 +421,10 @@ def solve(delta_tau, single_scat_albedo, phase_moments, config: BatchConfig, pla
     0-dim tensor that replaces ``config.surface_albedo``) through the
     hand-written reverse pass (num_quadrature <= 16); above 16 to
     planck_levels, surface_emission and top_emission (so to temperatures
-    through ``planck_levels``) from forward solves with unit sources.  flux_direct is not differentiable here (it is
-    the closed form F0 mu0 exp(-tau_cum / mu0)).
+    through ``planck_levels``) from forward solves with unit sources.  flux_direct,
+    F0 mu0 exp(-tau_cum / mu0) with tau_cum the cumulative delta-M-scaled depth,
+    carries gradients to delta_tau, single_scat_albedo and (through delta-M) the
+    phase moments like the other outputs.
     """
     outs = _SolveFunction.apply(config, precision, max_workspace_bytes, delta_tau,
                                 single_scat_albedo, phase_moments, planck_levels,
@@ -547,10 +579,11 @@ def _planck_levels_raw(wn_lo, wn_hi, temperature, derivative):
     out = torch.empty((W, nlev), dtype=torch.float64, device=wn_lo.device)
     ws = get_workspace(1, 1, 2, 1)
     stream = C.c_void_p(torch.cuda.current_stream().cuda_stream)
-    _lib.check(_lib.lib().adrt_planck_levels(
-        _dev_ptr(wn_lo, "wn_lo", (W,)), _dev_ptr(wn_hi, "wn_hi", (W,)),
-        _dev_ptr(temperature.contiguous(), "temperature"), int(shared), W, nlev,
-        int(bool(derivative)), _dev_ptr(out, "out"), ws.handle, stream))
+    with ws.lock:
+        _lib.check(_lib.lib().adrt_planck_levels(
+            _dev_ptr(wn_lo, "wn_lo", (W,)), _dev_ptr(wn_hi, "wn_hi", (W,)),
+            _dev_ptr(temperature.contiguous(), "temperature"), int(shared), W, nlev,
+            int(bool(derivative)), _dev_ptr(out, "out"), ws.handle, stream))
     return out
 
 

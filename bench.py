@@ -49,17 +49,31 @@ TRAFFIC_FILES = {form: os.path.join(ROOT, "profiles",
 
 def profiled_traffic(kernel, W, config, form):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE /
-    WRITE_SIZE summary, for the workload it was measured on; None otherwise."""
+    WRITE_SIZE summary, for the workload it was measured on, and a note on
+    where it comes from.  The summary carries the digest of the HIP sources it
+    was measured on; if the sources have changed since, the figure is stale and
+    None is returned."""
+    from addingdoublingrt_amd.build import source_digest
+
     path = TRAFFIC_FILES.get(form)
     if (W, config) != (65536, 5) or path is None or not os.path.exists(path):
-        return None
+        return None, "not measured for this workload"
+    stamp, val = None, None
     with open(path) as f:
         for line in f:
+            if line.startswith("source-digest:"):
+                stamp = line.split(":", 1)[1].strip()
             if line.startswith("bench-traffic:"):
-                name, val = line[len("bench-traffic:"):].strip().rsplit(None, 1)
+                name, v = line[len("bench-traffic:"):].strip().rsplit(None, 1)
                 if name == kernel:
-                    return float(val)
-    return None
+                    val = float(v)
+    rel = os.path.relpath(path, ROOT)
+    if val is None:
+        return None, f"{kernel} not in {rel}"
+    if stamp != source_digest():
+        return None, (f"{rel} was measured on HIP sources {stamp}, these are "
+                      f"{source_digest()}: stale, not reported")
+    return val, f"{rel}, rocprofv3 FETCH_SIZE + WRITE_SIZE on HIP sources {stamp}"
 
 
 def parse():
@@ -85,14 +99,28 @@ def parse():
     return p.parse_args()
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(cols, planck_host, seconds):
-    """FP64 oracle on a bounded sample of the same columns (forward only)."""
+    """FP64 oracle on a bounded sample of the same columns (forward only), on
+    every core this process may use (at most 16, the host share of one GPU),
+    and on one thread."""
     import oracle
 
-    threads = min(os.cpu_count() or 1, 16)
+    avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    threads = max(1, min(avail or 1, 16))
     B, Bs, Bt = planck_host
 
-    def run(n):
+    def run(n, threads=threads):
         sl = slice(0, n)
         mom = cols.phase_moments if cols.phase_moments.ndim == 2 else cols.phase_moments[sl]
         t0 = time.perf_counter()
@@ -110,9 +138,18 @@ def cpu_baseline(cols, planck_host, seconds):
     dt = run(n)
     n2 = int(min(cols.W, max(n, n * seconds / max(dt, 1e-3))))
     dt2 = run(n2)
+    # one thread: a quarter of the wall time on a proportional sample
+    n1 = max(1, int(n2 / threads * (dt2 > 0) / 4) or 1)
+    dt1 = run(n1, threads=1)
     return dict(value=n2 / dt2, unit="column-wavenumbers/s", cores=threads, kind="port",
-                sample=f"first {n2} of the {cols.W} columns, FP64 forward only "
-                       f"(the reference has no adjoint), {dt2:.1f} s on {threads} threads")
+                nproc=os.cpu_count(), cores_available=avail, cpu_model=cpu_model(),
+                single_thread_value=n1 / dt1,
+                sample=f"first {n2} of the {cols.W} columns, FP64 forward only (the reference "
+                       f"has no adjoint), {dt2:.1f} s on {threads} threads; one thread: first "
+                       f"{n1} columns in {dt1:.1f} s",
+                note="the FP64 C restatement of the reference column solver in oracle/ "
+                     "(naive loops, no Eigen: the reference's Eigen build cannot be made "
+                     "offline), one column per task over a thread pool, gcc -O3 -march=x86-64-v3 -ffp-contract=off")
 
 
 def compare_mixed(a, step_with, out, grads, W):
@@ -245,6 +282,7 @@ def main():
     avg_ms = ms[dom] / max(nl[dom], 1)
     achieved = (fl_step / launches) / (avg_ms * 1e-3) / 1e12
     peak = FP64_PEAK_TFLOPS if a.form == "f64" else FP32_MFMA_PEAK_TFLOPS
+    traffic, traffic_note = profiled_traffic(kname, W, a.config, a.form)
     res = {
         "metric": METRIC if not a.forward_only else "column-wavenumbers/s forward",
         "value": W / (ms_per_step * 1e-3),
@@ -274,7 +312,8 @@ def main():
         "roofline": {
             "bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
             "frac": achieved / peak,
-            "traffic": profiled_traffic(kname, W, a.config, a.form),
+            "traffic": traffic,
+            "traffic_source": traffic_note,
             "kernel": f"{kname}: {name}",
             "launches_per_step": launches, "avg_launch_ms": avg_ms,
         },

@@ -161,9 +161,10 @@ int adrt_solve_batch(const adrt_batch_config* cfg, const adrt_device_inputs* in,
                      const adrt_device_outputs* out, adrt_precision precision,
                      adrt_workspace* ws, hipStream_t stream);
 
-/* Counter of adrt_solve_batch calls on this workspace; pass the value read
- * right after a forward solve to adrt_adjoint_batch to let it reuse that
- * solve's layer records and interface intensities. */
+/* Token of the last adrt_solve_batch on this workspace (0 before the first),
+ * unique among all forward solves of the process, on any workspace; pass the
+ * value read right after a forward solve to adrt_adjoint_batch to let it
+ * reuse that solve's layer records and interface intensities. */
 int adrt_workspace_generation(const adrt_workspace* ws, unsigned long long* generation);
 
 /* Reverse pass: gradients of sum(cotangent * output) with respect to the

@@ -766,7 +766,10 @@ struct HostResult {
 };
 
 /// Host-side convenience (src_cuda/cuda_solver.cuh:91-98): copies the inputs
-/// to the device, solves, copies the TOA results back, synchronises.
+/// to the device, solves, copies the TOA results back, synchronises.  Host
+/// vectors in the layouts of DeviceData (temperature and planck_levels
+/// [nwav * nlev], or empty); a vector of another length throws
+/// std::invalid_argument before anything is copied.
 inline HostResult solveBatchHost(const BatchConfig& config, const std::vector<double>& delta_tau,
                                  const std::vector<double>& single_scat_albedo,
                                  const std::vector<double>& phase_moments,
@@ -775,6 +778,22 @@ inline HostResult solveBatchHost(const BatchConfig& config, const std::vector<do
                                  const std::vector<double>& temperature = {}) {
   using adrt::detail::DeviceBuffer;
   const std::size_t W = (std::size_t)std::max(config.num_wavenumbers, 0);
+  const std::size_t L = (std::size_t)std::max(config.num_layers, 0);
+  const std::size_t M = (std::size_t)std::max(config.num_moments_max, 0);
+  // the device buffers are read with these extents: a shorter host vector
+  // would be read past its end
+  auto need = [](const std::vector<double>& v, std::size_t n, bool optional, const char* what) {
+    if (optional && v.empty()) return;
+    if (v.size() != n)
+      throw std::invalid_argument(std::string("solveBatchHost: ") + what + " has " +
+                                  std::to_string(v.size()) + " values, expected " +
+                                  std::to_string(n));
+  };
+  need(delta_tau, W * L, false, "delta_tau");
+  need(single_scat_albedo, W * L, false, "single_scat_albedo");
+  need(phase_moments, (phase_moments_shared ? 1 : W) * L * M, false, "phase_moments");
+  need(planck_levels, W * (L + 1), true, "planck_levels");
+  need(temperature, W * (L + 1), true, "temperature");
   DeviceBuffer b[5], o[3];
   hipStream_t s = 0;
   DeviceData d;

@@ -8,6 +8,7 @@
 //           albedo Bs Ts Bt Tt F0 mu0 wn_lo wn_hi
 //           tau[L] ssa[L] M chi[L][M] nT T[nT] nB B[nB]
 //   batch:  W L N M shared thermal delta_m diffusion from_bottom all_interfaces
+//           (host: the same input through adrt::hip::solveBatchHost, TOA results)
 //           albedo Bs Ts Bt Tt F0 mu0 wn_lo wn_hi scaling
 //           tau[W][L] ssa[W][L] chi[(W)][L][M] nT T[nT] nB B[nB]
 //   raw:    W L N M shared delta_m diffusion all_interfaces cloud
@@ -94,7 +95,7 @@ double* to_device(const std::vector<double>& v) {
   return p;
 }
 
-int batch() {
+int batch(bool host) {
   adrt::hip::BatchConfig c;
   int shared, thermal, dm, diff, fb, all;
   std::cin >> c.num_wavenumbers >> c.num_layers >> c.num_quadrature >> c.num_moments_max >>
@@ -114,6 +115,19 @@ int batch() {
   const auto T = read_n(nT);
   std::cin >> nB;
   const auto B = read_n(nB);
+  if (host) {  // adrt::hip::solveBatchHost: host vectors in, TOA results out
+    adrt::hip::HostResult r;
+    try {
+      r = adrt::hip::solveBatchHost(c, tau, ssa, mom, shared, B, T);
+    } catch (const std::invalid_argument& e) {
+      std::printf("error %s\n", e.what());
+      return 0;
+    }
+    print("flux_up", r.flux_up);
+    print("flux_down", r.flux_down);
+    print("flux_direct", r.flux_direct);
+    return 0;
+  }
   adrt::hip::DeviceData d;
   d.delta_tau = to_device(tau);
   d.single_scat_albedo = to_device(ssa);
@@ -217,7 +231,8 @@ int main() {
   std::cin >> mode;
   try {
     if (mode == "column") return column();
-    if (mode == "batch") return batch();
+    if (mode == "batch") return batch(false);
+    if (mode == "host") return batch(true);
     if (mode == "raw") return raw();
   } catch (const std::exception& e) {
     std::printf("failure %s\n", e.what());

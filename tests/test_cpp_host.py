@@ -13,7 +13,9 @@ tests/cpp/adrt_cpp_check, against the CPU oracle.
 * adrt::hip::solveBatch with temperatures on the device (band Planck in the
   library) in TOA mode and all-interfaces mode against the oracle per column;
 * adrt::hip::solveBatchFromCoefficients (raw coefficients) in both modes
-  against oracle.raw_optics + the oracle solve.
+  against oracle.raw_optics + the oracle solve;
+* adrt::hip::solveBatchHost (host vectors in, TOA results out) bit for bit
+  against adrt::hip::solveBatch on the same inputs, and against the oracle.
 """
 import os
 import subprocess
@@ -231,3 +233,46 @@ def test_batch_from_coefficients(all_interfaces):
             atol = 1e-10 * 4 * np.pi * (np.abs(ref["mean_intensity"]) + r["planck_levels"])
         bad = np.abs(g - rk) > 1e-10 * np.abs(rk) + atol
         assert not bad.any(), (k, g[bad], rk[bad])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("thermal", [True, False])
+def test_batch_host_matches_device_batch(thermal):
+    """solveBatchHost (src_cuda/cuda_solver.cuh:91-98) uploads, solves and
+    downloads: the same numbers as solveBatch on device buffers, bit for bit."""
+    rng = np.random.default_rng(17)
+    W, L, N, M = 6, 5, 8, 16
+    tau = 10 ** rng.uniform(-2, 0.5, (W, L))
+    ssa = rng.uniform(0.1, 0.95, (W, L))
+    mom = rng.uniform(0.3, 0.8, (W, L))[..., None] ** np.arange(M)
+    T = np.linspace(230.0, 290.0, L + 1)[None, :] + rng.normal(0, 2, (W, L + 1))  # [nwav][nlev]
+    lo, hi = 700.0, 760.0
+    head = (f"{W} {L} {N} {M} 0 {int(thermal)} 0 0 0 0\n"
+            f"0.25 0.0 {295.0 if thermal else -1.0!r} 0.0 -1.0 1.3 0.65 {lo!r} {hi!r} 1.0\n"
+            f"{_fmt(tau)}\n{_fmt(ssa)}\n{_fmt(mom)}\n"
+            f"{T.size if thermal else 0} {_fmt(T) if thermal else ''}\n0\n")
+    dev = run_driver("batch\n" + head)
+    host = run_driver("host\n" + head)
+    assert "error" not in dev and "error" not in host, (dev, host)
+    for k in ("flux_up", "flux_down", "flux_direct"):
+        assert np.array_equal(dev[k], host[k]), k
+    for w in range(W):
+        kw = dict(temperature=T[w], use_thermal_emission=True, surface_temperature=295.0,
+                  wavenumber_low=lo, wavenumber_high=hi) if thermal else {}
+        ref = oracle.solve(L, N, tau[w], ssa[w], mom[w], surface_albedo=0.25, solar_flux=1.3,
+                           solar_mu=0.65, **kw)
+        for k, lev in (("flux_up", 0), ("flux_down", 0), ("flux_direct", L)):
+            r = ref[k][lev]
+            assert abs(host[k][w] - r) <= 1e-10 * abs(r) + 1e-13, (w, k, host[k][w], r)
+
+
+def test_batch_host_rejects_short_vectors():
+    """A host vector shorter than its device extent is refused before any copy
+    (checked on the host; no GPU work is started)."""
+    W, L, N, M = 2, 3, 4, 8
+    text = (f"host\n{W} {L} {N} {M} 1 1 0 0 0 0\n"
+            f"0.1 0.0 290.0 0.0 -1.0 0.0 1.0 700.0 710.0 1.0\n"
+            f"{_fmt(np.full(W * L, 0.3))}\n{_fmt(np.full(W * L, 0.5))}\n"
+            f"{_fmt(np.ones(L * M))}\n{L + 1} {_fmt(np.full(L + 1, 250.0))}\n0\n")
+    got = run_driver(text)
+    assert "temperature has 4 values, expected 8" in got.get("error", ""), got

@@ -417,3 +417,58 @@ def test_source_gradients_above_order_16(n, diffusion):
     out = solve(xtau, t(ssa), t(mom), cfg, planck_levels=t(B))
     with pytest.raises(NotImplementedError):
         out["flux_up"].sum().backward()
+
+
+def test_autograd_net_flux_with_direct_beam():
+    """flux_direct = F0 mu0 exp(-tau_cum / mu0) with the delta-M-scaled depth is
+    differentiable through solve(): a loss on the net flux F_up - F_down - F_dir
+    against central differences of the extended-precision oracle, for tau,
+    omega and (through delta-M) the per-column moments."""
+    n, W, L = 4, 4, 6
+    tau, ssa, mom, B = smooth_batch(n, W=W, L=L, seed=41, M=2 * n + 3)
+    cfg = BatchConfig(num_quadrature=n, surface_albedo=0.2, solar_flux=1.5, solar_mu=0.55,
+                      use_delta_m=True)
+    rng = np.random.default_rng(8)
+    u = rng.uniform(0.2, 1.0, (W, L + 1))
+    x = {k: t(v).requires_grad_(True) for k, v in dict(tau=tau, ssa=ssa, mom=mom).items()}
+    out = solve(x["tau"], x["ssa"], x["mom"], cfg, planck_levels=t(B))
+    tu = t(u)
+    ((out["flux_up"] - out["flux_down"] - out["flux_direct"]) * tu).sum().backward()
+    cot = {"flux_up": u, "flux_down": -u, "flux_direct": -u}
+    fd, nz = fd_grads(tau, ssa, mom, cfg, cot, B, which=("tau", "ssa", "mom"))
+    loose = unstable_layers(tau, ssa, mom, cfg)
+    for k, v in fd.items():
+        assert_close(x[k].grad.cpu().numpy(), v, f"net flux d/d{NAMES[k]}", loose,
+                     skip_loose=(k == "mom"), noise=nz[k])
+    # the beam's part alone is the closed form's derivative: without it the
+    # tau gradients would differ by -sum_l u_l dF_dir(l)/dtau
+    x2 = {k: t(v).requires_grad_(True) for k, v in dict(tau=tau, ssa=ssa, mom=mom).items()}
+    o2 = solve(x2["tau"], x2["ssa"], x2["mom"], cfg, planck_levels=t(B))
+    (o2["flux_direct"] * tu).sum().backward()
+    gd = x2["tau"].grad.cpu().numpy()
+    assert np.abs(gd).max() > 1e-3 * np.abs(fd["tau"]).max()
+
+
+def test_backward_after_workspace_rebuild():
+    """Two differentiable solves of one column shape, the second larger so that
+    the cached workspace is rebuilt in between: each backward uses the
+    workspace that holds its own forward, and matches a separate run."""
+    n, L = 8, 5
+    tau, ssa, mom, B = smooth_batch(n, W=9, L=L, seed=77)
+    cfg = BatchConfig(num_quadrature=n, surface_albedo=0.2, solar_flux=1.0, solar_mu=0.6)
+    rng = np.random.default_rng(12)
+    cot = {"flux_up": rng.uniform(-1, 1, (9, L + 1)), "mean_intensity": rng.uniform(-1, 1, (9, L + 1))}
+    xs = {k: t(v[:2]).requires_grad_(True) for k, v in dict(tau=tau, ssa=ssa).items()}
+    xl = {k: t(v).requires_grad_(True) for k, v in dict(tau=tau, ssa=ssa).items()}
+    # a shape of its own, so that the small solve builds a workspace sized for 2 columns
+    cfg_s = BatchConfig(**{**cfg.__dict__})
+    o_small = solve(xs["tau"], xs["ssa"], t(mom[:2]), cfg_s, planck_levels=t(B[:2]))
+    o_large = solve(xl["tau"], xl["ssa"], t(mom), cfg, planck_levels=t(B))
+    loss = sum((o_small[k] * t(v[:2])).sum() + (o_large[k] * t(v)).sum() for k, v in cot.items())
+    loss.backward()
+    for sl, x in ((slice(0, 2), xs), (slice(0, 9), xl)):
+        g, _ = gpu_grads(tau[sl], ssa[sl], mom[sl], cfg, {k: v[sl] for k, v in cot.items()},
+                         B[sl])
+        for k, name in (("tau", "delta_tau"), ("ssa", "single_scat_albedo")):
+            np.testing.assert_allclose(x[k].grad.cpu().numpy(), g[name], rtol=1e-12,
+                                       atol=1e-15 * np.abs(g[name]).max())

@@ -12,7 +12,17 @@ the extended solve), so the bar is accuracy class, per wavenumber w and output:
 with err = max over interfaces of |x - x_ext| / |x_ext| (an absolute floor of
 1e-12 x max|column| covers exact zeros).  flux_divergence = 4 pi (1-omega)(J-B)
 is judged on the scale of its terms, 4 pi (1-omega)(|J| + |B|), because J ~ B
-cancels.  The GPU-vs-FP64-oracle difference is printed alongside.
+cancels.
+
+On every well-conditioned column -- one where the FP64 oracle itself is within
+2.5e-11 of the extended solve in that output -- the GPU must also match the
+FP64 oracle directly:
+
+    |x_gpu - x_fp64_oracle| / |x| <= 1e-10      (same error measure)
+
+which is the plain form's parity bar; the accuracy-class bound above is what
+remains for the columns where the FP64 algorithm's own rounding exceeds it.
+Each check prints the fraction of (column, output) pairs in the strict class.
 """
 import numpy as np
 import pytest
@@ -28,6 +38,7 @@ OUTS = ("flux_up", "flux_down", "mean_intensity", "flux_direct", "flux_divergenc
 RTOL = 1e-10
 CLASS_FACTOR = 4.0
 ATOL_REL = 1e-12
+WELL_CONDITIONED = 2.5e-11  # FP64 oracle error vs extended that puts a column in the strict class
 
 
 def _col_err(x, ref, scale):
@@ -49,7 +60,8 @@ def compare(gpu, ref, ext, label, div_scale, ref_runs=(), class_factor=CLASS_FAC
     ulp-perturbed copies of the inputs) whose errors widen the FP64 error that
     sets the class; ``normwise`` measures errors against the column maximum
     instead of element by element."""
-    worst_g, worst_vs_orc = 0.0, 0.0
+    worst_g, worst_vs_orc, worst_strict = 0.0, 0.0, 0.0
+    n_strict, n_all, n_direct = 0, 0, 0
     for k in OUTS:
         g = gpu[k].detach().cpu().numpy() if hasattr(gpu[k], "cpu") else gpu[k]
         r, e = ref[k], ext[k]
@@ -63,7 +75,22 @@ def compare(gpu, ref, ext, label, div_scale, ref_runs=(), class_factor=CLASS_FAC
             eo = np.maximum(eo, _col_err(rr[k], e, sc))
         bound = np.maximum(RTOL, class_factor * eo)
         worst_g = max(worst_g, float(eg.max()))
-        worst_vs_orc = max(worst_vs_orc, float(_col_err(g, r, sc).max()))
+        ev = _col_err(g, r, sc)
+        worst_vs_orc = max(worst_vs_orc, float(ev.max()))
+        strict = eo <= WELL_CONDITIONED
+        n_strict += int(strict.sum())
+        n_all += strict.size
+        n_direct += int((ev <= RTOL).sum())
+        if strict.any():
+            worst_strict = max(worst_strict, float(ev[strict].max()))
+        bad = np.where(strict & (ev > RTOL))[0]
+        if len(bad):
+            w = bad[0]
+            l = int(np.argmax(np.abs(g[w] - r[w]) / sc[w]))
+            raise AssertionError(
+                f"{label}: {k} w={w} l={l}: gpu vs fp64 oracle {ev[w]:.2e} > {RTOL:.0e} on a "
+                f"well-conditioned column (fp64 oracle err {eo[w]:.2e}); gpu={g[w, l]!r} "
+                f"orc={r[w, l]!r}; {len(bad)} columns")
         bad = np.where(eg > bound)[0]
         if len(bad):
             w = bad[0]
@@ -72,7 +99,11 @@ def compare(gpu, ref, ext, label, div_scale, ref_runs=(), class_factor=CLASS_FAC
                 f"{label}: {k} w={w} l={l}: gpu err {eg[w]:.2e} > bound {bound[w]:.2e} "
                 f"(fp64 oracle err {eo[w]:.2e}); gpu={g[w, l]!r} ext={e[w, l]!r} "
                 f"orc={r[w, l]!r}; {len(bad)} columns")
-    print(f"{label}: gpu err vs extended {worst_g:.2e}; gpu vs fp64 oracle {worst_vs_orc:.2e}")
+    print(f"{label}: gpu err vs extended {worst_g:.2e}; gpu vs fp64 oracle {worst_vs_orc:.2e}; "
+          f"strict class {n_strict}/{n_all} column-outputs "
+          f"({100.0 * n_strict / max(n_all, 1):.1f} %), worst there {worst_strict:.2e}; "
+          f"within {RTOL:.0e} of the fp64 oracle: {n_direct}/{n_all} "
+          f"({100.0 * n_direct / max(n_all, 1):.1f} %)")
     return worst_g
 
 
