@@ -17,6 +17,7 @@ PRECISION_F64 = 0
 PRECISION_MIXED = 1
 WORKSPACE_ADJOINT = 1
 PROFILE_KINDS = 7
+NOALLOC_SCRATCH_BYTES = 64 << 20  # ADRT_NOALLOC_SCRATCH_BYTES
 
 _dp = C.POINTER(C.c_double)
 
@@ -116,6 +117,8 @@ SIGNATURES = {
     "adrt_workspace_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_size_t), C.POINTER(C.c_int)]),
     "adrt_solve_batch": (C.c_int, [C.POINTER(BatchConfig), C.POINTER(DeviceInputs),
                                    C.POINTER(DeviceOutputs), C.c_int, C.c_void_p, C.c_void_p]),
+    "adrt_solve_batch_noalloc": (C.c_int, [C.POINTER(BatchConfig), C.POINTER(DeviceInputs),
+                                           C.POINTER(DeviceOutputs), C.c_void_p]),
     "adrt_solve_batch_from_coefficients": (C.c_int, [C.POINTER(BatchConfig),
                                                      C.POINTER(DeviceCoefficients), C.c_double,
                                                      C.c_double, C.POINTER(DeviceOutputs), C.c_int,

@@ -13,6 +13,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -970,6 +971,157 @@ int adrt_adjoint_status(adrt_workspace* ws, hipStream_t stream) {
   return ADRT_OK;
 }
 
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Allocation-free forward solve: the scratch of the FP64 forward (quadrature
+// tables, per-chunk optics, layer store, base composites, phase records) lives
+// in one file-scope device array instead of a workspace.  Each GPU holds its
+// own copy for the life of the process; its device address is looked up once
+// per device (hipGetSymbolAddress) and only that address is used on the host.
+// ---------------------------------------------------------------------------
+namespace {
+
+static __device__ __align__(256) double scratch[ADRT_NOALLOC_SCRATCH_BYTES / sizeof(double)];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+char* g_scratch_ptr[kMaxDevices] = {};
+
+// the device address of `scratch` on the current device (dev), looked up on
+// the first call on that device
+int scratch_address(int dev, char** out) {
+  if (dev < 0 || dev >= kMaxDevices) return ADRT_ERR_HIP;
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || p == nullptr)
+      return ADRT_ERR_HIP;
+    g_scratch_ptr[dev] = (char*)p;
+  }
+  *out = g_scratch_ptr[dev];
+  return ADRT_OK;
+}
+
+struct QuadTables {
+  double mu[ADRT_MAX_QUADRATURE], wt[ADRT_MAX_QUADRATURE];
+  double mu0;
+  int n, Np, Lmax;
+};
+
+// the tables upload_tables writes, computed on the device from the host's
+// nodes and weights: mu, wt padded to Np (pad nodes mu = 1, w = 0, P_l = 0),
+// P_l(mu_i) [Lmax][Np] and P_l(mu0) [Lmax], by the recurrence of
+// legendre_column with the same operations in the same order (no contraction),
+// so the tables are identical bit for bit
+__global__ void tables_kernel(QuadTables q, double* tab) {
+#pragma clang fp contract(off)
+  const int i = threadIdx.x;
+  double* mu = tab;
+  double* wt = mu + q.Np;
+  double* Pl = wt + q.Np;
+  double* Pl0 = Pl + (size_t)q.Lmax * q.Np;
+  if (i > q.Np) return;
+  const bool node = i < q.Np;
+  if (node) {
+    mu[i] = (i < q.n) ? q.mu[i] : 1.0;
+    wt[i] = (i < q.n) ? q.wt[i] : 0.0;
+  }
+  const bool live = !node || i < q.n;
+  const double x = node ? q.mu[i < q.n ? i : 0] : q.mu0;
+  double pm2 = 0.0, pm1 = 0.0;
+  for (int l = 0; l < q.Lmax; ++l) {
+    double p;
+    if (l == 0) p = 1.0;
+    else if (l == 1) p = x;
+    else p = ((2 * l - 1) * x * pm1 - (l - 1) * pm2) / l;
+    pm2 = pm1;
+    pm1 = p;
+    if (node) Pl[(size_t)l * q.Np + i] = live ? p : 0.0;
+    else Pl0[l] = p;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int adrt_solve_batch_noalloc(const adrt_batch_config* cfg, const adrt_device_inputs* in,
+                             const adrt_device_outputs* out, hipStream_t stream) {
+  int rc = check_config(cfg);
+  if (rc) return rc;
+  if (!in || !out) return ADRT_ERR_NULL;
+  if (cfg->num_wavenumbers == 0) return ADRT_OK;
+  // a host-side view of the scratch array with the shape of a forward
+  // workspace (no member of it allocates: its vectors stay empty)
+  adrt_workspace ws;
+  ws.L = cfg->num_layers;
+  ws.n_real = cfg->num_quadrature;
+  ws.Np = padded_order(cfg->num_quadrature);
+  ws.M = cfg->num_moments;
+  ws.Lmax = std::max(cfg->num_moments, 2 * cfg->num_quadrature);
+  rc = check_solve_args(cfg, in, out, ADRT_PRECISION_F64, &ws);
+  if (rc) return rc;
+  QuadTables q{};
+  gauss_legendre01(ws.n_real, q.mu, q.wt);
+  double s = 0.0;
+  for (int i = 0; i < ws.n_real; ++i) s += q.mu[i] * q.wt[i];
+  ws.xfac = 0.5 / s;  // src/solver.cpp:378-383
+  const bool has_solar = cfg->solar_flux > 0.0 && cfg->solar_mu > 0.0;
+  q.mu0 = has_solar ? cfg->solar_mu : 1.0;
+  q.n = ws.n_real;
+  q.Np = ws.Np;
+  q.Lmax = ws.Lmax;
+  // layout: tables and phase records once, then the chunk arrays
+  const size_t L = ws.L, Np = ws.Np;
+  const size_t tab_bytes = align256((size_t)(2 * Np + ws.Lmax * Np + ws.Lmax) * 8);
+  const size_t phase_bytes =
+      align256(Np == (size_t)adrt::dev::w16::N ? L * adrt::dev::w16::PH16 * 8 : 0);
+  const size_t per_w = per_wavenumber_bytes(ws.L, ws.Np, ws.M, false);
+  const size_t fixed = tab_bytes + phase_bytes + 12 * 256;  // + alignment of the chunk arrays
+  if (fixed + per_w > ADRT_NOALLOC_SCRATCH_BYTES) return ADRT_ERR_SCRATCH;
+  const size_t C = std::min((ADRT_NOALLOC_SCRATCH_BYTES - fixed) / per_w,
+                            (size_t)cfg->num_wavenumbers);
+  int dev = 0;
+  ADRT_HIP_TRY(hipGetDevice(&dev));
+  char* base = nullptr;
+  rc = scratch_address(dev, &base);
+  if (rc) return rc;
+  size_t off = 0;
+  auto take = [&](size_t b) {
+    char* p = base + off;
+    off += align256(b);
+    return p;
+  };
+  ws.chunk = (int)C;
+  ws.mu = (double*)take(tab_bytes);
+  ws.wt = ws.mu + Np;
+  ws.Pl = ws.wt + Np;
+  ws.Pl0 = ws.Pl + (size_t)ws.Lmax * Np;
+  ws.phase = phase_bytes ? (double*)take(phase_bytes) : nullptr;
+  ws.tau_s = (double*)take(C * L * 8);
+  ws.omega_s = (double*)take(C * L * 8);
+  ws.fdm = (double*)take(C * L * 8);
+  ws.tau_cum = (double*)take(C * (L + 1) * 8);
+  ws.mode = (int*)take(C * L * 4);
+  ws.lflag = (int*)take(C * L * 4);
+  ws.redo = (int*)take(C * L * 4);
+  ws.flag = (int*)take(4);
+  ws.lay = (double*)take(C * L * (2 * Np * Np + 2 * Np) * 8);
+  ws.rb = (double*)take(C * L * (2 * Np * Np + Np) * 8);
+  if (off > ADRT_NOALLOC_SCRATCH_BYTES) return ADRT_ERR_SCRATCH;  // not reached: sized above
+  tables_kernel<<<1, 64, 0, stream>>>(q, ws.mu);
+  if (hipGetLastError() != hipSuccess) return ADRT_ERR_HIP;
+  adrt::dev::FwdArgs a = make_args(cfg, in, out, &ws, has_solar);
+  for (int w0 = 0; w0 < cfg->num_wavenumbers; w0 += ws.chunk) {
+    a.w0 = w0;
+    a.W = std::min(ws.chunk, cfg->num_wavenumbers - w0);
+    rc = forward_chunk(a, &ws, stream, false);
+    if (rc) return rc;
+  }
+  return ADRT_OK;
+}
+
 int adrt_validate_inputs(const adrt_batch_config* cfg, const adrt_device_inputs* in,
                          adrt_workspace* ws, hipStream_t stream) {
   int rc = check_config(cfg);
@@ -1083,6 +1235,9 @@ const char* adrt_status_string(int s) {
       return "reverse pass needs a workspace created with ADRT_WORKSPACE_ADJOINT (num_quadrature <= 16)";
     case ADRT_ERR_DOUBLING_STEPS:
       return "a layer needs more doubling steps than the reverse pass checkpoints (delta_tau too large)";
+    case ADRT_ERR_SCRATCH:
+      return "adrt_solve_batch_noalloc: one column of this shape (num_layers, num_quadrature, "
+             "num_moments) needs more than the ADRT_NOALLOC_SCRATCH_BYTES of scratch it holds";
     default: return "unknown status";
   }
 }

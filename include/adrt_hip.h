@@ -29,7 +29,8 @@
  *
  * Without a counterpart (the reference has no derivatives): adrt_adjoint_batch
  * and its workspace flag, adrt_raw_optics / adrt_raw_optics_adjoint (the
- * raw-coefficient optics and their reverse), adrt_profile / adrt_profile_read.
+ * raw-coefficient optics and their reverse), adrt_profile / adrt_profile_read;
+ * adrt_solve_batch_noalloc (for callers that cannot allocate).
  */
 #ifndef ADRT_HIP_H
 #define ADRT_HIP_H
@@ -68,6 +69,7 @@ extern "C" {
 #define ADRT_ERR_NULL 23               /* NULL argument */
 #define ADRT_ERR_ADJOINT 24            /* workspace not created with ADRT_WORKSPACE_ADJOINT, or N > 16 */
 #define ADRT_ERR_DOUBLING_STEPS 25     /* a layer needs more doubling steps than the reverse pass holds */
+#define ADRT_ERR_SCRATCH 26            /* adrt_solve_batch_noalloc: one column exceeds its scratch */
 
 #define ADRT_MAX_QUADRATURE 32
 
@@ -160,6 +162,28 @@ int adrt_workspace_info(const adrt_workspace* ws, size_t* bytes, int* chunk_wave
 int adrt_solve_batch(const adrt_batch_config* cfg, const adrt_device_inputs* in,
                      const adrt_device_outputs* out, adrt_precision precision,
                      adrt_workspace* ws, hipStream_t stream);
+
+/* Scratch held by adrt_solve_batch_noalloc on each GPU it runs on, for the
+ * life of the process (a file-scope device array of the library). */
+#define ADRT_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+
+/* The FP64 forward solve of adrt_solve_batch without a workspace and without
+ * allocating anything: no hipMalloc / hipFree, no allocating library call, no
+ * synchronisation, for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid allocation).  No
+ * counterpart in the reference, whose solveBatch mallocs and frees its
+ * temporaries inside every call (src_cuda/cuda_solver.cu:899-912).  Its scratch
+ * -- quadrature tables, per-chunk optics, layer records, base composites --
+ * is the library's ADRT_NOALLOC_SCRATCH_BYTES array on the current GPU; the
+ * batch runs in chunks of wavenumbers that fit it, and a column shape of which
+ * not even one column fits returns ADRT_ERR_SCRATCH (at quadrature order 16,
+ * about 0.87 MB per column of 100 layers, so 76 columns per chunk).  Results
+ * equal adrt_solve_batch's with precision ADRT_PRECISION_F64.  Every call on a
+ * GPU shares that GPU's array and leaves its contents behind (nothing in it
+ * needs to start at zero): calls that overlap on one GPU -- on two streams, or
+ * from two threads -- race, so serialise them.  Asynchronous on stream. */
+int adrt_solve_batch_noalloc(const adrt_batch_config* cfg, const adrt_device_inputs* in,
+                             const adrt_device_outputs* out, hipStream_t stream);
 
 /* Token of the last adrt_solve_batch on this workspace (0 before the first),
  * unique among all forward solves of the process, on any workspace; pass the
