@@ -15,6 +15,7 @@ LIB_PATH = os.path.join(_HERE, "libadrt_hip.so")
 ADRT_OK = 0
 PRECISION_F64 = 0
 PRECISION_MIXED = 1
+PRECISION_MIXED_REVERSE = 2
 WORKSPACE_ADJOINT = 1
 PROFILE_KINDS = 7
 NOALLOC_SCRATCH_BYTES = 64 << 20  # ADRT_NOALLOC_SCRATCH_BYTES

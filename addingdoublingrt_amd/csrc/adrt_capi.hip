@@ -712,8 +712,9 @@ int check_solve_args(const adrt_batch_config* cfg, const adrt_device_inputs* in,
   if (ws->L != cfg->num_layers || ws->n_real != cfg->num_quadrature || ws->M != cfg->num_moments)
     return ADRT_ERR_WORKSPACE;
   // the mixed form runs at quadrature order 16 (native 9..16, or padded on a
-  // reverse-pass workspace)
-  if (precision != ADRT_PRECISION_F64 && !(precision == ADRT_PRECISION_MIXED && ws->Np == 16))
+  // reverse-pass workspace); the mixed-reverse form's forward is the FP64 one
+  if (precision != ADRT_PRECISION_F64 && precision != ADRT_PRECISION_MIXED_REVERSE &&
+      !(precision == ADRT_PRECISION_MIXED && ws->Np == 16))
     return ADRT_ERR_PRECISION;
   return ADRT_OK;
 }
@@ -744,7 +745,7 @@ int adrt_solve_batch(const adrt_batch_config* cfg, const adrt_device_inputs* in,
   }
   ws->generation = ++g_forward_tokens;
   ws->kept_valid = ws->Iv != nullptr && cfg->num_wavenumbers <= ws->chunk;
-  ws->kept_mixed = precision == ADRT_PRECISION_MIXED;
+  ws->kept_mixed = precision == ADRT_PRECISION_MIXED;  // FP32 records (both other forms: FP64)
   return ADRT_OK;
 }
 
@@ -900,7 +901,11 @@ int adrt_adjoint_batch(const adrt_batch_config* cfg, const adrt_device_inputs* i
     const long n = (long)cfg->num_layers * cfg->num_moments;
     w16::fill_kernel<<<(unsigned)((n + 255) / 256), 256, 0, stream>>>(b.d_mom, n, 0.0);
   }
+  // forward records: FP32 doubling (MIXED) or FP64 (F64, MIXED_REVERSE); reverse
+  // doubling in FP32 for both reduced-precision forms
   const bool mixed = precision == ADRT_PRECISION_MIXED;
+  const bool rev_mixed = precision != ADRT_PRECISION_F64;
+  if (precision == ADRT_PRECISION_MIXED_REVERSE && ws->Np != 16) return ADRT_ERR_PRECISION;
   if (mixed != ws->kept_mixed) forward_generation = 0;  // kept records are of the other form
   const bool reuse = forward_generation != 0 && ws->kept_valid &&
                      forward_generation == ws->generation && cfg->num_wavenumbers <= ws->chunk;
@@ -923,7 +928,7 @@ int adrt_adjoint_batch(const adrt_batch_config* cfg, const adrt_device_inputs* i
     b.nwaves = (int)(blocks * 4);
     e = prof_begin(ws, 4, stream);
     const unsigned task_blocks = (unsigned)((tasks + 3) / 4);
-    if (mixed) {
+    if (rev_mixed) {
       // its own persistent grid: MREV_WAVES_PER_SIMD waves per SIMD
       w16::AdjArgs bm = b;
       const long mblocks = std::min((tasks + 3) / 4, (long)ws->nwaves / w16::REV_WAVES_PER_SIMD *

@@ -75,15 +75,45 @@ struct FwdArgs {
 __device__ __forceinline__ int lin(const FwdArgs& a, int l) { return a.rev ? a.L - 1 - l : l; }
 __device__ __forceinline__ int lev(const FwdArgs& a, int l) { return a.rev ? a.L - l : l; }
 
+__device__ __forceinline__ int ipow0(double omega) {  // src/doubling.hpp:20-26
+  return omega < 0.01 ? 4 : (omega < 0.1 ? 10 : 16);
+}
+
+// A scattering layer whose doubling start tau0 = tau / 2^nn lies beyond the
+// smallest quadrature cosine mu_min (reachable only at omega < 0.01, where
+// ipow0 = 4): the thin-layer transmission 1 - tau0 / mu has negative entries
+// and the doubling amplifies them by orders of magnitude before they cancel.
+__device__ __forceinline__ bool unstable_start(double tau_s, double omega_s, double mu_min) {
+  if (!(tau_s > 0.0 && omega_s > 0.0)) return false;
+  const double om = omega_s > 1.0 ? 1.0 : omega_s;
+  const int nn0 = (int)(log(tau_s) / log(2.0)) + ipow0(om);
+  const int nn = nn0 < 1 ? 1 : nn0;
+  return tau_s * ldexp(1.0, -nn) > mu_min;
+}
+
+// Thick, strongly scattering layers (omega >= 0.8, tau >= 4): their FP32
+// reverse doubling loses up to a few 1e-4 of the omega gradient (measured on
+// the 65536 x 100 and 16384 x 100 N = 16 workloads: the columns beyond 1e-4
+// each had such a layer, tau 7 - 10, omega 0.88 - 0.999).
+__device__ __forceinline__ bool thick_conservative(double tau_s, double omega_s) {
+  return omega_s >= 0.8 && tau_s >= 4.0;
+}
+
 // ---------------------------------------------------------------------------
-// prep: delta-M and cumulative optical depth (reference src/solver.cpp:430-513)
+// prep: delta-M and cumulative optical depth (reference src/solver.cpp:430-513),
+// and the scattering layers the reduced-precision forms leave to FP64 (redo):
+// those with an unstable doubling start and their neighbours in the column
+// (whose rounding errors the unstable layers amplify), and thick, strongly
+// scattering layers
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) prep_kernel(FwdArgs a) {
   const int w = blockIdx.x * blockDim.x + threadIdx.x;
   if (w >= a.W) return;
   const long gw = (long)(a.w0 + w);
   const int L = a.L, M = a.M, twoM = 2 * a.n_real;
+  const double mu_min = a.mu[0];
   double cum = 0.0;
+  bool unst_prev = false;  // layer l - 1 has an unstable start
   for (int l = 0; l < L; ++l) {
     const int li = lin(a, l);
     const double tau = a.tau[gw * L + li];
@@ -113,6 +143,13 @@ __global__ void __launch_bounds__(256) prep_kernel(FwdArgs a) {
     a.mode[t] = mode;
     a.tau_cum[(long)w * (L + 1) + l] = cum;
     cum += tl;
+    const bool unst = mode != 0 && unstable_start(tl, ol, mu_min);
+    a.redo[t] = (mode != 0 && (unst || thick_conservative(tl, ol))) ? 1 : 0;
+    if (unst || unst_prev) {  // the scattering neighbours of an unstable layer
+      if (l > 0 && a.mode[t - 1] != 0) a.redo[t - 1] = 1;
+      if (mode != 0) a.redo[t] = 1;
+    }
+    unst_prev = unst;
   }
   a.tau_cum[(long)w * (L + 1) + L] = cum;
 }
@@ -128,10 +165,6 @@ struct Lds {
   static constexpr int prow = v3 + N;        // 2N
   static constexpr int pval = prow + 2 * N;  // N
 };
-
-__device__ __forceinline__ int ipow0(double omega) {  // src/doubling.hpp:20-26
-  return omega < 0.01 ? 4 : (omega < 0.1 ? 10 : 16);
-}
 
 // ---------------------------------------------------------------------------
 // layer: phase matrices + doubling (src/phase_matrix.hpp:61-181,

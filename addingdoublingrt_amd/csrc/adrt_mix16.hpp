@@ -106,10 +106,7 @@ layer16m_kernel(FwdArgs a) {
       os[j] = su;
       os[N + j] = sd;
     }
-    if (L.l == 0) {
-      a.lflag[task] = 0;
-      a.redo[task] = 0;
-    }
+    if (L.l == 0) a.lflag[task] = 0;
     return;
   }
 
@@ -119,10 +116,10 @@ layer16m_kernel(FwdArgs a) {
   // omega < 0.01 starts at tau0 = tau / 2^(trunc(log2 tau) + 4) (src/doubling.hpp:
   // 20-26); past the smallest mu the thin-layer T0 has negative entries that
   // the doubling amplifies by orders of magnitude and cancels, which FP32
-  // cannot follow: such layers are left to the FP64 kernel (REDO pass)
-  const bool redo = st.tau0 > a.mu[0];
-  if (L.l == 0) a.redo[task] = redo ? 1 : 0;
-  if (redo) return;
+  // cannot follow, and the adding amplifies the FP32 rounding of the layers
+  // next to such a layer: all of these (marked by prep_kernel) are left to the
+  // FP64 kernel (REDO pass)
+  if (a.redo[task]) return;
   Mat<R> Rm, Sm;
 #pragma unroll
   for (int r = 0; r < 4; ++r) {

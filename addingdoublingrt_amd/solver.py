@@ -135,7 +135,8 @@ def _outputs_struct(out, W, L):
                                  else None for k in OUTPUT_NAMES})
 
 
-_PREC = {"f64": _lib.PRECISION_F64, "mixed": _lib.PRECISION_MIXED}
+_PREC = {"f64": _lib.PRECISION_F64, "mixed": _lib.PRECISION_MIXED,
+         "mixed_reverse": _lib.PRECISION_MIXED_REVERSE}
 
 
 def solve_batch(delta_tau, single_scat_albedo, phase_moments, config: BatchConfig,

@@ -10,17 +10,24 @@ shared by all wavenumbers, with every input already resident in HBM.  The
 workspace holds the whole batch, so the reverse pass reuses the forward's
 layer records and recomputes only the doubling steps.  Prints one JSON line.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--wavenumbers W] [--form f64]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--wavenumbers W] [--form auto]
 
-roofline: the dominant kernel of the step, with its algorithmic work from
-addingdoublingrt_amd/flops.py (forward doubling (2 + 2/3 + 2 + 2 + 2 + 2) N^3
-per step, reverse doubling 20 N^3 per step, counted from the inputs) over its
-time from HIP events the library records around each launch on the solve
-stream; peak = MI355X FP64 78.6 TFLOP/s (FP32 matrix cores, 157.3, for the
-mixed form).
-mixed_vs_f64 (forward + reverse FP64 line at N <= 16): the mixed form run
-on the same inputs after the timed region, with its time per step and its
-largest differences from the FP64 results.
+forms: the step is timed in each form the library has for this workload --
+plain FP64 ("f64"), the mixed form ("mixed": FP32 doubling, FP64 adding; layers
+with an unstable doubling start and their neighbours in FP64) and the FP64
+forward with an FP32 reverse doubling ("mixed_reverse") -- each with its own
+warm-up and K timed steps.  Each reduced-precision form is compared with the
+FP64 form on the same inputs: fluxes per column relative to the column's
+largest value (bar 1e-6), gradients per column norm-wise (bar 1e-4).  value and
+ms_per_step are those of the fastest form within both bars on every column
+(FP64 when none is); "forms" lists every form's time and differences.
+
+roofline: the dominant kernel of the chosen form's step, with its algorithmic
+work from addingdoublingrt_amd/flops.py (forward doubling (2 + 2/3 + 2 + 2 + 2
++ 2) N^3 per step, reverse doubling 20 N^3 per step, counted from the inputs)
+over its time from HIP events the library records around each launch on the
+solve stream; peak = MI355X FP64 78.6 TFLOP/s (FP32 matrix cores, 157.3, for a
+kernel whose products run in FP32).
 cpu_baseline: the FP64 CPU oracle (a restatement of the reference solver,
 forward only: the reference has no adjoint) on a bounded sample of the same
 columns, on this host's cores.
@@ -44,7 +51,7 @@ KINDS = ("prep", "layer_doubling", "adding_interfaces", "reverse_adding_interfac
          "reverse_layer_doubling", "reverse_column_terms", "reverse_start_terms")
 TRAFFIC_FILES = {form: os.path.join(ROOT, "profiles",
                                     f"fwd_adj_{form}_n16_65536x100_hbm_traffic.md")
-                 for form in ("f64", "mixed")}
+                 for form in ("f64", "mixed", "mixed_reverse")}
 
 
 def profiled_traffic(kernel, W, config, form):
@@ -83,15 +90,13 @@ def parse():
     p.add_argument("--warmup", type=int, default=1)
     p.add_argument("--wavenumbers", type=int, default=65536)
     p.add_argument("--config", type=int, default=5)
-    p.add_argument("--form", choices=("f64", "mixed"), default="f64")
+    p.add_argument("--form", choices=("auto", "f64", "mixed", "mixed_reverse"), default="auto",
+                   help="auto: time every form, report the fastest within the accuracy bar")
     p.add_argument("--cpu-seconds", type=float, default=12.0)
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--forward-only", action="store_true",
                    help="time the forward solve alone (any quadrature order; the default "
                         "line is forward + reverse pass)")
-    p.add_argument("--no-compare", action="store_true",
-                   help="skip the untimed mixed-form run that the forward + reverse FP64 line "
-                        "reports beside its value (speed and largest differences)")
     p.add_argument("--workspace-gb", type=float, default=96.0)
     p.add_argument("--copy-mib", type=int, default=0,
                    help="only run a plain device copy of this many MiB, three times (the "
@@ -152,29 +157,67 @@ def cpu_baseline(cols, planck_host, seconds):
                      "offline), one column per task over a thread pool, gcc -O3 -march=x86-64-v3 -ffp-contract=off")
 
 
-def compare_mixed(a, step_with, out, grads, W):
-    """The mixed form on the same inputs, after the timed region: its time per
-    step (same protocol) and its largest differences from the FP64 step just
-    measured -- outputs relative to each column's largest magnitude, gradients
-    relative to each array's largest magnitude."""
+FORM_TEXT = {
+    "f64": "plain FP64 forward",
+    "mixed": "mixed forward (FP32 doubling with FP64-refined solves, FP64 adding; unstable-start "
+             "layers and their neighbours in FP64)",
+    "mixed_reverse": "FP64 forward (outputs identical to the plain form)",
+}
+REV_TEXT = {"f64": " + FP64 reverse pass", "mixed": " + reverse pass with FP32 doubling",
+            "mixed_reverse": " + reverse pass with FP32 doubling (routed layers FP64)"}
+FLUX_BAR, GRAD_BAR = 1e-6, 1e-4
+
+
+def accuracy(out, grads, ref_out, ref_grads, W):
+    """Differences of one form from the FP64 form on the same inputs.
+
+    fluxes: per column, the largest |x - x_f64| over the interfaces of flux_up,
+    flux_down, mean_intensity and flux_direct, relative to that output's largest
+    |x_f64| in the column; gradients: per column, ||g - g_f64|| / ||g_f64|| over
+    the column's entries of each [W][...] gradient, and for the shared moments
+    and the surface albedo (sums over the columns) the relative difference of
+    the whole array.  Reports the largest value, the 99th percentile and the
+    fraction of columns within the bar (fluxes 1e-6, gradients 1e-4)."""
     import torch
 
-    mout = {k: torch.empty_like(v) for k, v in out.items()}
-    mgrads = {}
-    step_with("mixed", mout, mgrads)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(a.steps):
-        step_with("mixed", mout, mgrads)
-    torch.cuda.synchronize()
-    ms = (time.perf_counter() - t0) * 1e3 / a.steps
-    err = {}
+    def summary(e):
+        e = e.double().cpu()
+        return {"max": float(e.max()), "p99": float(torch.quantile(e, 0.99)) if e.numel() > 1
+                else float(e.max()), "median": float(e.median()),
+                "frac_within_bar": None}
+
+    res = {}
+    col = torch.zeros(W, dtype=torch.float64, device=out["flux_up"].device)
     for k in ("flux_up", "flux_down", "mean_intensity", "flux_direct"):
-        scale = out[k].abs().amax(dim=1, keepdim=True).clamp_min(1e-300)
-        err[k] = float(((mout[k] - out[k]).abs() / scale).max())
-    for k, v in grads.items():
-        err["d_" + k] = float((mgrads[k] - v).abs().max() / v.abs().max().clamp_min(1e-300))
-    return {"ms_per_step": ms, "value": W / (ms * 1e-3), "max_rel_diff": err}
+        scale = ref_out[k].abs().amax(dim=1).clamp_min(1e-300)
+        e = ((out[k] - ref_out[k]).abs().amax(dim=1) / scale)
+        col = torch.maximum(col, e)
+    fl = summary(col)
+    fl["frac_within_bar"] = float((col <= FLUX_BAR).double().mean())
+    res["fluxes"] = fl
+    worst_grad = 0.0
+    gcol = torch.zeros(W, dtype=torch.float64, device=col.device)
+    for k, v in ref_grads.items():
+        m = grads[k]
+        if v.dim() >= 1 and v.shape[0] == W and not (k == "surface_albedo"):
+            a, b = m.reshape(W, -1), v.reshape(W, -1)
+            nb = b.norm(dim=1)
+            e = torch.where(nb > 0, (a - b).norm(dim=1) / nb.clamp_min(1e-300),
+                            torch.zeros_like(nb))
+            gcol = torch.maximum(gcol, e)
+            r = summary(e)
+            r["frac_within_bar"] = float((e <= GRAD_BAR).double().mean())
+        else:  # one array for all columns
+            tot_b = v.sum() if k == "surface_albedo" else v
+            tot_a = m.sum() if k == "surface_albedo" else m
+            e = float((tot_a - tot_b).norm() / tot_b.norm().clamp_min(1e-300))
+            r = {"max": e, "p99": e, "median": e, "frac_within_bar": float(e <= GRAD_BAR)}
+        res["d_" + k] = r
+        worst_grad = max(worst_grad, r["max"])
+    both = torch.maximum(col / FLUX_BAR, gcol / GRAD_BAR)
+    res["columns_within_both_bars"] = float((both <= 1.0).double().mean())
+    res["meets_bar"] = fl["max"] <= FLUX_BAR and worst_grad <= GRAD_BAR
+    return res
 
 
 def main():
@@ -204,73 +247,101 @@ def main():
     cfg = workloads.batch_config(cols)
     W, L = cols.W, cols.L
     ws_bytes = int(a.workspace_gb * (1 << 30))
-    out = {k: torch.empty((W, L + 1), dtype=torch.float64, device="cuda")
-           for k in ("flux_up", "flux_down", "mean_intensity", "flux_direct", "flux_divergence")}
     g = torch.Generator(device="cuda").manual_seed(5)
     cot = {k: torch.rand((W, L + 1), dtype=torch.float64, device="cuda", generator=g)
            for k in ("flux_up", "flux_down", "mean_intensity")}
     inputs = (d["delta_tau"], d["single_scat_albedo"], d["phase_moments"], cfg)
     kw = dict(planck_levels=d["planck_levels"], surface_emission=d["surface_emission"],
               top_emission=d["top_emission"])
-    grads = {}
-
-    adjoint_ws = not a.forward_only or a.form == "mixed"
-
-    def step_with(form, out, grads):
-        o = solve_batch(*inputs, **kw, out=out, max_workspace_bytes=ws_bytes,
-                        adjoint_workspace=adjoint_ws, precision=form)
-        gen = o.pop("_generation", None)
-        if not a.forward_only:
-            grads.update(adjoint_batch(*inputs, out, cot, **kw, forward_generation=gen,
-                                       max_workspace_bytes=ws_bytes, check_status=False,
-                                       precision=form))
-
-    def step():
-        step_with(a.form, out, grads)
-
-    for _ in range(a.warmup):
-        step()
-    torch.cuda.synchronize()
-    ws = get_workspace(W, L, cols.N, cols.phase_moments.shape[-1], ws_bytes, adjoint_ws)
+    reverse_ok = cols.N <= 16
+    if a.form == "auto":
+        forms = ["f64"] + (["mixed", "mixed_reverse"] if reverse_ok else [])
+        if a.forward_only:
+            forms = ["f64"] + (["mixed"] if reverse_ok else [])
+    else:
+        forms = [a.form]
     lib = _lib.lib()
-    _lib.check(lib.adrt_profile(ws.handle, 1))
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(a.steps):
-        step()
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    ms = (C.c_double * _lib.PROFILE_KINDS)()
-    nl = (C.c_int * _lib.PROFILE_KINDS)()
-    _lib.check(lib.adrt_profile_read(ws.handle, ms, nl))
-    _lib.check(lib.adrt_profile(ws.handle, 0))
-    stream = C.c_void_p(torch.cuda.current_stream().cuda_stream)
-    if not a.forward_only:
-        _lib.check(lib.adrt_adjoint_status(ws.handle, stream))
-
-    ms_per_step = elapsed * 1e3 / a.steps
     Bs_host = d["surface_emission"].cpu().numpy() if d["surface_emission"] is not None else None
     work = flops.forward_flops(cols.delta_tau, cols.single_scat_albedo, cols.phase_moments,
                                cols.N, cols.use_delta_m, cols.surface_albedo, Bs_host,
                                cols.use_diffusion_lower_bc)
+
+    def run_form(form):
+        adjoint_ws = not a.forward_only or form != "f64"
+        out = {k: torch.empty((W, L + 1), dtype=torch.float64, device="cuda")
+               for k in ("flux_up", "flux_down", "mean_intensity", "flux_direct",
+                         "flux_divergence")}
+        grads = {}
+
+        def step():
+            o = solve_batch(*inputs, **kw, out=out, max_workspace_bytes=ws_bytes,
+                            adjoint_workspace=adjoint_ws, precision=form)
+            gen = o.pop("_generation", None)
+            if not a.forward_only:
+                grads.update(adjoint_batch(*inputs, out, cot, **kw, forward_generation=gen,
+                                           max_workspace_bytes=ws_bytes, check_status=False,
+                                           precision=form))
+
+        for _ in range(a.warmup):
+            step()
+        torch.cuda.synchronize()
+        ws = get_workspace(W, L, cols.N, cols.phase_moments.shape[-1], ws_bytes, adjoint_ws)
+        _lib.check(lib.adrt_profile(ws.handle, 1))
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(a.steps):
+            step()
+        torch.cuda.synchronize()
+        elapsed = time.perf_counter() - t0
+        ms = (C.c_double * _lib.PROFILE_KINDS)()
+        nl = (C.c_int * _lib.PROFILE_KINDS)()
+        _lib.check(lib.adrt_profile_read(ws.handle, ms, nl))
+        _lib.check(lib.adrt_profile(ws.handle, 0))
+        if not a.forward_only:
+            stream = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+            _lib.check(lib.adrt_adjoint_status(ws.handle, stream))
+        return dict(ms_per_step=elapsed * 1e3 / a.steps, kind_ms=list(ms), launches=list(nl),
+                    out=out, grads=grads)
+
+    runs = {f: run_form(f) for f in forms}
+    acc = {}
+    if "f64" in runs:
+        for f in forms:
+            if f != "f64":
+                acc[f] = accuracy(runs[f]["out"], runs[f]["grads"], runs["f64"]["out"],
+                                  runs["f64"]["grads"], W)
+    # the value: the fastest form whose fluxes and gradients are within the bar
+    # of the FP64 form on every column of this workload (FP64 itself qualifies)
+    ok = [f for f in forms if f == "f64" or acc.get(f, {}).get("meets_bar")]
+    chosen = min(ok or forms, key=lambda f: runs[f]["ms_per_step"])
+    r = runs[chosen]
+    ms_per_step = r["ms_per_step"]
+    ms, nl = r["kind_ms"], r["launches"]
     per_kind = {k: ms[i] / a.steps for i, k in enumerate(KINDS)}
     # dominant kernel and its algorithmic work
-    model = {1: ("layer16_kernel<false>", "phase setup + doubling (forward)", work["doubling"]),
-             4: ("layer16_adjoint_kernel<false>",
-                 "recomputed doubling + its reverse (reverse counted)", work["doubling_reverse"])}
-    if a.form == "mixed":
+    if chosen == "f64":
+        model = {1: ("layer16_kernel<false>", "phase setup + doubling (forward)", work["doubling"]),
+                 4: ("layer16_adjoint_kernel<false>",
+                     "recomputed doubling + its reverse (reverse counted)",
+                     work["doubling_reverse"])}
+    elif chosen == "mixed":
         model = {1: ("layer16m_kernel", "mixed doubling (forward), FP64 flop count",
                      work["doubling"]),
                  4: ("layer16m_adjoint_kernel",
                      "mixed recomputed doubling + reverse, FP64 flop count",
                      work["doubling_reverse"])}
+    else:
+        model = {1: ("layer16_kernel<false>", "phase setup + doubling (forward)", work["doubling"]),
+                 4: ("layer16m_adjoint_kernel",
+                     "FP32 recomputed doubling + reverse, FP64 flop count",
+                     work["doubling_reverse"])}
     if a.forward_only:
         npad = 1 << max(cols.N - 1, 1).bit_length()  # the library's padded order
         names = ({1: "layer16_kernel<false>", 2: "adding16_kernel<false, false>"}
-                 if npad == 16 or adjoint_ws else
+                 if npad == 16 or chosen != "f64" else
                  {1: "layer32_kernel", 2: "adding32_kernel"} if npad == 32 else
                  {1: f"layer_kernel<{npad}>", 2: f"adding_kernel<{npad}>"})
-        if a.form == "mixed":
+        if chosen == "mixed":
             names = {1: "layer16m_kernel + layer16_kernel<true>",
                      2: "adding16_kernel<false, true>"}
         model = {1: (names[1], "phase setup + doubling (forward)", work["doubling"]),
@@ -281,8 +352,9 @@ def main():
     launches = nl[dom] / a.steps
     avg_ms = ms[dom] / max(nl[dom], 1)
     achieved = (fl_step / launches) / (avg_ms * 1e-3) / 1e12
-    peak = FP64_PEAK_TFLOPS if a.form == "f64" else FP32_MFMA_PEAK_TFLOPS
-    traffic, traffic_note = profiled_traffic(kname, W, a.config, a.form)
+    fp32_kernel = (dom == 4 and chosen != "f64") or (dom == 1 and chosen == "mixed")
+    peak = FP32_MFMA_PEAK_TFLOPS if fp32_kernel else FP64_PEAK_TFLOPS
+    traffic, traffic_note = profiled_traffic(kname, W, a.config, chosen)
     res = {
         "metric": METRIC if not a.forward_only else "column-wavenumbers/s forward",
         "value": W / (ms_per_step * 1e-3),
@@ -294,14 +366,13 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if a.form == "f64" else "f32+f64",
+        "dtype": "f64" if chosen == "f64" else "f32+f64",
         "data": "synthetic (numpy PCG64 seed 20261203+k; Planck levels from the device kernel)",
         "config": {
             "workload": f"config {a.config}: {cols.name}, {W} wavenumbers x {L} layers, "
                         f"N={cols.N}",
-            "form": ("plain FP64 forward" if a.form == "f64" else
-                     "mixed forward (FP32 doubling with FP64-refined solves, FP64 adding)")
-                    + ("" if a.forward_only else " + reverse pass"),
+            "form": chosen + ": " + FORM_TEXT[chosen]
+                    + ("" if a.forward_only else REV_TEXT[chosen]),
             "num_wavenumbers": W, "num_layers": L, "num_quadrature": cols.N,
             "algorithmic_gflop_per_step": (work["total"] if a.forward_only else
                                            2 * work["total"] + work["doubling_reverse"]
@@ -317,9 +388,19 @@ def main():
             "kernel": f"{kname}: {name}",
             "launches_per_step": launches, "avg_launch_ms": avg_ms,
         },
+        "forms": {f: {"ms_per_step": runs[f]["ms_per_step"],
+                      "value": W / (runs[f]["ms_per_step"] * 1e-3),
+                      "speedup_vs_f64": (runs["f64"]["ms_per_step"] / runs[f]["ms_per_step"]
+                                         if "f64" in runs else None),
+                      "kernel_ms_per_step": {k: runs[f]["kind_ms"][i] / a.steps
+                                             for i, k in enumerate(KINDS)},
+                      "vs_f64": acc.get(f)}
+                  for f in forms},
+        "bar": {"fluxes": FLUX_BAR, "gradients": GRAD_BAR,
+                "rule": "value is the fastest form whose fluxes (per column, relative to the "
+                        "column's largest value) and gradients (per column, norm-wise) are "
+                        "within the bar of the FP64 form on every column"},
     }
-    if a.form == "f64" and not a.forward_only and not a.no_compare and cols.N <= 16:
-        res["mixed_vs_f64"] = compare_mixed(a, step_with, out, grads, W)
     if not a.no_cpu:
         ph = (None, None, None)
         if cols.thermal:

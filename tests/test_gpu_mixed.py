@@ -1,35 +1,28 @@
-"""Mixed-precision form (precision="mixed", N <= 16 on a reverse-pass
-workspace) against the FP64 oracle and the FP64 GPU reverse pass.
+"""The reduced-precision forms against the FP64 form and the FP64 oracle.
 
-The mixed form runs the doubling in FP32 on matrix cores in S = I - T form,
-with the (I - R^2)^-1 solve refined by one FP64-residual step, and keeps the
-thin-layer start, the source vectors, the adding and the interface fluxes in
-FP64; layers whose reference start is unstable (omega < 0.01 with tau0 past
-the smallest quadrature cosine) are recomputed by the FP64 kernel.  Its reverse
-pass runs the recomputed doubling and its reverse in FP32 with the same FP64
-parts.
+Bar (the accuracy the reduced-precision forms are held to): fluxes and mean
+intensity within 1e-6 of FP64, per column relative to that output's largest
+value in the column; gradients within 1e-4 of the FP64 reverse pass, per
+column norm-wise (||g - g_f64|| / ||g_f64|| over the column's entries), and
+for arrays summed over the columns (shared moments, surface albedo) as one
+relative norm.
 
-Accuracy measured on these workloads (relative to |oracle| with a floor of
-1e-12 of the column maximum):
-  flux_up / flux_down   max 5.7e-6, median 5e-7 .. 2e-6
-  mean_intensity        max 7.2e-6 here; up to 3.0e-5 measured on other draws of
-                        the 65536 x 100 workload, in columns with unstable-start layers
-Gradients against the FP64 reverse pass, norm-wise per column
-(||g_mixed - g_f64|| / ||g_f64|| over each column's entries), leaving out the
-entries of unstable-start layers (below):
-  Planck levels, surface and top emission   <= 2.5e-6
-  delta_tau                                  <= 4.9e-5
-  phase moments                              <= 9.8e-5
-  surface albedo                             <= 1.3e-4
-  single_scat_albedo                         <= 1.4e-3 (N = 16, omega ~ U(0, 1), tau up to 10)
-The bounds below are those figures with a margin of 2 to 2.5; they are the
-accuracy this form delivers (fluxes short of the 1e-6 it was meant to reach).
-
-On 4096 columns of the 65536 x 100 workload, against the FP64 form and
-relative to each column's largest value (test_mixed_unstable_columns):
-fluxes at most 2.2e-5 and mean intensity 6.8e-5, in columns with
-unstable-start layers.  Before the layers redone in FP64 kept FP64 records
-the mean intensity reached 2.6e-4 there (9.3e-3 over all 65536 columns).
+* "mixed_reverse": the FP64 forward (outputs identical to the plain form, bit
+  for bit) with the reverse pass's doubling recomputation and its reverse in
+  FP32 on the matrix cores.  Layers the FP32 reverse does not hold to the bar
+  run the FP64 reverse kernels: layers with an unstable doubling start (omega
+  < 0.01 with tau0 past the smallest quadrature cosine) and their scattering
+  neighbours, and thick, strongly scattering layers (omega >= 0.8, tau >= 4).
+  Held to the bar on every column.
+* "mixed": the FP32 doubling in the forward as well (S = I - T form, FP64
+  start, FP64-refined solves, FP64 adding and fluxes; the same layers in
+  FP64).  On the 65536 x 100, N = 16 workload it leaves 23 % of the columns
+  beyond 1e-6 in the fluxes (largest 3.6e-4, median 4.8e-7), so its tests at
+  the bar are expected to fail (xfail, with the measured figures printed); a
+  regression bound at its measured ceiling guards it.  Its gradients of
+  unstable-start layers (which both forms compute in FP64, but from interface
+  vectors that differ by the FP32 rounding of the other layers' records) are
+  reported apart from the others.
 """
 import numpy as np
 import pytest
@@ -42,12 +35,10 @@ from test_gpu_adjoint import unstable_layers  # noqa: E402
 
 pytestmark = pytest.mark.gpu
 
-WORKLOADS = [(1, 48), (2, 16), (4, 48), (5, 24)]
-FLUX_BOUND = 1.2e-5
-MEAN_INTENSITY_BOUND = 6e-5
-GRAD_BOUND = {"planck_levels": 6e-6, "surface_emission": 6e-6, "top_emission": 6e-6,
-              "delta_tau": 1.2e-4, "phase_moments": 2.5e-4, "single_scat_albedo": 3.5e-3,
-              "surface_albedo": 3e-4}
+WORKLOADS = [(1, 48), (2, 16), (4, 48), (5, 24), (5, 4096)]
+FLUX_BAR = 1e-6
+GRAD_BAR = 1e-4
+OUTS = ("flux_up", "flux_down", "mean_intensity", "flux_direct")
 
 
 def _inputs(k, W):
@@ -56,8 +47,97 @@ def _inputs(k, W):
     return cols, d, workloads.batch_config(cols)
 
 
+def _run(cols, d, cfg, prec, seed):
+    W, L = cols.W, cols.L
+    g = torch.Generator(device="cuda").manual_seed(seed)
+    cot = {n: torch.rand((W, L + 1), dtype=torch.float64, device="cuda", generator=g)
+           for n in ("flux_up", "flux_down", "mean_intensity")}
+    args = (d["delta_tau"], d["single_scat_albedo"], d["phase_moments"], cfg)
+    kw = dict(planck_levels=d["planck_levels"], surface_emission=d["surface_emission"],
+              top_emission=d["top_emission"])
+    out = solve_batch(*args, **kw, adjoint_workspace=True, precision=prec)
+    gen = out.pop("_generation")
+    gr = adjoint_batch(*args, out, cot, **kw, forward_generation=gen, precision=prec)
+    torch.cuda.synchronize()
+    return ({n: v.cpu().numpy() for n, v in out.items()},
+            {n: v.cpu().numpy() for n, v in gr.items() if v is not None})
+
+
+def flux_errors(out, ref):
+    """per column: the largest |x - x_ref| over interfaces and outputs,
+    relative to that output's largest |x_ref| in the column"""
+    e = 0.0
+    for n in OUTS:
+        scale = np.maximum(np.abs(ref[n]).max(axis=1), 1e-300)
+        e = np.maximum(e, np.abs(out[n] - ref[n]).max(axis=1) / scale)
+    return e
+
+
+def grad_errors(g, ref, W, mask=None):
+    """{name: per-column norm-wise relative differences} (one value for arrays
+    summed over the columns); mask [W][L]: entries left out (delta_tau,
+    single_scat_albedo, per-column moments)."""
+    res = {}
+    for n, b in ref.items():
+        a = g[n]
+        if n == "surface_albedo":
+            res[n] = np.array([abs(a.sum() - b.sum()) / max(abs(b.sum()), 1e-300)])
+            continue
+        if not (n == "phase_moments" and b.ndim == 2):  # [W][...] per column
+            a, b = a.reshape(W, -1).copy(), b.reshape(W, -1).copy()
+            if mask is not None and n in ("delta_tau", "single_scat_albedo", "phase_moments"):
+                m = mask if n != "phase_moments" else np.repeat(mask, ref[n].shape[-1], axis=1)
+                a[m], b[m] = 0.0, 0.0
+            nb = np.linalg.norm(b, axis=1)
+            res[n] = np.where(nb > 0, np.linalg.norm(a - b, axis=1) / np.maximum(nb, 1e-300), 0.0)
+        else:  # shared moments: one array
+            res[n] = np.array([np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)])
+    return res
+
+
 @pytest.mark.parametrize("k,W", WORKLOADS)
-def test_mixed_forward_vs_oracle(k, W):
+def test_mixed_reverse_forward_is_the_f64_forward(k, W):
+    cols, d, cfg = _inputs(k, W)
+    o64, _ = _run(cols, d, cfg, "f64", 11 + k)
+    omr, _ = _run(cols, d, cfg, "mixed_reverse", 11 + k)
+    for n in o64:
+        assert np.array_equal(o64[n], omr[n]), n
+
+
+@pytest.mark.parametrize("k,W", WORKLOADS)
+def test_mixed_reverse_gradients_within_bar(k, W):
+    cols, d, cfg = _inputs(k, W)
+    _, g64 = _run(cols, d, cfg, "f64", 11 + k)
+    _, gmr = _run(cols, d, cfg, "mixed_reverse", 11 + k)
+    for n, e in grad_errors(gmr, g64, W).items():
+        print(f"mixed_reverse config {k} W={W} d/d{n}: per column max {e.max():.2e}, "
+              f"median {np.median(e):.2e}")
+        assert e.max() <= GRAD_BAR, (n, float(e.max()), int(e.argmax()))
+
+
+@pytest.mark.parametrize("k,W", WORKLOADS)
+def test_mixed_forward_regression(k, W):
+    """The FP32-forward form at its measured ceiling (largest 3.6e-4 over all
+    65536 columns of the headline workload, in a column with two
+    unstable-start layers): a regression guard, not the bar."""
+    cols, d, cfg = _inputs(k, W)
+    out = solve_batch(d["delta_tau"], d["single_scat_albedo"], d["phase_moments"], cfg,
+                      d["planck_levels"], d["surface_emission"], d["top_emission"],
+                      precision="mixed", adjoint_workspace=True)
+    out.pop("_generation")
+    ref = solve_batch(d["delta_tau"], d["single_scat_albedo"], d["phase_moments"], cfg,
+                      d["planck_levels"], d["surface_emission"], d["top_emission"])
+    torch.cuda.synchronize()
+    e = flux_errors({n: out[n].cpu().numpy() for n in OUTS}, {n: ref[n].cpu().numpy() for n in OUTS})
+    print(f"mixed config {k} W={W}: fluxes per column max {e.max():.2e}, median "
+          f"{np.median(e):.2e}, within 1e-6: {np.mean(e <= FLUX_BAR):.3f}")
+    assert e.max() <= 4e-4, (float(e.max()), int(e.argmax()))
+
+
+@pytest.mark.xfail(reason="the FP32-forward form misses the 1e-6 flux bar on about a quarter of "
+                          "the headline workload's columns (largest 3.6e-4)", strict=False)
+@pytest.mark.parametrize("k,W", WORKLOADS)
+def test_mixed_forward_vs_oracle_at_bar(k, W):
     cols, d, cfg = _inputs(k, W)
     out = solve_batch(d["delta_tau"], d["single_scat_albedo"], d["phase_moments"], cfg,
                       d["planck_levels"], d["surface_emission"], d["top_emission"],
@@ -72,73 +152,31 @@ def test_mixed_forward_vs_oracle(k, W):
         top_emission=0.0 if h["top_emission"] is None else h["top_emission"],
         surface_albedo=cfg.surface_albedo, solar_flux=cfg.solar_flux, solar_mu=cfg.solar_mu,
         use_delta_m=cfg.use_delta_m, use_diffusion_lower_bc=cfg.use_diffusion_lower_bc)
-    for name, bound in (("flux_up", FLUX_BOUND), ("flux_down", FLUX_BOUND),
-                        ("mean_intensity", MEAN_INTENSITY_BOUND), ("flux_direct", FLUX_BOUND)):
-        g, r = out[name].cpu().numpy(), ref[name]
-        scale = np.maximum(np.abs(r), 1e-12 * np.abs(r).max(axis=1, keepdims=True) + 1e-300)
-        err = np.abs(g - r) / scale
-        print(f"mixed config {k} {name}: max rel err {err.max():.2e}, median {np.median(err):.2e}")
-        assert err.max() <= bound, (name, float(err.max()), np.unravel_index(err.argmax(), err.shape))
+    e = flux_errors({n: out[n].cpu().numpy() for n in OUTS}, ref)
+    print(f"mixed vs FP64 oracle config {k} W={W}: per column max {e.max():.2e}, "
+          f"within 1e-6: {np.mean(e <= FLUX_BAR):.3f}")
+    assert e.max() <= FLUX_BAR, (float(e.max()), int(e.argmax()))
 
 
+@pytest.mark.xfail(reason="the FP32-forward form's gradients miss 1e-4 where its interface "
+                          "vectors differ from FP64 (unstable-start layers, thick layers)",
+                   strict=False)
 @pytest.mark.parametrize("k,W", WORKLOADS)
-def test_mixed_gradients_vs_f64(k, W):
+def test_mixed_gradients_at_bar(k, W):
     cols, d, cfg = _inputs(k, W)
-    L = cols.L
-    g = torch.Generator(device="cuda").manual_seed(11 + k)
-    cot = {n: torch.rand((W, L + 1), dtype=torch.float64, device="cuda", generator=g)
-           for n in ("flux_up", "flux_down", "mean_intensity")}
-    args = (d["delta_tau"], d["single_scat_albedo"], d["phase_moments"], cfg)
-    kw = dict(planck_levels=d["planck_levels"], surface_emission=d["surface_emission"],
-              top_emission=d["top_emission"])
-    res = {}
-    for prec in ("f64", "mixed"):
-        out = solve_batch(*args, **kw, adjoint_workspace=True, precision=prec)
-        gen = out.pop("_generation")
-        gr = adjoint_batch(*args, out, cot, **kw, forward_generation=gen, precision=prec)
-        torch.cuda.synchronize()
-        res[prec] = {n: v.cpu().numpy() for n, v in gr.items() if v is not None}
+    _, g64 = _run(cols, d, cfg, "f64", 11 + k)
+    _, gmx = _run(cols, d, cfg, "mixed", 11 + k)
     unstable = unstable_layers(cols.delta_tau, cols.single_scat_albedo, cols.phase_moments, cfg)
-    for name, bound in GRAD_BOUND.items():
-        if name not in res["f64"]:
-            continue
-        a, b = res["mixed"][name], res["f64"][name]
-        # entries of unstable-start layers carry derivatives that cancel terms
-        # orders of magnitude larger (FP64 recovers about one digit of them, see
-        # test_gpu_adjoint); both forms compute those layers with the FP64
-        # kernels, so they are left out of the comparison
-        if name in ("delta_tau", "single_scat_albedo"):
-            a, b = np.where(unstable, 0.0, a), np.where(unstable, 0.0, b)
-        elif name == "phase_moments":
-            m = unstable[..., None] if b.ndim == 3 else unstable.any(axis=0)[:, None]
-            a, b = np.where(m, 0.0, a), np.where(m, 0.0, b)
-        if b.ndim and b.shape[0] == W:
-            a, b = a.reshape(W, -1), b.reshape(W, -1)
-        else:  # shared moments: one vector
-            a, b = a.reshape(1, -1), b.reshape(1, -1)
-        nb = np.linalg.norm(b, axis=1)
-        live = nb > 0
-        if not live.any():
-            continue
-        rel = np.linalg.norm(a - b, axis=1)[live] / nb[live]
-        print(f"mixed config {k} d/d{name}: norm-wise per column max {rel.max():.2e}")
-        assert rel.max() <= bound, (name, float(rel.max()))
-
-
-def test_mixed_unstable_columns():
-    """The mixed form against the FP64 form on 4096 columns of the 65536 x 100
-    workload, about half of which hold an unstable-start layer."""
-    cols, d, cfg = _inputs(5, 4096)
-    args = (d["delta_tau"], d["single_scat_albedo"], d["phase_moments"], cfg,
-            d["planck_levels"], d["surface_emission"], d["top_emission"])
-    res = {}
-    for prec in ("f64", "mixed"):
-        out = solve_batch(*args, precision=prec, adjoint_workspace=True)
-        out.pop("_generation")
-        torch.cuda.synchronize()
-        res[prec] = {n: v.cpu().numpy() for n, v in out.items()}
-    for name, bound in (("flux_up", 5e-5), ("flux_down", 5e-5), ("mean_intensity", 1.5e-4)):
-        a, b = res["mixed"][name], res["f64"][name]
-        err = (np.abs(a - b) / np.abs(b).max(axis=1, keepdims=True)).max(axis=1)
-        print(f"mixed unstable columns {name}: max {err.max():.2e}, median {np.median(err):.2e}")
-        assert err.max() <= bound, (name, float(err.max()), int(err.argmax()))
+    worst = 0.0
+    # reported apart: the entries of unstable-start layers, then all others
+    for label, mask in (("without unstable-start layers", unstable), ("all entries", None)):
+        for n, e in grad_errors(gmx, g64, cols.W, mask).items():
+            print(f"mixed config {k} W={W} d/d{n} ({label}): per column max {e.max():.2e}")
+            if mask is not None:
+                worst = max(worst, float(e.max()))
+    if unstable.any():
+        a, b = gmx["delta_tau"][unstable], g64["delta_tau"][unstable]
+        print(f"mixed config {k} W={W}: {int(unstable.sum())} unstable-start layers, their "
+              f"d/ddelta_tau relative differences up to "
+              f"{np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-300)):.2e}")
+    assert worst <= GRAD_BAR, worst
