@@ -98,6 +98,12 @@ def parse():
                    help="time the forward solve alone (any quadrature order; the default "
                         "line is forward + reverse pass)")
     p.add_argument("--workspace-gb", type=float, default=96.0)
+    p.add_argument("--summarize-stats", metavar="DIR")
+    p.add_argument("--summarize-traffic", nargs=4, metavar=("FETCH", "WRITE", "COPY_FETCH",
+                                                            "COPY_WRITE"))
+    p.add_argument("--copy-mib-size", type=float, default=1024.0)
+    p.add_argument("--title", default="rocprofv3 summary")
+    p.add_argument("--cmd", action="append", default=[])
     p.add_argument("--copy-mib", type=int, default=0,
                    help="only run a plain device copy of this many MiB, three times (the "
                         "reference kernel that scales the FETCH_SIZE / WRITE_SIZE counters)")
@@ -220,8 +226,89 @@ def accuracy(out, grads, ref_out, ref_grads, W):
     return res
 
 
+def summarize(a):
+    """Markdown summaries of rocprofv3 output for profiles/, stamped with the
+    digest of the HIP sources they were measured on:
+      --summarize-stats DIR            kernel statistics (--kernel-trace --stats)
+      --summarize-traffic F W CF CW    FETCH_SIZE / WRITE_SIZE per dispatch of the
+                                       step (F, W) and of the calibration copy
+                                       (CF, CW; bench.py --copy-mib 1024)"""
+    import csv
+    import glob
+
+    from addingdoublingrt_amd.build import source_digest
+
+    def rows(d, pattern):
+        files = glob.glob(os.path.join(d, "**", pattern), recursive=True)
+        if not files:
+            raise SystemExit(f"no {pattern} under {d}")
+        with open(files[0]) as f:
+            return list(csv.DictReader(f))
+
+    def short(name):
+        name = name.replace("(anonymous namespace)", "{anonymous}")
+        return name.split("(")[0].replace("void ", "").strip() or name
+
+    print(f"# {a.title}\n")
+    print(f"source-digest: {source_digest()}\n")
+    print("Commands (from the repository root, after `cd /tmp && export TMPDIR=/tmp`):\n")
+    print("```")
+    for c in a.cmd:
+        print(c)
+    print("```\n")
+    if a.summarize_stats:
+        r = rows(a.summarize_stats, "*kernel_stats.csv")
+        print("| kernel | calls | average ms | total ms | % |")
+        print("|---|---:|---:|---:|---:|")
+        for x in sorted(r, key=lambda x: -float(x["TotalDurationNs"])):
+            print(f"| `{short(x['Name'])}` | {x['Calls']} | {float(x['AverageNs']) / 1e6:.3f} | "
+                  f"{float(x['TotalDurationNs']) / 1e6:.2f} | {float(x['Percentage']):.2f} |")
+        return
+    fd, wd, cfd, cwd = a.summarize_traffic
+
+    def per_dispatch(d, counter):
+        acc = {}
+        for x in rows(d, "*counter_collection.csv"):
+            if x["Counter_Name"] != counter:
+                continue
+            key = (short(x["Kernel_Name"]), x["Dispatch_Id"])
+            acc[key] = acc.get(key, 0.0) + float(x["Counter_Value"])
+        per = {}
+        for (k, _), v in acc.items():
+            per.setdefault(k, []).append(v)
+        return {k: sum(v) / len(v) for k, v in per.items()}
+
+    cf = per_dispatch(cfd, "FETCH_SIZE")
+    cw = per_dispatch(cwd, "WRITE_SIZE")
+    copy = [k for k in cf if "copyBuffer" in k][0]
+    copy_kib = float(a.copy_mib_size) * 1024
+    ff, fw = copy_kib / cf[copy], copy_kib / cw[copy]
+    print(f"Calibration: a {a.copy_mib_size} MiB device copy reads FETCH_SIZE = {cf[copy]:.0f} KiB and "
+          f"writes WRITE_SIZE = {cw[copy]:.0f} KiB per dispatch, so bytes = FETCH_SIZE x 1024 x "
+          f"{ff:.3f} and WRITE_SIZE x 1024 x {fw:.3f}.\n")
+    F = per_dispatch(fd, "FETCH_SIZE")
+    Wr = per_dispatch(wd, "WRITE_SIZE")
+    print("| kernel | FETCH_SIZE KiB | WRITE_SIZE KiB | read GB | written GB | total GB |")
+    print("|---|---:|---:|---:|---:|---:|")
+    tot = {}
+    for k in sorted(F, key=lambda k: -(F[k] + Wr.get(k, 0.0))):
+        if "adrt" not in k:
+            continue
+        rd, wr = F[k] * 1024 * ff, Wr.get(k, 0.0) * 1024 * fw
+        tot[k] = rd + wr
+        print(f"| `{k}` | {F[k]:.0f} | {Wr.get(k, 0.0):.0f} | {rd / 1e9:.1f} | {wr / 1e9:.1f} | "
+              f"{(rd + wr) / 1e9:.1f} |")
+    print("\nPer-launch HBM-side bytes (read + written) that bench.py reports as roofline.traffic:\n")
+    for k, v in tot.items():
+        name = k.split("::")[-1]
+        print(f"bench-traffic: {name} {v:.6e}")
+
+
 def main():
     a = parse()
+    if a.summarize_stats or a.summarize_traffic:
+        summarize(a)
+        return
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {a.gpus}); columns are independent, "
               "so more GPUs would each solve their own share of the wavenumbers.",
@@ -325,16 +412,17 @@ def main():
                      "recomputed doubling + its reverse (reverse counted)",
                      work["doubling_reverse"])}
     elif chosen == "mixed":
-        model = {1: ("layer16m_kernel", "mixed doubling (forward), FP64 flop count",
+        model = {1: ("layer16m_kernel + layer16_kernel<true>",
+                     "mixed doubling (forward; FP64 for the routed layers), FP64 flop count",
                      work["doubling"]),
-                 4: ("layer16m_adjoint_kernel",
-                     "mixed recomputed doubling + reverse, FP64 flop count",
-                     work["doubling_reverse"])}
+                 4: ("layer16m_adjoint_kernel + layer16_adjoint_kernel<true>",
+                     "FP32 recomputed doubling + reverse (FP64 for the routed layers), "
+                     "FP64 flop count", work["doubling_reverse"])}
     else:
         model = {1: ("layer16_kernel<false>", "phase setup + doubling (forward)", work["doubling"]),
-                 4: ("layer16m_adjoint_kernel",
-                     "FP32 recomputed doubling + reverse, FP64 flop count",
-                     work["doubling_reverse"])}
+                 4: ("layer16m_adjoint_kernel + layer16_adjoint_kernel<true>",
+                     "FP32 recomputed doubling + reverse (FP64 for the routed layers), "
+                     "FP64 flop count", work["doubling_reverse"])}
     if a.forward_only:
         npad = 1 << max(cols.N - 1, 1).bit_length()  # the library's padded order
         names = ({1: "layer16_kernel<false>", 2: "adding16_kernel<false, false>"}
@@ -354,7 +442,9 @@ def main():
     achieved = (fl_step / launches) / (avg_ms * 1e-3) / 1e12
     fp32_kernel = (dom == 4 and chosen != "f64") or (dom == 1 and chosen == "mixed")
     peak = FP32_MFMA_PEAK_TFLOPS if fp32_kernel else FP64_PEAK_TFLOPS
-    traffic, traffic_note = profiled_traffic(kname, W, a.config, chosen)
+    vals = [profiled_traffic(part, W, a.config, chosen) for part in kname.split(" + ")]
+    traffic = sum(v for v, _ in vals) if all(v is not None for v, _ in vals) else None
+    traffic_note = "; ".join(dict.fromkeys(n for _, n in vals))
     res = {
         "metric": METRIC if not a.forward_only else "column-wavenumbers/s forward",
         "value": W / (ms_per_step * 1e-3),
