@@ -10,14 +10,16 @@ package is its host-side face:
 * ``solve``                         -- differentiable solve (torch.autograd)
 * ``raw_optics`` / ``solve_from_coefficients``
                                     -- differentiable raw-coefficient optics and solve
+* ``source_jacobians``              -- d outputs / d thermal sources (doubling once)
 * ``planck_levels``                 -- device band Planck, differentiable in T
 * ``workloads``                     -- synthetic inputs of the benchmark configs
 """
 from ._lib import AdrtError, AdrtLibraryError, lib  # noqa: F401
 from .solver import (BatchConfig, Workspace, adjoint_batch, gauss_legendre, planck,  # noqa: F401
                      planck_deriv, planck_levels, raw_optics, solve, solve_batch,
-                     solve_batch_from_coefficients, solve_from_coefficients)
+                     solve_batch_from_coefficients, solve_from_coefficients, source_jacobians)
 
 __all__ = ["AdrtError", "AdrtLibraryError", "BatchConfig", "Workspace", "adjoint_batch",
            "gauss_legendre", "planck", "planck_deriv", "planck_levels", "raw_optics", "solve",
-           "solve_batch", "solve_batch_from_coefficients", "solve_from_coefficients", "lib"]
+           "solve_batch", "solve_batch_from_coefficients", "solve_from_coefficients",
+           "source_jacobians", "lib"]

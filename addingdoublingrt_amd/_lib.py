@@ -17,6 +17,7 @@ PRECISION_F64 = 0
 PRECISION_MIXED = 1
 PRECISION_MIXED_REVERSE = 2
 WORKSPACE_ADJOINT = 1
+WORKSPACE_JACOBIAN = 2
 PROFILE_KINDS = 7
 NOALLOC_SCRATCH_BYTES = 64 << 20  # ADRT_NOALLOC_SCRATCH_BYTES
 
@@ -78,6 +79,13 @@ class DeviceGradients(C.Structure):
     ]
 
 
+class DeviceJacobians(C.Structure):
+    _fields_ = [
+        ("flux_up", C.c_void_p), ("flux_down", C.c_void_p), ("mean_intensity", C.c_void_p),
+        ("flux_divergence", C.c_void_p),
+    ]
+
+
 class DeviceCoefficients(C.Structure):
     _fields_ = [
         ("absorption_coeff", C.c_void_p), ("scattering_coeff", C.c_void_p),
@@ -120,6 +128,8 @@ SIGNATURES = {
                                    C.POINTER(DeviceOutputs), C.c_int, C.c_void_p, C.c_void_p]),
     "adrt_solve_batch_noalloc": (C.c_int, [C.POINTER(BatchConfig), C.POINTER(DeviceInputs),
                                            C.POINTER(DeviceOutputs), C.c_void_p]),
+    "adrt_source_jacobian_batch": (C.c_int, [C.POINTER(BatchConfig), C.POINTER(DeviceInputs),
+                                             C.POINTER(DeviceJacobians), C.c_void_p, C.c_void_p]),
     "adrt_solve_batch_from_coefficients": (C.c_int, [C.POINTER(BatchConfig),
                                                      C.POINTER(DeviceCoefficients), C.c_double,
                                                      C.c_double, C.POINTER(DeviceOutputs), C.c_int,

@@ -65,6 +65,10 @@ struct LdsA {
   static constexpr int t0 = 0, t1 = TILE, t2 = 2 * TILE, t3 = 3 * TILE, vs = 4 * TILE;
   static constexpr int size = vs + 8 * N;  // 8 vector slots; the last one for column sums
 };
+static_assert(LdsA::t3 + TILE <= LdsA::vs && LdsA::vs + 8 * N <= LdsA::size,
+              "reverse per-wave LDS regions overlap");
+// start16_adjoint_kernel: per wave one tile, a column-sum slot and two tiles
+constexpr int LDS_START16 = 3 * TILE + N;
 
 // phase matrices and thin-layer scalars of a scattering layer, recomputed for
 // the reverse pass in the register layout of RowT, keeping the sums before the
@@ -697,7 +701,7 @@ __device__ void layer16_adjoint_task(const FwdArgs& a, const AdjArgs& b, long ta
 // FP32 register order of layer16m_adjoint_kernel).
 template <bool REDO, typename RowT = double>
 __global__ void __launch_bounds__(256) start16_adjoint_kernel(FwdArgs a, AdjArgs b) {
-  __shared__ double smem[4 * (3 * TILE + N)];
+  __shared__ double smem[4 * LDS_START16];
   const int wv = wave_index();
   const long task = (long)blockIdx.x * 4 + wv;
   const int Lay = a.L;
@@ -714,7 +718,7 @@ __global__ void __launch_bounds__(256) start16_adjoint_kernel(FwdArgs a, AdjArgs
   const int j = L.j;
   const int w = (int)(task / Lay), l = (int)(task % Lay);
   const long gw = (long)(a.w0 + w);
-  double* dtile = smem + wv * (3 * TILE + N);
+  double* dtile = smem + wv * LDS_START16;
   double* csl = dtile + TILE;
   double* tiles = csl + N;  // two tiles for the Legendre sums
   Phase16<RowT> p;

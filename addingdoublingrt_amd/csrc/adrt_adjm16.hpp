@@ -33,6 +33,11 @@ struct LdsAM {
   static constexpr int td = 0, tf0 = TILE, vs = TILE + 8 * TF, csl = vs + 4 * N;
   static constexpr int size = csl + N;
 };
+static_assert(N * TS * sizeof(float) <= LdsAM::TF * sizeof(double) &&
+                  LdsAM::td + TILE <= LdsAM::tf0 && LdsAM::tf0 + 8 * LdsAM::TF <= LdsAM::vs &&
+                  8 * N * sizeof(float) <= 4 * N * sizeof(double) &&
+                  LdsAM::vs + 4 * N <= LdsAM::csl && LdsAM::csl + N <= LdsAM::size,
+              "mixed reverse per-wave LDS regions overlap");
 
 // per-step checkpoint (within the SLAB stride): R, S, Q in FP32 register
 // order, then y, z, s_up, s_dn (FP64, element j) and gamma

@@ -352,6 +352,12 @@ struct adrt_workspace {
          *g_Bb = nullptr, *g_Ftop = nullptr, *g_f = nullptr, *g_chi = nullptr, *scot = nullptr, *slab = nullptr,
          *mom_part = nullptr, *phase = nullptr, *raw = nullptr;
   int* status = nullptr;
+  // source Jacobians (ADRT_WORKSPACE_JACOBIAN): the thermal basis of the
+  // chunk's columns and (L + 3) pseudo-columns per column
+  double *basis = nullptr, *jlay = nullptr, *jrb = nullptr, *jtau_s = nullptr,
+         *jtau_cum = nullptr, *jB = nullptr, *jsurf = nullptr, *jtop = nullptr, *jssa = nullptr,
+         *jfdir = nullptr;
+  int* jlflag = nullptr;
   int nwaves = 0, nn_cap = 0;
   unsigned long long generation = 0;  // token of the last forward solve (unique in the process)
   bool kept_valid = false;            // the last forward's state (one chunk) is still held
@@ -437,8 +443,9 @@ void prof_end(adrt_workspace* ws, int k, hipStream_t stream) {
   if (k >= 0) (void)hipEventRecord(ws->ev_pool[2 * k + 1], stream);
 }
 
+// prep, phase records and the layer kernel (layer records) of one chunk
 template <int N>
-int launch_chunk(const adrt::dev::FwdArgs& a, adrt_workspace* ws, hipStream_t stream,
+int layers_chunk(const adrt::dev::FwdArgs& a, adrt_workspace* ws, hipStream_t stream,
                  bool mixed) {
   using namespace adrt::dev;
   constexpr int GPB = Geo<N>::GPB;  // lane groups per 256-thread block
@@ -446,8 +453,8 @@ int launch_chunk(const adrt::dev::FwdArgs& a, adrt_workspace* ws, hipStream_t st
   prep_kernel<<<(a.W + 255) / 256, 256, 0, stream>>>(a);
   prof_end(ws, e, stream);
   const long tasks = (long)a.W * a.L;
+  e = prof_begin(ws, 1, stream);
   if constexpr (N == 16) {  // one wave per problem on the matrix cores
-    e = prof_begin(ws, 1, stream);
     if (a.phase) w16::phase16_kernel<<<(unsigned)a.L, 64, 0, stream>>>(a, ws->phase);
     if (mixed) {
       w16::layer16m_kernel<<<(unsigned)((tasks + 3) / 4), 256, 0, stream>>>(a);
@@ -456,31 +463,109 @@ int launch_chunk(const adrt::dev::FwdArgs& a, adrt_workspace* ws, hipStream_t st
     } else {
       w16::layer16_kernel<false><<<(unsigned)((tasks + 3) / 4), 256, 0, stream>>>(a);
     }
-    prof_end(ws, e, stream);
-    e = prof_begin(ws, 2, stream);
-    if (mixed)
-      w16::adding16_kernel<false, true><<<(unsigned)((a.W + 3) / 4), 256, 0, stream>>>(a);
-    else
-      w16::adding16_kernel<false, false><<<(unsigned)((a.W + 3) / 4), 256, 0, stream>>>(a);
-    prof_end(ws, e, stream);
   } else {
     if (mixed) return ADRT_ERR_PRECISION;
-    e = prof_begin(ws, 1, stream);
     if constexpr (N == 32)  // one wave per problem on the matrix cores
       w32::layer32_kernel<<<(unsigned)((tasks + w32::WAVES - 1) / w32::WAVES), 64 * w32::WAVES,
                             0, stream>>>(a);
     else
       layer_kernel<N><<<(unsigned)((tasks + GPB - 1) / GPB), 256, 0, stream>>>(a);
-    prof_end(ws, e, stream);
-    e = prof_begin(ws, 2, stream);
-    if constexpr (N == 32)
-      w32::adding32_kernel<<<(unsigned)((a.W + w32::WAVES - 1) / w32::WAVES), 64 * w32::WAVES, 0,
-                             stream>>>(a);
-    else
-      adding_kernel<N><<<(unsigned)((a.W + GPB - 1) / GPB), 256, 0, stream>>>(a);
-    prof_end(ws, e, stream);
   }
+  prof_end(ws, e, stream);
   return hipGetLastError() == hipSuccess ? ADRT_OK : ADRT_ERR_HIP;
+}
+
+// adding sweeps, interface solves and outputs of one chunk
+template <int N>
+int adding_chunk(const adrt::dev::FwdArgs& a, adrt_workspace* ws, hipStream_t stream,
+                 bool mixed) {
+  using namespace adrt::dev;
+  constexpr int GPB = Geo<N>::GPB;
+  const int e = prof_begin(ws, 2, stream);
+  if constexpr (N == 16) {
+    if (mixed)
+      w16::adding16_kernel<false, true><<<(unsigned)((a.W + 3) / 4), 256, 0, stream>>>(a);
+    else
+      w16::adding16_kernel<false, false><<<(unsigned)((a.W + 3) / 4), 256, 0, stream>>>(a);
+  } else if constexpr (N == 32) {
+    w32::adding32_kernel<<<(unsigned)((a.W + w32::WAVES - 1) / w32::WAVES), 64 * w32::WAVES, 0,
+                           stream>>>(a);
+  } else {
+    adding_kernel<N><<<(unsigned)((a.W + GPB - 1) / GPB), 256, 0, stream>>>(a);
+  }
+  prof_end(ws, e, stream);
+  return hipGetLastError() == hipSuccess ? ADRT_OK : ADRT_ERR_HIP;
+}
+
+template <int N>
+int launch_chunk(const adrt::dev::FwdArgs& a, adrt_workspace* ws, hipStream_t stream,
+                 bool mixed) {
+  const int rc = layers_chunk<N>(a, ws, stream, mixed);
+  return rc ? rc : adding_chunk<N>(a, ws, stream, mixed);
+}
+
+// Source Jacobians: every output is affine in the thermal sources and R, T do
+// not depend on them, so d(output)/d(source u) is the solve with source u set
+// to one and every other source (and the beam) zero.  Pseudo-column
+// p = c (L + 3) + u of real column c takes the real column's R, T and its
+// sources rebuilt from the thermal basis for unit source u: u <= L the Planck
+// value of level u (caller order), L + 1 the surface emission, L + 2 the top.
+struct JacArgs {
+  int C, L, Np, rev;
+  const double* lay;    // [C][L][REC] real records
+  const double* basis;  // [C][L][2 Np]
+  const int* lflag;     // [C][L]
+  const double* tau_s;  // [C][L]
+  const double* tau_cum;  // [C][L+1]
+  const double* ssa;    // [caller columns][L] (the chunk's columns from w0), caller order
+  long w0;
+  double* jlay;         // [P][L][REC]
+  int* jlflag;          // [P][L]
+  double *jtau_s, *jtau_cum, *jB, *jsurf, *jtop, *jssa;
+};
+
+__global__ void jac_records_kernel(JacArgs j) {
+  const long task = blockIdx.x;  // pseudo-column * L + layer
+  const int U = j.L + 3;
+  const long p = task / j.L;
+  const int l = (int)(task % j.L);
+  const long c = p / U;
+  const int u = (int)(p % U);
+  const int REC = 2 * j.Np * j.Np + 2 * j.Np;
+  const double* src = j.lay + (c * j.L + l) * REC;
+  double* dst = j.jlay + task * REC;
+  for (int k = threadIdx.x; k < 2 * j.Np * j.Np; k += blockDim.x) dst[k] = src[k];
+  const int e = threadIdx.x;
+  if (e < j.Np) {
+    const int lt = j.rev ? j.L - l : l, lb = j.rev ? j.L - l - 1 : l + 1;  // caller levels
+    const double Bt = (u == lt) ? 1.0 : 0.0, Bb = (u == lb) ? 1.0 : 0.0;
+    const double tau = j.tau_s[c * j.L + l];
+    const double B_bar = (Bb + Bt) / 2.0;
+    const double B_d = (tau > 0.0) ? (Bb - Bt) / tau : 0.0;
+    const double* bs = j.basis + (c * j.L + l) * 2 * j.Np;
+    const double y = bs[e], z = bs[j.Np + e];
+    dst[2 * j.Np * j.Np + e] = y * B_bar + z * B_d;
+    dst[2 * j.Np * j.Np + j.Np + e] = y * B_bar - z * B_d;
+  }
+}
+
+__global__ void jac_columns_kernel(JacArgs j, long P) {
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= P) return;
+  const int U = j.L + 3, L = j.L;
+  const long c = p / U;
+  const int u = (int)(p % U);
+  for (int l = 0; l < L; ++l) {
+    j.jlflag[p * L + l] = j.lflag[c * L + l];
+    j.jtau_s[p * L + l] = j.tau_s[c * L + l];
+    if (j.jssa) j.jssa[p * L + l] = j.ssa[(j.w0 + c) * L + l];
+  }
+  for (int l = 0; l <= L; ++l) {
+    j.jtau_cum[p * (L + 1) + l] = j.tau_cum[c * (L + 1) + l];
+    j.jB[p * (L + 1) + l] = (u == l) ? 1.0 : 0.0;
+  }
+  j.jsurf[p] = (u == L + 1) ? 1.0 : 0.0;
+  j.jtop[p] = (u == L + 2) ? 1.0 : 0.0;
 }
 
 }  // namespace
@@ -498,7 +583,9 @@ int adrt_workspace_create_ex(const adrt_batch_config* cfg, size_t max_bytes, int
   int rc = check_config(cfg);
   if (rc) return rc;
   const bool adjoint = (flags & ADRT_WORKSPACE_ADJOINT) != 0;
+  const bool jac = (flags & ADRT_WORKSPACE_JACOBIAN) != 0;
   if (adjoint && cfg->num_quadrature > 16) return ADRT_ERR_ADJOINT;
+  if (adjoint && jac) return ADRT_ERR_WORKSPACE;
   adrt_workspace* ws = new (std::nothrow) adrt_workspace();
   if (!ws) return ADRT_ERR_WORKSPACE;
   ws->flags = flags;
@@ -529,7 +616,13 @@ int adrt_workspace_create_ex(const adrt_batch_config* cfg, size_t max_bytes, int
     slab_bytes = (size_t)ws->nwaves * adrt::dev::w16::slab_stride(ws->nn_cap) * sizeof(double);
   }
   if (max_bytes == 0) max_bytes = (size_t)4 << 30;
-  const size_t per_w = per_wavenumber_bytes(ws->L, ws->Np, ws->M, adjoint);
+  size_t per_w = per_wavenumber_bytes(ws->L, ws->Np, ws->M, adjoint);
+  if (jac) {  // thermal basis + L + 3 pseudo-columns per column
+    const size_t Lj = ws->L, Nj = ws->Np;
+    const size_t pseudo = Lj * (2 * Nj * Nj + 2 * Nj) * 8 + Lj * (2 * Nj * Nj + Nj) * 8 +
+                          Lj * 4 + 2 * Lj * 8 + 3 * (Lj + 1) * 8 + 2 * 8;
+    per_w += Lj * 2 * Nj * 8 + (Lj + 3) * pseudo;
+  }
   const size_t phase_bytes =
       ws->Np == adrt::dev::w16::N ? (size_t)ws->L * adrt::dev::w16::PH16 * 8 : 0;
   const size_t fixed =
@@ -568,6 +661,22 @@ int adrt_workspace_create_ex(const adrt_batch_config* cfg, size_t max_bytes, int
     o_slab = take(slab_bytes);
     o_status = take(4);
   }
+  size_t o_basis = 0, o_jlay = 0, o_jrb = 0, o_jlflag = 0, o_jtau_s = 0, o_jtau_cum = 0,
+         o_jB = 0, o_jsurf = 0, o_jtop = 0, o_jssa = 0, o_jfdir = 0;
+  if (jac) {
+    const size_t P = C * (L + 3);
+    o_basis = take(C * L * 2 * Np * 8);
+    o_jlay = take(P * L * (2 * Np * Np + 2 * Np) * 8);
+    o_jrb = take(P * L * (2 * Np * Np + Np) * 8);
+    o_jlflag = take(P * L * 4);
+    o_jtau_s = take(P * L * 8);
+    o_jtau_cum = take(P * (L + 1) * 8);
+    o_jB = take(P * (L + 1) * 8);
+    o_jsurf = take(P * 8);
+    o_jtop = take(P * 8);
+    o_jssa = take(P * L * 8);
+    o_jfdir = take(P * (L + 1) * 8);
+  }
   if (hipMalloc((void**)&ws->base, off) != hipSuccess) {
     delete ws;
     return ADRT_ERR_HIP;
@@ -589,6 +698,19 @@ int adrt_workspace_create_ex(const adrt_batch_config* cfg, size_t max_bytes, int
   ws->rb = (double*)(ws->base + o_rb);
   ws->phase = phase_bytes ? (double*)(ws->base + o_phase) : nullptr;
   ws->raw = (double*)(ws->base + o_raw);
+  if (jac) {
+    ws->basis = (double*)(ws->base + o_basis);
+    ws->jlay = (double*)(ws->base + o_jlay);
+    ws->jrb = (double*)(ws->base + o_jrb);
+    ws->jlflag = (int*)(ws->base + o_jlflag);
+    ws->jtau_s = (double*)(ws->base + o_jtau_s);
+    ws->jtau_cum = (double*)(ws->base + o_jtau_cum);
+    ws->jB = (double*)(ws->base + o_jB);
+    ws->jsurf = (double*)(ws->base + o_jsurf);
+    ws->jtop = (double*)(ws->base + o_jtop);
+    ws->jssa = (double*)(ws->base + o_jssa);
+    ws->jfdir = (double*)(ws->base + o_jfdir);
+  }
   if (adjoint) {
     ws->Iv = (double*)(ws->base + o_iv);
     ws->Av = (double*)(ws->base + o_av);
@@ -699,6 +821,36 @@ int forward_chunk(const adrt::dev::FwdArgs& a, adrt_workspace* ws, hipStream_t s
   }
 }
 
+int layers_dispatch(const adrt::dev::FwdArgs& a, adrt_workspace* ws, hipStream_t stream) {
+  switch (ws->Np) {
+#ifndef ADRT_ONLY_N
+    case 2: return layers_chunk<2>(a, ws, stream, false);
+    case 4: return layers_chunk<4>(a, ws, stream, false);
+    case 8: return layers_chunk<8>(a, ws, stream, false);
+    case 16: return layers_chunk<16>(a, ws, stream, false);
+    case 32: return layers_chunk<32>(a, ws, stream, false);
+#else
+    case ADRT_ONLY_N: return layers_chunk<ADRT_ONLY_N>(a, ws, stream, false);
+#endif
+    default: return ADRT_ERR_QUADRATURE_MAX;
+  }
+}
+
+int adding_dispatch(const adrt::dev::FwdArgs& a, adrt_workspace* ws, hipStream_t stream) {
+  switch (ws->Np) {
+#ifndef ADRT_ONLY_N
+    case 2: return adding_chunk<2>(a, ws, stream, false);
+    case 4: return adding_chunk<4>(a, ws, stream, false);
+    case 8: return adding_chunk<8>(a, ws, stream, false);
+    case 16: return adding_chunk<16>(a, ws, stream, false);
+    case 32: return adding_chunk<32>(a, ws, stream, false);
+#else
+    case ADRT_ONLY_N: return adding_chunk<ADRT_ONLY_N>(a, ws, stream, false);
+#endif
+    default: return ADRT_ERR_QUADRATURE_MAX;
+  }
+}
+
 int check_solve_args(const adrt_batch_config* cfg, const adrt_device_inputs* in,
                      const adrt_device_outputs* out, adrt_precision precision,
                      adrt_workspace* ws) {
@@ -746,6 +898,75 @@ int adrt_solve_batch(const adrt_batch_config* cfg, const adrt_device_inputs* in,
   ws->generation = ++g_forward_tokens;
   ws->kept_valid = ws->Iv != nullptr && cfg->num_wavenumbers <= ws->chunk;
   ws->kept_mixed = precision == ADRT_PRECISION_MIXED;  // FP32 records (both other forms: FP64)
+  return ADRT_OK;
+}
+
+int adrt_source_jacobian_batch(const adrt_batch_config* cfg, const adrt_device_inputs* in,
+                               const adrt_device_jacobians* jac, adrt_workspace* ws,
+                               hipStream_t stream) {
+  int rc = check_config(cfg);
+  if (rc) return rc;
+  if (!in || !jac) return ADRT_ERR_NULL;
+  if (cfg->num_wavenumbers == 0) return ADRT_OK;
+  if (!in->delta_tau) return ADRT_ERR_DELTA_TAU;
+  if (!in->single_scat_albedo) return ADRT_ERR_SSA;
+  if (!in->phase_moments) return ADRT_ERR_MOMENTS;
+  if (!jac->flux_up || !jac->flux_down || !jac->mean_intensity) return ADRT_ERR_OUTPUTS;
+  if (!ws || !ws->basis || ws->L != cfg->num_layers || ws->n_real != cfg->num_quadrature ||
+      ws->M != cfg->num_moments)
+    return ADRT_ERR_WORKSPACE;
+  // the records need no beam and no sources: R, T and the thermal basis only
+  adrt_batch_config c = *cfg;
+  c.solar_flux = 0.0;
+  rc = upload_tables(ws, 1.0, stream);
+  if (rc) return rc;
+  adrt_device_inputs ri = *in;
+  ri.planck_levels = nullptr;
+  ri.surface_emission = nullptr;
+  ri.top_emission = nullptr;
+  adrt_device_outputs none{nullptr, nullptr, nullptr, nullptr, nullptr};
+  adrt::dev::FwdArgs a = make_args(&c, &ri, &none, ws, false);
+  a.basis = ws->basis;
+  ws->kept_valid = false;
+  const int L = cfg->num_layers, U = L + 3;
+  for (int w0 = 0; w0 < cfg->num_wavenumbers; w0 += ws->chunk) {
+    a.w0 = w0;
+    a.W = std::min(ws->chunk, cfg->num_wavenumbers - w0);
+    rc = layers_dispatch(a, ws, stream);
+    if (rc) return rc;
+    const long P = (long)a.W * U;
+    JacArgs j{a.W, L, ws->Np, a.rev, ws->lay, ws->basis, ws->lflag, ws->tau_s, ws->tau_cum,
+              in->single_scat_albedo, (long)w0, ws->jlay, ws->jlflag, ws->jtau_s, ws->jtau_cum,
+              ws->jB, ws->jsurf, ws->jtop, jac->flux_divergence ? ws->jssa : nullptr};
+    jac_records_kernel<<<(unsigned)(P * L), 256, 0, stream>>>(j);
+    jac_columns_kernel<<<(unsigned)((P + 127) / 128), 128, 0, stream>>>(j, P);
+    if (hipGetLastError() != hipSuccess) return ADRT_ERR_HIP;
+    // the adding sweeps and interfaces of the pseudo-columns, outputs into the
+    // Jacobians ([W][L+3][L+1]: interface index fastest)
+    adrt::dev::FwdArgs p = a;
+    p.W = (int)P;
+    p.w0 = 0;
+    p.has_solar = 0;
+    p.B = ws->jB;
+    p.surf_w = ws->jsurf;
+    p.top_w = ws->jtop;
+    p.ssa = ws->jssa;
+    p.lay = ws->jlay;
+    p.rb = ws->jrb;
+    p.lflag = ws->jlflag;
+    p.tau_s = ws->jtau_s;
+    p.tau_cum = ws->jtau_cum;
+    p.basis = nullptr;
+    p.Iv = p.Av = nullptr;
+    const long o = (long)w0 * U * (L + 1);
+    p.fu = jac->flux_up + o;
+    p.fd = jac->flux_down + o;
+    p.mi = jac->mean_intensity + o;
+    p.fdir = ws->jfdir;
+    p.fdiv = jac->flux_divergence ? jac->flux_divergence + o : nullptr;
+    rc = adding_dispatch(p, ws, stream);
+    if (rc) return rc;
+  }
   return ADRT_OK;
 }
 

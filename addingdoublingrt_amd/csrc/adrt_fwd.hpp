@@ -62,6 +62,8 @@ struct FwdArgs {
   int* lflag;       // [W][L]   is_scattering
   int* redo;        // [W][L]   mixed form: layer recomputed by the FP64 kernel
   double* rb;       // [W][L][2N^2 + N]    base composites R_ab, T_ba, s_up (n_base = 1..L)
+  double* basis;    // [W][L][2N] thermal source basis or null: y, z of the layer
+                    // (s_up = y B_bar + z B_d, s_dn = y B_bar - z B_d), element j
   // reverse pass (N = 16 kernels)
   double* Iv;         // [W][L+1][2][N] forward interface intensities (up, down); null = not kept
   double* Av;         // [W][L+1][2][N] adjoint interface vectors (up, down)
@@ -208,6 +210,10 @@ __global__ void __launch_bounds__(256) layer_kernel(FwdArgs a) {
     if (h == 0) {
       out[2 * N * N + j] = 0.0;
       out[2 * N * N + N + j] = 0.0;
+      if (a.basis) {
+        a.basis[task * 2 * N + j] = 0.0;
+        a.basis[task * 2 * N + N + j] = 0.0;
+      }
     }
     if (j == 0 && h == 0) a.lflag[task] = 0;
     return;
@@ -225,6 +231,10 @@ __global__ void __launch_bounds__(256) layer_kernel(FwdArgs a) {
     if (h == 0) {
       out[2 * N * N + j] = B_bar * omt + B_d * slope;
       out[2 * N * N + N + j] = B_bar * omt - B_d * slope;
+      if (a.basis) {
+        a.basis[task * 2 * N + j] = omt;
+        a.basis[task * 2 * N + N + j] = slope;
+      }
     }
     if (j == 0 && h == 0) a.lflag[task] = 0;
     return;
@@ -425,6 +435,10 @@ __global__ void __launch_bounds__(256) layer_kernel(FwdArgs a) {
   if (h == 0) {
     out[2 * N * N + j] = (y * B_bar + z * B_d) + sus;
     out[2 * N * N + N + j] = (y * B_bar - z * B_d) + sds;
+    if (a.basis) {
+      a.basis[task * 2 * N + j] = y;
+      a.basis[task * 2 * N + N + j] = z;
+    }
   }
   if (j == 0 && h == 0) a.lflag[task] = 1;
 }

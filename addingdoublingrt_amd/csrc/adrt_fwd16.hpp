@@ -13,10 +13,18 @@ namespace adrt {
 namespace dev {
 namespace w16 {
 
+// Per-wave LDS region of the N = 16 layer and adding kernels (a workgroup of
+// four waves holds four consecutive regions).  Every path -- Newton-Schulz,
+// the LU fallback (which runs in registers and lane permutes, no LDS), the
+// interface solves -- stays inside its wave's region: a write past it would
+// land in the next wave's tiles, which that wave reads without any fault.
 struct Lds16 {
   static constexpr int tile = 0, tile2 = TILE, pv = 2 * TILE, vs = pv + N;  // 4 vector slots
   static constexpr int size = vs + 4 * N;
 };
+static_assert(Lds16::tile + TILE <= Lds16::tile2 && Lds16::tile2 + TILE <= Lds16::pv &&
+                  Lds16::pv + N <= Lds16::vs && Lds16::vs + 4 * N <= Lds16::size,
+              "N = 16 per-wave LDS regions overlap");
 
 constexpr int REC16 = 2 * N * N + 2 * N;  // R, T (lane order), s_up, s_down
 constexpr int RBR16 = 2 * N * N + N;      // R_ab, T_ba (lane order), s_up
@@ -285,6 +293,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))
     if (L.h == 0) {
       out[2 * N * N + j] = su;
       out[2 * N * N + N + j] = sd;
+      if (a.basis) {  // thermal basis: (1 - t, slope) (zero for a transparent layer)
+        const double omt = tau > 0.0 ? 1.0 - trans : 0.0;
+        const double slope = tau > 0.0 ? mu_j * omt - 0.5 * tau * (1.0 + trans) : 0.0;
+        a.basis[task * 2 * N + j] = omt;
+        a.basis[task * 2 * N + N + j] = slope;
+      }
     }
     if (L.l == 0) a.lflag[task] = 0;
     return;
@@ -383,6 +397,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))
   if (L.h == 0) {
     out[2 * N * N + j] = (y * B_bar + z * B_d) + sus;
     out[2 * N * N + N + j] = (y * B_bar - z * B_d) + sds;
+    if (a.basis) {
+      a.basis[task * 2 * N + j] = y;
+      a.basis[task * 2 * N + N + j] = z;
+    }
   }
   if (L.l == 0) a.lflag[task] = REDO ? 2 : 1;
 }

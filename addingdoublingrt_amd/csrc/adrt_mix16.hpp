@@ -31,6 +31,13 @@ struct LdsM {
                        vs = pv + N;  // 4 FP32 vector slots (2N doubles)
   static constexpr int size = vs + 2 * N;
 };
+// an FP32 tile (N * TS floats) fits the double tile it is given; four FP32
+// vector slots of N fit 2 N doubles
+static_assert(N * TS * sizeof(float) <= TILE * sizeof(double) &&
+                  LdsM::tf2 + TILE <= LdsM::td0 && LdsM::td0 + TILE <= LdsM::pv &&
+                  LdsM::pv + N <= LdsM::vs && LdsM::vs + 2 * N <= LdsM::size &&
+                  4 * N * sizeof(float) <= 2 * N * sizeof(double),
+              "mixed per-wave LDS regions overlap");
 
 // mixed record: R, S = I - T as FP32 (register order) then s_up, s_dn as FP64
 __device__ __forceinline__ float* mrec_mats(double* lay, long task) {

@@ -29,6 +29,8 @@ namespace w16 {
 constexpr int N = 16;
 constexpr int TS = 17;           // LDS tile column stride (padding against bank conflicts)
 constexpr int TILE = N * TS;     // doubles/floats per tile
+// every tile access (put, get, get_t, mul, matvec) indexes tile[k * TS + i], 0 <= i, k < N
+static_assert((N - 1) * TS + (N - 1) < TILE, "tile index past the tile");
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));

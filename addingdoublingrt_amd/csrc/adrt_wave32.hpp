@@ -43,7 +43,11 @@ constexpr int N = 32;
 constexpr int TS = 33;          // LDS tile column stride
 constexpr int TILE = N * TS;    // doubles per tile
 constexpr int WAVES = 2;        // waves per 128-thread block
-constexpr int SLOT = 2 * TILE + 4 * N;  // LDS doubles per wave
+constexpr int SLOT = 2 * TILE + 4 * N;  // LDS doubles per wave: two tiles, four vector slots
+// every tile access indexes tile[k * TS + i], 0 <= i, k < N; the LU fallback
+// keeps its 32 x 64 augmented system in the wave's two tiles
+static_assert((N - 1) * TS + (N - 1) < TILE && 2 * TILE + 4 * N <= SLOT,
+              "N = 32 per-wave LDS regions overlap");
 
 struct Mat {  // this lane's 16 entries: v[b][r] = M(16 (b & 1) + h + 4r, 16 (b >> 1) + j)
   double v[4][4];
@@ -363,6 +367,10 @@ layer32_kernel(FwdArgs a) {
       for (int J = 0; J < 2; ++J) {
         out[2 * N * N + 16 * J + j] = 0.0;
         out[2 * N * N + N + 16 * J + j] = 0.0;
+        if (a.basis) {
+          a.basis[task * 2 * N + 16 * J + j] = 0.0;
+          a.basis[task * 2 * N + N + 16 * J + j] = 0.0;
+        }
       }
     if (j == 0 && h == 0) a.lflag[task] = 0;
     return;
@@ -391,6 +399,10 @@ layer32_kernel(FwdArgs a) {
       for (int J = 0; J < 2; ++J) {
         out[2 * N * N + 16 * J + j] = B_bar * omt[J] + B_d * slope[J];
         out[2 * N * N + N + 16 * J + j] = B_bar * omt[J] - B_d * slope[J];
+        if (a.basis) {
+          a.basis[task * 2 * N + 16 * J + j] = omt[J];
+          a.basis[task * 2 * N + N + 16 * J + j] = slope[J];
+        }
       }
     if (j == 0 && h == 0) a.lflag[task] = 0;
     return;
@@ -600,6 +612,13 @@ layer32_kernel(FwdArgs a) {
     out[2 * N * N + 16 + j] = (y.hi * B_bar + z.hi * B_d) + sus.hi;
     out[2 * N * N + N + j] = (y.lo * B_bar - z.lo * B_d) + sds.lo;
     out[2 * N * N + N + 16 + j] = (y.hi * B_bar - z.hi * B_d) + sds.hi;
+    if (a.basis) {
+      double* bs = a.basis + task * 2 * N;
+      bs[j] = y.lo;
+      bs[16 + j] = y.hi;
+      bs[N + j] = z.lo;
+      bs[N + 16 + j] = z.hi;
+    }
   }
   if (j == 0 && h == 0) a.lflag[task] = 1;
 }

@@ -38,16 +38,18 @@ class Workspace:
     """Owns an ``adrt_workspace`` (device scratch for one column shape)."""
 
     def __init__(self, num_wavenumbers, num_layers, num_quadrature, num_moments, max_bytes=0,
-                 adjoint=False):
+                 adjoint=False, jacobian=False):
         cfg = _lib.BatchConfig(num_wavenumbers=num_wavenumbers, num_layers=num_layers,
                                num_quadrature=num_quadrature, num_moments=num_moments,
                                surface_albedo=0.0, solar_mu=1.0, spectrum_scaling=1.0)
         h = C.c_void_p()
-        flags = _lib.WORKSPACE_ADJOINT if adjoint else 0
+        flags = (_lib.WORKSPACE_ADJOINT if adjoint else 0) | (
+            _lib.WORKSPACE_JACOBIAN if jacobian else 0)
         _lib.check(_lib.lib().adrt_workspace_create_ex(C.byref(cfg), int(max_bytes), flags,
                                                        C.byref(h)))
         self.handle = h
         self.adjoint = bool(adjoint)
+        self.jacobian = bool(jacobian)
         self.shape = (num_layers, num_quadrature, num_moments)
         b = C.c_size_t()
         ch = C.c_int()
@@ -79,19 +81,19 @@ _ws_lock = threading.Lock()
 
 
 def get_workspace(num_wavenumbers, num_layers, num_quadrature, num_moments, max_bytes=0,
-                  adjoint=False):
+                  adjoint=False, jacobian=False):
     """The cached workspace for this column shape on the current device and
     stream: work on one stream runs in order, so solves on one stream may share
     scratch; another stream gets a workspace of its own."""
     key = (torch.cuda.current_device(), torch.cuda.current_stream().cuda_stream, num_layers,
-           num_quadrature, num_moments, int(max_bytes), bool(adjoint))
+           num_quadrature, num_moments, int(max_bytes), bool(adjoint), bool(jacobian))
     with _ws_lock:
         ws = _ws_cache.get(key)
         # a workspace serves any batch size (the library chunks); rebuild it
         # only when its chunk was capped by a smaller earlier batch
         if ws is None or (ws.chunk < num_wavenumbers and ws.chunk == ws.created_for):
             ws = Workspace(num_wavenumbers, num_layers, num_quadrature, num_moments, max_bytes,
-                           adjoint)
+                           adjoint, jacobian)
             _ws_cache[key] = ws
         return ws
 
@@ -287,51 +289,62 @@ def adjoint_batch(delta_tau, single_scat_albedo, phase_moments, config: BatchCon
 REVERSE_MAX_QUADRATURE = 16
 
 
+JACOBIAN_NAMES = ("flux_up", "flux_down", "mean_intensity", "flux_divergence")
+
+
+def source_jacobians(delta_tau, single_scat_albedo, phase_moments, config: BatchConfig,
+                     divergence=True, max_workspace_bytes=0):
+    """Derivatives of the outputs with respect to the thermal sources
+    (``adrt_source_jacobian_batch``): a dict of ``[W][L+3][L+1]`` tensors,
+    ``J[w, u, i] = d output(w, interface i) / d source u`` with u = 0..L the
+    Planck value of level u, L+1 the surface emission and L+2 the top emission
+    (the part of the reference's computeTemperatureJacobian, src/solver.cpp:
+    117-341, before the dB/dT chain rule).  Each column's doubling runs once;
+    any quadrature order."""
+    W, L, M, shared, cfg, inp = _prepare(delta_tau, single_scat_albedo, phase_moments, config,
+                                         None, None, None)
+    dev = delta_tau.device
+    names = JACOBIAN_NAMES if divergence else JACOBIAN_NAMES[:3]
+    jac = {k: torch.empty((W, L + 3, L + 1), dtype=torch.float64, device=dev) for k in names}
+    js = _lib.DeviceJacobians(**{k: C.c_void_p(v.data_ptr()) for k, v in jac.items()})
+    ws = get_workspace(W, L, config.num_quadrature, M, max_workspace_bytes, jacobian=True)
+    stream = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    with ws.lock:
+        _lib.check(_lib.lib().adrt_source_jacobian_batch(C.byref(cfg), C.byref(inp), C.byref(js),
+                                                         ws.handle, stream))
+    return jac
+
+
 def planck_source_vjp(delta_tau, single_scat_albedo, phase_moments, config: BatchConfig,
-                      cotangents, max_columns=1 << 16, max_workspace_bytes=0):
+                      cotangents, max_columns=1 << 14, max_workspace_bytes=0):
     """Gradients of ``sum(cotangent * output)`` w.r.t. the Planck levels and the
-    surface and top emission, from forward solves alone (any quadrature order).
+    surface and top emission, from the source Jacobians (any quadrature order).
 
     R and T do not depend on the thermal sources and every output is affine in
-    them, so d(output)/d(B_m) is the solve with B = e_m, no beam and no other
-    emission (the linear recursion the reference's computeTemperatureJacobian
-    runs in forward mode, src/solver.cpp:117-341).  Each column is solved L + 3
-    times (unit level, surface and top sources), in batches of at most
-    ``max_columns`` columns.  Returns ``[W][L+1]``, ``[W]``, ``[W]``.
+    them, so these gradients are the cotangents contracted with
+    ``source_jacobians`` (the linear recursion the reference's
+    computeTemperatureJacobian runs in forward mode, src/solver.cpp:117-341),
+    in batches of at most ``max_columns`` columns.  Returns ``[W][L+1]``,
+    ``[W]``, ``[W]``.
     """
     W, L = delta_tau.shape
     dev = delta_tau.device
-    nu = L + 3
-    cfg = BatchConfig(**{**config.__dict__, "solar_flux": 0.0, "surface_emission": 0.0,
-                         "top_emission": 0.0, "spectrum_scaling": config.spectrum_scaling})
-    eye = torch.eye(L + 1, dtype=torch.float64, device=dev)
-    unitB = torch.cat([eye, torch.zeros((2, L + 1), dtype=torch.float64, device=dev)])
-    unitS = torch.zeros(nu, dtype=torch.float64, device=dev)
-    unitS[L + 1] = 1.0
-    unitT = torch.zeros(nu, dtype=torch.float64, device=dev)
-    unitT[L + 2] = 1.0
     gB = torch.zeros((W, L + 1), dtype=torch.float64, device=dev)
     gS = torch.zeros(W, dtype=torch.float64, device=dev)
     gT = torch.zeros(W, dtype=torch.float64, device=dev)
-    names = ("flux_up", "flux_down", "mean_intensity", "flux_divergence")
-    step = max(1, max_columns // nu)
-    for w0 in range(0, W, step):
-        w1 = min(W, w0 + step)
-        n = w1 - w0
-        rep = lambda t: t[w0:w1].repeat_interleave(nu, dim=0).contiguous()
-        mom = phase_moments if phase_moments.dim() == 2 else rep(phase_moments)
-        out = solve_batch(rep(delta_tau), rep(single_scat_albedo), mom, cfg,
-                          planck_levels=unitB.repeat(n, 1).contiguous(),
-                          surface_emission=unitS.repeat(n).contiguous(),
-                          top_emission=unitT.repeat(n).contiguous(),
-                          max_workspace_bytes=max_workspace_bytes)
-        acc = torch.zeros((n, nu), dtype=torch.float64, device=dev)
-        for k in names:
+    div = cotangents.get("flux_divergence") is not None
+    for w0 in range(0, W, max_columns):
+        w1 = min(W, w0 + max_columns)
+        mom = phase_moments if phase_moments.dim() == 2 else phase_moments[w0:w1].contiguous()
+        J = source_jacobians(delta_tau[w0:w1].contiguous(),
+                             single_scat_albedo[w0:w1].contiguous(), mom, config,
+                             divergence=div, max_workspace_bytes=max_workspace_bytes)
+        acc = torch.zeros((w1 - w0, L + 3), dtype=torch.float64, device=dev)
+        for k in JACOBIAN_NAMES:
             c = cotangents.get(k)
-            if c is None:
+            if c is None or k not in J:
                 continue
-            J = out[k].view(n, nu, L + 1)                       # [w][source][interface]
-            acc += torch.einsum("wsi,wi->ws", J, c[w0:w1].to(torch.float64))
+            acc += torch.bmm(J[k], c[w0:w1].to(torch.float64).unsqueeze(-1)).squeeze(-1)
         gB[w0:w1] = acc[:, :L + 1]
         gS[w0:w1] = acc[:, L + 1]
         gT[w0:w1] = acc[:, L + 2]

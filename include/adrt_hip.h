@@ -23,6 +23,8 @@
  *   status codes 1..17      <- the std::invalid_argument cases of
  *                              adrt::ADConfig::validate     src/adding_doubling.hpp:123-174
  *
+ *   adrt_source_jacobian_batch <- adrt::computeTemperatureJacobian (the Planck-value part)
+ *                                                            src/solver.cpp:117-341
  *   adrt_planck_levels      <- the device Planck             src_cuda/cuda_planck.cuh:33-122
  *   adrt_validate_inputs    <- the tau / omega range checks of adrt::ADConfig::validate,
  *                              on the device
@@ -145,7 +147,8 @@ typedef struct adrt_device_gradients {
 typedef struct adrt_workspace adrt_workspace;
 
 /* Workspace flags. */
-#define ADRT_WORKSPACE_ADJOINT 1 /* keep what the reverse pass needs (N <= 16) */
+#define ADRT_WORKSPACE_ADJOINT 1  /* keep what the reverse pass needs (N <= 16) */
+#define ADRT_WORKSPACE_JACOBIAN 2 /* room for adrt_source_jacobian_batch (not with ADJOINT) */
 
 /* Create a workspace for columns of this shape (num_layers, num_quadrature,
  * num_moments).  max_bytes bounds its device memory (0 = 4 GiB); the batch is
@@ -167,6 +170,35 @@ int adrt_workspace_info(const adrt_workspace* ws, size_t* bytes, int* chunk_wave
 int adrt_solve_batch(const adrt_batch_config* cfg, const adrt_device_inputs* in,
                      const adrt_device_outputs* out, adrt_precision precision,
                      adrt_workspace* ws, hipStream_t stream);
+
+/* Derivatives of the outputs with respect to the thermal sources, FP64:
+ * array[(w * (L + 3) + u) * (L + 1) + i] = d output(w, interface i) / d source u,
+ * u = 0..L the Planck value of level u, u = L + 1 the surface emission, u = L + 2
+ * the top emission (levels and interfaces in the caller's order). */
+typedef struct adrt_device_jacobians {
+  double* flux_up;
+  double* flux_down;
+  double* mean_intensity;
+  double* flux_divergence;  /* may be NULL */
+} adrt_device_jacobians;
+
+/* Source Jacobians of num_wavenumbers columns (FP64, any quadrature order):
+ * replaces the forward-mode adrt::computeTemperatureJacobian
+ * (src/solver.cpp:117-341), which differentiates the adding recursion and the
+ * interface solves with respect to every level's Planck value, the surface
+ * and the top emission, reusing the factorisations of the solve.  R and T do
+ * not depend on the sources and every output is affine in them, so each
+ * column's doubling runs once (no beam, no sources), its layers keep the
+ * thermal source basis (s_up = y B_bar + z B_d, s_dn = y B_bar - z B_d), and the
+ * adding sweeps and interface solves run for L + 3 pseudo-columns per column,
+ * each with one unit source.  Temperature Jacobians follow by the chain rule
+ * with dB/dT (adrt_planck_levels with derivative != 0).  The inputs'
+ * planck_levels, surface_emission and top_emission are not read, nor is the
+ * beam.  Needs a workspace created with ADRT_WORKSPACE_JACOBIAN (the batch runs
+ * in chunks of the columns that fit it).  Asynchronous on stream. */
+int adrt_source_jacobian_batch(const adrt_batch_config* cfg, const adrt_device_inputs* in,
+                               const adrt_device_jacobians* jac, adrt_workspace* ws,
+                               hipStream_t stream);
 
 /* Scratch held by adrt_solve_batch_noalloc on each GPU it runs on, for the
  * life of the process (a file-scope device array of the library). */

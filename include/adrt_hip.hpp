@@ -382,51 +382,33 @@ inline Column column_of(const ADConfig& c) {
 /// (src/solver.cpp:117-341).  R and T do not depend on temperature and every
 /// output is affine in the Planck values, so the derivative of the outputs by
 /// level value B_m is the solve with B = e_m and no beam (the reference runs
-/// the same linear recursion in forward mode); one batch holds the L+1 level
-/// columns and the surface- and top-emission columns.  Any quadrature order.
+/// the same linear recursion in forward mode).  adrt_source_jacobian_batch
+/// runs the column's doubling once and the adding and interface solves for
+/// the L+1 level sources and the surface and top emission.  Any quadrature
+/// order.
 inline void temperature_jacobians(const ADConfig& c, const detail::Column& k, RTOutput& r,
                                   SolverWorkspace& ws) {
   const int L = c.num_layers, nif = L + 1, ndof = L + 2;
   const int top = c.index_from_bottom ? L : 0, bottom = c.index_from_bottom ? 0 : L;
   adrt_batch_config cfg = k.cfg;
-  const int W = nif + 2;  // column m <= L: B = e_m; L+1: surface emission 1; L+2: top 1
-  cfg.num_wavenumbers = W;
-  cfg.solar_flux = 0.0;
-  cfg.surface_emission = 0.0;
-  cfg.top_emission = 0.0;
+  cfg.num_wavenumbers = 1;
   cfg.spectrum_scaling = 1.0;
   hipStream_t s = ws.stream();
-  adrt_workspace* jw = ws.jacobian.get(cfg, 0);
-  std::vector<double> rep;
-  auto replicate = [&](const std::vector<double>& v) {
-    rep.resize(v.size() * W);
-    for (int w = 0; w < W; ++w) std::copy(v.begin(), v.end(), rep.begin() + (std::size_t)w * v.size());
-    return rep;
-  };
-  std::vector<double> unitB((std::size_t)W * nif, 0.0), unitS(W, 0.0), unitT(W, 0.0);
-  for (int m = 0; m < nif; ++m) unitB[(std::size_t)m * nif + m] = 1.0;
-  unitS[nif] = 1.0;
-  unitT[nif + 1] = 1.0;
-  double* dtau = ws.in[0].get((std::size_t)W * L);
-  double* dssa = ws.in[1].get((std::size_t)W * L);
+  adrt_workspace* jw = ws.jacobian.get(cfg, ADRT_WORKSPACE_JACOBIAN);
+  const int U = nif + 2;  // sources: level m <= L, L+1 surface emission, L+2 top emission
+  double* dtau = ws.in[0].get((std::size_t)L);
+  double* dssa = ws.in[1].get((std::size_t)L);
   double* dmom = ws.in[2].get(k.mom.size());
-  double* dB = ws.in[3].get((std::size_t)W * nif);
-  double* dS = ws.in[4].get(W);
-  double* dT = ws.in[5].get(W);
-  detail::upload(dtau, replicate(k.tau), s);
-  detail::upload(dssa, replicate(k.ssa), s);
+  detail::upload(dtau, k.tau, s);
+  detail::upload(dssa, k.ssa, s);
   detail::upload(dmom, k.mom, s);
-  detail::upload(dB, unitB, s);
-  detail::upload(dS, unitS, s);
-  detail::upload(dT, unitT, s);
-  adrt_device_inputs in{dtau, dssa, dmom, 1, dB, dS, dT};
-  double* o[5];
-  for (int q = 0; q < 5; ++q) o[q] = ws.out[q].get((std::size_t)W * nif);
-  adrt_device_outputs out{o[0], o[1], o[2], o[3], o[4]};
-  detail::check(adrt_solve_batch(&cfg, &in, &out, ADRT_PRECISION_F64, jw, s));
+  adrt_device_inputs in{dtau, dssa, dmom, 1, nullptr, nullptr, nullptr};
+  double* o[4];
+  for (int q = 0; q < 4; ++q) o[q] = ws.out[q].get((std::size_t)U * nif);
+  adrt_device_jacobians jac{o[0], o[1], o[2], o[3]};
+  detail::check(adrt_source_jacobian_batch(&cfg, &in, &jac, jw, s));
   std::vector<double> h[4];
-  const int oq[4] = {0, 1, 2, 4};  // flux_up, flux_down, mean_intensity, flux_divergence
-  for (int a = 0; a < 4; ++a) detail::download(h[a], o[oq[a]], (std::size_t)W * nif, s);
+  for (int a = 0; a < 4; ++a) detail::download(h[a], o[a], (std::size_t)U * nif, s);
   detail::hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
   // d(output a at interface i) / d(B_m): column m, interface i
   auto d = [&](int a, int i, int col) { return h[a][(std::size_t)col * nif + i]; };
