@@ -345,7 +345,7 @@ struct adrt_workspace {
   double *mu = nullptr, *wt = nullptr, *Pl = nullptr, *Pl0 = nullptr;
   double *tau_s = nullptr, *omega_s = nullptr, *tau_cum = nullptr, *fdm = nullptr;
   double *lay = nullptr, *rb = nullptr;
-  int *mode = nullptr, *lflag = nullptr, *flag = nullptr, *redo = nullptr;
+  int *mode = nullptr, *lflag = nullptr, *flag = nullptr, *redo = nullptr, *ucol = nullptr;
   // reverse pass (ADRT_WORKSPACE_ADJOINT)
   int flags = 0;
   double *Iv = nullptr, *Av = nullptr, *g_taus = nullptr, *g_omgs = nullptr, *g_Bt = nullptr,
@@ -357,7 +357,7 @@ struct adrt_workspace {
   double *basis = nullptr, *jlay = nullptr, *jrb = nullptr, *jtau_s = nullptr,
          *jtau_cum = nullptr, *jB = nullptr, *jsurf = nullptr, *jtop = nullptr, *jssa = nullptr,
          *jfdir = nullptr;
-  int* jlflag = nullptr;
+  int *jlflag = nullptr, *jucol = nullptr;
   int nwaves = 0, nn_cap = 0;
   unsigned long long generation = 0;  // token of the last forward solve (unique in the process)
   bool kept_valid = false;            // the last forward's state (one chunk) is still held
@@ -380,9 +380,9 @@ size_t per_wavenumber_bytes(int L, int Np, int M, bool adjoint) {
   const size_t d = sizeof(double);
   size_t b = 0;
   b += 4 * (size_t)L * d + (size_t)(L + 1) * d;      // tau_s, omega_s, fdm, spare, tau_cum
-  b += 3 * (size_t)L * sizeof(int);                   // mode, lflag, redo
+  b += 3 * (size_t)L * sizeof(int) + sizeof(int);     // mode, lflag, redo; ucol
   b += (size_t)L * (2 * Np * Np + 2 * Np) * d;        // layer store
-  b += (size_t)L * (2 * Np * Np + Np) * d;            // base composites
+  b += (size_t)L * (2 * Np * Np + 4 * Np) * d;        // adding records
   b += (size_t)(3 * L + 3) * d;                       // raw-coefficient optics
   if (adjoint) {
     b += 2 * (size_t)(L + 1) * 2 * Np * d;            // forward and adjoint interface vectors
@@ -483,10 +483,15 @@ int adding_chunk(const adrt::dev::FwdArgs& a, adrt_workspace* ws, hipStream_t st
   constexpr int GPB = Geo<N>::GPB;
   const int e = prof_begin(ws, 2, stream);
   if constexpr (N == 16) {
-    if (mixed)
-      w16::adding16_kernel<false, true><<<(unsigned)((a.W + 3) / 4), 256, 0, stream>>>(a);
-    else
-      w16::adding16_kernel<false, false><<<(unsigned)((a.W + 3) / 4), 256, 0, stream>>>(a);
+    // the base sweep; columns flagged by the prep kernel take the composite form
+    const unsigned g = (unsigned)((a.W + 3) / 4);
+    if (mixed) {
+      w16::adding16_kernel<false, true><<<g, 256, 0, stream>>>(a);
+      w16::adding16_composite_kernel<false, true><<<g, 256, 0, stream>>>(a);
+    } else {
+      w16::adding16_kernel<false, false><<<g, 256, 0, stream>>>(a);
+      w16::adding16_composite_kernel<false, false><<<g, 256, 0, stream>>>(a);
+    }
   } else if constexpr (N == 32) {
     w32::adding32_kernel<<<(unsigned)((a.W + w32::WAVES - 1) / w32::WAVES), 64 * w32::WAVES, 0,
                            stream>>>(a);
@@ -521,6 +526,8 @@ struct JacArgs {
   long w0;
   double* jlay;         // [P][L][REC]
   int* jlflag;          // [P][L]
+  const int* ucol;      // [C]
+  int* jucol;           // [P]
   double *jtau_s, *jtau_cum, *jB, *jsurf, *jtop, *jssa;
 };
 
@@ -564,6 +571,7 @@ __global__ void jac_columns_kernel(JacArgs j, long P) {
     j.jtau_cum[p * (L + 1) + l] = j.tau_cum[c * (L + 1) + l];
     j.jB[p * (L + 1) + l] = (u == l) ? 1.0 : 0.0;
   }
+  j.jucol[p] = j.ucol[c];
   j.jsurf[p] = (u == L + 1) ? 1.0 : 0.0;
   j.jtop[p] = (u == L + 2) ? 1.0 : 0.0;
 }
@@ -644,9 +652,9 @@ int adrt_workspace_create_ex(const adrt_batch_config* cfg, size_t max_bytes, int
   const size_t o_tau_s = take(C * L * 8), o_om = take(C * L * 8), o_fdm = take(C * L * 8);
   const size_t o_cum = take(C * (L + 1) * 8);
   const size_t o_mode = take(C * L * 4), o_lflag = take(C * L * 4), o_flag = take(4);
-  const size_t o_redo = take(C * L * 4);
+  const size_t o_redo = take(C * L * 4), o_ucol = take(C * 4);
   const size_t o_lay = take(C * L * (2 * Np * Np + 2 * Np) * 8);
-  const size_t o_rb = take(C * L * (2 * Np * Np + Np) * 8);
+  const size_t o_rb = take(C * L * (2 * Np * Np + 4 * Np) * 8);
   const size_t o_phase = take(phase_bytes);
   const size_t o_raw = take(C * (3 * L + 3) * 8);
   size_t o_iv = 0, o_av = 0, o_g = 0, o_chi = 0, o_slab = 0, o_status = 0, o_part = 0,
@@ -662,13 +670,14 @@ int adrt_workspace_create_ex(const adrt_batch_config* cfg, size_t max_bytes, int
     o_status = take(4);
   }
   size_t o_basis = 0, o_jlay = 0, o_jrb = 0, o_jlflag = 0, o_jtau_s = 0, o_jtau_cum = 0,
-         o_jB = 0, o_jsurf = 0, o_jtop = 0, o_jssa = 0, o_jfdir = 0;
+         o_jB = 0, o_jsurf = 0, o_jtop = 0, o_jssa = 0, o_jfdir = 0, o_jucol = 0;
   if (jac) {
     const size_t P = C * (L + 3);
     o_basis = take(C * L * 2 * Np * 8);
     o_jlay = take(P * L * (2 * Np * Np + 2 * Np) * 8);
-    o_jrb = take(P * L * (2 * Np * Np + Np) * 8);
+    o_jrb = take(P * L * (2 * Np * Np + 4 * Np) * 8);
     o_jlflag = take(P * L * 4);
+    o_jucol = take(P * 4);
     o_jtau_s = take(P * L * 8);
     o_jtau_cum = take(P * (L + 1) * 8);
     o_jB = take(P * (L + 1) * 8);
@@ -693,6 +702,7 @@ int adrt_workspace_create_ex(const adrt_batch_config* cfg, size_t max_bytes, int
   ws->mode = (int*)(ws->base + o_mode);
   ws->lflag = (int*)(ws->base + o_lflag);
   ws->redo = (int*)(ws->base + o_redo);
+  ws->ucol = (int*)(ws->base + o_ucol);
   ws->flag = (int*)(ws->base + o_flag);
   ws->lay = (double*)(ws->base + o_lay);
   ws->rb = (double*)(ws->base + o_rb);
@@ -703,6 +713,7 @@ int adrt_workspace_create_ex(const adrt_batch_config* cfg, size_t max_bytes, int
     ws->jlay = (double*)(ws->base + o_jlay);
     ws->jrb = (double*)(ws->base + o_jrb);
     ws->jlflag = (int*)(ws->base + o_jlflag);
+    ws->jucol = (int*)(ws->base + o_jucol);
     ws->jtau_s = (double*)(ws->base + o_jtau_s);
     ws->jtau_cum = (double*)(ws->base + o_jtau_cum);
     ws->jB = (double*)(ws->base + o_jB);
@@ -797,6 +808,7 @@ adrt::dev::FwdArgs make_args(const adrt_batch_config* cfg, const adrt_device_inp
   a.lay = ws->lay;
   a.lflag = ws->lflag;
   a.redo = ws->redo;
+  a.ucol = ws->ucol;
   a.rb = ws->rb;
   a.Iv = ws->Iv;  // kept for the reverse pass (adjoint workspaces only)
   a.Av = ws->Av;
@@ -936,7 +948,8 @@ int adrt_source_jacobian_batch(const adrt_batch_config* cfg, const adrt_device_i
     if (rc) return rc;
     const long P = (long)a.W * U;
     JacArgs j{a.W, L, ws->Np, a.rev, ws->lay, ws->basis, ws->lflag, ws->tau_s, ws->tau_cum,
-              in->single_scat_albedo, (long)w0, ws->jlay, ws->jlflag, ws->jtau_s, ws->jtau_cum,
+              in->single_scat_albedo, (long)w0, ws->jlay, ws->jlflag, ws->ucol, ws->jucol,
+              ws->jtau_s, ws->jtau_cum,
               ws->jB, ws->jsurf, ws->jtop, jac->flux_divergence ? ws->jssa : nullptr};
     jac_records_kernel<<<(unsigned)(P * L), 256, 0, stream>>>(j);
     jac_columns_kernel<<<(unsigned)((P + 127) / 128), 128, 0, stream>>>(j, P);
@@ -954,6 +967,7 @@ int adrt_source_jacobian_batch(const adrt_batch_config* cfg, const adrt_device_i
     p.lay = ws->jlay;
     p.rb = ws->jrb;
     p.lflag = ws->jlflag;
+    p.ucol = ws->jucol;
     p.tau_s = ws->jtau_s;
     p.tau_cum = ws->jtau_cum;
     p.basis = nullptr;
@@ -1139,10 +1153,14 @@ int adrt_adjoint_batch(const adrt_batch_config* cfg, const adrt_device_inputs* i
       ws->kept_valid = false;
     }
     int e = prof_begin(ws, 3, stream);
-    if (mixed)
-      w16::adding16_kernel<true, true><<<(unsigned)((a.W + 3) / 4), 256, 0, stream>>>(a);
-    else
-      w16::adding16_kernel<true, false><<<(unsigned)((a.W + 3) / 4), 256, 0, stream>>>(a);
+    const unsigned g = (unsigned)((a.W + 3) / 4);
+    if (mixed) {
+      w16::adding16_kernel<true, true><<<g, 256, 0, stream>>>(a);
+      w16::adding16_composite_kernel<true, true><<<g, 256, 0, stream>>>(a);
+    } else {
+      w16::adding16_kernel<true, false><<<g, 256, 0, stream>>>(a);
+      w16::adding16_composite_kernel<true, false><<<g, 256, 0, stream>>>(a);
+    }
     prof_end(ws, e, stream);
     const long tasks = (long)a.W * a.L;
     const long blocks = std::min((tasks + 3) / 4, (long)ws->nwaves / 4);
@@ -1332,9 +1350,10 @@ int adrt_solve_batch_noalloc(const adrt_batch_config* cfg, const adrt_device_inp
   ws.mode = (int*)take(C * L * 4);
   ws.lflag = (int*)take(C * L * 4);
   ws.redo = (int*)take(C * L * 4);
+  ws.ucol = (int*)take(C * 4);
   ws.flag = (int*)take(4);
   ws.lay = (double*)take(C * L * (2 * Np * Np + 2 * Np) * 8);
-  ws.rb = (double*)take(C * L * (2 * Np * Np + Np) * 8);
+  ws.rb = (double*)take(C * L * (2 * Np * Np + 4 * Np) * 8);
   if (off > ADRT_NOALLOC_SCRATCH_BYTES) return ADRT_ERR_SCRATCH;  // not reached: sized above
   tables_kernel<<<1, 64, 0, stream>>>(q, ws.mu);
   if (hipGetLastError() != hipSuccess) return ADRT_ERR_HIP;

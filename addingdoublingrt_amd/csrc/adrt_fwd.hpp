@@ -61,7 +61,10 @@ struct FwdArgs {
   double* lay;      // [W][L][2N^2 + 2N]   R, T (column-major), s_up, s_down
   int* lflag;       // [W][L]   is_scattering
   int* redo;        // [W][L]   mixed form: layer recomputed by the FP64 kernel
-  double* rb;       // [W][L][2N^2 + N]    base composites R_ab, T_ba, s_up (n_base = 1..L)
+  int* ucol;        // [W]      N = 16 adding in the reference's composite form (consecutive
+                    //          scattering layers with unstable doubling starts)
+  double* rb;       // [W][L][2N^2 + 4N]   adding records (base composites; N = 16: the
+                    //                     downward-sweep operators of each layer)
   double* basis;    // [W][L][2N] thermal source basis or null: y, z of the layer
                     // (s_up = y B_bar + z B_d, s_dn = y B_bar - z B_d), element j
   // reverse pass (N = 16 kernels)
@@ -107,6 +110,8 @@ __device__ __forceinline__ bool thick_conservative(double tau_s, double omega_s)
 // those with an unstable doubling start and their neighbours in the column
 // (whose rounding errors the unstable layers amplify), and thick, strongly
 // scattering layers
+// -- and the columns whose N = 16 adding runs in the reference's composite
+// form (two consecutive scattering layers with unstable starts)
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) prep_kernel(FwdArgs a) {
   const int w = blockIdx.x * blockDim.x + threadIdx.x;
@@ -116,6 +121,8 @@ __global__ void __launch_bounds__(256) prep_kernel(FwdArgs a) {
   const double mu_min = a.mu[0];
   double cum = 0.0;
   bool unst_prev = false;  // layer l - 1 has an unstable start
+  bool unst_scat = false;  // the last scattering layer above has an unstable start
+  int ucol = 0;
   for (int l = 0; l < L; ++l) {
     const int li = lin(a, l);
     const double tau = a.tau[gw * L + li];
@@ -152,8 +159,13 @@ __global__ void __launch_bounds__(256) prep_kernel(FwdArgs a) {
       if (mode != 0) a.redo[t] = 1;
     }
     unst_prev = unst;
+    if (mode != 0) {
+      if (unst && unst_scat) ucol = 1;
+      unst_scat = unst;
+    }
   }
   a.tau_cum[(long)w * (L + 1) + L] = cum;
+  if (a.ucol) a.ucol[w] = ucol;
 }
 
 template <int N>

@@ -27,7 +27,12 @@ static_assert(Lds16::tile + TILE <= Lds16::tile2 && Lds16::tile2 + TILE <= Lds16
               "N = 16 per-wave LDS regions overlap");
 
 constexpr int REC16 = 2 * N * N + 2 * N;  // R, T (lane order), s_up, s_down
-constexpr int RBR16 = 2 * N * N + N;      // R_ab, T_ba (lane order), s_up
+// per layer, for the downward sweep of the adding: G_l (lane order; the
+// diagonal of T_l for a non-scattering layer), the base below it Rb(l+1)
+// (lane order), d_l, sb(l+1), and whether Rb(l+1) is non-zero
+constexpr int RB_G = 0, RB_R = N * N, RB_D = 2 * N * N, RB_S = RB_D + N, RB_F = RB_S + N;
+constexpr int RBR16 = 2 * N * N + 4 * N;
+static_assert(RB_F < RBR16, "adding record fields past the record");
 
 // ---------------------------------------------------------------------------
 // phase matrices and thin-layer start of a scattering layer, in FP64, for the
@@ -406,22 +411,34 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))
 }
 
 // ---------------------------------------------------------------------------
-// adding sweeps + interfaces, one wave per wavenumber
+// adding and interface intensities, one wave per wavenumber
 //
 // The interface intensities solve the two-stream system
 //   I_up(l)   = R_l I_dn(l) + T_l I_up(l+1) + s_up(l)
 //   I_dn(l+1) = T_l I_dn(l) + R_l I_up(l+1) + s_dn(l)
-// with I_dn(0) and (no surface) I_up(L) given; the adding sweeps and the
-// interface solves are a direct solver for it.  Its transpose has the same
-// form with R_l^T, T_l^T, the output cotangents as sources and the roles of
-// up and down exchanged, so the reverse pass runs this kernel with ADJ = true.
+// with I_dn(0) given and, at the bottom, I_up(L) = R_s I_dn(L) + s_s (the
+// surface) or I_up(L) given.  The reference solves it by adding: composites
+// of all layers above and of all layers below every interface (two sweeps,
+// two inverses per composite) and a solve of the two composites at each
+// interface (src/solver.cpp:24-73, 560-700; src/adding.hpp:31-258).  Here the
+// same system is solved by one sweep of the base composites only -- Rb(l),
+// the reflectance of everything below interface l seen from above, and
+// sb(l), the upward intensity leaving it with nothing incident from above
+// (its sources and the bottom boundary's intensity carried through it):
+//   X_l = (I - R_l Rb(l+1))^-1,   G_l = X_l T_l,
+//   d_l = X_l (R_l sb(l+1) + s_dn(l)),
+//   Rb(l) = R_l + T_l Rb(l+1) G_l,   sb(l) = s_up(l) + T_l (Rb(l+1) d_l + sb(l+1)),
+// from Rb(L) = R_s, sb(L) = s_s (or Rb(L) = 0, sb(L) = the given I_up(L));
+// then one downward sweep of matrix-vector products
+//   I_dn(l+1) = G_l I_dn(l) + d_l,   I_up(l) = Rb(l) I_dn(l) + sb(l),
+// whose I_dn(l+1), I_up(l+1) satisfy both equations of layer l for the
+// I_dn(l) above it and the base below it.  One inverse and four matrix
+// products per scattering layer, against six inverses and about twenty
+// products for the two composite sweeps and the interface solves.  The
+// transposed system has the same form with R_l^T, T_l^T, the output
+// cotangents as sources and the roles of up and down exchanged, so the
+// reverse pass runs this kernel with ADJ = true.
 // ---------------------------------------------------------------------------
-struct Comp16 {
-  Mat<double> Rab, Rba, Tab, Tba;
-  double su, sd;
-  int scat;
-};
-
 struct Ws16 {
   double *tile, *tile2, *pv, *v0, *v1, *v2, *v3;
 };
@@ -494,6 +511,10 @@ struct LayerRef {
       load64(rec + N * N, L, ADJ, T);
     }
   }
+  __device__ void refl(const Lane& L, Mat<double>& R) const {
+    if (narrow()) load(rec, L, ADJ, false, R);
+    else load64(rec, L, ADJ, R);
+  }
   __device__ void trans(const Lane& L, Mat<double>& T) const {
     if (narrow()) load(tmat(), L, ADJ, true, T);
     else load64(rec + N * N, L, ADJ, T);
@@ -519,6 +540,294 @@ __device__ __forceinline__ LayerRef<ADJ, F32> layer_ref(const FwdArgs& a, const 
                                                         double mu_j, double wt_j) {
   return LayerRef<ADJ, F32>{a, lay + (long)l * REC16, gw, mu_j, wt_j, lflag[l]};
 }
+
+// the base below an interface: Rb (lane order; all zero unless scat) and sb
+// (element j)
+struct Base16 {
+  Mat<double> R;
+  double s;
+  int scat;
+};
+
+// Layer l (y) on top of the base B: stores at o what the downward sweep needs
+// at layer l -- G_l (the diagonal of T_l for a non-scattering layer), d_l and
+// the base below, Rb(l+1), sb(l+1) -- and replaces B by the base below
+// interface l.
+template <class LR>
+__device__ void base16_add(Base16& B, const LR& y, int l, double* o, const Lane& L,
+                           const Ws16& s) {
+  using R = double;
+  const int j = L.j;
+  R su, sd;
+  y.sources(L, l, su, sd);
+  if (B.scat) gstore<R>(o + RB_R, L, B.R);
+  if (L.h == 0) {
+    o[RB_S + j] = B.s;
+    if (j == 0) o[RB_F] = B.scat ? 1.0 : 0.0;
+  }
+  if (!y.scat) {
+    // T_l = diag(t), R_l = 0: X = I, G = T_l, d = s_dn,
+    // Rb(l) = T_l Rb T_l, sb(l) = s_up + T_l (Rb s_dn + sb)
+    Mat<R> T;
+    y.trans(L, T);
+    const R tj = diag<R>(T, L);
+    R rsd = 0.0;
+    if (B.scat) {
+      put<R>(s.tile, L, B.R);
+      if (L.h == 0) {
+        s.v0[j] = tj;
+        s.v1[j] = sd;
+      }
+      wave_sync();
+      rsd = matvec<R>(s.tile, s.v1, L);
+      R tv[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) tv[r] = s.v0[row<R>(L.h, r)];
+      wave_sync();
+#pragma unroll
+      for (int r = 0; r < 4; ++r) B.R.v[r] = tv[r] * B.R.v[r] * tj;
+    }
+    if (L.h == 0) {
+      o[RB_G + j] = tj;
+      o[RB_D + j] = sd;
+    }
+    B.s = su + tj * (rsd + B.s);
+    return;
+  }
+  if (!B.scat) {
+    // nothing below reflects: X = I, G = T_l, d = R_l sb + s_dn, Rb(l) = R_l,
+    // sb(l) = s_up + T_l sb
+    Mat<R> T;
+    y.mats(L, B.R, T);
+    put<R>(s.tile, L, B.R);
+    put<R>(s.tile2, L, T);
+    if (L.h == 0) s.v0[j] = B.s;
+    wave_sync();
+    const R d = matvec<R>(s.tile, s.v0, L) + sd;
+    const R ts = matvec<R>(s.tile2, s.v0, L);
+    wave_sync();
+    gstore<R>(o + RB_G, L, T);
+    if (L.h == 0) o[RB_D + j] = d;
+    B.s = su + ts;
+    B.scat = 1;
+    return;
+  }
+  Mat<R> Rl, P, X, G, T;
+  y.refl(L, Rl);
+  put<R>(s.tile, L, Rl);
+  if (L.h == 0) s.v0[j] = B.s;
+  wave_sync();
+  mul<R>(s.tile, B.R, L, P);                      // P = R_l Rb(l+1)
+  const R v = matvec<R>(s.tile, s.v0, L) + sd;    // R_l sb(l+1) + s_dn
+  wave_sync();
+  put<R>(s.tile, L, P);
+  if (L.h == 0) s.v1[j] = v;
+  wave_sync();
+  R d;
+  if (inverse_ns<R, false>(P, s.tile, s.tile2, X, L)) {
+    // I - P close to singular, or the iteration slow: G and d by LU solves
+    y.trans(L, T);
+    left_solve_mat_tile<R>(s.tile, T, G, s.tile2, L);
+    d = left_solve_tile<R>(s.tile, v, L);
+  } else {
+    y.trans(L, T);
+    put<R>(s.tile2, L, X);
+    wave_sync();
+    mul<R>(s.tile2, T, L, G);                     // G = X T_l
+    d = matvec<R>(s.tile2, s.v1, L);              // d = X (R_l sb + s_dn)
+  }
+  wave_sync();
+  gstore<R>(o + RB_G, L, G);
+  if (L.h == 0) o[RB_D + j] = d;
+  put<R>(s.tile, L, B.R);
+  if (L.h == 0) s.v2[j] = d;
+  wave_sync();
+  Mat<R> M;
+  mul<R>(s.tile, G, L, M);                        // Rb(l+1) G
+  const R rd = matvec<R>(s.tile, s.v2, L);        // Rb(l+1) d
+  wave_sync();
+  put<R>(s.tile, L, T);
+  if (L.h == 0) s.v3[j] = rd + B.s;
+  wave_sync();
+  y.refl(L, Rl);
+  mul_add<R>(s.tile, M, Rl, L, B.R);              // Rb(l) = R_l + T_l Rb(l+1) G
+  const R ts = matvec<R>(s.tile, s.v3, L);
+  wave_sync();
+  B.s = su + ts;
+}
+
+template <bool ADJ, bool F32 = false>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8)))
+adding16_kernel(FwdArgs a) {
+  using R = double;
+  __shared__ R smem[4 * Lds16::size];
+  const int wv = wave_index();
+  const int w = blockIdx.x * 4 + wv;
+  if (w >= a.W || (a.ucol && a.ucol[w])) return;
+  const Lane L;
+  const int j = L.j;
+  const long gw = (long)(a.w0 + w);
+  const int Lay = a.L;
+  R* sm = smem + wv * Lds16::size;
+  Ws16 s{sm + Lds16::tile, sm + Lds16::tile2, sm + Lds16::pv,
+         sm + Lds16::vs, sm + Lds16::vs + N, sm + Lds16::vs + 2 * N, sm + Lds16::vs + 3 * N};
+  const double* lay = a.lay + (long)w * Lay * REC16;
+  const int* lflag = a.lflag + (long)w * Lay;
+  double* rbw = a.rb + (long)w * Lay * RBR16;
+  const double mu_j = a.mu[j], wt_j = a.wt[j];
+  const double* Bw = a.B ? a.B + gw * (Lay + 1) : nullptr;
+  auto Blev = [&](int l) -> double { return Bw ? Bw[lev(a, l)] : 0.0; };
+  double* vec = ADJ ? (a.Av ? a.Av + (long)w * (Lay + 1) * 2 * N : nullptr)
+                    : (a.Iv ? a.Iv + (long)w * (Lay + 1) * 2 * N : nullptr);
+
+  // boundary conditions (src/solver.cpp:385-419, 516-551, 585-603)
+  const double Bs = a.surf_w ? a.surf_w[gw] : a.surf_em;
+  const double Btop = a.top_w ? a.top_w[gw] : a.top_em;
+  const double A = a.albedo;
+  const int has_surface = !a.diffusion && (A > 0.0 || Bs > 0.0);
+  const double tau_tot = a.tau_cum[(long)w * (Lay + 1) + Lay];
+  double Itd, Ibu = 0.0;
+  if constexpr (ADJ) {
+    double u, d;
+    ibar16(a, gw, 0, mu_j, wt_j, u, d);
+    Itd = u;  // transposed system: the top boundary carries the cotangent of I_up(0)
+    ibar16(a, gw, Lay, mu_j, wt_j, u, d);
+    if (!has_surface) Ibu = d;  // ... and a fixed I_up(L) the cotangent of I_dn(L)
+  } else {
+    Itd = Btop;
+    if (a.diffusion) {
+      const double Bb = Blev(Lay);
+      const double dt = a.tau_s[(long)w * Lay + Lay - 1];
+      const double dBdt = (dt > 0.0) ? (Bb - Blev(Lay - 1)) / dt : 0.0;
+      Ibu = Bb + mu_j * dBdt;
+    } else if (!has_surface) {
+      Ibu = Bs;
+    }
+  }
+
+  // ---- the base below the last layer (src/solver.cpp:385-419, 516-551, 585-603) ----
+  Base16 B;
+  if (has_surface) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int i = row<R>(L.h, r);
+      if constexpr (ADJ)  // R_s^T(i, j) = R_s(j, i)
+        B.R.v[r] = (j < a.n_real) ? 2.0 * A * a.mu[i] * a.wt[i] * a.xfac : 0.0;
+      else
+        B.R.v[r] = (i < a.n_real) ? 2.0 * A * mu_j * wt_j * a.xfac : 0.0;
+    }
+    if constexpr (ADJ) {
+      double u, d;
+      ibar16(a, gw, Lay, mu_j, wt_j, u, d);
+      B.s = d;
+    } else {
+      double ssol = 0.0;
+      if (a.has_solar && A > 0.0) ssol = (A / kPi) * a.F0 * a.mu0 * exp(-tau_tot / a.mu0);
+      B.s = (1.0 - A) * Bs + ssol;
+    }
+    B.scat = A > 0.0;
+  } else {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) B.R.v[r] = 0.0;
+    B.s = Ibu;
+    B.scat = 0;
+  }
+
+  // ---- base sweep, bottom to top ----
+  for (int l = Lay - 1; l >= 0; --l)
+    base16_add(B, layer_ref<ADJ, F32>(a, lay, lflag, gw, l, mu_j, wt_j), l,
+               rbw + (long)l * RBR16, L, s);
+
+  auto emit = [&](int l, double Iup, double Idn) {
+    if (vec && L.h == 0) {
+      vec[(long)l * 2 * N + j] = Iup;
+      vec[(long)l * 2 * N + N + j] = Idn;
+    }
+    if constexpr (!ADJ) {
+      const double c = 2.0 * kPi * wt_j * mu_j;
+      const double fu = colsum<R>(c * Iup, L, s.pv);
+      const double fdn = colsum<R>(c * Idn, L, s.pv);
+      double J = colsum<R>(0.5 * wt_j * (Iup + Idn), L, s.pv);
+      if (L.l == 0) {
+        const long o = gw * (Lay + 1) + lev(a, l);
+        double fdir = 0.0;
+        if (a.has_solar) {
+          fdir = (l == 0) ? a.F0 * a.mu0
+                          : a.F0 * a.mu0 * exp(-a.tau_cum[(long)w * (Lay + 1) + l] / a.mu0);
+          J += fdir * (1.0 / (4.0 * kPi * a.mu0));
+        }
+        a.fu[o] = fu * a.scaling;
+        a.fd[o] = fdn;
+        a.mi[o] = J;
+        a.fdir[o] = fdir;
+        if (a.fdiv) {
+          const int lyr = (l == 0) ? 0 : l - 1;
+          const double om = a.ssa[gw * Lay + lin(a, lyr)];
+          a.fdiv[o] = 4.0 * kPi * (1.0 - om) * (J - Blev(l));
+        }
+      }
+    }
+  };
+
+  // ---- downward sweep (the interface intensities of src/solver.cpp:628-678) ----
+  double Idn = Itd, Iup = B.s;
+  if (B.scat) {
+    put<R>(s.tile, L, B.R);
+    if (L.h == 0) s.v0[j] = Itd;
+    wave_sync();
+    Iup += matvec<R>(s.tile, s.v0, L);
+    wave_sync();
+  }
+  emit(0, Iup, Idn);
+  for (int l = 0; l < Lay; ++l) {
+    const double* o = rbw + (long)l * RBR16;
+    const double d = o[RB_D + j], sbn = o[RB_S + j];
+    const bool bscat = o[RB_F] != 0.0;
+    if (lflag[l]) {
+      Mat<R> G;
+      gload<R>(o + RB_G, L, G);
+      put<R>(s.tile, L, G);
+      if (L.h == 0) s.v0[j] = Idn;
+      wave_sync();
+      Idn = matvec<R>(s.tile, s.v0, L) + d;
+      wave_sync();
+    } else {
+      Idn = o[RB_G + j] * Idn + d;
+    }
+    Iup = sbn;
+    if (bscat) {
+      Mat<R> Rb;
+      gload<R>(o + RB_R, L, Rb);
+      put<R>(s.tile, L, Rb);
+      if (L.h == 0) s.v0[j] = Idn;
+      wave_sync();
+      Iup += matvec<R>(s.tile, s.v0, L);
+      wave_sync();
+    }
+    emit(l + 1, Iup, Idn);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// adding in the reference's form, for the columns the base sweep does not
+// hold to the reference: columns in which two scattering layers with an
+// unstable doubling start follow each other (non-scattering layers between
+// them aside).  Their records are far from physical (entries of R and T up to
+// 1e8 where the reference's start amplifies its rounding), and at the
+// interface between two such layers I_up = Rb I_dn + sb cancels terms of
+// that size; the reference's composites of the layers above and below each
+// interface and its solve there (src/adding.hpp:69-258, src/solver.cpp:24-73,
+// 560-700) land closer to the reference's own result.  The same system,
+// solved as the reference does: composites C (R_ab, R_ba, T_ab, T_ba, sources)
+// of everything below each interface (bottom-up), then of everything above it
+// (top-down), and at each interface the two 16 x 16 solves for I_up and I_dn.
+// ---------------------------------------------------------------------------
+struct Comp16 {
+  Mat<double> Rab, Rba, Tab, Tba;
+  double su, sd;
+  int scat;
+};
+
 
 // rbase step: new = add(top = layer y, bot = C)  (src/adding.hpp:246-258)
 template <class LR>
@@ -781,12 +1090,12 @@ __device__ void add16_below(Comp16& C, const LR& y, int l, const Lane& L,
 
 template <bool ADJ, bool F32 = false>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8)))
-adding16_kernel(FwdArgs a) {
+adding16_composite_kernel(FwdArgs a) {
   using R = double;
   __shared__ R smem[4 * Lds16::size];
   const int wv = wave_index();
   const int w = blockIdx.x * 4 + wv;
-  if (w >= a.W) return;
+  if (w >= a.W || !a.ucol || !a.ucol[w]) return;
   const Lane L;
   const int j = L.j;
   const long gw = (long)(a.w0 + w);
@@ -1012,6 +1321,7 @@ adding16_kernel(FwdArgs a) {
     if (l < Lay) add16_below(C, layer_ref<ADJ, F32>(a, lay, lflag, gw, l, mu_j, wt_j), l, L, s);
   }
 }
+
 
 }  // namespace w16
 }  // namespace dev

@@ -431,6 +431,26 @@ __device__ __forceinline__ void right_solve_tile(const R* tP, const Mat<R>& T, M
   right_solve<R>(A, D, L);
 }
 
+// G = (I - P)^-1 T for P in the tile tP, by LU with partial pivoting on the
+// transposed system G^T (I - P)^T = T^T (tS: scratch tile)
+template <typename R>
+__device__ __forceinline__ void left_solve_mat_tile(const R* tP, const Mat<R>& T, Mat<R>& G,
+                                                    R* tS, const Lane& L) {
+  Mat<R> A;
+  get_t<R>(tP, L, A);
+  put<R>(tS, L, T);
+  wave_sync();
+  get_t<R>(tS, L, G);
+  wave_sync();
+#pragma unroll
+  for (int r = 0; r < 4; ++r) A.v[r] = ((row<R>(L.h, r) == L.j) ? R(1) : R(0)) - A.v[r];
+  right_solve<R>(A, G, L);
+  put<R>(tS, L, G);
+  wave_sync();
+  get_t<R>(tS, L, G);
+  wave_sync();
+}
+
 // Q = P (I - P)^-1 = (I - P)^-1 - I, for P in Q on entry and in the tile tP
 // (tX: scratch tile), in Q form so that a small Q keeps its relative
 // precision in FP32: the Newton-Schulz iteration of X = I + Q from Q0 = P,
