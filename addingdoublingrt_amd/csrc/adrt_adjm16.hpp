@@ -14,7 +14,8 @@
 //   R^   = R'^ + TGR^ - G^T TGR^ + P^ R^T + R^T P^ + v2^ s_dn^T + gamma v3^ s_up^T
 // (v2^ = s_up'^ - G^T s_up'^, v3^ = s_dn'^ - G^T s_dn'^): matrix products only,
 // on the FP32 matrix cores, from per-step checkpoints of R, S and Q (FP32) and
-// the FP64 vectors.  The doubling is recomputed exactly as the forward does it.
+// the FP64 vectors.  The doubling is recomputed as the forward does it, but
+// with Q's refinement step in FP32 (the forward takes its residual in FP64).
 // Layers the forward left to the FP64 kernel (redo flag) are skipped here and
 // handled by layer16_adjoint_kernel<true>.
 #pragma once
@@ -156,29 +157,23 @@ __device__ void layer16m_adjoint_task(const FwdArgs& a, const AdjArgs& b, long t
     put<R>(tA, L, Q);
     wave_sync();
     q_form_ns<R>(Q, tA, tB, L);
-#pragma unroll
-    for (int r = 0; r < 4; ++r) td[j * TS + row<R>(L.h, r)] = (double)Rm.v[r];
-    wave_sync();
-    d4 P = {0.0, 0.0, 0.0, 0.0};
-    mac64_f32(P, td, Rm, L);
-    wave_sync();
-#pragma unroll
-    for (int r = 0; r < 4; ++r) td[j * TS + row<R>(L.h, r)] = (double)Q.v[r];
-    wave_sync();
-    Mat<double> Pm{{P[0], P[1], P[2], P[3]}};
-    Mat<double> QP;
-    mul<double>(td, Pm, L, QP);
-    double E[4];
-#pragma unroll
-    for (int r = 0; r < 4; ++r) E[r] = (P[r] - td[j * TS + row<double>(L.h, r)]) + QP.v[r];
-    wave_sync();
-#pragma unroll
-    for (int r = 0; r < 4; ++r) tB[j * TS + row<double>(L.h, r)] = (float)E[r];
-    wave_sync();
+    // one refinement step in FP32, E = P - Q + Q P, Q <- Q + E + E Q: Q to
+    // FP32 rounding of the P = R R of this FP32 recursion (the forward's
+    // refinement takes E in FP64 for its 1e-6 flux accuracy; the gradients'
+    // 1e-4 needs only FP32 consistency)
     {
-      Mat<R> Zm;
+      Mat<R> Pm, QP, Zm;
+      get<R>(tA, L, Pm);
+      put<R>(tB, L, Q);
+      wave_sync();
+      mul<R>(tB, Pm, L, QP);
 #pragma unroll
-      for (int r = 0; r < 4; ++r) Zm.v[r] = Q.v[r] + tB[j * TS + row<R>(L.h, r)];
+      for (int r = 0; r < 4; ++r) Pm.v[r] = (Pm.v[r] - Q.v[r]) + QP.v[r];  // E
+      wave_sync();
+      put<R>(tB, L, Pm);
+      wave_sync();
+#pragma unroll
+      for (int r = 0; r < 4; ++r) Zm.v[r] = Q.v[r] + Pm.v[r];
       mul_add<R>(tB, Q, Zm, L, Q);
     }
     gstore<R>(sf + 2 * N * N, L, Q);
