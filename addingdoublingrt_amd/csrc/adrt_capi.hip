@@ -358,7 +358,7 @@ struct adrt_workspace {
          *jtau_cum = nullptr, *jB = nullptr, *jsurf = nullptr, *jtop = nullptr, *jssa = nullptr,
          *jfdir = nullptr;
   int *jlflag = nullptr, *jucol = nullptr;
-  int nwaves = 0, nn_cap = 0;
+  int nwaves = 0, nn_cap = 0, cus = 0;  // slabs (waves of the larger reverse grid); CUs
   unsigned long long generation = 0;  // token of the last forward solve (unique in the process)
   bool kept_valid = false;            // the last forward's state (one chunk) is still held
   bool kept_mixed = false;            // ... and was computed by the mixed-precision form
@@ -614,12 +614,16 @@ int adrt_workspace_create_ex(const adrt_batch_config* cfg, size_t max_bytes, int
   }
   size_t slab_bytes = 0;
   if (adjoint) {
-    // persistent reverse-layer grid: REV_WAVES_PER_SIMD waves on each of 4 SIMDs per CU
+    // persistent reverse-layer grids: REV_WAVES_PER_SIMD (FP64) or
+    // MREV_WAVES_PER_SIMD (FP32) waves on each of 4 SIMDs per CU; one slab
+    // per wave of the larger grid
     int cus = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ws->device) !=
             hipSuccess || cus <= 0)
       cus = 256;
-    ws->nwaves = cus * 4 * adrt::dev::w16::REV_WAVES_PER_SIMD;
+    ws->cus = cus;
+    ws->nwaves = cus * 4 * std::max(adrt::dev::w16::REV_WAVES_PER_SIMD,
+                                    adrt::dev::w16::MREV_WAVES_PER_SIMD);
     ws->nn_cap = 64;
     slab_bytes = (size_t)ws->nwaves * adrt::dev::w16::slab_stride(ws->nn_cap) * sizeof(double);
   }
@@ -1163,15 +1167,14 @@ int adrt_adjoint_batch(const adrt_batch_config* cfg, const adrt_device_inputs* i
     }
     prof_end(ws, e, stream);
     const long tasks = (long)a.W * a.L;
-    const long blocks = std::min((tasks + 3) / 4, (long)ws->nwaves / 4);
+    const long blocks = std::min((tasks + 3) / 4, (long)ws->cus * w16::REV_WAVES_PER_SIMD);
     b.nwaves = (int)(blocks * 4);
     e = prof_begin(ws, 4, stream);
     const unsigned task_blocks = (unsigned)((tasks + 3) / 4);
     if (rev_mixed) {
       // its own persistent grid: MREV_WAVES_PER_SIMD waves per SIMD
       w16::AdjArgs bm = b;
-      const long mblocks = std::min((tasks + 3) / 4, (long)ws->nwaves / w16::REV_WAVES_PER_SIMD *
-                                                         w16::MREV_WAVES_PER_SIMD / 4);
+      const long mblocks = std::min((tasks + 3) / 4, (long)ws->cus * w16::MREV_WAVES_PER_SIMD);
       bm.nwaves = (int)(mblocks * 4);
       w16::layer16m_adjoint_kernel<<<(unsigned)mblocks, 256, 0, stream>>>(a, bm);
       w16::layer16_adjoint_kernel<true><<<(unsigned)blocks, 256, 0, stream>>>(a, b);
