@@ -493,8 +493,9 @@ int adding_chunk(const adrt::dev::FwdArgs& a, adrt_workspace* ws, hipStream_t st
       w16::adding16_composite_kernel<false, false><<<g, 256, 0, stream>>>(a);
     }
   } else if constexpr (N == 32) {
-    w32::adding32_kernel<<<(unsigned)((a.W + w32::WAVES - 1) / w32::WAVES), 64 * w32::WAVES, 0,
-                           stream>>>(a);
+    const unsigned g = (unsigned)((a.W + w32::WAVES - 1) / w32::WAVES);
+    w32::adding32_kernel<<<g, 64 * w32::WAVES, 0, stream>>>(a);
+    w32::adding32_composite_kernel<<<g, 64 * w32::WAVES, 0, stream>>>(a);
   } else {
     adding_kernel<N><<<(unsigned)((a.W + GPB - 1) / GPB), 256, 0, stream>>>(a);
   }

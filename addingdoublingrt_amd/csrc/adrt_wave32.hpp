@@ -976,13 +976,16 @@ __device__ __forceinline__ void add32_below(Comp& C, const double* rec, int lsca
   C.scat = 1;
 }
 
+// The reference's form, for the columns the prep kernel flags (consecutive
+// scattering layers with unstable doubling starts; the N = 16 adding's
+// composite kernel gives the reason); adding32_kernel below does the others.
 // 2 waves per SIMD measured 16 % faster than one wave with all 512 registers
 __global__ void __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(2, 8)))
-adding32_kernel(FwdArgs a) {
+adding32_composite_kernel(FwdArgs a) {
   __shared__ double smem[WAVES * SLOT];
   const int wv = wave_index();
   const int w = blockIdx.x * WAVES + wv;
-  if (w >= a.W) return;
+  if (w >= a.W || !a.ucol || !a.ucol[w]) return;
   const Lane L;
   const int j = L.j;
   const long gw = (long)(a.w0 + w);
@@ -1166,6 +1169,276 @@ adding32_kernel(FwdArgs a) {
     }
     emit(l, Iup, Idn);
     if (l < Lay) add32_below(C, lay + (long)l * REC, lflag[l], L, s);
+  }
+}
+
+
+// ---------------------------------------------------------------------------
+// adding by the base sweep: the same system as the N = 16 adding16_kernel
+// solves (see adrt_fwd16.hpp) -- one sweep of the base composites Rb(l), sb(l)
+// from the bottom, one inverse and four products per scattering layer, then a
+// downward sweep of matrix-vector products for the interface intensities
+// ---------------------------------------------------------------------------
+// per layer: G_l (column-major; the diagonal of T_l for a non-scattering
+// layer), Rb(l+1) (column-major), d_l, sb(l+1), whether Rb(l+1) is non-zero
+constexpr int RB_G = 0, RB_R = N * N, RB_D = 2 * N * N, RB_S = RB_D + N, RB_F = RB_S + N;
+constexpr int RBR32 = 2 * N * N + 4 * N;
+static_assert(RB_F < RBR32, "adding record fields past the record");
+
+struct Base {
+  Mat R;
+  Vec s;
+  int scat;
+};
+
+// G = (I - P)^-1 T for P in the tile t1 (t2: scratch), by LU: G^T (I - P)^T =
+// T^T is X A = T' with A^T = I - P and T'^T = T, both column-major as they are
+__device__ __forceinline__ void left_solve_mat_tile(double* t1, double* t2, const Mat& T, Mat& G,
+                                                    const Lane& L) {
+  Mat A;
+  get(t1, L, A);
+#pragma unroll
+  for (int b = 0; b < 4; ++b)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) A.v[b][r] = delta(L, b, r) - A.v[b][r];
+  wave_sync();
+  put(t1, L, A);
+  put(t2, L, T);
+  wave_sync();
+  gauss_jordan(t1, t2, L);
+  wave_sync();
+  get(t2, L, G);
+  wave_sync();
+}
+
+__device__ __forceinline__ void store_vec(double* g, const Lane& L, const Vec& x) {
+  if (L.h == 0) {
+    g[L.j] = x.lo;
+    g[16 + L.j] = x.hi;
+  }
+}
+
+// layer record rec on top of the base B: stores at o what the downward sweep
+// needs at this layer, then replaces B by the base above the layer
+__device__ __forceinline__ void base32_add(Base& B, const double* rec, int lscat, double* o,
+                                           const Lane& L, const Ws& s) {
+  const Vec su = gvec(rec + 2 * N * N, L), sd = gvec(rec + 2 * N * N + N, L);
+  if (B.scat) gstore(o + RB_R, L, B.R);
+  store_vec(o + RB_S, L, B.s);
+  if (L.l == 0) o[RB_F] = B.scat ? 1.0 : 0.0;
+  if (!lscat) {
+    // T = diag(t), R = 0: G = T, d = s_dn, Rb = T Rb T, sb = s_up + T (Rb s_dn + sb)
+    const double* Tt = rec + N * N;
+    const Vec t{Tt[L.j * N + L.j], Tt[(16 + L.j) * N + 16 + L.j]};
+    Vec rsd{0.0, 0.0};
+    if (B.scat) {
+      put_vec(s.v0, L, t);
+      put_vec(s.v1, L, sd);
+      put(s.tile, L, B.R);
+      wave_sync();
+      rsd = matvec(s.tile, s.v1, L);
+      double tv[2][4];
+      rows_of(s.v0, L, tv);
+      wave_sync();
+#pragma unroll
+      for (int b = 0; b < 4; ++b)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) B.R.v[b][r] = tv[b & 1][r] * B.R.v[b][r] * col_of(t, b);
+    }
+    store_vec(o + RB_G, L, t);
+    store_vec(o + RB_D, L, sd);
+    B.s = vadd(su, vmul(t, vadd(rsd, B.s)));
+    return;
+  }
+  if (!B.scat) {
+    // nothing below reflects: G = T, d = R sb + s_dn, Rb = R, sb = s_up + T sb
+    Mat T;
+    gload(rec, L, B.R);
+    gload(rec + N * N, L, T);
+    put(s.tile, L, B.R);
+    put(s.tile2, L, T);
+    put_vec(s.v0, L, B.s);
+    wave_sync();
+    const Vec d = vadd(matvec(s.tile, s.v0, L), sd);
+    const Vec ts = matvec(s.tile2, s.v0, L);
+    wave_sync();
+    gstore(o + RB_G, L, T);
+    store_vec(o + RB_D, L, d);
+    B.s = vadd(su, ts);
+    B.scat = 1;
+    return;
+  }
+  Mat A, P, G;
+  gload(rec, L, A);  // R_l
+  put(s.tile, L, A);
+  put_vec(s.v0, L, B.s);
+  wave_sync();
+  mul(s.tile, B.R, L, P);                           // P = R_l Rb(l+1)
+  const Vec v = vadd(matvec(s.tile, s.v0, L), sd);  // R_l sb(l+1) + s_dn
+  wave_sync();
+  put(s.tile, L, P);
+  put_vec(s.v1, L, v);
+  wave_sync();
+  Vec d;
+  if (inverse_ns(P, s.tile, s.tile2, G, L)) {
+    // I - P close to singular or the iteration slow: d and G by LU solves
+    d = left_solve_tile(s.tile, s.tile2, s.v1, L);
+    put(s.tile, L, P);
+    gload(rec + N * N, L, A);
+    wave_sync();
+    left_solve_mat_tile(s.tile, s.tile2, A, G, L);
+  } else {
+    put(s.tile2, L, G);
+    gload(rec + N * N, L, A);
+    wave_sync();
+    mul(s.tile2, A, L, G);          // G = X T_l
+    d = matvec(s.tile2, s.v1, L);   // d = X (R_l sb + s_dn)
+  }
+  wave_sync();
+  gstore(o + RB_G, L, G);
+  store_vec(o + RB_D, L, d);
+  put(s.tile, L, B.R);
+  put_vec(s.v2, L, d);
+  wave_sync();
+  mul(s.tile, G, L, P);                             // Rb(l+1) G
+  const Vec rd = matvec(s.tile, s.v2, L);           // Rb(l+1) d
+  wave_sync();
+  gload(rec + N * N, L, A);
+  put(s.tile, L, A);
+  put_vec(s.v3, L, vadd(rd, B.s));
+  wave_sync();
+  gload(rec, L, A);
+  mul_add(s.tile, P, A, L, B.R);                    // Rb(l) = R_l + T_l Rb(l+1) G
+  const Vec ts = matvec(s.tile, s.v3, L);
+  wave_sync();
+  B.s = vadd(su, ts);
+}
+
+__global__ void __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(2, 8)))
+adding32_kernel(FwdArgs a) {
+  __shared__ double smem[WAVES * SLOT];
+  const int wv = wave_index();
+  const int w = blockIdx.x * WAVES + wv;
+  if (w >= a.W || (a.ucol && a.ucol[w])) return;
+  const Lane L;
+  const int j = L.j;
+  const long gw = (long)(a.w0 + w);
+  const int Lay = a.L;
+  double* sm = smem + wv * SLOT;
+  const Ws s{sm, sm + TILE, sm + 2 * TILE, sm + 2 * TILE + N, sm + 2 * TILE + 2 * N,
+             sm + 2 * TILE + 3 * N};
+  constexpr int REC = 2 * N * N + 2 * N;
+  const double* lay = a.lay + (long)w * Lay * REC;
+  const int* lflag = a.lflag + (long)w * Lay;
+  double* rbw = a.rb + (long)w * Lay * RBR32;
+  const double mu_c[2] = {a.mu[j], a.mu[16 + j]};
+  const double wt_c[2] = {a.wt[j], a.wt[16 + j]};
+  const double* Bw = a.B ? a.B + gw * (Lay + 1) : nullptr;
+  auto Blev = [&](int l) -> double { return Bw ? Bw[lev(a, l)] : 0.0; };
+
+  // boundary conditions (src/solver.cpp:385-419, 516-551, 585-603)
+  const double Bs = a.surf_w ? a.surf_w[gw] : a.surf_em;
+  const double Btop = a.top_w ? a.top_w[gw] : a.top_em;
+  const double A = a.albedo;
+  const int has_surface = !a.diffusion && (A > 0.0 || Bs > 0.0);
+  const double tau_tot = a.tau_cum[(long)w * (Lay + 1) + Lay];
+  const Vec Itd{Btop, Btop};
+  Vec Ibu{0.0, 0.0};
+  if (a.diffusion) {
+    const double Bb = Blev(Lay);
+    const double dt = a.tau_s[(long)w * Lay + Lay - 1];
+    const double dBdt = (dt > 0.0) ? (Bb - Blev(Lay - 1)) / dt : 0.0;
+    Ibu = Vec{Bb + mu_c[0] * dBdt, Bb + mu_c[1] * dBdt};
+  } else if (!has_surface) {
+    Ibu = Vec{Bs, Bs};
+  }
+
+  // ---- the base below the last layer, then the base sweep ----
+  Base B;
+  if (has_surface) {
+    const double rr[2] = {2.0 * A * mu_c[0] * wt_c[0] * a.xfac, 2.0 * A * mu_c[1] * wt_c[1] * a.xfac};
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) B.R.v[b][r] = (bi(b) + rw(L.h, r) < a.n_real) ? rr[b >> 1] : 0.0;
+    double ssol = 0.0;
+    if (a.has_solar && A > 0.0) ssol = (A / kPi) * a.F0 * a.mu0 * exp(-tau_tot / a.mu0);
+    const double su = (1.0 - A) * Bs + ssol;
+    B.s = Vec{su, su};
+    B.scat = A > 0.0;
+  } else {
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) B.R.v[b][r] = 0.0;
+    B.s = Ibu;
+    B.scat = 0;
+  }
+  for (int l = Lay - 1; l >= 0; --l)
+    base32_add(B, lay + (long)l * REC, lflag[l], rbw + (long)l * RBR32, L, s);
+
+  auto emit = [&](int l, const Vec& Iup, const Vec& Idn) {
+    const double c0 = 2.0 * kPi * wt_c[0] * mu_c[0], c1 = 2.0 * kPi * wt_c[1] * mu_c[1];
+    const double fu = row_sum<double>(c0 * Iup.lo + c1 * Iup.hi);
+    const double fdn = row_sum<double>(c0 * Idn.lo + c1 * Idn.hi);
+    double J = row_sum<double>(0.5 * wt_c[0] * (Iup.lo + Idn.lo) +
+                               0.5 * wt_c[1] * (Iup.hi + Idn.hi));
+    if (j == 0 && L.h == 0) {
+      const long o = gw * (Lay + 1) + lev(a, l);
+      double fdir = 0.0;
+      if (a.has_solar) {
+        fdir = (l == 0) ? a.F0 * a.mu0
+                        : a.F0 * a.mu0 * exp(-a.tau_cum[(long)w * (Lay + 1) + l] / a.mu0);
+        J += fdir * (1.0 / (4.0 * kPi * a.mu0));
+      }
+      a.fu[o] = fu * a.scaling;
+      a.fd[o] = fdn;
+      a.mi[o] = J;
+      a.fdir[o] = fdir;
+      if (a.fdiv) {
+        const int lyr = (l == 0) ? 0 : l - 1;
+        const double om = a.ssa[gw * Lay + lin(a, lyr)];
+        a.fdiv[o] = 4.0 * kPi * (1.0 - om) * (J - Blev(l));
+      }
+    }
+  };
+
+  // ---- downward sweep (the interface intensities of src/solver.cpp:628-678) ----
+  Vec Idn = Itd, Iup = B.s;
+  if (B.scat) {
+    put(s.tile, L, B.R);
+    put_vec(s.v0, L, Itd);
+    wave_sync();
+    Iup = vadd(Iup, matvec(s.tile, s.v0, L));
+    wave_sync();
+  }
+  emit(0, Iup, Idn);
+  for (int l = 0; l < Lay; ++l) {
+    const double* o = rbw + (long)l * RBR32;
+    const Vec d = gvec(o + RB_D, L), sbn = gvec(o + RB_S, L);
+    const bool bscat = o[RB_F] != 0.0;
+    if (lflag[l]) {
+      Mat G;
+      gload(o + RB_G, L, G);
+      put(s.tile, L, G);
+      put_vec(s.v0, L, Idn);
+      wave_sync();
+      Idn = vadd(matvec(s.tile, s.v0, L), d);
+      wave_sync();
+    } else {
+      Idn = vadd(vmul(gvec(o + RB_G, L), Idn), d);
+    }
+    Iup = sbn;
+    if (bscat) {
+      Mat Rb;
+      gload(o + RB_R, L, Rb);
+      put(s.tile, L, Rb);
+      put_vec(s.v0, L, Idn);
+      wave_sync();
+      Iup = vadd(matvec(s.tile, s.v0, L), sbn);
+      wave_sync();
+    }
+    emit(l + 1, Iup, Idn);
   }
 }
 
