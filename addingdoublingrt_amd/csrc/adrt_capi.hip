@@ -497,7 +497,9 @@ int adding_chunk(const adrt::dev::FwdArgs& a, adrt_workspace* ws, hipStream_t st
     w32::adding32_kernel<<<g, 64 * w32::WAVES, 0, stream>>>(a);
     w32::adding32_composite_kernel<<<g, 64 * w32::WAVES, 0, stream>>>(a);
   } else {
-    adding_kernel<N><<<(unsigned)((a.W + GPB - 1) / GPB), 256, 0, stream>>>(a);
+    const unsigned g = (unsigned)((a.W + GPB - 1) / GPB);
+    adding_kernel<N><<<g, 256, 0, stream>>>(a);
+    adding_composite_kernel<N><<<g, 256, 0, stream>>>(a);
   }
   prof_end(ws, e, stream);
   return hipGetLastError() == hipSuccess ? ADRT_OK : ADRT_ERR_HIP;

@@ -744,8 +744,12 @@ __device__ __forceinline__ double left_solve(const double (&Mc)[Geo<N>::RP], dou
   return right_solve_vec<N, double>(a, b, j, h, sm + Lds<N>::prow, sm + Lds<N>::pval, tile);
 }
 
+// The reference's form, for the columns the prep kernel flags (consecutive
+// scattering layers with unstable doubling starts; the N = 16 adding's
+// composite kernel gives the reason); adding_kernel below does the others.
 template <int N>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 8))) adding_kernel(FwdArgs a) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 8)))
+adding_composite_kernel(FwdArgs a) {
   using R = double;
   constexpr int RP = Geo<N>::RP, G = Geo<N>::G, GPB = Geo<N>::GPB, H = Geo<N>::H;
   constexpr int S = Lds<N>::stride;
@@ -754,7 +758,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 8))
   const int j = (threadIdx.x % G) / H;
   const int h = threadIdx.x % H;
   const int w = blockIdx.x * GPB + g;
-  if (w >= a.W) return;
+  if (w >= a.W || !a.ucol || !a.ucol[w]) return;
   const long gw = (long)(a.w0 + w);
   const int L = a.L;
   R* sm = smem + g * S;
@@ -942,6 +946,256 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 8))
     }
     emit(l, Iup, Idn);
     if (l < L) add_layer_below<N>(C, lay + (long)l * REC, lflag[l], j, h, sm);
+  }
+}
+
+
+// ---------------------------------------------------------------------------
+// adding by the base sweep on column slices: the same system as the N = 16
+// adding16_kernel solves (see adrt_fwd16.hpp) -- one sweep of the base
+// composites Rb(l), sb(l) from the bottom, then a downward sweep of
+// matrix-vector products for the interface intensities.  Per scattering
+// layer two LU solves with I - R_l Rb(l+1) (for G_l and d_l) and three
+// products, against six solves and about twenty products for the composites
+// of both sweeps and the interface solves.
+// ---------------------------------------------------------------------------
+// per layer: G_l (column-major; the diagonal of T_l for a non-scattering
+// layer), Rb(l+1) (column-major), d_l, sb(l+1), whether Rb(l+1) is non-zero
+template <int N>
+struct RbRec {
+  static constexpr int G = 0, R = N * N, D = 2 * N * N, S = D + N, F = S + N;
+  static constexpr int size = 2 * N * N + 4 * N;
+  static_assert(F < size, "adding record fields past the record");
+};
+
+template <int N>
+struct Base {
+  double R[Geo<N>::RP];  // Rb, this lane's slice of column j (all zero unless scat)
+  double s;              // sb, element j
+  int scat;
+};
+
+// layer rec on top of the base B: stores at o what the downward sweep needs at
+// this layer, then replaces B by the base above the layer
+template <int N>
+__device__ void base_add(Base<N>& B, const double* rec, int lscat, double* o, int j, int h,
+                         double* sm) {
+  using R = double;
+  using F = RbRec<N>;
+  constexpr int RP = Geo<N>::RP;
+  R* tile = sm + Lds<N>::tile;
+  R* v0 = sm + Lds<N>::v0;
+  R* v1 = sm + Lds<N>::v1;
+  R* prow = sm + Lds<N>::prow;
+  R* pval = sm + Lds<N>::pval;
+  const R* Rl = rec;
+  const R* Tl = rec + N * N;
+  const R su = rec[2 * N * N + j];
+  const R sd = rec[2 * N * N + N + j];
+  if (B.scat) store_col<N, R>(o + F::R, j, h, B.R);
+  if (h == 0) {
+    o[F::S + j] = B.s;
+    if (j == 0) o[F::F] = B.scat ? 1.0 : 0.0;
+  }
+  if (!lscat) {
+    // T_l = diag(t), R_l = 0: G = T_l, d = s_dn, Rb = T_l Rb T_l,
+    // sb = s_up + T_l (Rb s_dn + sb)
+    const R tj = Tl[j * N + j];
+    R rsd = 0.0;
+    if (B.scat) {
+      put_col<N, R>(tile, j, h, B.R);
+      if (h == 0) {
+        v0[j] = tj;
+        v1[j] = sd;
+      }
+      wave_sync();
+      rsd = matvec<N, R>(tile, v1, j);
+      R tv[RP];
+#pragma unroll
+      for (int r = 0; r < RP; ++r) tv[r] = v0[h * RP + r];
+      wave_sync();
+#pragma unroll
+      for (int r = 0; r < RP; ++r) B.R[r] = tv[r] * B.R[r] * tj;
+    }
+    if (h == 0) {
+      o[F::G + j] = tj;
+      o[F::D + j] = sd;
+    }
+    B.s = su + tj * (rsd + B.s);
+    return;
+  }
+  if (h == 0) v0[j] = B.s;
+  wave_sync();
+  const R v = gmatvec<N>(Rl, v0, j) + sd;  // R_l sb(l+1) + s_dn
+  R Lc[RP], Mc[RP];
+  if (!B.scat) {
+    // nothing below reflects: G = T_l, d = R_l sb + s_dn, Rb = R_l, sb = s_up + T_l sb
+    const R ts = gmatvec<N>(Tl, v0, j);
+    wave_sync();
+    load_col<N, R>(Tl, j, h, Lc);
+    store_col<N, R>(o + F::G, j, h, Lc);
+    load_col<N, R>(Rl, j, h, B.R);
+    if (h == 0) o[F::D + j] = v;
+    B.s = su + ts;
+    B.scat = 1;
+    return;
+  }
+  wave_sync();
+  load_col<N, R>(Rl, j, h, Lc);
+  put_col<N, R>(tile, j, h, Lc);
+  wave_sync();
+  eye_minus_mul<N, R>(tile, B.R, j, h, Mc);  // M = I - R_l Rb(l+1)
+  wave_sync();
+  const R d = left_solve<N>(Mc, v, j, h, sm);  // d = M^-1 (R_l sb + s_dn)
+  // G = M^-1 T_l as the right solve G^T M^T = T_l^T: this lane holds its slice
+  // of row j of M (column j of M^T) and of row j of T_l, and gets its slice of
+  // row j of G
+  put_col<N, R>(tile, j, h, Mc);
+  wave_sync();
+  R ra[RP], rt[RP], Gc[RP];
+  get_row<N, R>(tile, j, h, ra);
+  wave_sync();
+#pragma unroll
+  for (int r = 0; r < RP; ++r) rt[r] = Tl[(h * RP + r) * N + j];
+  right_solve<N, R>(ra, rt, j, h, prow, pval, tile);
+  put_col<N, R>(tile, j, h, rt);  // G^T, column-major
+  wave_sync();
+  get_row<N, R>(tile, j, h, Gc);  // this lane's slice of column j of G
+  wave_sync();
+  store_col<N, R>(o + F::G, j, h, Gc);
+  if (h == 0) o[F::D + j] = d;
+  // Rb(l) = R_l + T_l (Rb(l+1) G), sb(l) = s_up + T_l (Rb(l+1) d + sb(l+1))
+  put_col<N, R>(tile, j, h, B.R);
+  if (h == 0) v1[j] = d;
+  wave_sync();
+  mul<N, R>(tile, Gc, h, Mc);  // Rb(l+1) G
+  const R rd = matvec<N, R>(tile, v1, j);
+  wave_sync();
+  load_col<N, R>(Tl, j, h, Lc);
+  put_col<N, R>(tile, j, h, Lc);
+  if (h == 0) v0[j] = rd + B.s;
+  wave_sync();
+  mul<N, R>(tile, Mc, h, Gc);  // T_l Rb(l+1) G
+  const R ts = matvec<N, R>(tile, v0, j);
+  wave_sync();
+  load_col<N, R>(Rl, j, h, Lc);
+#pragma unroll
+  for (int r = 0; r < RP; ++r) B.R[r] = Lc[r] + Gc[r];
+  B.s = su + ts;
+}
+
+template <int N>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 8))) adding_kernel(FwdArgs a) {
+  using R = double;
+  using F = RbRec<N>;
+  constexpr int RP = Geo<N>::RP, G = Geo<N>::G, GPB = Geo<N>::GPB, H = Geo<N>::H;
+  constexpr int S = Lds<N>::stride;
+  __shared__ R smem[GPB * S];
+  const int g = threadIdx.x / G;
+  const int j = (threadIdx.x % G) / H;
+  const int h = threadIdx.x % H;
+  const int w = blockIdx.x * GPB + g;
+  if (w >= a.W || (a.ucol && a.ucol[w])) return;
+  const long gw = (long)(a.w0 + w);
+  const int L = a.L;
+  R* sm = smem + g * S;
+  R* tile = sm + Lds<N>::tile;
+  R* v0 = sm + Lds<N>::v0;
+  constexpr int REC = 2 * N * N + 2 * N;
+  const double* lay = a.lay + (long)w * L * REC;
+  const int* lflag = a.lflag + (long)w * L;
+  double* rbw = a.rb + (long)w * L * F::size;
+  const double mu_j = a.mu[j], wt_j = a.wt[j];
+  const double* Bw = a.B ? a.B + gw * (L + 1) : nullptr;
+  auto Blev = [&](int l) -> double { return Bw ? Bw[lev(a, l)] : 0.0; };
+
+  // boundary conditions (src/solver.cpp:385-419, 516-551, 585-603)
+  const double Bs = a.surf_w ? a.surf_w[gw] : a.surf_em;
+  const double Btop = a.top_w ? a.top_w[gw] : a.top_em;
+  const double A = a.albedo;
+  const int has_surface = !a.diffusion && (A > 0.0 || Bs > 0.0);
+  const double tau_tot = a.tau_cum[(long)w * (L + 1) + L];
+  const double Itd = Btop;
+  double Ibu = 0.0;
+  if (a.diffusion) {
+    const double Bb = Blev(L);
+    const double dt = a.tau_s[(long)w * L + L - 1];
+    const double dBdt = (dt > 0.0) ? (Bb - Blev(L - 1)) / dt : 0.0;
+    Ibu = Bb + mu_j * dBdt;
+  } else if (!has_surface) {
+    Ibu = Bs;
+  }
+
+  // ---- the base below the last layer, then the base sweep ----
+  Base<N> B;
+  if (has_surface) {
+    const double rr = 2.0 * A * mu_j * wt_j * a.xfac;
+#pragma unroll
+    for (int r = 0; r < RP; ++r) B.R[r] = (h * RP + r < a.n_real) ? rr : 0.0;
+    double ssol = 0.0;
+    if (a.has_solar && A > 0.0) ssol = (A / kPi) * a.F0 * a.mu0 * exp(-tau_tot / a.mu0);
+    B.s = (1.0 - A) * Bs + ssol;
+    B.scat = A > 0.0;
+  } else {
+#pragma unroll
+    for (int r = 0; r < RP; ++r) B.R[r] = 0.0;
+    B.s = Ibu;
+    B.scat = 0;
+  }
+  for (int l = L - 1; l >= 0; --l)
+    base_add<N>(B, lay + (long)l * REC, lflag[l], rbw + (long)l * F::size, j, h, sm);
+
+  auto emit = [&](int l, double Iup, double Idn) {
+    const double c = 2.0 * kPi * wt_j * mu_j;
+    const double fu = group_sum_ordered<N, R>(c * Iup, j, h, sm + Lds<N>::pval);
+    const double fdn = group_sum_ordered<N, R>(c * Idn, j, h, sm + Lds<N>::pval);
+    double J = group_sum_ordered<N, R>(0.5 * wt_j * (Iup + Idn), j, h, sm + Lds<N>::pval);
+    if (j == 0 && h == 0) {
+      const long o = gw * (L + 1) + lev(a, l);
+      double fdir = 0.0;
+      if (a.has_solar) {
+        fdir = (l == 0) ? a.F0 * a.mu0
+                        : a.F0 * a.mu0 * exp(-a.tau_cum[(long)w * (L + 1) + l] / a.mu0);
+        J += fdir * (1.0 / (4.0 * kPi * a.mu0));
+      }
+      a.fu[o] = fu * a.scaling;
+      a.fd[o] = fdn;
+      a.mi[o] = J;
+      a.fdir[o] = fdir;
+      if (a.fdiv) {
+        const int lyr = (l == 0) ? 0 : l - 1;
+        const double om = a.ssa[gw * L + lin(a, lyr)];
+        a.fdiv[o] = 4.0 * kPi * (1.0 - om) * (J - Blev(l));
+      }
+    }
+  };
+
+  // ---- downward sweep (the interface intensities of src/solver.cpp:628-678) ----
+  double Idn = Itd, Iup = B.s;
+  if (B.scat) {
+    put_col<N, R>(tile, j, h, B.R);
+    if (h == 0) v0[j] = Itd;
+    wave_sync();
+    Iup += matvec<N, R>(tile, v0, j);
+    wave_sync();
+  }
+  emit(0, Iup, Idn);
+  for (int l = 0; l < L; ++l) {
+    const double* o = rbw + (long)l * F::size;
+    const double d = o[F::D + j], sbn = o[F::S + j];
+    const bool bscat = o[F::F] != 0.0;
+    if (h == 0) v0[j] = Idn;
+    wave_sync();
+    Idn = (lflag[l] ? gmatvec<N>(o + F::G, v0, j) : o[F::G + j] * Idn) + d;
+    wave_sync();
+    Iup = sbn;
+    if (bscat) {
+      if (h == 0) v0[j] = Idn;
+      wave_sync();
+      Iup = gmatvec<N>(o + F::R, v0, j) + sbn;
+      wave_sync();
+    }
+    emit(l + 1, Iup, Idn);
   }
 }
 
