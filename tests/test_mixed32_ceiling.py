@@ -1,0 +1,149 @@
+"""Why the reduced-precision forms stop at quadrature order 16.
+
+The 8192 x 80, N = 32 workload (addingdoublingrt_amd.workloads config 3:
+1 - omega log-uniform in [1e-6, 1e-1], tau log-uniform in [1e-2, 100], 64
+Henyey-Greenstein moments, solar + thermal sources, diffusion lower boundary)
+asks for FP32 doubling against a 1e-6 flux bar.  On columns of it, the
+doubling (a numpy restatement of src/doubling.hpp:29-230, checked here
+against the oracle's doubling records) runs in float64 three ways:
+
+  * as is;
+  * as the FP32 S-form recursion of the N = 16 mixed kernel (adrt_mix16.hpp:
+    float32 products, Q = P (I - P)^-1 from a float64 solve rounded to float32,
+    i.e. the FP64-refined solve at its best);
+  * from the thin-layer start R0, S0 = I - T0 rounded to float32, every step
+    after that in float64;
+  * as is, with the final records R and S = I - T rounded to float32;
+
+and one float64 direct solve of the interface system (the adding,
+tests/test_oracle_interfaces.py) turns each set of records into fluxes.
+
+The FP32 recursion misses 1e-6 of the column maximum on every sampled column
+(by about an order of magnitude).  Rounding the start alone already moves the
+fluxes by 3e-7 - 3e-6: a layer with 1 - omega ~ 1e-6 absorbs a fraction of its
+thin-layer scattering far below float32's resolution of R0 and S0, and the
+doubling carries that error to every later step, so that floor holds for any
+FP32 doubling arithmetic.  Rounding the final records instead moves the fluxes
+by less than 1e-7: thick layers carry their absorption at a scale float32
+resolves, so FP32 records are not what fails.  No FP32 N = 32 kernel is built
+for that reason.
+"""
+import math
+
+import numpy as np
+import oracle
+from addingdoublingrt_amd import workloads
+from test_oracle_interfaces import direct_solve
+
+
+def ipow0(omega):  # src/doubling.hpp:20-26
+    return 4 if omega < 0.01 else (10 if omega < 0.1 else 16)
+
+
+def doubling(n, tau, omega, Ppp, Ppm, mu, wt, Ftop, mu0, pp, pm, start32=False):
+    """R, T, y, z, s_up_solar, s_dn_solar of one scattering layer (src/doubling.hpp:80-219),
+    float64; start32 rounds the thin-layer start R0, S0 to float32 first."""
+    con = 2.0 * omega * math.pi
+    Gpp = (np.eye(n) - con * Ppp * wt[None, :]) / mu[:, None]
+    Gpm = con * Ppm * wt[None, :] / mu[:, None]
+    nn = max(int(math.log(tau) / math.log(2.0)) + ipow0(omega), 1)
+    tau0 = tau / 2.0 ** nn
+    S, R = tau0 * Gpp, tau0 * Gpm
+    if start32:
+        S = S.astype(np.float32).astype(np.float64)
+        R = R.astype(np.float32).astype(np.float64)
+    T = np.eye(n) - S
+    y, z = (1.0 - omega) * tau0 / mu, np.zeros(n)
+    su = omega * tau0 / mu * Ftop * pm
+    sd = omega * tau0 / mu * Ftop * pp
+    g, gam = 0.5 * tau0, math.exp(-tau0 / mu0)
+    for _ in range(nn):
+        TG = np.linalg.solve((np.eye(n) - R @ R).T, T.T).T
+        TGR = TG @ R
+        u = z + g * y
+        R, T, z, y, su, sd = (R + TGR @ T, TG @ T, (TG @ u - TGR @ u) + z - g * y,
+                              TG @ y + TGR @ y + y, TG @ (R @ sd + gam * su) + su,
+                              TG @ (gam * (R @ su) + sd) + gam * sd)
+        gam, g = gam * gam, 2.0 * g
+    return R, T, y, z, su, sd
+
+
+def doubling_fp32(n, tau, omega, Ppp, Ppm, mu, wt, Ftop, mu0, pp, pm):
+    """The FP32 S-form recursion (T = I - S): per step P = R R, Q = P (I - P)^-1,
+    G = S - Q + S Q, TGR = R - G R, R' = R + TGR - TGR S, S' = G + S - G S, with
+    float32 matrices and float64 source vectors (src/doubling.hpp:155-219
+    rewritten in S)."""
+    f32 = np.float32
+    con = 2.0 * omega * math.pi
+    Gpp = (np.eye(n) - con * Ppp * wt[None, :]) / mu[:, None]
+    Gpm = con * Ppm * wt[None, :] / mu[:, None]
+    nn = max(int(math.log(tau) / math.log(2.0)) + ipow0(omega), 1)
+    tau0 = tau / 2.0 ** nn
+    S, R = (tau0 * Gpp).astype(f32), (tau0 * Gpm).astype(f32)
+    y, z = (1.0 - omega) * tau0 / mu, np.zeros(n)
+    su = omega * tau0 / mu * Ftop * pm
+    sd = omega * tau0 / mu * Ftop * pp
+    g, gam = 0.5 * tau0, math.exp(-tau0 / mu0)
+    I = np.eye(n)
+    for _ in range(nn):
+        P = (R @ R).astype(np.float64)
+        Q = np.linalg.solve((I - P).T, P.T).T.astype(f32)
+        G = (S - Q) + S @ Q
+        TGR = R - G @ R
+        Gd, Td, Rd = G.astype(np.float64), TGR.astype(np.float64), R.astype(np.float64)
+        u = z + g * y
+        v2, v3 = Rd @ sd + gam * su, gam * (Rd @ su) + sd
+        R, S, z, y, su, sd = ((R + TGR) - TGR @ S, (G + S) - G @ S,
+                              (u - Gd @ u) - Td @ u + z - g * y, (y - Gd @ y) + Td @ y + y,
+                              (v2 - Gd @ v2) + su, (v3 - Gd @ v3) + gam * sd)
+        gam, g = gam * gam, 2.0 * g
+    return R.astype(np.float64), I - S.astype(np.float64), y, z, su, sd
+
+
+def column_fluxes(cols, w, B, mu, wt, mode, check_oracle=False):
+    n, L = cols.N, cols.L
+    mu0, F0 = cols.solar_mu, cols.solar_flux
+    recs, tc = [], 0.0
+    for l in range(L):
+        tau, om = cols.delta_tau[w, l], cols.single_scat_albedo[w, l]
+        chi = cols.phase_moments[w, l]
+        Ppp, Ppm = oracle.phase_matrices(chi, mu, wt)
+        pp, pm = oracle.solar_phase_vectors(chi, mu, wt, mu0)
+        Ftop = F0 * math.exp(-tc / mu0)
+        if mode == "fp32":
+            R, T, y, z, su, sd = doubling_fp32(n, tau, om, Ppp, Ppm, mu, wt, Ftop, mu0, pp, pm)
+        else:
+            R, T, y, z, su, sd = doubling(n, tau, om, Ppp, Ppm, mu, wt, Ftop, mu0, pp, pm,
+                                          start32=(mode == "start32"))
+        if check_oracle and l < 8:
+            ref = oracle.doubling(n, tau, om, B[l], B[l + 1], Ppp, Ppm, mu, wt, solar_flux=F0,
+                                  solar_mu=mu0, tau_cumulative=tc, p_plus=pp, p_minus=pm)
+            for a, b in ((R, ref["R"]), (T, ref["T"]), (su, ref["s_up_solar"]),
+                         (sd, ref["s_down_solar"])):
+                # same algorithm, other operation order: thick near-conservative
+                # layers amplify FP64 rounding to ~1e-10 of the record maximum
+                assert np.max(np.abs(a - b)) <= 1e-8 * max(np.max(np.abs(b)), 1e-300)
+        if mode == "rec32":
+            R = R.astype(np.float32).astype(np.float64)
+            T = np.eye(n) - (np.eye(n) - T).astype(np.float32).astype(np.float64)
+        Bbar, Bd = 0.5 * (B[l] + B[l + 1]), (B[l + 1] - B[l]) / tau
+        recs.append((R, T, y * Bbar + z * Bd + su, y * Bbar - z * Bd + sd))
+        tc += tau
+    dB = (B[L] - B[L - 1]) / cols.delta_tau[w, L - 1]  # diffusion lower boundary
+    return direct_solve(n, recs, mu, wt, np.full(n, B[0]), ("given", B[L] + mu * dB))
+
+
+def test_fp32_doubling_misses_the_bar_fp32_records_do_not():
+    cols = workloads.make(3, W=2)
+    B, _, _ = workloads.planck_inputs_host(cols, oracle.planck)
+    mu, wt = oracle.gauss_legendre(cols.N)
+    err = {"fp32": [], "start32": [], "rec32": []}
+    for w in range(cols.W):
+        ref = column_fluxes(cols, w, B[w], mu, wt, "f64", check_oracle=(w == 0))
+        for mode, acc in err.items():
+            out = column_fluxes(cols, w, B[w], mu, wt, mode)
+            acc.append(max(np.max(np.abs(o - r)) / np.max(np.abs(r)) for o, r in zip(out, ref)))
+    print({k: ["%.2e" % e for e in v] for k, v in err.items()})
+    assert min(err["fp32"]) > 3e-6          # FP32 doubling: beyond the 1e-6 bar everywhere
+    assert min(err["start32"]) > 2e-7       # the FP32 start alone: a floor near the bar
+    assert max(err["rec32"]) < 1e-7         # FP32 final records: well inside it
