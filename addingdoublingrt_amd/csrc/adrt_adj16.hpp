@@ -750,8 +750,9 @@ __global__ void __launch_bounds__(256) start16_adjoint_kernel(FwdArgs a, AdjArgs
   }
 }
 
-// REDO: only the layers the mixed-precision forward left to the FP64 kernel
-template <bool REDO>
+// REDO: only the layers the mixed-precision forward left to the FP64 kernel;
+// UNPAIRED: only the layers pair mode leaves (a.pflag, adrt_pair16.hpp)
+template <bool REDO, bool UNPAIRED = false>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(REV_WAVES_PER_SIMD, 8)))
 layer16_adjoint_kernel(FwdArgs a, AdjArgs b) {
   __shared__ double smem[4 * LdsA::size];
@@ -764,6 +765,9 @@ layer16_adjoint_kernel(FwdArgs a, AdjArgs b) {
   for (long task = gwave; task < ntask; task += b.nwaves) {
     if constexpr (REDO) {
       if (!a.redo[task]) continue;
+    }
+    if constexpr (UNPAIRED) {
+      if (a.pflag[task]) continue;
     }
     layer16_adjoint_task(a, b, task, L, sm, slab);
   }

@@ -22,6 +22,7 @@
 #include "adrt_adj16.hpp"
 #include "adrt_mix16.hpp"
 #include "adrt_adjm16.hpp"
+#include "adrt_pair16.hpp"
 #include "adrt_wave32.hpp"
 
 namespace {
@@ -348,6 +349,7 @@ struct adrt_workspace {
   int *mode = nullptr, *lflag = nullptr, *flag = nullptr, *redo = nullptr, *ucol = nullptr;
   // reverse pass (ADRT_WORKSPACE_ADJOINT)
   int flags = 0;
+  int *pord = nullptr, *pflag = nullptr;  // pair mode (orders <= 8 padded to 16)
   double *Iv = nullptr, *Av = nullptr, *g_taus = nullptr, *g_omgs = nullptr, *g_Bt = nullptr,
          *g_Bb = nullptr, *g_Ftop = nullptr, *g_f = nullptr, *g_chi = nullptr, *scot = nullptr, *slab = nullptr,
          *mom_part = nullptr, *phase = nullptr, *raw = nullptr;
@@ -387,6 +389,7 @@ size_t per_wavenumber_bytes(int L, int Np, int M, bool adjoint) {
   if (adjoint) {
     b += 2 * (size_t)(L + 1) * 2 * Np * d;            // forward and adjoint interface vectors
     b += 6 * (size_t)L * d;                           // per-layer gradient terms
+    b += 2 * (size_t)L * sizeof(int);                 // pair order and flags (orders <= 8)
     b += (size_t)L * M * d;                           // per-layer moment gradients
     if (Np == adrt::dev::w16::N)
       b += (size_t)L * adrt::dev::w16::SCOT * d;      // doubling-start cotangents
@@ -460,6 +463,12 @@ int layers_chunk(const adrt::dev::FwdArgs& a, adrt_workspace* ws, hipStream_t st
       w16::layer16m_kernel<<<(unsigned)((tasks + 3) / 4), 256, 0, stream>>>(a);
       // layers whose doubling start is outside FP32's reach, again in FP64
       w16::layer16_kernel<true><<<(unsigned)((tasks + 3) / 4), 256, 0, stream>>>(a);
+    } else if (a.pord) {
+      // orders up to 8 padded to 16 (reverse-pass workspaces): two layers per wave
+      const long pairs = (long)a.W * (a.L / 2);
+      w16::pair_order_kernel<<<(unsigned)((a.W + 3) / 4), 256, 0, stream>>>(a);
+      if (pairs > 0) w16::layer16p_kernel<<<(unsigned)((pairs + 3) / 4), 256, 0, stream>>>(a);
+      w16::layer16_kernel<false, true><<<(unsigned)((tasks + 3) / 4), 256, 0, stream>>>(a);
     } else {
       w16::layer16_kernel<false><<<(unsigned)((tasks + 3) / 4), 256, 0, stream>>>(a);
     }
@@ -665,11 +674,13 @@ int adrt_workspace_create_ex(const adrt_batch_config* cfg, size_t max_bytes, int
   const size_t o_phase = take(phase_bytes);
   const size_t o_raw = take(C * (3 * L + 3) * 8);
   size_t o_iv = 0, o_av = 0, o_g = 0, o_chi = 0, o_slab = 0, o_status = 0, o_part = 0,
-         o_scot = 0;
+         o_scot = 0, o_pord = 0, o_pflag = 0;
   if (adjoint) {
     o_iv = take(C * (L + 1) * 2 * Np * 8);
     o_av = take(C * (L + 1) * 2 * Np * 8);
     o_g = take(6 * C * L * 8);
+    o_pord = take(C * L * 4);
+    o_pflag = take(C * L * 4);
     o_part = take(L * adrt::dev::w16::MOM_SLICES * M * 8);
     o_chi = take(C * L * M * 8);
     o_scot = take(Np == adrt::dev::w16::N ? C * L * adrt::dev::w16::SCOT * 8 : 0);
@@ -732,6 +743,10 @@ int adrt_workspace_create_ex(const adrt_batch_config* cfg, size_t max_bytes, int
   if (adjoint) {
     ws->Iv = (double*)(ws->base + o_iv);
     ws->Av = (double*)(ws->base + o_av);
+    if (ws->n_real <= 8 && ws->L <= adrt::dev::w16::PAIR_MAX_L) {
+      ws->pord = (int*)(ws->base + o_pord);
+      ws->pflag = (int*)(ws->base + o_pflag);
+    }
     double* g = (double*)(ws->base + o_g);
     ws->g_taus = g;
     ws->g_omgs = g + C * L;
@@ -817,6 +832,8 @@ adrt::dev::FwdArgs make_args(const adrt_batch_config* cfg, const adrt_device_inp
   a.redo = ws->redo;
   a.ucol = ws->ucol;
   a.rb = ws->rb;
+  a.pord = ws->pord;  // two layers per wave (orders <= 8 on reverse-pass workspaces)
+  a.pflag = ws->pflag;
   a.Iv = ws->Iv;  // kept for the reverse pass (adjoint workspaces only)
   a.Av = ws->Av;
   // one phase record per layer when every wavenumber shares the moments
@@ -1185,6 +1202,17 @@ int adrt_adjoint_batch(const adrt_batch_config* cfg, const adrt_device_inputs* i
       e = prof_begin(ws, 6, stream);
       w16::start16_adjoint_kernel<true><<<task_blocks, 256, 0, stream>>>(a, b);
       w16::start16_adjoint_kernel<false, float><<<task_blocks, 256, 0, stream>>>(a, b);
+    } else if (a.pord) {
+      // two layers per wave (orders up to 8 padded to 16)
+      w16::AdjArgs bp = b;
+      const long pairs = (long)a.W * (a.L / 2);
+      const long pblocks = std::min((pairs + 3) / 4, (long)ws->cus * w16::REV_WAVES_PER_SIMD);
+      bp.nwaves = (int)(pblocks * 4);
+      if (pblocks > 0) w16::layer16p_adjoint_kernel<<<(unsigned)pblocks, 256, 0, stream>>>(a, bp);
+      w16::layer16_adjoint_kernel<false, true><<<(unsigned)blocks, 256, 0, stream>>>(a, b);
+      prof_end(ws, e, stream);
+      e = prof_begin(ws, 6, stream);
+      w16::start16_adjoint_kernel<false><<<task_blocks, 256, 0, stream>>>(a, b);
     } else {
       w16::layer16_adjoint_kernel<false><<<(unsigned)blocks, 256, 0, stream>>>(a, b);
       prof_end(ws, e, stream);

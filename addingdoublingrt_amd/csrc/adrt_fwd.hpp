@@ -61,6 +61,10 @@ struct FwdArgs {
   double* lay;      // [W][L][2N^2 + 2N]   R, T (column-major), s_up, s_down
   int* lflag;       // [W][L]   is_scattering
   int* redo;        // [W][L]   mixed form: layer recomputed by the FP64 kernel
+  int* pord;        // [W][L]   pair mode (orders <= 8 at 16): each column's layers by
+                    //          doubling count, non-scattering first, pairs (2q, 2q + 1)
+                    //          (pair_order_kernel); null = off
+  int* pflag;       // [W][L]   pair mode: the layer runs in a pair (both layers of it scatter)
   int* ucol;        // [W]      N = 16 adding in the reference's composite form (consecutive
                     //          scattering layers with unstable doubling starts)
   double* rb;       // [W][L][2N^2 + 4N]   adding records (base composites; N = 16: the
@@ -82,6 +86,11 @@ __device__ __forceinline__ int lev(const FwdArgs& a, int l) { return a.rev ? a.L
 
 __device__ __forceinline__ int ipow0(double omega) {  // src/doubling.hpp:20-26
   return omega < 0.01 ? 4 : (omega < 0.1 ? 10 : 16);
+}
+__device__ __forceinline__ int doubling_count(double tau, double omega_in) {  // src/doubling.hpp:108-112
+  const double omega = omega_in > 1.0 ? 1.0 : omega_in;
+  const int nn0 = (int)(log(tau) / log(2.0)) + ipow0(omega);
+  return nn0 < 1 ? 1 : nn0;
 }
 
 // A scattering layer whose doubling start tau0 = tau / 2^nn lies beyond the

@@ -246,20 +246,12 @@ __device__ __forceinline__ void layer16_start(const FwdArgs& a, long task, int w
 // S = I - T in FP32 register order of the FP32 matrix instruction, sources FP64.
 // ---------------------------------------------------------------------------
 template <bool REDO>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) layer16_kernel(FwdArgs a) {
+__device__ __forceinline__ void layer16_task(const FwdArgs& a, long task, const Lane& L,
+                                             double* sm) {
   using R = double;
-  __shared__ R smem[4 * Lds16::size];
-  const int wv = wave_index();
-  const long task = (long)blockIdx.x * 4 + wv;
   const int L_ = a.L;
-  if (task >= (long)a.W * L_) return;
-  if constexpr (REDO) {
-    if (!a.redo[task]) return;
-  }
-  const Lane L;
   const int w = (int)(task / L_), l = (int)(task % L_);
   const long gw = (long)(a.w0 + w);
-  R* sm = smem + wv * Lds16::size;
   R* tile = sm + Lds16::tile;
   R* tile2 = sm + Lds16::tile2;
   R* pv = sm + Lds16::pv;
@@ -408,6 +400,24 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))
     }
   }
   if (L.l == 0) a.lflag[task] = REDO ? 2 : 1;
+}
+
+// UNPAIRED: only the layers pair mode leaves (orders up to 8 on reverse-pass
+// workspaces: a.pflag, adrt_pair16.hpp)
+template <bool REDO, bool UNPAIRED = false>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) layer16_kernel(FwdArgs a) {
+  __shared__ double smem[4 * Lds16::size];
+  const int wv = wave_index();
+  const long task = (long)blockIdx.x * 4 + wv;
+  if (task >= (long)a.W * a.L) return;
+  if constexpr (REDO) {
+    if (!a.redo[task]) return;
+  }
+  if constexpr (UNPAIRED) {
+    if (a.pflag[task]) return;
+  }
+  const Lane L;
+  layer16_task<REDO>(a, task, L, smem + wv * Lds16::size);
 }
 
 // ---------------------------------------------------------------------------

@@ -146,6 +146,93 @@ __device__ __forceinline__ void mul_add(const R* X, const Mat<R>& Y, const Mat<R
   fence_sched();
 }
 
+// ---- pair mode: two problems of order <= 8 in one 16 x 16 problem ----------
+// Problem A in rows and columns 0..7, problem B in 8..15, zeros elsewhere
+// (block diagonal).  A product of two such matrices needs only the K range of
+// each lane's own block: for the FP64 layout, K chunks r = 0, 1 cover rows
+// 0..7 and r = 2, 3 rows 8..15, so each lane feeds the two chunks of its block
+// (A operand: row j of X at k = row(h, s) + 8 half(j); B operand: register
+// s + 2 half(j)) and two instructions give both diagonal blocks.  The other
+// output registers (rows of the other block) hold cross products and are
+// zeroed by pmask, which keeps the result block diagonal.  Everything else --
+// tiles, transposed reads, matrix-vector products, Newton-Schulz, the LU
+// solve -- works on the block-diagonal 16 x 16 matrix unchanged (the LU of a
+// block-diagonal matrix never mixes the blocks: the multipliers across them
+// are exact zeros).
+__device__ __forceinline__ int pair_half(const Lane& L) { return L.j >> 3; }
+
+__device__ __forceinline__ void pmask(d4& acc, const Lane& L) {
+  const bool hb = pair_half(L) != 0;
+  acc[0] = hb ? 0.0 : acc[0];
+  acc[1] = hb ? 0.0 : acc[1];
+  acc[2] = hb ? acc[2] : 0.0;
+  acc[3] = hb ? acc[3] : 0.0;
+}
+__device__ __forceinline__ void pmask(Mat<double>& m, const Lane& L) {
+  const bool hb = pair_half(L) != 0;
+  m.v[0] = hb ? 0.0 : m.v[0];
+  m.v[1] = hb ? 0.0 : m.v[1];
+  m.v[2] = hb ? m.v[2] : 0.0;
+  m.v[3] = hb ? m.v[3] : 0.0;
+}
+
+// acc += X Y (X in a tile, Y in registers), both block diagonal; cross blocks
+// of acc are left unmasked
+__device__ __forceinline__ void pmac(d4& acc, const double* X, const Mat<double>& Y,
+                                     const Lane& L) {
+  fence_sched();
+  const int hb = pair_half(L);
+#pragma unroll
+  for (int s = 0; s < 2; ++s) {
+    const double a = X[(row<double>(L.h, s) + 8 * hb) * TS + L.j];
+    const double b = hb ? Y.v[s + 2] : Y.v[s];
+    acc = mfma(a, b, acc);
+  }
+  fence_sched();
+}
+// acc += X^T Y (X in a tile, read transposed)
+__device__ __forceinline__ void pmac_t(d4& acc, const double* X, const Mat<double>& Y,
+                                       const Lane& L) {
+  fence_sched();
+  const int hb = pair_half(L);
+#pragma unroll
+  for (int s = 0; s < 2; ++s) {
+    const double a = X[L.j * TS + row<double>(L.h, s) + 8 * hb];
+    const double b = hb ? Y.v[s + 2] : Y.v[s];
+    acc = mfma(a, b, acc);
+  }
+  fence_sched();
+}
+__device__ __forceinline__ void pmul(const double* X, const Mat<double>& Y, const Lane& L,
+                                     Mat<double>& C) {
+  d4 acc = {0.0, 0.0, 0.0, 0.0};
+  pmac(acc, X, Y, L);
+  pmask(acc, L);
+#pragma unroll
+  for (int r = 0; r < 4; ++r) C.v[r] = acc[r];
+}
+__device__ __forceinline__ void pmul_add(const double* X, const Mat<double>& Y,
+                                         const Mat<double>& Z, const Lane& L, Mat<double>& C) {
+  d4 acc = {Z.v[0], Z.v[1], Z.v[2], Z.v[3]};
+  pmac(acc, X, Y, L);
+  pmask(acc, L);
+#pragma unroll
+  for (int r = 0; r < 4; ++r) C.v[r] = acc[r];
+}
+
+// products of one problem per wave (PAIR = false) or of a block-diagonal pair
+template <typename R, bool PAIR>
+__device__ __forceinline__ void mulx(const R* X, const Mat<R>& Y, const Lane& L, Mat<R>& C) {
+  if constexpr (PAIR) pmul(X, Y, L, C);
+  else mul<R>(X, Y, L, C);
+}
+template <typename R, bool PAIR>
+__device__ __forceinline__ void mulx_add(const R* X, const Mat<R>& Y, const Mat<R>& Z,
+                                         const Lane& L, Mat<R>& C) {
+  if constexpr (PAIR) pmul_add(X, Y, Z, L, C);
+  else mul_add<R>(X, Y, Z, L, C);
+}
+
 // C = I - X Y
 template <typename R>
 __device__ __forceinline__ void eye_minus_mul(const R* X, const Mat<R>& Y, const Lane& L,
@@ -354,16 +441,17 @@ constexpr int NS_MAX_STEPS = 8;
 // once; where the start is poor (e > 1/2: (I - P) close to singular) or the
 // iteration has not finished in NS_MAX_STEPS steps, LU with partial
 // pivoting (right_solve) solves X (I - P) = I instead.  Every branch is
-// wave-uniform.  Returns true if the LU solve ran.  With FALLBACK = false the
-// fallback is left to the caller: the function returns true at once, X unset,
+// wave-uniform.  PAIR: P is a block-diagonal pair (pair mode above); the
+// residual test then covers both blocks.  Returns true if the LU solve ran.
+// With FALLBACK = false the fallback is left to the caller: the function returns true at once, X unset,
 // so that a caller with a right-hand side can solve for it directly (a solve
 // keeps the error of an ill-conditioned I - P at that of an LU solve; the
 // product with a computed inverse loses up to cond(I - P) more).
-template <typename R, bool FALLBACK = true>
+template <typename R, bool FALLBACK = true, bool PAIR = false>
 __device__ __forceinline__ bool inverse_ns(const Mat<R>& P, const R* tP, R* tX, Mat<R>& X,
                                            const Lane& L) {
   Mat<R> E;
-  mul<R>(tP, P, L, E);  // P^2
+  mulx<R, PAIR>(tP, P, L, E);  // P^2
 #pragma unroll
   for (int r = 0; r < 4; ++r) X.v[r] = ((row<R>(L.h, r) == L.j) ? R(1) : R(0)) + P.v[r];
   float e = residual_size<R>(E);
@@ -386,26 +474,26 @@ __device__ __forceinline__ bool inverse_ns(const Mat<R>& P, const R* tP, R* tX, 
     put<R>(tX, L, X);
     wave_sync();
     if (e <= NS_NEWTON) {
-      mul_add<R>(tX, E, X, L, X);
+      mulx_add<R, PAIR>(tX, E, X, L, X);
       wave_sync();
       break;
     }
     Mat<R> XE;
-    mul<R>(tX, E, L, XE);
+    mulx<R, PAIR>(tX, E, L, XE);
     wave_sync();
     if (e <= NS_CUBIC) {
       put<R>(tX, L, XE);
       wave_sync();
 #pragma unroll
       for (int r = 0; r < 4; ++r) X.v[r] += XE.v[r];
-      mul_add<R>(tX, E, X, L, X);
+      mulx_add<R, PAIR>(tX, E, X, L, X);
       wave_sync();
       break;
     }
 #pragma unroll
     for (int r = 0; r < 4; ++r) X.v[r] += XE.v[r];
     Mat<R> PX;
-    mul<R>(tP, X, L, PX);
+    mulx<R, PAIR>(tP, X, L, PX);
 #pragma unroll
     for (int r = 0; r < 4; ++r)
       E.v[r] = (((row<R>(L.h, r) == L.j) ? R(1) : R(0)) - X.v[r]) + PX.v[r];

@@ -217,7 +217,8 @@ def test_adjoint_workspace_forward_matches_plain():
                  label=f"reverse-pass workspace N={n}", adjoint_workspace=True)
 
 
-@pytest.mark.parametrize("n,delta_m", [(8, False), (4, True), (16, False)])
+@pytest.mark.parametrize("n,delta_m", [(8, False), (4, True), (16, False), (2, False), (3, False),
+                                        (6, True)])
 def test_adjoint_matches_oracle_temperature_jacobian(n, delta_m):
     """u^T dF/dT from the reverse pass vs the oracle's forward-mode Jacobian."""
     rng = np.random.default_rng(7 + n)
