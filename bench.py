@@ -406,31 +406,37 @@ def main():
     ms, nl = r["kind_ms"], r["launches"]
     per_kind = {k: ms[i] / a.steps for i, k in enumerate(KINDS)}
     # dominant kernel and its algorithmic work
-    if chosen == "f64":
-        model = {1: ("layer16_kernel<false>", "phase setup + doubling (forward)", work["doubling"]),
-                 4: ("layer16_adjoint_kernel<false>",
+    if chosen == "f64" and cols.N <= 8:  # two layers per wave (pair mode)
+        model = {1: ("layer16p_kernel + layer16_kernel<false, true>",
+                     "phase setup + doubling (forward), two layers per wave", work["doubling"]),
+                 4: ("layer16p_adjoint_kernel + layer16_adjoint_kernel<false, true>",
+                     "recomputed doubling + its reverse (reverse counted), two layers per wave",
+                     work["doubling_reverse"])}
+    elif chosen == "f64":
+        model = {1: ("layer16_kernel<false, false>", "phase setup + doubling (forward)", work["doubling"]),
+                 4: ("layer16_adjoint_kernel<false, false>",
                      "recomputed doubling + its reverse (reverse counted)",
                      work["doubling_reverse"])}
     elif chosen == "mixed":
-        model = {1: ("layer16m_kernel + layer16_kernel<true>",
+        model = {1: ("layer16m_kernel + layer16_kernel<true, false>",
                      "mixed doubling (forward; FP64 for the routed layers), FP64 flop count",
                      work["doubling"]),
-                 4: ("layer16m_adjoint_kernel + layer16_adjoint_kernel<true>",
+                 4: ("layer16m_adjoint_kernel + layer16_adjoint_kernel<true, false>",
                      "FP32 recomputed doubling + reverse (FP64 for the routed layers), "
                      "FP64 flop count", work["doubling_reverse"])}
     else:
-        model = {1: ("layer16_kernel<false>", "phase setup + doubling (forward)", work["doubling"]),
-                 4: ("layer16m_adjoint_kernel + layer16_adjoint_kernel<true>",
+        model = {1: ("layer16_kernel<false, false>", "phase setup + doubling (forward)", work["doubling"]),
+                 4: ("layer16m_adjoint_kernel + layer16_adjoint_kernel<true, false>",
                      "FP32 recomputed doubling + reverse (FP64 for the routed layers), "
                      "FP64 flop count", work["doubling_reverse"])}
     if a.forward_only:
         npad = 1 << max(cols.N - 1, 1).bit_length()  # the library's padded order
-        names = ({1: "layer16_kernel<false>", 2: "adding16_kernel<false, false>"}
+        names = ({1: "layer16_kernel<false, false>", 2: "adding16_kernel<false, false>"}
                  if npad == 16 or chosen != "f64" else
                  {1: "layer32_kernel", 2: "adding32_kernel"} if npad == 32 else
                  {1: f"layer_kernel<{npad}>", 2: f"adding_kernel<{npad}>"})
         if chosen == "mixed":
-            names = {1: "layer16m_kernel + layer16_kernel<true>",
+            names = {1: "layer16m_kernel + layer16_kernel<true, false>",
                      2: "adding16_kernel<false, true>"}
         model = {1: (names[1], "phase setup + doubling (forward)", work["doubling"]),
                  2: (names[2], "adding sweeps + interfaces (forward)",
