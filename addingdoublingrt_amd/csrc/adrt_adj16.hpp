@@ -696,11 +696,14 @@ __device__ void layer16_adjoint_task(const FwdArgs& a, const AdjArgs& b, long ta
 // each scattering layer, reversed, from the start cotangents that
 // layer16_adjoint_kernel left in b.scot; one wave per (wavenumber, layer).
 // Kept apart from the doubling reverse so that kernel keeps to 128 registers.
+// 4 waves per SIMD (measured: 24.0 ms on the 65536 x 100 workload against
+// 27.6 ms at 3 and 32.4 ms at 5).
 // REDO: only the layers the mixed-precision forward left to the FP64 kernels;
 // RowT = float: the other layers of the mixed form (start cotangents in the
 // FP32 register order of layer16m_adjoint_kernel).
 template <bool REDO, typename RowT = double>
-__global__ void __launch_bounds__(256) start16_adjoint_kernel(FwdArgs a, AdjArgs b) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8)))
+start16_adjoint_kernel(FwdArgs a, AdjArgs b) {
   __shared__ double smem[4 * LDS_START16];
   const int wv = wave_index();
   const long task = (long)blockIdx.x * 4 + wv;

@@ -28,14 +28,17 @@ namespace dev {
 namespace w16 {
 
 struct LdsAM {
-  // in doubles: one FP64 tile, eight FP32 tiles (half a double tile each),
-  // eight FP32 vector slots, one FP64 column-sum slot
+  // in doubles: eight FP32 tiles (half a double tile each), the first two of
+  // which hold the FP64 tile of the thin-layer start before the doubling loops
+  // use them; eight FP32 vector slots, one FP64 column-sum slot (9.4 KB per
+  // wave, so four 4-wave workgroups fit a CU's 160 KB)
   static constexpr int TF = TILE / 2 + 1;
-  static constexpr int td = 0, tf0 = TILE, vs = TILE + 8 * TF, csl = vs + 4 * N;
+  static constexpr int td = 0, tf0 = 0, vs = 8 * TF, csl = vs + 4 * N;
   static constexpr int size = csl + N;
 };
 static_assert(N * TS * sizeof(float) <= LdsAM::TF * sizeof(double) &&
-                  LdsAM::td + TILE <= LdsAM::tf0 && LdsAM::tf0 + 8 * LdsAM::TF <= LdsAM::vs &&
+                  LdsAM::td + TILE <= LdsAM::tf0 + 2 * LdsAM::TF &&
+                  LdsAM::tf0 + 8 * LdsAM::TF <= LdsAM::vs &&
                   8 * N * sizeof(float) <= 4 * N * sizeof(double) &&
                   LdsAM::vs + 4 * N <= LdsAM::csl && LdsAM::csl + N <= LdsAM::size,
               "mixed reverse per-wave LDS regions overlap");
@@ -435,7 +438,7 @@ __device__ void layer16m_adjoint_task(const FwdArgs& a, const AdjArgs& b, long t
   }
 }
 
-constexpr int MREV_WAVES_PER_SIMD = 3;  // register budget 168 per lane
+constexpr int MREV_WAVES_PER_SIMD = 4;  // register budget 128 per lane
 
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MREV_WAVES_PER_SIMD, 8)))
 layer16m_adjoint_kernel(FwdArgs a, AdjArgs b) {
