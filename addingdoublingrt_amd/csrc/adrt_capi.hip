@@ -372,6 +372,11 @@ struct adrt_workspace {
   // every launch (kinds as adrt_profile_read)
   bool profiling = false;
   std::vector<hipEvent_t> ev_pool;
+  // the composite-form adding of the flagged columns runs on this stream beside
+  // the base sweep on the caller's stream (fork / join by the two events);
+  // null for a workspace that has none (then both run on the caller's stream)
+  hipStream_t side = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr;
   std::vector<int> ev_kind;  // one per recorded pair
   size_t ev_used = 0;        // pairs in use
 };
@@ -484,6 +489,30 @@ int layers_chunk(const adrt::dev::FwdArgs& a, adrt_workspace* ws, hipStream_t st
   return hipGetLastError() == hipSuccess ? ADRT_OK : ADRT_ERR_HIP;
 }
 
+// The adding of a chunk runs as two kernels over disjoint columns: the base
+// sweep, and the reference's composite form for the columns the prep kernel
+// flagged.  Each wave sweeps one column's layers in sequence, so a launch lasts
+// about one column's latency however few columns it has; the composite
+// kernel therefore runs beside the base sweep, on the workspace's side stream,
+// forked from and joined back into the caller's stream by events.
+template <class Base, class Side>
+int fork_join(adrt_workspace* ws, hipStream_t stream, Base base, Side side) {
+  if (!ws->side) {
+    base(stream);
+    side(stream);
+    return ADRT_OK;
+  }
+  if (hipEventRecord(ws->fork, stream) != hipSuccess ||
+      hipStreamWaitEvent(ws->side, ws->fork, 0) != hipSuccess)
+    return ADRT_ERR_HIP;
+  side(ws->side);
+  base(stream);
+  if (hipEventRecord(ws->join, ws->side) != hipSuccess ||
+      hipStreamWaitEvent(stream, ws->join, 0) != hipSuccess)
+    return ADRT_ERR_HIP;
+  return ADRT_OK;
+}
+
 // adding sweeps, interface solves and outputs of one chunk
 template <int N>
 int adding_chunk(const adrt::dev::FwdArgs& a, adrt_workspace* ws, hipStream_t stream,
@@ -491,26 +520,34 @@ int adding_chunk(const adrt::dev::FwdArgs& a, adrt_workspace* ws, hipStream_t st
   using namespace adrt::dev;
   constexpr int GPB = Geo<N>::GPB;
   const int e = prof_begin(ws, 2, stream);
+  int rc;
   if constexpr (N == 16) {
     // the base sweep; columns flagged by the prep kernel take the composite form
     const unsigned g = (unsigned)((a.W + 3) / 4);
-    if (mixed) {
-      w16::adding16_kernel<false, true><<<g, 256, 0, stream>>>(a);
-      w16::adding16_composite_kernel<false, true><<<g, 256, 0, stream>>>(a);
-    } else {
-      w16::adding16_kernel<false, false><<<g, 256, 0, stream>>>(a);
-      w16::adding16_composite_kernel<false, false><<<g, 256, 0, stream>>>(a);
-    }
+    if (mixed)
+      rc = fork_join(
+          ws, stream,
+          [&](hipStream_t s) { w16::adding16_kernel<false, true><<<g, 256, 0, s>>>(a); },
+          [&](hipStream_t s) { w16::adding16_composite_kernel<false, true><<<g, 256, 0, s>>>(a); });
+    else
+      rc = fork_join(
+          ws, stream,
+          [&](hipStream_t s) { w16::adding16_kernel<false, false><<<g, 256, 0, s>>>(a); },
+          [&](hipStream_t s) { w16::adding16_composite_kernel<false, false><<<g, 256, 0, s>>>(a); });
   } else if constexpr (N == 32) {
     const unsigned g = (unsigned)((a.W + w32::WAVES - 1) / w32::WAVES);
-    w32::adding32_kernel<<<g, 64 * w32::WAVES, 0, stream>>>(a);
-    w32::adding32_composite_kernel<<<g, 64 * w32::WAVES, 0, stream>>>(a);
+    rc = fork_join(
+        ws, stream,
+        [&](hipStream_t s) { w32::adding32_kernel<<<g, 64 * w32::WAVES, 0, s>>>(a); },
+        [&](hipStream_t s) { w32::adding32_composite_kernel<<<g, 64 * w32::WAVES, 0, s>>>(a); });
   } else {
     const unsigned g = (unsigned)((a.W + GPB - 1) / GPB);
-    adding_kernel<N><<<g, 256, 0, stream>>>(a);
-    adding_composite_kernel<N><<<g, 256, 0, stream>>>(a);
+    rc = fork_join(
+        ws, stream, [&](hipStream_t s) { adding_kernel<N><<<g, 256, 0, s>>>(a); },
+        [&](hipStream_t s) { adding_composite_kernel<N><<<g, 256, 0, s>>>(a); });
   }
   prof_end(ws, e, stream);
+  if (rc) return rc;
   return hipGetLastError() == hipSuccess ? ADRT_OK : ADRT_ERR_HIP;
 }
 
@@ -765,6 +802,12 @@ int adrt_workspace_create_ex(const adrt_batch_config* cfg, size_t max_bytes, int
       return ADRT_ERR_HIP;
     }
   }
+  if (hipStreamCreateWithFlags(&ws->side, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&ws->fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&ws->join, hipEventDisableTiming) != hipSuccess) {
+    (void)adrt_workspace_destroy(ws);  // already failing
+    return ADRT_ERR_HIP;
+  }
   *out = ws;
   return ADRT_OK;
 }
@@ -773,6 +816,9 @@ int adrt_workspace_destroy(adrt_workspace* ws) {
   if (!ws) return ADRT_OK;
   int rc = ADRT_OK;
   for (hipEvent_t e : ws->ev_pool) (void)hipEventDestroy(e);
+  if (ws->fork) (void)hipEventDestroy(ws->fork);
+  if (ws->join) (void)hipEventDestroy(ws->join);
+  if (ws->side) (void)hipStreamDestroy(ws->side);
   if (ws->base && hipFree(ws->base) != hipSuccess) rc = ADRT_ERR_HIP;
   delete ws;
   return rc;
@@ -1178,14 +1224,18 @@ int adrt_adjoint_batch(const adrt_batch_config* cfg, const adrt_device_inputs* i
     }
     int e = prof_begin(ws, 3, stream);
     const unsigned g = (unsigned)((a.W + 3) / 4);
-    if (mixed) {
-      w16::adding16_kernel<true, true><<<g, 256, 0, stream>>>(a);
-      w16::adding16_composite_kernel<true, true><<<g, 256, 0, stream>>>(a);
-    } else {
-      w16::adding16_kernel<true, false><<<g, 256, 0, stream>>>(a);
-      w16::adding16_composite_kernel<true, false><<<g, 256, 0, stream>>>(a);
-    }
+    if (mixed)
+      rc = fork_join(
+          ws, stream,
+          [&](hipStream_t s) { w16::adding16_kernel<true, true><<<g, 256, 0, s>>>(a); },
+          [&](hipStream_t s) { w16::adding16_composite_kernel<true, true><<<g, 256, 0, s>>>(a); });
+    else
+      rc = fork_join(
+          ws, stream,
+          [&](hipStream_t s) { w16::adding16_kernel<true, false><<<g, 256, 0, s>>>(a); },
+          [&](hipStream_t s) { w16::adding16_composite_kernel<true, false><<<g, 256, 0, s>>>(a); });
     prof_end(ws, e, stream);
+    if (rc) return rc;
     const long tasks = (long)a.W * a.L;
     const long blocks = std::min((tasks + 3) / 4, (long)ws->cus * w16::REV_WAVES_PER_SIMD);
     b.nwaves = (int)(blocks * 4);
@@ -1206,7 +1256,7 @@ int adrt_adjoint_batch(const adrt_batch_config* cfg, const adrt_device_inputs* i
       // two layers per wave (orders up to 8 padded to 16)
       w16::AdjArgs bp = b;
       const long pairs = (long)a.W * (a.L / 2);
-      const long pblocks = std::min((pairs + 3) / 4, (long)ws->cus * w16::REV_WAVES_PER_SIMD);
+      const long pblocks = std::min((pairs + 3) / 4, (long)ws->cus * w16::PAIR_REV_WAVES_PER_SIMD);
       bp.nwaves = (int)(pblocks * 4);
       if (pblocks > 0) w16::layer16p_adjoint_kernel<<<(unsigned)pblocks, 256, 0, stream>>>(a, bp);
       w16::layer16_adjoint_kernel<false, true><<<(unsigned)blocks, 256, 0, stream>>>(a, b);

@@ -778,7 +778,10 @@ adding_composite_kernel(FwdArgs a) {
   constexpr int RBR = 2 * N * N + N;
   const double* lay = a.lay + (long)w * L * REC;
   const int* lflag = a.lflag + (long)w * L;
-  double* rbw = a.rb + (long)w * L * RBR;
+  // the column's region of the adding store is L (2N^2 + 4N) doubles, as the
+  // base sweep's (RbRec<N>::size per layer): this kernel runs beside
+  // adding_kernel, so its columns must not reach into that kernel's columns
+  double* rbw = a.rb + (long)w * L * (2 * N * N + 4 * N);
   const double mu_j = a.mu[j], wt_j = a.wt[j];
   const double* Bw = a.B ? a.B + gw * (L + 1) : nullptr;
   auto Blev = [&](int l) -> double { return Bw ? Bw[lev(a, l)] : 0.0; };

@@ -729,8 +729,10 @@ __device__ void layer16p_adjoint_pair(const FwdArgs& a, const AdjArgs& b, long t
   }
 }
 
-// persistent grid over the pairs of tasks
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(REV_WAVES_PER_SIMD, 8)))
+// persistent grid over the pairs; 2 waves per SIMD (256 registers: the pair
+// state spills at 3 -- 7.2 against 7.8 ms on the 4096 x 60, N = 8 workload)
+constexpr int PAIR_REV_WAVES_PER_SIMD = 2;
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PAIR_REV_WAVES_PER_SIMD, 8)))
 layer16p_adjoint_kernel(FwdArgs a, AdjArgs b) {
   __shared__ double smem[4 * LdsA::size];
   const int wv = wave_index();

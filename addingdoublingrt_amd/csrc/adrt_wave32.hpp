@@ -997,7 +997,10 @@ adding32_composite_kernel(FwdArgs a) {
   constexpr int RBR = 2 * N * N + N;
   const double* lay = a.lay + (long)w * Lay * REC;
   const int* lflag = a.lflag + (long)w * Lay;
-  double* rbw = a.rb + (long)w * Lay * RBR;
+  // the column's region of the adding store is L (2N^2 + 4N) doubles, as the
+  // base sweep's (RBR32 per layer): this kernel runs beside adding32_kernel, so
+  // its columns must not reach into that kernel's columns
+  double* rbw = a.rb + (long)w * Lay * (2 * N * N + 4 * N);
   const double mu_c[2] = {a.mu[j], a.mu[16 + j]};
   const double wt_c[2] = {a.wt[j], a.wt[16 + j]};
   const double* Bw = a.B ? a.B + gw * (Lay + 1) : nullptr;
