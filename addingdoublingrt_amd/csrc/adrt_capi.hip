@@ -24,6 +24,7 @@
 #include "adrt_adjm16.hpp"
 #include "adrt_pair16.hpp"
 #include "adrt_wave32.hpp"
+#include "adrt_mix32.hpp"
 
 namespace {
 
@@ -478,12 +479,19 @@ int layers_chunk(const adrt::dev::FwdArgs& a, adrt_workspace* ws, hipStream_t st
       w16::layer16_kernel<false><<<(unsigned)((tasks + 3) / 4), 256, 0, stream>>>(a);
     }
   } else {
-    if (mixed) return ADRT_ERR_PRECISION;
-    if constexpr (N == 32)  // one wave per problem on the matrix cores
-      w32::layer32_kernel<<<(unsigned)((tasks + w32::WAVES - 1) / w32::WAVES), 64 * w32::WAVES,
-                            0, stream>>>(a);
-    else
+    if constexpr (N == 32) {  // one wave per problem on the matrix cores
+      const unsigned g = (unsigned)((tasks + w32::WAVES - 1) / w32::WAVES);
+      if (mixed) {
+        w32::layer32m_kernel<<<g, 64 * w32::WAVES, 0, stream>>>(a);
+        // the routed and the non-scattering layers, in FP64
+        w32::layer32_kernel<true><<<g, 64 * w32::WAVES, 0, stream>>>(a);
+      } else {
+        w32::layer32_kernel<false><<<g, 64 * w32::WAVES, 0, stream>>>(a);
+      }
+    } else {
+      if (mixed) return ADRT_ERR_PRECISION;
       layer_kernel<N><<<(unsigned)((tasks + GPB - 1) / GPB), 256, 0, stream>>>(a);
+    }
   }
   prof_end(ws, e, stream);
   return hipGetLastError() == hipSuccess ? ADRT_OK : ADRT_ERR_HIP;
@@ -945,10 +953,11 @@ int check_solve_args(const adrt_batch_config* cfg, const adrt_device_inputs* in,
   if (!ws) return ADRT_ERR_WORKSPACE;
   if (ws->L != cfg->num_layers || ws->n_real != cfg->num_quadrature || ws->M != cfg->num_moments)
     return ADRT_ERR_WORKSPACE;
-  // the mixed form runs at quadrature order 16 (native 9..16, or padded on a
-  // reverse-pass workspace); the mixed-reverse form's forward is the FP64 one
+  // the mixed form runs at quadrature orders 16 (native 9..16, or padded on a
+  // reverse-pass workspace) and 32; the mixed-reverse form's forward is the
+  // FP64 one
   if (precision != ADRT_PRECISION_F64 && precision != ADRT_PRECISION_MIXED_REVERSE &&
-      !(precision == ADRT_PRECISION_MIXED && ws->Np == 16))
+      !(precision == ADRT_PRECISION_MIXED && (ws->Np == 16 || ws->Np == 32)))
     return ADRT_ERR_PRECISION;
   return ADRT_OK;
 }

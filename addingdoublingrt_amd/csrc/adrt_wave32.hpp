@@ -310,12 +310,177 @@ __device__ __forceinline__ void right_solve_tile(double* t1, double* t2, const M
   wave_sync();
 }
 
+// row of register r in lane group h for the register layout of RowT: the FP64
+// instruction's interleaved rows h + 4 r, the FP32 instruction's blocked 4 h + r
+template <typename RowT>
+__device__ __forceinline__ int rrow(int h, int r) {
+  return sizeof(RowT) == 8 ? h + 4 * r : 4 * h + r;
+}
+
+// phase matrices and thin-layer start of a scattering layer
+// (src/phase_matrix.hpp:61-181, src/doubling.hpp:83-147), entries in the
+// register layout of RowT: T0 (SFORM: S0 = I - T0 = tau0 G++, formed without
+// cancellation) and R0 of rows 16 (b & 1) + rrow(h, r), columns 16 (b >> 1) + j
+struct Start32 {
+  Mat R, T;
+  Vec y, sus, sds;
+  double gam, gk, tau0;
+  int nn;
+};
+template <typename RowT, bool SFORM>
+__device__ __forceinline__ void start32(const FwdArgs& a, long task, int w, int l, long gw,
+                                        const Lane& L, double tau, double omega_in, int mode,
+                                        Start32& st) {
+  const int Lay = a.L, j = L.j, h = L.h;
+  const double mu_c[2] = {a.mu[j], a.mu[16 + j]};
+  const double wt_c[2] = {a.wt[j], a.wt[16 + j]};
+  auto row = [&](int b, int r) { return bi(b) + rrow<RowT>(h, r); };
+  auto dlt = [&](int b, int r) { return (row(b, r) == bj(b) + j) ? 1.0 : 0.0; };
+  const double omega = omega_in > 1.0 ? 1.0 : omega_in;
+  const double con = 2.0 * omega * kPi;
+  const int li = lin(a, l);
+  const double* chi = a.shared_moments ? a.mom + (long)li * a.M : a.mom + (gw * Lay + li) * a.M;
+  const int twoM = 2 * a.n_real;
+  int Lm = a.M;
+  double f = 0.0, inv1mf = 1.0;
+  if (mode == 3) {
+    f = a.fdm[task];
+    inv1mf = 1.0 / (1.0 - f);
+    Lm = twoM;
+  } else if (mode == 2) {
+    Lm = a.M < twoM ? a.M : twoM;
+  }
+
+  // phase matrices (src/phase_matrix.hpp:73-91): P(i, c) = sum_l c_l P_l(mu_i) P_l(+-mu_c)
+  Mat Ppp, Ppm;
+#pragma unroll
+  for (int b = 0; b < 4; ++b)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      Ppp.v[b][r] = 0.0;
+      Ppm.v[b][r] = 0.0;
+    }
+  double sign = 1.0;
+  double sp[2] = {0.0, 0.0}, smv[2] = {0.0, 0.0};  // solar phase vectors, elements j, 16 + j
+  for (int ll = 0; ll < Lm; ++ll) {
+    const double chil = (mode == 3) ? (chi[ll] - f) * inv1mf : chi[ll];
+    const double cl = (2 * ll + 1) * chil;
+    const double* Pl = a.Pl + ll * N;
+    const double plc[2] = {Pl[j], Pl[16 + j]};
+#pragma unroll
+    for (int I = 0; I < 2; ++I)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const double pi = cl * Pl[16 * I + rrow<RowT>(h, r)];
+#pragma unroll
+        for (int J = 0; J < 2; ++J) {
+          const double term = pi * plc[J];
+          Ppp.v[I + 2 * J][r] += term;
+          Ppm.v[I + 2 * J][r] += sign * term;
+        }
+      }
+    if (a.has_solar) {
+#pragma unroll
+      for (int J = 0; J < 2; ++J) {
+        const double term = cl * plc[J] * a.Pl0[ll];
+        sp[J] += term;
+        smv[J] += sign * term;
+      }
+    }
+    sign = -sign;
+  }
+  {
+    // Hansen column normalisation (src/phase_matrix.hpp:93-110)
+    double part[2] = {0.0, 0.0};
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        Ppp.v[b][r] = Ppp.v[b][r] / (2.0 * kPi);
+        Ppm.v[b][r] = Ppm.v[b][r] / (2.0 * kPi);
+        part[b >> 1] += (Ppp.v[b][r] + Ppm.v[b][r]) * a.wt[bi(b) + rrow<RowT>(h, r)];
+      }
+#pragma unroll
+    for (int J = 0; J < 2; ++J) {
+      const double csum = w16::col_reduce<double>(part[J], L);
+      if (csum > 0.0) {
+        const double corr = 1.0 / (2.0 * kPi * csum);
+#pragma unroll
+        for (int I = 0; I < 2; ++I)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            Ppp.v[I + 2 * J][r] *= corr;
+            Ppm.v[I + 2 * J][r] *= corr;
+          }
+      }
+    }
+  }
+  double pplus[2] = {0.0, 0.0}, pminus[2] = {0.0, 0.0};
+  if (a.has_solar) {  // src/phase_matrix.hpp:153-180
+#pragma unroll
+    for (int J = 0; J < 2; ++J) {
+      pplus[J] = sp[J] / (2.0 * kPi);
+      pminus[J] = smv[J] / (2.0 * kPi);
+    }
+    const double s = row_sum<double>((pplus[0] + pminus[0]) * wt_c[0] +
+                                     (pplus[1] + pminus[1]) * wt_c[1]);
+    if (s > 0.0) {
+      const double corr = 1.0 / (2.0 * kPi * s);
+#pragma unroll
+      for (int J = 0; J < 2; ++J) {
+        pplus[J] *= corr;
+        pminus[J] *= corr;
+      }
+    }
+  }
+
+  // thin-layer initialisation (src/doubling.hpp:83-147)
+  const int nn0 = (int)(log(tau) / log(2.0)) + ipow0(omega);
+  const int nn = nn0 < 1 ? 1 : nn0;
+  const double tau0 = tau * ldexp(1.0, -nn);
+  Mat& Rm = st.R;
+  Mat& Tm = st.T;
+#pragma unroll
+  for (int b = 0; b < 4; ++b)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const double mui = a.mu[bi(b) + rrow<RowT>(h, r)];
+      const double wc = wt_c[b >> 1];
+      const double d = dlt(b, r);
+      const double gpp = (d + (-con) * (Ppp.v[b][r] * wc)) / mui;
+      const double gpm = con * (Ppm.v[b][r] * wc) / mui;
+      Tm.v[b][r] = SFORM ? tau0 * gpp : d - tau0 * gpp;
+      Rm.v[b][r] = tau0 * gpm;
+    }
+  Vec& y = st.y;
+  Vec& sus = st.sus;
+  Vec& sds = st.sds;
+  double& gam = st.gam;
+  y = Vec{(1.0 - omega) * tau0 / mu_c[0], (1.0 - omega) * tau0 / mu_c[1]};
+  sus = Vec{0.0, 0.0};
+  sds = Vec{0.0, 0.0};
+  gam = 0.0;
+  if (a.has_solar) {
+    const double Ftop = a.F0 * exp(-a.tau_cum[(long)w * (Lay + 1) + l] / a.mu0);
+    const double b0 = omega * tau0 / mu_c[0] * Ftop, b1 = omega * tau0 / mu_c[1] * Ftop;
+    sus = Vec{b0 * pminus[0], b1 * pminus[1]};
+    sds = Vec{b0 * pplus[0], b1 * pplus[1]};
+    gam = exp(-tau0 / a.mu0);
+  }
+  st.gk = 0.5 * tau0;
+  st.tau0 = tau0;
+  st.nn = nn;
+}
+
 // ---------------------------------------------------------------------------
 // layer: phase matrices + doubling (src/phase_matrix.hpp:61-181,
-// src/doubling.hpp:29-230); one wave per (wavenumber, layer)
+// src/doubling.hpp:29-230); one wave per (wavenumber, layer).
+// REDO: only the layers the mixed-precision kernel leaves to FP64 (the routed
+// scattering layers, and the transparent and purely absorbing ones)
 // ---------------------------------------------------------------------------
 // 2 waves per SIMD (256 VGPRs, some spilled) measured 27 % faster than the
 // spill-free single wave
+template <bool REDO>
 __global__ void __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(2, 8)))
 layer32_kernel(FwdArgs a) {
   __shared__ double smem[WAVES * SLOT];
@@ -323,6 +488,9 @@ layer32_kernel(FwdArgs a) {
   const long task = (long)blockIdx.x * WAVES + wv;
   const int Lay = a.L;
   if (task >= (long)a.W * Lay) return;
+  if constexpr (REDO) {
+    if (!a.redo[task] && a.tau_s[task] > 0.0 && a.omega_s[task] > 0.0) return;
+  }
   const Lane L;
   const int j = L.j, h = L.h;
   const int w = (int)(task / Lay), l = (int)(task % Lay);
@@ -349,7 +517,6 @@ layer32_kernel(FwdArgs a) {
   const double B_bar = (Bbot + Btop) / 2.0;
   const double B_d = (tau > 0.0) ? (Bbot - Btop) / tau : 0.0;
   const double mu_c[2] = {a.mu[j], a.mu[16 + j]};
-  const double wt_c[2] = {a.wt[j], a.wt[16 + j]};
 
   if (tau <= 0.0) {  // transparent layer (src/doubling.hpp:52-53)
     Mat Z, I1;
@@ -409,133 +576,12 @@ layer32_kernel(FwdArgs a) {
   }
 
   // ---- scattering layer ----
-  const double omega = omega_in > 1.0 ? 1.0 : omega_in;
-  const double con = 2.0 * omega * kPi;
-  const int li = lin(a, l);
-  const double* chi = a.shared_moments ? a.mom + (long)li * a.M : a.mom + (gw * Lay + li) * a.M;
-  const int twoM = 2 * a.n_real;
-  int Lm = a.M;
-  double f = 0.0, inv1mf = 1.0;
-  if (mode == 3) {
-    f = a.fdm[task];
-    inv1mf = 1.0 / (1.0 - f);
-    Lm = twoM;
-  } else if (mode == 2) {
-    Lm = a.M < twoM ? a.M : twoM;
-  }
-
-  // phase matrices (src/phase_matrix.hpp:73-91): P(i, c) = sum_l c_l P_l(mu_i) P_l(+-mu_c)
-  Mat Ppp, Ppm;
-#pragma unroll
-  for (int b = 0; b < 4; ++b)
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      Ppp.v[b][r] = 0.0;
-      Ppm.v[b][r] = 0.0;
-    }
-  double sign = 1.0;
-  double sp[2] = {0.0, 0.0}, smv[2] = {0.0, 0.0};  // solar phase vectors, elements j, 16 + j
-  for (int ll = 0; ll < Lm; ++ll) {
-    const double chil = (mode == 3) ? (chi[ll] - f) * inv1mf : chi[ll];
-    const double cl = (2 * ll + 1) * chil;
-    const double* Pl = a.Pl + ll * N;
-    const double plc[2] = {Pl[j], Pl[16 + j]};
-#pragma unroll
-    for (int I = 0; I < 2; ++I)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const double pi = cl * Pl[16 * I + rw(h, r)];
-#pragma unroll
-        for (int J = 0; J < 2; ++J) {
-          const double term = pi * plc[J];
-          Ppp.v[I + 2 * J][r] += term;
-          Ppm.v[I + 2 * J][r] += sign * term;
-        }
-      }
-    if (a.has_solar) {
-#pragma unroll
-      for (int J = 0; J < 2; ++J) {
-        const double term = cl * plc[J] * a.Pl0[ll];
-        sp[J] += term;
-        smv[J] += sign * term;
-      }
-    }
-    sign = -sign;
-  }
-  {
-    // Hansen column normalisation (src/phase_matrix.hpp:93-110)
-    double part[2] = {0.0, 0.0};
-#pragma unroll
-    for (int b = 0; b < 4; ++b)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        Ppp.v[b][r] = Ppp.v[b][r] / (2.0 * kPi);
-        Ppm.v[b][r] = Ppm.v[b][r] / (2.0 * kPi);
-        part[b >> 1] += (Ppp.v[b][r] + Ppm.v[b][r]) * a.wt[bi(b) + rw(h, r)];
-      }
-#pragma unroll
-    for (int J = 0; J < 2; ++J) {
-      const double csum = w16::col_reduce<double>(part[J], L);
-      if (csum > 0.0) {
-        const double corr = 1.0 / (2.0 * kPi * csum);
-#pragma unroll
-        for (int I = 0; I < 2; ++I)
-#pragma unroll
-          for (int r = 0; r < 4; ++r) {
-            Ppp.v[I + 2 * J][r] *= corr;
-            Ppm.v[I + 2 * J][r] *= corr;
-          }
-      }
-    }
-  }
-  double pplus[2] = {0.0, 0.0}, pminus[2] = {0.0, 0.0};
-  if (a.has_solar) {  // src/phase_matrix.hpp:153-180
-#pragma unroll
-    for (int J = 0; J < 2; ++J) {
-      pplus[J] = sp[J] / (2.0 * kPi);
-      pminus[J] = smv[J] / (2.0 * kPi);
-    }
-    const double s = row_sum<double>((pplus[0] + pminus[0]) * wt_c[0] +
-                                     (pplus[1] + pminus[1]) * wt_c[1]);
-    if (s > 0.0) {
-      const double corr = 1.0 / (2.0 * kPi * s);
-#pragma unroll
-      for (int J = 0; J < 2; ++J) {
-        pplus[J] *= corr;
-        pminus[J] *= corr;
-      }
-    }
-  }
-
-  // thin-layer initialisation (src/doubling.hpp:83-147)
-  const int nn0 = (int)(log(tau) / log(2.0)) + ipow0(omega);
-  const int nn = nn0 < 1 ? 1 : nn0;
-  const double tau0 = tau * ldexp(1.0, -nn);
-  Mat Rm, Tm;
-#pragma unroll
-  for (int b = 0; b < 4; ++b)
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const double mui = a.mu[bi(b) + rw(h, r)];
-      const double wc = wt_c[b >> 1];
-      const double d = delta(L, b, r);
-      const double gpp = (d + (-con) * (Ppp.v[b][r] * wc)) / mui;
-      const double gpm = con * (Ppm.v[b][r] * wc) / mui;
-      Tm.v[b][r] = d - tau0 * gpp;
-      Rm.v[b][r] = tau0 * gpm;
-    }
-  Vec y{(1.0 - omega) * tau0 / mu_c[0], (1.0 - omega) * tau0 / mu_c[1]};
-  Vec z{0.0, 0.0};
-  Vec sus{0.0, 0.0}, sds{0.0, 0.0};
-  double gam = 0.0;
-  if (a.has_solar) {
-    const double Ftop = a.F0 * exp(-a.tau_cum[(long)w * (Lay + 1) + l] / a.mu0);
-    const double b0 = omega * tau0 / mu_c[0] * Ftop, b1 = omega * tau0 / mu_c[1] * Ftop;
-    sus = Vec{b0 * pminus[0], b1 * pminus[1]};
-    sds = Vec{b0 * pplus[0], b1 * pplus[1]};
-    gam = exp(-tau0 / a.mu0);
-  }
-  double gk = 0.5 * tau0;
+  Start32 st;
+  start32<double, false>(a, task, w, l, gw, L, tau, omega_in, mode, st);
+  const int nn = st.nn;
+  Mat Rm = st.R, Tm = st.T;
+  Vec y = st.y, z{0.0, 0.0}, sus = st.sus, sds = st.sds;
+  double gam = st.gam, gk = st.gk;
 
   double* v1 = v0 + N;
   double* v2 = v1 + N;

@@ -225,8 +225,9 @@ def solve_batch_from_coefficients(absorption_coeff, scattering_coeff, altitude, 
         out = {k: torch.empty((W, L + 1), dtype=torch.float64, device=wavenumber.device)
                for k in OUTPUT_NAMES}
     o = _outputs_struct(out, W, L)
+    # the mixed form below order 9 runs padded to 16 on a reverse-pass workspace
     ws = get_workspace(W, L, config.num_quadrature, M, max_workspace_bytes,
-                       precision == "mixed")
+                       precision == "mixed" and config.num_quadrature <= 16)
     stream = C.c_void_p(torch.cuda.current_stream().cuda_stream)
     with ws.lock:
         _lib.check(_lib.lib().adrt_solve_batch_from_coefficients(

@@ -341,10 +341,11 @@ def main():
     kw = dict(planck_levels=d["planck_levels"], surface_emission=d["surface_emission"],
               top_emission=d["top_emission"])
     reverse_ok = cols.N <= 16
+    npad = 1 << max(cols.N - 1, 1).bit_length()  # the library's padded order
     if a.form == "auto":
         forms = ["f64"] + (["mixed", "mixed_reverse"] if reverse_ok else [])
-        if a.forward_only:
-            forms = ["f64"] + (["mixed"] if reverse_ok else [])
+        if a.forward_only:  # the mixed forward also runs at N = 32
+            forms = ["f64"] + (["mixed"] if reverse_ok or npad == 32 else [])
     else:
         forms = [a.form]
     lib = _lib.lib()
@@ -354,7 +355,8 @@ def main():
                                cols.use_diffusion_lower_bc)
 
     def run_form(form):
-        adjoint_ws = not a.forward_only or form != "f64"
+        # reduced precision below N = 16 runs padded to 16 on a reverse-pass workspace
+        adjoint_ws = not a.forward_only or (form != "f64" and reverse_ok)
         out = {k: torch.empty((W, L + 1), dtype=torch.float64, device="cuda")
                for k in ("flux_up", "flux_down", "mean_intensity", "flux_direct",
                          "flux_divergence")}
@@ -430,14 +432,15 @@ def main():
                      "FP32 recomputed doubling + reverse (FP64 for the routed layers), "
                      "FP64 flop count", work["doubling_reverse"])}
     if a.forward_only:
-        npad = 1 << max(cols.N - 1, 1).bit_length()  # the library's padded order
-        names = ({1: "layer16_kernel<false, false>", 2: "adding16_kernel<false, false>"}
+        names = ({1: "layer32_kernel<false>", 2: "adding32_kernel"} if npad == 32 else
+                 {1: "layer16_kernel<false, false>", 2: "adding16_kernel<false, false>"}
                  if npad == 16 or chosen != "f64" else
-                 {1: "layer32_kernel", 2: "adding32_kernel"} if npad == 32 else
                  {1: f"layer_kernel<{npad}>", 2: f"adding_kernel<{npad}>"})
         if chosen == "mixed":
-            names = {1: "layer16m_kernel + layer16_kernel<true, false>",
-                     2: "adding16_kernel<false, true>"}
+            names = ({1: "layer32m_kernel + layer32_kernel<true>", 2: "adding32_kernel"}
+                     if npad == 32 else
+                     {1: "layer16m_kernel + layer16_kernel<true, false>",
+                      2: "adding16_kernel<false, true>"})
         model = {1: (names[1], "phase setup + doubling (forward)", work["doubling"]),
                  2: (names[2], "adding sweeps + interfaces (forward)",
                      work["adding"] + work["interfaces"])}

@@ -79,7 +79,9 @@ typedef enum adrt_precision {
   ADRT_PRECISION_F64 = 0,   /* plain form: FP64 throughout */
   ADRT_PRECISION_MIXED = 1, /* FP32 doubling (FP64 start, FP64-refined solves), FP64 adding and
                                fluxes; layers with an unstable doubling start and their
-                               neighbours in FP64; reverse doubling in FP32 likewise (N <= 16) */
+                               neighbours in FP64; reverse doubling in FP32 likewise.  Padded
+                               orders 16 (9..16, or any order on a reverse-pass workspace) and
+                               32 (17..32, forward only) */
   ADRT_PRECISION_MIXED_REVERSE = 2 /* the FP64 forward (outputs identical to ADRT_PRECISION_F64)
                                       with the reverse pass's doubling recomputation and its
                                       reverse in FP32, routed as for ADRT_PRECISION_MIXED */

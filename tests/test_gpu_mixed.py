@@ -22,7 +22,9 @@ relative norm.
   regression bound at its measured ceiling guards it.  Its gradients of
   unstable-start layers (which both forms compute in FP64, but from interface
   vectors that differ by the FP32 rounding of the other layers' records) are
-  reported apart from the others.
+  reported apart from the others.  At quadrature order 32 (forward only) it
+  misses the bar on most columns of config 3 and is held to a regression
+  bound; layers it routes to FP64 come out bit for bit.
 """
 import numpy as np
 import pytest
@@ -180,3 +182,70 @@ def test_mixed_gradients_at_bar(k, W):
               f"d/ddelta_tau relative differences up to "
               f"{np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-300)):.2e}")
     assert worst <= GRAD_BAR, worst
+
+
+# quadrature order 32 (workload config 3: 1 - omega down to 1e-6, 64 moments):
+# the mixed forward alone (the reverse pass runs up to order 16).  Measured on
+# 8192 columns: per column largest 1.6e-4, median 7.7e-6, 5.5 % within 1e-6 --
+# the floor that rounding the thin-layer start to FP32 already sets
+# (tests/test_mixed32_ceiling.py) -- and no faster than FP64.
+N32_COLUMNS = 512
+
+
+def _mixed32_errors(W):
+    cols, d, cfg = _inputs(3, W)
+    args = (d["delta_tau"], d["single_scat_albedo"], d["phase_moments"], cfg,
+            d["planck_levels"], d["surface_emission"], d["top_emission"])
+    out = solve_batch(*args, precision="mixed")
+    ref = solve_batch(*args)
+    torch.cuda.synchronize()
+    return cols, d, cfg, out, flux_errors({n: out[n].cpu().numpy() for n in OUTS},
+                                          {n: ref[n].cpu().numpy() for n in OUTS})
+
+
+def test_mixed32_forward_regression():
+    """N = 32 mixed forward at its measured ceiling: finite everywhere, every
+    column within 5e-4 of FP64 and the median within 3e-5 (a regression guard,
+    not the bar)."""
+    _, _, _, _, e = _mixed32_errors(N32_COLUMNS)
+    print(f"mixed N=32 config 3 W={N32_COLUMNS}: per column max {e.max():.2e}, median "
+          f"{np.median(e):.2e}, within 1e-6: {np.mean(e <= FLUX_BAR):.3f}")
+    assert np.isfinite(e).all()
+    assert e.max() <= 5e-4, (float(e.max()), int(e.argmax()))
+    assert np.median(e) <= 3e-5, float(np.median(e))
+
+
+def test_mixed32_routed_layers_are_fp64():
+    """Columns whose every scattering layer the prep kernel routes to FP64
+    (thick, strongly scattering: omega >= 0.8, tau >= 4) come out of the mixed
+    form identical to the FP64 form, bit for bit."""
+    cols, d, cfg = _inputs(3, 8)
+    tau = d["delta_tau"].clone()
+    tau.clamp_(min=4.5)  # every layer thick; config 3's omega >= 1 - 1e-1
+    args = (tau, d["single_scat_albedo"], d["phase_moments"], cfg,
+            d["planck_levels"], d["surface_emission"], d["top_emission"])
+    assert float(d["single_scat_albedo"].min()) >= 0.8
+    out = solve_batch(*args, precision="mixed")
+    ref = solve_batch(*args)
+    torch.cuda.synchronize()
+    for n in OUTS:
+        assert torch.equal(out[n], ref[n]), n
+
+
+@pytest.mark.xfail(reason="FP32 doubling at N = 32 misses the 1e-6 flux bar on most columns of "
+                          "config 3 (rounding the thin-layer start to FP32 alone moves the "
+                          "fluxes by 3e-7 - 3e-6)", strict=False)
+def test_mixed32_forward_vs_oracle_at_bar():
+    cols, d, cfg, out, _ = _mixed32_errors(64)
+    h = {n: (None if v is None else v.cpu().numpy()) for n, v in d.items()}
+    ref = oracle.solve_batch(
+        cols.delta_tau, cols.single_scat_albedo, cols.phase_moments, cols.N,
+        planck_levels=h["planck_levels"],
+        surface_emission=0.0 if h["surface_emission"] is None else h["surface_emission"],
+        top_emission=0.0 if h["top_emission"] is None else h["top_emission"],
+        surface_albedo=cfg.surface_albedo, solar_flux=cfg.solar_flux, solar_mu=cfg.solar_mu,
+        use_delta_m=cfg.use_delta_m, use_diffusion_lower_bc=cfg.use_diffusion_lower_bc)
+    e = flux_errors({n: out[n].cpu().numpy() for n in OUTS}, ref)
+    print(f"mixed N=32 vs FP64 oracle config 3: per column max {e.max():.2e}, "
+          f"within 1e-6: {np.mean(e <= FLUX_BAR):.3f}")
+    assert e.max() <= FLUX_BAR, (float(e.max()), int(e.argmax()))

@@ -1,4 +1,4 @@
-"""Why the reduced-precision forms stop at quadrature order 16.
+"""Why the reduced-precision forms cannot meet the flux bar at quadrature order 32.
 
 The 8192 x 80, N = 32 workload (addingdoublingrt_amd.workloads config 3:
 1 - omega log-uniform in [1e-6, 1e-1], tau log-uniform in [1e-2, 100], 64
@@ -25,8 +25,10 @@ thin-layer scattering far below float32's resolution of R0 and S0, and the
 doubling carries that error to every later step, so that floor holds for any
 FP32 doubling arithmetic.  Rounding the final records instead moves the fluxes
 by less than 1e-7: thick layers carry their absorption at a scale float32
-resolves, so FP32 records are not what fails.  No FP32 N = 32 kernel is built
-for that reason.
+resolves, so FP32 records are not what fails.  The N = 32 mixed kernel
+(adrt_mix32.hpp) measures accordingly on the GPU: on 8192 columns of this
+workload its fluxes differ from FP64 by a median 7.7e-6 per column (largest
+1.6e-4, 5.5 % of the columns within 1e-6; tests/test_gpu_mixed.py).
 """
 import math
 
