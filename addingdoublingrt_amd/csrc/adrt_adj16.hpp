@@ -776,14 +776,14 @@ layer16_adjoint_kernel(FwdArgs a, AdjArgs b) {
   }
 }
 
+// the column terms of the reverse pass for interface vectors of NV elements;
 // one thread per wavenumber
-__global__ void __launch_bounds__(256) column16_adjoint_kernel(FwdArgs a, AdjArgs b) {
-  const int w = blockIdx.x * blockDim.x + threadIdx.x;
-  if (w >= a.W) return;
+template <int NV>
+__device__ __forceinline__ void column_adjoint(const FwdArgs& a, const AdjArgs& b, int w) {
   const long gw = (long)(a.w0 + w);
   const int Lay = a.L;
-  const double* Aw = a.Av + (long)w * (Lay + 1) * 2 * N;
-  const double* Iw = a.Iv + (long)w * (Lay + 1) * 2 * N;
+  const double* Aw = a.Av + (long)w * (Lay + 1) * 2 * NV;
+  const double* Iw = a.Iv + (long)w * (Lay + 1) * 2 * NV;
   double* gts = b.g_taus + (long)w * Lay;
   const double* Bw = a.B ? a.B + gw * (Lay + 1) : nullptr;
   auto Blev = [&](int l) -> double { return Bw ? Bw[lev(a, l)] : 0.0; };
@@ -791,13 +791,13 @@ __global__ void __launch_bounds__(256) column16_adjoint_kernel(FwdArgs a, AdjArg
   const double A = a.albedo;
   const int has_surface = !a.diffusion && (A > 0.0 || Bs > 0.0);
   const double tau_tot = a.tau_cum[(long)w * (Lay + 1) + Lay];
-  const double* DL = Aw + (long)Lay * 2 * N + N;
+  const double* DL = Aw + (long)Lay * 2 * NV + NV;
   double sumD = 0.0, sumDn = 0.0, sumU0 = 0.0, muwI = 0.0, muD = 0.0;
-  for (int i = 0; i < N; ++i) {
+  for (int i = 0; i < NV; ++i) {
     sumD += DL[i];
     if (i < a.n_real) sumDn += DL[i];
     sumU0 += Aw[i];
-    muwI += a.mu[i] * a.wt[i] * Iw[(long)Lay * 2 * N + N + i];
+    muwI += a.mu[i] * a.wt[i] * Iw[(long)Lay * 2 * NV + NV + i];
     muD += DL[i] * a.mu[i];
   }
   // cumulative optical depth: solar beam at layer tops, direct flux and beam
@@ -886,6 +886,11 @@ __global__ void __launch_bounds__(256) column16_adjoint_kernel(FwdArgs a, AdjArg
     if (b.d_tau) b.d_tau[gw * Lay + li] = taub;
     if (b.d_ssa) b.d_ssa[gw * Lay + li] = omb;
   }
+}
+
+__global__ void __launch_bounds__(256) column16_adjoint_kernel(FwdArgs a, AdjArgs b) {
+  const int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w < a.W) column_adjoint<N>(a, b, w);
 }
 
 // shared moments: d_mom[l'][m] += sum over the chunk's wavenumbers, in a fixed

@@ -25,6 +25,7 @@
 #include "adrt_pair16.hpp"
 #include "adrt_wave32.hpp"
 #include "adrt_mix32.hpp"
+#include "adrt_adj32.hpp"
 
 namespace {
 
@@ -399,6 +400,8 @@ size_t per_wavenumber_bytes(int L, int Np, int M, bool adjoint) {
     b += (size_t)L * M * d;                           // per-layer moment gradients
     if (Np == adrt::dev::w16::N)
       b += (size_t)L * adrt::dev::w16::SCOT * d;      // doubling-start cotangents
+    else if (Np == adrt::dev::w32::N)
+      b += (size_t)L * adrt::dev::w32::SCOT32 * d;
   }
   return b;
 }
@@ -546,7 +549,7 @@ int adding_chunk(const adrt::dev::FwdArgs& a, adrt_workspace* ws, hipStream_t st
     const unsigned g = (unsigned)((a.W + w32::WAVES - 1) / w32::WAVES);
     rc = fork_join(
         ws, stream,
-        [&](hipStream_t s) { w32::adding32_kernel<<<g, 64 * w32::WAVES, 0, s>>>(a); },
+        [&](hipStream_t s) { w32::adding32_kernel<false><<<g, 64 * w32::WAVES, 0, s>>>(a); },
         [&](hipStream_t s) { w32::adding32_composite_kernel<<<g, 64 * w32::WAVES, 0, s>>>(a); });
   } else {
     const unsigned g = (unsigned)((a.W + GPB - 1) / GPB);
@@ -649,14 +652,14 @@ int adrt_workspace_create_ex(const adrt_batch_config* cfg, size_t max_bytes, int
   if (rc) return rc;
   const bool adjoint = (flags & ADRT_WORKSPACE_ADJOINT) != 0;
   const bool jac = (flags & ADRT_WORKSPACE_JACOBIAN) != 0;
-  if (adjoint && cfg->num_quadrature > 16) return ADRT_ERR_ADJOINT;
   if (adjoint && jac) return ADRT_ERR_WORKSPACE;
   adrt_workspace* ws = new (std::nothrow) adrt_workspace();
   if (!ws) return ADRT_ERR_WORKSPACE;
   ws->flags = flags;
   ws->L = cfg->num_layers;
   ws->n_real = cfg->num_quadrature;
-  ws->Np = adjoint ? 16 : padded_order(cfg->num_quadrature);
+  // reverse-pass workspaces: orders up to 16 padded to 16, 17..32 to 32
+  ws->Np = adjoint ? (cfg->num_quadrature <= 16 ? 16 : 32) : padded_order(cfg->num_quadrature);
   ws->M = cfg->num_moments;
   ws->Lmax = std::max(cfg->num_moments, 2 * cfg->num_quadrature);
   ws->h_mu.assign(ws->n_real, 0.0);
@@ -679,10 +682,17 @@ int adrt_workspace_create_ex(const adrt_batch_config* cfg, size_t max_bytes, int
             hipSuccess || cus <= 0)
       cus = 256;
     ws->cus = cus;
-    ws->nwaves = cus * 4 * std::max(adrt::dev::w16::REV_WAVES_PER_SIMD,
-                                    adrt::dev::w16::MREV_WAVES_PER_SIMD);
-    ws->nn_cap = 64;
-    slab_bytes = (size_t)ws->nwaves * adrt::dev::w16::slab_stride(ws->nn_cap) * sizeof(double);
+    if (ws->Np == adrt::dev::w32::N) {
+      // one wave per SIMD; 48 steps hold layers up to tau 2^32
+      ws->nwaves = cus * 4 * adrt::dev::w32::REV32_WAVES_PER_SIMD;
+      ws->nn_cap = 48;
+      slab_bytes = (size_t)ws->nwaves * adrt::dev::w32::slab_stride32(ws->nn_cap) * sizeof(double);
+    } else {
+      ws->nwaves = cus * 4 * std::max(adrt::dev::w16::REV_WAVES_PER_SIMD,
+                                      adrt::dev::w16::MREV_WAVES_PER_SIMD);
+      ws->nn_cap = 64;
+      slab_bytes = (size_t)ws->nwaves * adrt::dev::w16::slab_stride(ws->nn_cap) * sizeof(double);
+    }
   }
   if (max_bytes == 0) max_bytes = (size_t)4 << 30;
   size_t per_w = per_wavenumber_bytes(ws->L, ws->Np, ws->M, adjoint);
@@ -728,7 +738,9 @@ int adrt_workspace_create_ex(const adrt_batch_config* cfg, size_t max_bytes, int
     o_pflag = take(C * L * 4);
     o_part = take(L * adrt::dev::w16::MOM_SLICES * M * 8);
     o_chi = take(C * L * M * 8);
-    o_scot = take(Np == adrt::dev::w16::N ? C * L * adrt::dev::w16::SCOT * 8 : 0);
+    o_scot = take(Np == adrt::dev::w16::N   ? C * L * adrt::dev::w16::SCOT * 8
+                  : Np == adrt::dev::w32::N ? C * L * adrt::dev::w32::SCOT32 * 8
+                                            : 0);
     o_slab = take(slab_bytes);
     o_status = take(4);
   }
@@ -939,6 +951,39 @@ int adding_dispatch(const adrt::dev::FwdArgs& a, adrt_workspace* ws, hipStream_t
 #endif
     default: return ADRT_ERR_QUADRATURE_MAX;
   }
+}
+
+// the reverse pass of one chunk at quadrature order 32 (the forward's records
+// and interface intensities are in the workspace): the transposed adding for
+// every column, the doubling reverse on a persistent grid, the start reverse,
+// the column terms and the shared-moment sums
+int adjoint_chunk32(const adrt::dev::FwdArgs& a, adrt::dev::w16::AdjArgs b, adrt_workspace* ws,
+                    hipStream_t stream, bool shared_moments) {
+  using namespace adrt::dev;
+  int e = prof_begin(ws, 3, stream);
+  w32::adding32_kernel<true><<<(unsigned)((a.W + w32::WAVES - 1) / w32::WAVES), 64 * w32::WAVES,
+                               0, stream>>>(a);
+  prof_end(ws, e, stream);
+  const long tasks = (long)a.W * a.L;
+  const long waves = std::min(tasks, (long)ws->cus * 4 * w32::REV32_WAVES_PER_SIMD);
+  b.nwaves = (int)waves;
+  e = prof_begin(ws, 4, stream);
+  w32::layer32_adjoint_kernel<<<(unsigned)waves, 64, 0, stream>>>(a, b);
+  prof_end(ws, e, stream);
+  e = prof_begin(ws, 6, stream);
+  w32::start32_adjoint_kernel<<<(unsigned)tasks, 64, 0, stream>>>(a, b);
+  prof_end(ws, e, stream);
+  e = prof_begin(ws, 5, stream);
+  w32::column32_adjoint_kernel<<<(unsigned)((a.W + 255) / 256), 256, 0, stream>>>(a, b);
+  if (shared_moments) {
+    const long n = (long)a.L * a.M;
+    w16::moments16_partial_kernel<<<dim3((unsigned)a.L, w16::MOM_SLICES), 256, 0, stream>>>(
+        a, b, ws->mom_part);
+    w16::moments16_reduce_kernel<<<(unsigned)((n + 255) / 256), 256, 0, stream>>>(a, b,
+                                                                                ws->mom_part);
+  }
+  prof_end(ws, e, stream);
+  return hipGetLastError() == hipSuccess ? ADRT_OK : ADRT_ERR_HIP;
 }
 
 int check_solve_args(const adrt_batch_config* cfg, const adrt_device_inputs* in,
@@ -1180,7 +1225,9 @@ int adrt_adjoint_batch(const adrt_batch_config* cfg, const adrt_device_inputs* i
   if (cfg->num_wavenumbers == 0) return ADRT_OK;
   rc = check_solve_args(cfg, in, out, precision, ws);
   if (rc) return rc;
-  if (!ws->Iv || ws->Np != 16) return ADRT_ERR_ADJOINT;
+  if (!ws->Iv || (ws->Np != 16 && ws->Np != 32)) return ADRT_ERR_ADJOINT;
+  // above order 16 the reverse pass runs in FP64 only
+  if (ws->Np == 32 && precision != ADRT_PRECISION_F64) return ADRT_ERR_PRECISION;
   const bool has_solar = cfg->solar_flux > 0.0 && cfg->solar_mu > 0.0;
   rc = upload_tables(ws, has_solar ? cfg->solar_mu : 1.0, stream);
   if (rc) return rc;
@@ -1230,6 +1277,11 @@ int adrt_adjoint_batch(const adrt_batch_config* cfg, const adrt_device_inputs* i
       rc = forward_chunk(a, ws, stream, mixed);
       if (rc) return rc;
       ws->kept_valid = false;
+    }
+    if (ws->Np == 32) {
+      rc = adjoint_chunk32(a, b, ws, stream, shared && b.d_mom);
+      if (rc) return rc;
+      continue;
     }
     int e = prof_begin(ws, 3, stream);
     const unsigned g = (unsigned)((a.W + 3) / 4);

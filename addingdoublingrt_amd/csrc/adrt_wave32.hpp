@@ -19,6 +19,7 @@
 // column per lane.
 #pragma once
 
+#include "adrt_fwd16.hpp"
 #include "adrt_wave16.hpp"
 
 namespace adrt {
@@ -723,6 +724,21 @@ __device__ __forceinline__ void gstore(double* g, const Lane& L, const Mat& m) {
 #pragma unroll
     for (int r = 0; r < 4; ++r) g[(bj(b) + L.j) * N + bi(b) + rw(L.h, r)] = m.v[b][r];
 }
+// elements j, 16 + j of a vector in global memory (lanes of group 0)
+__device__ __forceinline__ void store_vec(double* g, const Lane& L, const Vec& x) {
+  if (L.h == 0) {
+    g[L.j] = x.lo;
+    g[16 + L.j] = x.hi;
+  }
+}
+
+// the transpose of a column-major 32 x 32 in global memory
+__device__ __forceinline__ void gload_t(const double* g, const Lane& L, Mat& m) {
+#pragma unroll
+  for (int b = 0; b < 4; ++b)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) m.v[b][r] = g[(bi(b) + rw(L.h, r)) * N + bj(b) + L.j];
+}
 __device__ __forceinline__ Vec gvec(const double* g, const Lane& L) {
   return Vec{g[L.j], g[16 + L.j]};
 }
@@ -1115,6 +1131,11 @@ adding32_composite_kernel(FwdArgs a) {
   }
 
   auto emit = [&](int l, const Vec& Iup, const Vec& Idn) {
+    if (a.Iv && L.h == 0) {  // interface intensities for the reverse pass
+      double* vec = a.Iv + ((long)w * (Lay + 1) + l) * 2 * N;
+      store_vec(vec, L, Iup);
+      store_vec(vec + N, L, Idn);
+    }
     const double c0 = 2.0 * kPi * wt_c[0] * mu_c[0], c1 = 2.0 * kPi * wt_c[1] * mu_c[1];
     const double fu = row_sum<double>(c0 * Iup.lo + c1 * Iup.hi);
     const double fdn = row_sum<double>(c0 * Idn.lo + c1 * Idn.hi);
@@ -1260,18 +1281,29 @@ __device__ __forceinline__ void left_solve_mat_tile(double* t1, double* t2, cons
   wave_sync();
 }
 
-__device__ __forceinline__ void store_vec(double* g, const Lane& L, const Vec& x) {
-  if (L.h == 0) {
-    g[L.j] = x.lo;
-    g[16 + L.j] = x.hi;
-  }
+// cotangents of I_up(l) and I_dn(l) (elements j, 16 + j) from the output
+// cotangents, as the N = 16 reverse takes them (w16::ibar16)
+__device__ __forceinline__ void ibar32(const FwdArgs& a, long gw, int l, const Lane& L, Vec& up,
+                                       Vec& dn) {
+  w16::ibar16(a, gw, l, a.mu[L.j], a.wt[L.j], up.lo, dn.lo);
+  w16::ibar16(a, gw, l, a.mu[16 + L.j], a.wt[16 + L.j], up.hi, dn.hi);
+}
+
+// R or T of a layer record, or (ADJ) its transpose
+template <bool ADJ>
+__device__ __forceinline__ void rec_mat(const double* m, const Lane& L, Mat& M) {
+  if constexpr (ADJ) gload_t(m, L, M);
+  else gload(m, L, M);
 }
 
 // layer record rec on top of the base B: stores at o what the downward sweep
-// needs at this layer, then replaces B by the base above the layer
+// needs at this layer, then replaces B by the base above the layer.  ADJ: the
+// transposed system (transposed R and T; sources su, sd the cotangents of
+// I_dn(l) and I_up(l+1))
+template <bool ADJ>
 __device__ __forceinline__ void base32_add(Base& B, const double* rec, int lscat, double* o,
-                                           const Lane& L, const Ws& s) {
-  const Vec su = gvec(rec + 2 * N * N, L), sd = gvec(rec + 2 * N * N + N, L);
+                                           const Vec& su, const Vec& sd, const Lane& L,
+                                           const Ws& s) {
   if (B.scat) gstore(o + RB_R, L, B.R);
   store_vec(o + RB_S, L, B.s);
   if (L.l == 0) o[RB_F] = B.scat ? 1.0 : 0.0;
@@ -1302,8 +1334,8 @@ __device__ __forceinline__ void base32_add(Base& B, const double* rec, int lscat
   if (!B.scat) {
     // nothing below reflects: G = T, d = R sb + s_dn, Rb = R, sb = s_up + T sb
     Mat T;
-    gload(rec, L, B.R);
-    gload(rec + N * N, L, T);
+    rec_mat<ADJ>(rec, L, B.R);
+    rec_mat<ADJ>(rec + N * N, L, T);
     put(s.tile, L, B.R);
     put(s.tile2, L, T);
     put_vec(s.v0, L, B.s);
@@ -1318,7 +1350,7 @@ __device__ __forceinline__ void base32_add(Base& B, const double* rec, int lscat
     return;
   }
   Mat A, P, G;
-  gload(rec, L, A);  // R_l
+  rec_mat<ADJ>(rec, L, A);  // R_l
   put(s.tile, L, A);
   put_vec(s.v0, L, B.s);
   wave_sync();
@@ -1333,12 +1365,12 @@ __device__ __forceinline__ void base32_add(Base& B, const double* rec, int lscat
     // I - P close to singular or the iteration slow: d and G by LU solves
     d = left_solve_tile(s.tile, s.tile2, s.v1, L);
     put(s.tile, L, P);
-    gload(rec + N * N, L, A);
+    rec_mat<ADJ>(rec + N * N, L, A);
     wave_sync();
     left_solve_mat_tile(s.tile, s.tile2, A, G, L);
   } else {
     put(s.tile2, L, G);
-    gload(rec + N * N, L, A);
+    rec_mat<ADJ>(rec + N * N, L, A);
     wave_sync();
     mul(s.tile2, A, L, G);          // G = X T_l
     d = matvec(s.tile2, s.v1, L);   // d = X (R_l sb + s_dn)
@@ -1352,23 +1384,27 @@ __device__ __forceinline__ void base32_add(Base& B, const double* rec, int lscat
   mul(s.tile, G, L, P);                             // Rb(l+1) G
   const Vec rd = matvec(s.tile, s.v2, L);           // Rb(l+1) d
   wave_sync();
-  gload(rec + N * N, L, A);
+  rec_mat<ADJ>(rec + N * N, L, A);
   put(s.tile, L, A);
   put_vec(s.v3, L, vadd(rd, B.s));
   wave_sync();
-  gload(rec, L, A);
+  rec_mat<ADJ>(rec, L, A);
   mul_add(s.tile, P, A, L, B.R);                    // Rb(l) = R_l + T_l Rb(l+1) G
   const Vec ts = matvec(s.tile, s.v3, L);
   wave_sync();
   B.s = vadd(su, ts);
 }
 
+// ADJ: the transposed system of the reverse pass (as adding16_kernel<true>),
+// for every column: the adjoint vectors of the columns the forward solved in
+// the composite form also come from this sweep
+template <bool ADJ>
 __global__ void __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(2, 8)))
 adding32_kernel(FwdArgs a) {
   __shared__ double smem[WAVES * SLOT];
   const int wv = wave_index();
   const int w = blockIdx.x * WAVES + wv;
-  if (w >= a.W || (a.ucol && a.ucol[w])) return;
+  if (w >= a.W || (!ADJ && a.ucol && a.ucol[w])) return;
   const Lane L;
   const int j = L.j;
   const long gw = (long)(a.w0 + w);
@@ -1384,6 +1420,8 @@ adding32_kernel(FwdArgs a) {
   const double wt_c[2] = {a.wt[j], a.wt[16 + j]};
   const double* Bw = a.B ? a.B + gw * (Lay + 1) : nullptr;
   auto Blev = [&](int l) -> double { return Bw ? Bw[lev(a, l)] : 0.0; };
+  double* vec = ADJ ? (a.Av ? a.Av + (long)w * (Lay + 1) * 2 * N : nullptr)
+                    : (a.Iv ? a.Iv + (long)w * (Lay + 1) * 2 * N : nullptr);
 
   // boundary conditions (src/solver.cpp:385-419, 516-551, 585-603)
   const double Bs = a.surf_w ? a.surf_w[gw] : a.surf_em;
@@ -1391,9 +1429,17 @@ adding32_kernel(FwdArgs a) {
   const double A = a.albedo;
   const int has_surface = !a.diffusion && (A > 0.0 || Bs > 0.0);
   const double tau_tot = a.tau_cum[(long)w * (Lay + 1) + Lay];
-  const Vec Itd{Btop, Btop};
+  Vec Itd{Btop, Btop};
   Vec Ibu{0.0, 0.0};
-  if (a.diffusion) {
+  if constexpr (ADJ) {
+    // transposed system: the top boundary carries the cotangent of I_up(0),
+    // a fixed I_up(L) the cotangent of I_dn(L)
+    Vec u, d;
+    ibar32(a, gw, 0, L, u, d);
+    Itd = u;
+    ibar32(a, gw, Lay, L, u, d);
+    if (!has_surface) Ibu = d;
+  } else if (a.diffusion) {
     const double Bb = Blev(Lay);
     const double dt = a.tau_s[(long)w * Lay + Lay - 1];
     const double dBdt = (dt > 0.0) ? (Bb - Blev(Lay - 1)) / dt : 0.0;
@@ -1409,11 +1455,23 @@ adding32_kernel(FwdArgs a) {
 #pragma unroll
     for (int b = 0; b < 4; ++b)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) B.R.v[b][r] = (bi(b) + rw(L.h, r) < a.n_real) ? rr[b >> 1] : 0.0;
-    double ssol = 0.0;
-    if (a.has_solar && A > 0.0) ssol = (A / kPi) * a.F0 * a.mu0 * exp(-tau_tot / a.mu0);
-    const double su = (1.0 - A) * Bs + ssol;
-    B.s = Vec{su, su};
+      for (int r = 0; r < 4; ++r) {
+        const int i = bi(b) + rw(L.h, r), c = bj(b) + j;
+        if constexpr (ADJ)  // R_s^T(i, c) = R_s(c, i)
+          B.R.v[b][r] = (c < a.n_real) ? 2.0 * A * a.mu[i] * a.wt[i] * a.xfac : 0.0;
+        else
+          B.R.v[b][r] = (i < a.n_real) ? rr[b >> 1] : 0.0;
+      }
+    if constexpr (ADJ) {
+      Vec u, d;
+      ibar32(a, gw, Lay, L, u, d);
+      B.s = d;
+    } else {
+      double ssol = 0.0;
+      if (a.has_solar && A > 0.0) ssol = (A / kPi) * a.F0 * a.mu0 * exp(-tau_tot / a.mu0);
+      const double su = (1.0 - A) * Bs + ssol;
+      B.s = Vec{su, su};
+    }
     B.scat = A > 0.0;
   } else {
 #pragma unroll
@@ -1423,10 +1481,28 @@ adding32_kernel(FwdArgs a) {
     B.s = Ibu;
     B.scat = 0;
   }
-  for (int l = Lay - 1; l >= 0; --l)
-    base32_add(B, lay + (long)l * REC, lflag[l], rbw + (long)l * RBR32, L, s);
+  for (int l = Lay - 1; l >= 0; --l) {
+    const double* rec = lay + (long)l * REC;
+    Vec su, sd;
+    if constexpr (ADJ) {
+      Vec u, d;
+      ibar32(a, gw, l, L, u, d);
+      su = d;
+      ibar32(a, gw, l + 1, L, u, d);
+      sd = u;
+    } else {
+      su = gvec(rec + 2 * N * N, L);
+      sd = gvec(rec + 2 * N * N + N, L);
+    }
+    base32_add<ADJ>(B, rec, lflag[l], rbw + (long)l * RBR32, su, sd, L, s);
+  }
 
   auto emit = [&](int l, const Vec& Iup, const Vec& Idn) {
+    if (vec && L.h == 0) {
+      store_vec(vec + (long)l * 2 * N, L, Iup);
+      store_vec(vec + (long)l * 2 * N + N, L, Idn);
+    }
+    if constexpr (ADJ) return;
     const double c0 = 2.0 * kPi * wt_c[0] * mu_c[0], c1 = 2.0 * kPi * wt_c[1] * mu_c[1];
     const double fu = row_sum<double>(c0 * Iup.lo + c1 * Iup.hi);
     const double fdn = row_sum<double>(c0 * Idn.lo + c1 * Idn.hi);

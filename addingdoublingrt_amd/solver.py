@@ -287,7 +287,7 @@ def adjoint_batch(delta_tau, single_scat_albedo, phase_moments, config: BatchCon
     return grads
 
 
-REVERSE_MAX_QUADRATURE = 16
+REVERSE_MAX_QUADRATURE = 32  # every order the forward takes
 
 
 JACOBIAN_NAMES = ("flux_up", "flux_down", "mean_intensity", "flux_divergence")
@@ -353,26 +353,13 @@ def planck_source_vjp(delta_tau, single_scat_albedo, phase_moments, config: Batc
 
 
 class _SolveFunction(torch.autograd.Function):
-    """Forward solve with a hand-written reverse pass (adrt_adjoint_batch); for
-    quadrature orders above 16 the reverse pass covers the thermal sources
-    only (planck_source_vjp)."""
+    """Forward solve with a hand-written reverse pass (adrt_adjoint_batch)."""
 
     @staticmethod
     def forward(ctx, config, precision, max_ws, delta_tau, single_scat_albedo, phase_moments,
                 planck_levels, surface_emission, top_emission, surface_albedo):
         if surface_albedo is not None:
             config = BatchConfig(**{**config.__dict__, "surface_albedo": float(surface_albedo)})
-        ctx.sources_only = config.num_quadrature > REVERSE_MAX_QUADRATURE
-        if ctx.sources_only:
-            out = solve_batch(delta_tau, single_scat_albedo, phase_moments, config,
-                              planck_levels, surface_emission, top_emission,
-                              precision=precision, max_workspace_bytes=max_ws)
-            ctx.config, ctx.precision, ctx.max_ws = config, precision, max_ws
-            ctx.has = (planck_levels is not None, surface_emission is not None,
-                       top_emission is not None, surface_albedo is not None)
-            outs = tuple(out[k] for k in OUTPUT_NAMES)
-            ctx.save_for_backward(delta_tau, single_scat_albedo, phase_moments)
-            return outs
         out, ws = _solve_batch_ws(delta_tau, single_scat_albedo, phase_moments, config,
                                   planck_levels, surface_emission, top_emission,
                                   precision=precision, max_workspace_bytes=max_ws,
@@ -391,19 +378,6 @@ class _SolveFunction(torch.autograd.Function):
 
     @staticmethod
     def backward(ctx, *gouts):
-        if ctx.sources_only:
-            if any(ctx.needs_input_grad[i] for i in (3, 4, 5, 9)):
-                raise NotImplementedError(
-                    f"gradients w.r.t. delta_tau, single_scat_albedo, phase_moments and "
-                    f"surface_albedo need num_quadrature <= {REVERSE_MAX_QUADRATURE}; above it "
-                    "the reverse pass covers planck_levels, surface_emission and top_emission")
-            tau, ssa, mom = ctx.saved_tensors
-            cot = {k: g for k, g in zip(OUTPUT_NAMES, gouts) if g is not None}
-            gB, gS, gT = planck_source_vjp(tau, ssa, mom, ctx.config, cot,
-                                           max_workspace_bytes=ctx.max_ws)
-            hasB, hasBs, hasBt, _ = ctx.has
-            return (None, None, None, None, None, None, gB if hasB else None,
-                    gS if hasBs else None, gT if hasBt else None, None)
         tau, ssa, mom, B, Bs, Bt, *outs = ctx.saved_tensors
         out = dict(zip(OUTPUT_NAMES, outs))
         cot = {k: g for k, g in zip(OUTPUT_NAMES, gouts) if g is not None}
@@ -434,9 +408,8 @@ def solve(delta_tau, single_scat_albedo, phase_moments, config: BatchConfig, pla
     Gradients flow to delta_tau, single_scat_albedo, phase_moments,
     planck_levels, surface_emission, top_emission and ``surface_albedo`` (a
     0-dim tensor that replaces ``config.surface_albedo``) through the
-    hand-written reverse pass (num_quadrature <= 16); above 16 to
-    planck_levels, surface_emission and top_emission (so to temperatures
-    through ``planck_levels``) from forward solves with unit sources.  flux_direct,
+    hand-written reverse pass (every quadrature order; above 16 in FP64
+    only).  flux_direct,
     F0 mu0 exp(-tau_cum / mu0) with tau_cum the cumulative delta-M-scaled depth,
     carries gradients to delta_tau, single_scat_albedo and (through delta-M) the
     phase moments like the other outputs.
@@ -556,8 +529,7 @@ def solve_from_coefficients(absorption_coeff, scattering_coeff, altitude, temper
     """Differentiable solve from raw retrieval coefficients: ``raw_optics``
     chained with ``solve``.  The forward result equals
     ``solve_batch_from_coefficients`` on the same inputs; gradients flow to the
-    coefficients, cloud depths, temperatures, phase moments and surface_albedo
-    (above quadrature order 16: to the temperatures only, as ``solve``)."""
+    coefficients, cloud depths, temperatures, phase moments and surface_albedo."""
     r = raw_optics(absorption_coeff, scattering_coeff, altitude, temperature, wavenumber,
                    cloud_optical_depth, surface_temperature, top_temperature)
     return solve(r["delta_tau"], r["single_scat_albedo"], phase_moments, config,

@@ -432,12 +432,12 @@ def main():
                      "FP32 recomputed doubling + reverse (FP64 for the routed layers), "
                      "FP64 flop count", work["doubling_reverse"])}
     if a.forward_only:
-        names = ({1: "layer32_kernel<false>", 2: "adding32_kernel"} if npad == 32 else
+        names = ({1: "layer32_kernel<false>", 2: "adding32_kernel<false>"} if npad == 32 else
                  {1: "layer16_kernel<false, false>", 2: "adding16_kernel<false, false>"}
                  if npad == 16 or chosen != "f64" else
                  {1: f"layer_kernel<{npad}>", 2: f"adding_kernel<{npad}>"})
         if chosen == "mixed":
-            names = ({1: "layer32m_kernel + layer32_kernel<true>", 2: "adding32_kernel"}
+            names = ({1: "layer32m_kernel + layer32_kernel<true>", 2: "adding32_kernel<false>"}
                      if npad == 32 else
                      {1: "layer16m_kernel + layer16_kernel<true, false>",
                       2: "adding16_kernel<false, true>"})

@@ -69,7 +69,7 @@ extern "C" {
 #define ADRT_ERR_HIP 21                /* a HIP runtime call failed */
 #define ADRT_ERR_PRECISION 22          /* unknown precision value */
 #define ADRT_ERR_NULL 23               /* NULL argument */
-#define ADRT_ERR_ADJOINT 24            /* workspace not created with ADRT_WORKSPACE_ADJOINT, or N > 16 */
+#define ADRT_ERR_ADJOINT 24            /* workspace not created with ADRT_WORKSPACE_ADJOINT */
 #define ADRT_ERR_DOUBLING_STEPS 25     /* a layer needs more doubling steps than the reverse pass holds */
 #define ADRT_ERR_SCRATCH 26            /* adrt_solve_batch_noalloc: one column exceeds its scratch */
 
@@ -149,7 +149,7 @@ typedef struct adrt_device_gradients {
 typedef struct adrt_workspace adrt_workspace;
 
 /* Workspace flags. */
-#define ADRT_WORKSPACE_ADJOINT 1  /* keep what the reverse pass needs (N <= 16) */
+#define ADRT_WORKSPACE_ADJOINT 1  /* keep what the reverse pass needs */
 #define ADRT_WORKSPACE_JACOBIAN 2 /* room for adrt_source_jacobian_batch (not with ADJOINT) */
 
 /* Create a workspace for columns of this shape (num_layers, num_quadrature,
@@ -158,8 +158,9 @@ typedef struct adrt_workspace adrt_workspace;
 int adrt_workspace_create(const adrt_batch_config* cfg, size_t max_bytes,
                           adrt_workspace** ws);
 /* As adrt_workspace_create, with flags (ADRT_WORKSPACE_ADJOINT).  With the
- * adjoint flag the columns are solved at quadrature order 16 (orders below 16
- * padded with zero-weight nodes, which leaves the results unchanged) and the
+ * adjoint flag the columns are solved at quadrature order 16 or, for orders
+ * 17..32, 32 (lower orders padded with zero-weight nodes, which leaves the
+ * results unchanged) and the
  * workspace also holds interface intensities, per-layer gradient terms and
  * the doubling checkpoints of the reverse pass. */
 int adrt_workspace_create_ex(const adrt_batch_config* cfg, size_t max_bytes, int flags,
@@ -239,8 +240,10 @@ int adrt_workspace_generation(const adrt_workspace* ws, unsigned long long* gene
  * chunk's forward is recomputed (checkpointing: layer records and interface
  * intensities are kept per chunk, doubling steps are always recomputed).
  * The reference has no adjoint; its derivative feature is the forward-mode
- * temperature Jacobian of src/solver.cpp:117-341.  Asynchronous on stream;
- * adrt_adjoint_status reports layers that exceeded the checkpoint depth. */
+ * temperature Jacobian of src/solver.cpp:117-341.  Every quadrature order;
+ * orders 17..32 in ADRT_PRECISION_F64 only (else ADRT_ERR_PRECISION).
+ * Asynchronous on stream; adrt_adjoint_status reports layers that exceeded
+ * the checkpoint depth (64 doubling steps up to order 16, 48 above). */
 int adrt_adjoint_batch(const adrt_batch_config* cfg, const adrt_device_inputs* in,
                        const adrt_device_outputs* out, const adrt_device_cotangents* cot,
                        const adrt_device_gradients* grad, adrt_precision precision,

@@ -373,11 +373,11 @@ def test_temperature_gradient_through_device_planck():
 
 
 @pytest.mark.parametrize("n,diffusion", [(32, False), (32, True)])
-def test_source_gradients_above_order_16(n, diffusion):
-    """Above order 16 solve() differentiates the thermal sources from forward
-    solves with unit sources.  Every output is affine in the sources, so central
-    differences of the GPU forward with a unit step are exact up to rounding;
-    the other inputs refuse gradients there."""
+def test_source_gradients_order_32(n, diffusion):
+    """At order 32 the thermal-source gradients of solve() (the N = 32 reverse
+    pass) against central differences of the GPU forward with a unit step,
+    exact up to rounding since every output is affine in the sources, and
+    against the source-Jacobian route (planck_source_vjp)."""
     W, L = 3, 5
     tau, ssa, mom, B = smooth_batch(n, W=W, L=L, seed=31)
     rng = np.random.default_rng(4)
@@ -414,10 +414,11 @@ def test_source_gradients_above_order_16(n, diffusion):
             ref = fd(which, w)
             got = g.cpu().numpy()[w]
             assert abs(got - ref) <= 1e-10 * max(abs(ref), 1.0), (which, w, got, ref)
-    xtau = t(tau).requires_grad_(True)
-    out = solve(xtau, t(ssa), t(mom), cfg, planck_levels=t(B))
-    with pytest.raises(NotImplementedError):
-        out["flux_up"].sum().backward()
+    from addingdoublingrt_amd.solver import planck_source_vjp
+    vB, vS, vT = planck_source_vjp(t(tau), t(ssa), t(mom), cfg, {k: t(v) for k, v in cot.items()})
+    np.testing.assert_allclose(gB, vB.cpu().numpy(), rtol=1e-10, atol=1e-12 * np.abs(gB).max())
+    np.testing.assert_allclose(xS.grad.cpu().numpy(), vS.cpu().numpy(), rtol=1e-10, atol=1e-12)
+    np.testing.assert_allclose(xT.grad.cpu().numpy(), vT.cpu().numpy(), rtol=1e-10, atol=1e-12)
 
 
 def test_autograd_net_flux_with_direct_beam():

@@ -1,0 +1,208 @@
+"""The reverse pass at quadrature orders 17..32 (adrt_adj32.hpp) vs the CPU oracle.
+
+The protocol of tests/test_gpu_adjoint.py, through the C-ABI:
+
+* central differences of the extended-precision oracle (relative step 1e-6,
+  bar 1e-4 with the same floors) for every differentiable input: solar +
+  thermal sources at N = 32 with per-column moments; delta-M with shared
+  moments and bottom-first indexing at N = 24 (padded to 32 with zero-weight
+  nodes); a column with a purely absorbing layer and a diffusion lower
+  boundary (tau and Planck values: omega = 0 is a kink of the forward);
+  near-conservative thick layers as config 3 draws them (1 - omega to 1e-3,
+  tau to 60, 64 HG moments; tau gradients);
+* the exact forward-mode temperature Jacobian of the oracle (u^T J against
+  the adjoint gradient chained with the device dB/dT, 1e-8) at N = 32 and
+  N = 20 with delta-M; on columns of the 8192 x 80 config-3 workload, whose
+  near-conservative thick layers leave the FP64 oracle's own Jacobian some
+  1e-7 from the extended-precision one, within the parity tests' accuracy
+  class against the extended-precision oracle;
+* the forward on an N = 32 reverse-pass workspace bit for bit equal to the
+  plain forward, and torch.autograd (adrt.solve) equal to adjoint_batch.
+
+Inputs stay away from the non-differentiable points of the forward (tau at
+a power of two, omega at the 0.01 / 0.1 thresholds) and from layers with an
+unstable doubling start (omega < 0.01), which tests/test_gpu_adjoint.py
+covers at N <= 16.
+"""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+import oracle  # noqa: E402
+from addingdoublingrt_amd import BatchConfig, adjoint_batch, solve, solve_batch, workloads  # noqa: E402
+from addingdoublingrt_amd.solver import planck_levels  # noqa: E402
+from test_gpu_adjoint import (NAMES, REL, albedo_fd, assert_close, fd_grads, gpu_grads,  # noqa: E402
+                              random_cot, t)
+
+pytestmark = pytest.mark.gpu
+
+
+def batch32(W, L, seed, M, per_column=True, g_range=(0.3, 0.85)):
+    rng = np.random.default_rng(seed)
+    tau = 10.0 ** rng.uniform(-2.0, 1.0, (W, L))
+    frac = np.log2(tau) % 1.0
+    tau = np.where((frac < 1e-3) | (frac > 1 - 1e-3), tau * 1.01, tau)
+    ssa = rng.uniform(0.15, 0.97, (W, L))
+    g = rng.uniform(*g_range, (W, L) if per_column else (L,))
+    mom = g[..., None] ** np.arange(M)
+    B = rng.uniform(0.5, 2.0, (W, L + 1))
+    return tau, ssa, mom, B
+
+
+def check_fd(tau, ssa, mom, B, Bs, Bt, cfg, which, label, seed=5, albedo=True):
+    W, L = tau.shape
+    keys = ("flux_up", "flux_down", "mean_intensity", "flux_direct")
+    cot = random_cot(W, L, seed, keys)
+    g, _ = gpu_grads(tau, ssa, mom, cfg, cot, B, Bs, Bt)
+    fd, nz = fd_grads(tau, ssa, mom, cfg, cot, B, Bs, Bt, which=which)
+    for k, v in fd.items():
+        assert_close(g[NAMES[k]], v, f"{label} d/d{NAMES[k]}", noise=nz[k])
+    if albedo and not cfg.use_diffusion_lower_bc and cfg.surface_albedo > 0.0:
+        gA, nA = albedo_fd(tau, ssa, mom, cfg, cot, B, Bs, Bt)
+        got = g["surface_albedo"].sum()
+        assert abs(got - gA) <= REL * abs(gA) + nA, (got, gA, nA)
+
+
+def test_adjoint32_vs_fd_solar_thermal():
+    tau, ssa, mom, B = batch32(W=3, L=5, seed=321, M=12)
+    ssa[0, 1] = 0.05  # ipow0 = 10 branch
+    cfg = BatchConfig(num_quadrature=32, surface_albedo=0.25, solar_flux=1.5, solar_mu=0.55)
+    check_fd(tau, ssa, mom, B, B[:, -1] * 1.1, B[:, 0] * 0.9, cfg,
+             ("tau", "ssa", "mom", "B", "Bs", "Bt"), "N=32 solar + thermal")
+
+
+def test_adjoint24_vs_fd_delta_m_shared_reversed():
+    n = 24
+    # delta-M fraction f = g^48 from 1.3e-4 up: above the 1e-12 threshold of the
+    # reference's delta-M switch by far more than the difference step
+    tau, ssa, mom, B = batch32(W=3, L=4, seed=24, M=2 * n + 3, per_column=False,
+                               g_range=(0.83, 0.9))
+    cfg = BatchConfig(num_quadrature=n, surface_albedo=0.2, solar_flux=1.0, solar_mu=0.5,
+                      use_delta_m=True, index_from_bottom=True)
+    check_fd(tau, ssa, mom, B, B[:, -1] * 1.05, B[:, 0] * 0.8, cfg,
+             ("tau", "ssa", "mom", "B", "Bs", "Bt"), "N=24 delta-M shared moments reversed")
+
+
+def test_adjoint32_vs_fd_absorbing_layer_diffusion():
+    tau, ssa, mom, B = batch32(W=3, L=5, seed=77, M=10)
+    ssa[1, 2] = 0.0  # purely absorbing
+    ssa[2, 0] = 0.0
+    cfg = BatchConfig(num_quadrature=32, use_diffusion_lower_bc=True, solar_flux=0.8,
+                      solar_mu=0.9)
+    check_fd(tau, ssa, mom, B, None, B[:, 0] * 0.9, cfg, ("tau", "B", "Bt"),
+             "N=32 absorbing layers, diffusion boundary")
+
+
+def test_adjoint32_vs_fd_near_conservative_thick():
+    """Layers as config 3 draws them, milder: 1 - omega in [1e-3, 1e-1], tau up to 60,
+    64 HG moments g in [0.6, 0.95]; many doubling steps per layer."""
+    rng = np.random.default_rng(33)
+    W, L = 2, 5
+    tau = 10.0 ** rng.uniform(-1.0, np.log10(60.0), (W, L))
+    frac = np.log2(tau) % 1.0
+    tau = np.where((frac < 1e-3) | (frac > 1 - 1e-3), tau * 1.01, tau)
+    ssa = 1.0 - 10.0 ** rng.uniform(-3.0, -1.0, (W, L))
+    g = rng.uniform(0.6, 0.95, (W, L))
+    mom = g[..., None] ** np.arange(64)
+    B = np.linspace(1.0, 2.0, L + 1)[None, :] * rng.uniform(0.9, 1.1, (W, 1))
+    cfg = BatchConfig(num_quadrature=32, surface_albedo=0.1, solar_flux=1.0, solar_mu=0.5)
+    check_fd(tau, ssa, mom, B, B[:, -1].copy(), B[:, 0].copy(), cfg, ("tau", "B"),
+             "N=32 near-conservative thick layers", albedo=False)
+
+
+def _jacobian_check(n, tau, ssa, mom, T, Ts, lo, hi, cfg, cot, label, classed=False):
+    """u^T J_T from the reverse pass against the oracle's forward-mode Jacobian:
+    at 1e-8 relative, or (classed) within the accuracy class of the parity
+    tests, max(1e-8, 4 x the FP64 oracle's own error) against the
+    extended-precision oracle, for columns whose FP64 conditioning is the limit."""
+    W, L = tau.shape
+    B = planck_levels(t(lo), t(hi), t(T)).cpu().numpy()
+    dBdT = planck_levels(t(lo), t(hi), t(T), derivative=True).cpu().numpy()
+    Bs = np.array([oracle.planck(lo[w], hi[w], Ts[w]) for w in range(W)])
+    dBsdT = np.array([oracle.planck_deriv(lo[w], hi[w], Ts[w]) for w in range(W)])
+    g, _ = gpu_grads(tau, ssa, mom, cfg, cot, B, Bs, B[:, 0].copy())
+    worst = 0.0
+    def ujac(w, extended):
+        r = oracle.solve(L, n, tau[w], ssa[w], mom[w], temperature=T[w], use_thermal_emission=True,
+                         use_delta_m=cfg.use_delta_m, compute_temperature_jacobian=True,
+                         surface_albedo=cfg.surface_albedo, surface_temperature=Ts[w],
+                         wavenumber_low=lo[w], wavenumber_high=hi[w],
+                         use_diffusion_lower_bc=cfg.use_diffusion_lower_bc, extended=extended)
+        return sum(cot[k][w] @ r[k + "_temperature_jac"] for k in cot)
+
+    for w in range(W):
+        ref = ujac(w, classed)
+        mine = g["planck_levels"][w] * dBdT[w]
+        mine[0] += g["top_emission"][w] * dBdT[w, 0]
+        mine = np.concatenate([mine, [g["surface_emission"][w] * dBsdT[w]]])
+        scale = np.abs(ref).max()
+        rel = lambda x: np.abs(x - ref) / (np.abs(ref) + 1e-10 * scale)
+        err = rel(mine)
+        bound = 1e-8
+        if classed:
+            e64 = rel(ujac(w, False)).max()
+            bound = max(1e-8, 4.0 * e64)
+            print(f"{label} column {w}: GPU {err.max():.2e}, FP64 oracle {e64:.2e} "
+                  f"(vs extended)")
+        worst = max(worst, err.max())
+        assert err.max() < bound, (label, w, err.max(), bound)
+    print(f"{label}: u^T J_T rel err {worst:.2e}")
+
+
+@pytest.mark.parametrize("n,delta_m", [(32, False), (20, True)])
+def test_adjoint32_matches_oracle_temperature_jacobian(n, delta_m):
+    rng = np.random.default_rng(17 + n)
+    W, L = 3, 7
+    tau, ssa, mom, _ = batch32(W, L, seed=5 + n, M=2 * n + (3 if delta_m else 0))
+    ssa[1, 3] = 0.0  # a purely absorbing layer
+    T = np.linspace(320.0, 200.0, L + 1)[None, :] + rng.normal(0, 5, (W, L + 1))
+    Ts = rng.uniform(300, 320, W)
+    lo = np.linspace(500, 1500, W)
+    hi = lo + 40.0
+    cfg = BatchConfig(num_quadrature=n, surface_albedo=0.1, use_delta_m=delta_m)
+    cot = random_cot(W, L, 99, keys=("flux_up", "flux_down", "mean_intensity", "flux_divergence"))
+    _jacobian_check(n, tau, ssa, mom, T, Ts, lo, hi, cfg, cot, f"N={n} delta_m={delta_m}")
+
+
+def test_adjoint32_temperature_jacobian_config3_columns():
+    """Two columns of the 8192 x 80 config-3 workload (1 - omega down to 1e-6, tau
+    to 100, 64 HG moments, diffusion lower boundary), thermal only."""
+    cols = workloads.make(3, W=2)
+    W, L = cols.W, cols.L
+    T = np.linspace(300.0, 180.0, L + 1)[None, :].repeat(W, 0)
+    lo = np.array([800.0, 1700.0])
+    hi = lo + 25.0
+    cfg = BatchConfig(num_quadrature=cols.N, use_diffusion_lower_bc=True)
+    cot = random_cot(W, L, 3)
+    _jacobian_check(cols.N, cols.delta_tau, cols.single_scat_albedo, cols.phase_moments, T,
+                    T[:, -1].copy(), lo, hi, cfg, cot, "config 3 columns", classed=True)
+
+
+def test_adjoint32_workspace_forward_is_plain_forward():
+    tau, ssa, mom, B = batch32(W=6, L=8, seed=8, M=64)
+    cfg = BatchConfig(num_quadrature=32, surface_albedo=0.3, solar_flux=2.0, solar_mu=0.7)
+    args = (t(tau), t(ssa), t(mom), cfg, t(B), t(B[:, -1].copy()), t(B[:, 0].copy()))
+    a = solve_batch(*args, adjoint_workspace=True)
+    a.pop("_generation")
+    b = solve_batch(*args)
+    torch.cuda.synchronize()
+    for k in b:
+        assert torch.equal(a[k], b[k]), k
+
+
+def test_adjoint32_autograd_matches_adjoint_batch():
+    tau, ssa, mom, B = batch32(W=4, L=6, seed=12, M=64)
+    cfg = BatchConfig(num_quadrature=32, surface_albedo=0.2, solar_flux=1.0, solar_mu=0.6)
+    Bs, Bt = B[:, -1].copy(), B[:, 0].copy()
+    cot = random_cot(4, 6, 21)
+    g_ref, _ = gpu_grads(tau, ssa, mom, cfg, cot, B, Bs, Bt)
+    x = {k: t(v).requires_grad_() for k, v in
+         dict(tau=tau, ssa=ssa, mom=mom, B=B, Bs=Bs, Bt=Bt).items()}
+    A = torch.tensor(cfg.surface_albedo, dtype=torch.float64, device="cuda", requires_grad=True)
+    out = solve(x["tau"], x["ssa"], x["mom"], cfg, x["B"], x["Bs"], x["Bt"], surface_albedo=A)
+    loss = sum((out[k] * t(v)).sum() for k, v in cot.items())
+    loss.backward()
+    for k, name in NAMES.items():
+        np.testing.assert_allclose(x[k].grad.cpu().numpy(), g_ref[name], rtol=1e-12, atol=1e-300)
+    np.testing.assert_allclose(A.grad.item(), g_ref["surface_albedo"].sum(), rtol=1e-12)
