@@ -340,7 +340,7 @@ def main():
     inputs = (d["delta_tau"], d["single_scat_albedo"], d["phase_moments"], cfg)
     kw = dict(planck_levels=d["planck_levels"], surface_emission=d["surface_emission"],
               top_emission=d["top_emission"])
-    reverse_ok = cols.N <= 16
+    reverse_ok = cols.N <= 16  # the reduced-precision reverse forms run up to order 16
     npad = 1 << max(cols.N - 1, 1).bit_length()  # the library's padded order
     if a.form == "auto":
         forms = ["f64"] + (["mixed", "mixed_reverse"] if reverse_ok else [])
@@ -413,6 +413,11 @@ def main():
                      "phase setup + doubling (forward), two layers per wave", work["doubling"]),
                  4: ("layer16p_adjoint_kernel + layer16_adjoint_kernel<false, true>",
                      "recomputed doubling + its reverse (reverse counted), two layers per wave",
+                     work["doubling_reverse"])}
+    elif chosen == "f64" and npad == 32:
+        model = {1: ("layer32_kernel<false>", "phase setup + doubling (forward)", work["doubling"]),
+                 4: ("layer32_adjoint_kernel",
+                     "recomputed doubling + its reverse (reverse counted)",
                      work["doubling_reverse"])}
     elif chosen == "f64":
         model = {1: ("layer16_kernel<false, false>", "phase setup + doubling (forward)", work["doubling"]),
