@@ -132,15 +132,17 @@ __device__ __forceinline__ float residual_sizef(const MatF& E) {
 }
 
 // (X v) for NV vectors in FP32 slots of 32 values (v + 32 q), X an FP32 tile:
-// elements j and 16 + j, accumulated in FP32
+// elements j and 16 + j, accumulated in FP32, split over the lane groups as
+// matvecs
 template <int NV>
 __device__ __forceinline__ void matvecsf(const float* X, const float* v, const Lane& L,
                                          float (&lo)[NV], float (&hi)[NV]) {
   fence_sched();
 #pragma unroll
   for (int q = 0; q < NV; ++q) lo[q] = hi[q] = 0.0f;
-#pragma unroll 4
-  for (int k = 0; k < N; ++k) {
+#pragma unroll
+  for (int kk = 0; kk < 8; ++kk) {
+    const int k = 8 * L.h + kk;
     const float xl = X[k * TS + L.j], xh = X[k * TS + 16 + L.j];
 #pragma unroll
     for (int q = 0; q < NV; ++q) {
@@ -148,6 +150,11 @@ __device__ __forceinline__ void matvecsf(const float* X, const float* v, const L
       lo[q] = __builtin_fmaf(xl, vk, lo[q]);
       hi[q] = __builtin_fmaf(xh, vk, hi[q]);
     }
+  }
+#pragma unroll
+  for (int q = 0; q < NV; ++q) {
+    lo[q] = w16::col_reduce<float>(lo[q], L);
+    hi[q] = w16::col_reduce<float>(hi[q], L);
   }
   fence_sched();
 }

@@ -243,28 +243,41 @@ __device__ __forceinline__ void eye_minus_mul(const R* X, const Mat<R>& Y, const
   for (int r = 0; r < 4; ++r) C.v[r] = ((row<R>(L.h, r) == L.j) ? R(1) : R(0)) - P.v[r];
 }
 
-// (X v)_j for X in a tile and v in an LDS slot of 16 values
+template <typename R>
+__device__ __forceinline__ R col_reduce(R p, const Lane& L);
+
+// (X v)_j for X in a tile and v in an LDS slot of 16 values: lane group h
+// takes k = 4h..4h+3, the four partial sums are added across the column
 template <typename R>
 __device__ __forceinline__ R matvec(const R* X, const R* v, const Lane& L) {
   fence_sched();
   R s = R(0);
-#pragma unroll 4
-  for (int k = 0; k < N; ++k) s = fma_(X[k * TS + L.j], v[k], s);
+#pragma unroll
+  for (int kk = 0; kk < 4; ++kk) {
+    const int k = 4 * L.h + kk;
+    s = fma_(X[k * TS + L.j], v[k], s);
+  }
+  s = col_reduce<R>(s, L);
   fence_sched();
   return s;
 }
 
+// (X v_q)_j for NV vectors in LDS slots of 16 values, v + 16 q, split over
+// the lane groups as matvec
 template <typename R, int NV>
 __device__ __forceinline__ void matvecs(const R* X, const R* v, const Lane& L, R (&s)[NV]) {
   fence_sched();
 #pragma unroll
   for (int q = 0; q < NV; ++q) s[q] = R(0);
-#pragma unroll 4
-  for (int k = 0; k < N; ++k) {
+#pragma unroll
+  for (int kk = 0; kk < 4; ++kk) {
+    const int k = 4 * L.h + kk;
     const R x = X[k * TS + L.j];
 #pragma unroll
     for (int q = 0; q < NV; ++q) s[q] = fma_(x, v[q * N + k], s[q]);
   }
+#pragma unroll
+  for (int q = 0; q < NV; ++q) s[q] = col_reduce<R>(s[q], L);
   fence_sched();
 }
 

@@ -128,15 +128,17 @@ __device__ __forceinline__ void mul_add(const double* X, const Mat& Y, const Mat
   mul_impl<true>(X, Y, Z, L, C);
 }
 
-// (X v) for NV vectors stored in LDS slots of 32 values, v + 32 q
+// (X v) for NV vectors stored in LDS slots of 32 values, v + 32 q: lane group
+// h takes k = 8h..8h+7, the four partial sums are added across the column
 template <int NV>
 __device__ __forceinline__ void matvecs(const double* X, const double* v, const Lane& L,
                                         Vec (&s)[NV]) {
   fence_sched();
 #pragma unroll
   for (int q = 0; q < NV; ++q) s[q] = Vec{0.0, 0.0};
-#pragma unroll 4
-  for (int k = 0; k < N; ++k) {
+#pragma unroll
+  for (int kk = 0; kk < 8; ++kk) {
+    const int k = 8 * L.h + kk;
     const double xl = X[k * TS + L.j], xh = X[k * TS + 16 + L.j];
 #pragma unroll
     for (int q = 0; q < NV; ++q) {
@@ -145,6 +147,9 @@ __device__ __forceinline__ void matvecs(const double* X, const double* v, const 
       s[q].hi = fma_(xh, vk, s[q].hi);
     }
   }
+#pragma unroll
+  for (int q = 0; q < NV; ++q)
+    s[q] = Vec{w16::col_reduce<double>(s[q].lo, L), w16::col_reduce<double>(s[q].hi, L)};
   fence_sched();
 }
 
