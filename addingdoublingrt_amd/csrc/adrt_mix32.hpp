@@ -120,15 +120,12 @@ __device__ __forceinline__ void mul64_f32(const double* X, const MatF& Y, const 
 
 // N max |E_ij| over the wave, wave-uniform
 __device__ __forceinline__ float residual_sizef(const MatF& E) {
-  float m = 0.0f;
+  unsigned m = 0u;
 #pragma unroll
   for (int b = 0; b < 4; ++b)
 #pragma unroll
-    for (int r = 0; r < 4; ++r) m = __builtin_fmaxf(m, __builtin_fabsf(E.v[b][r]));
-  m = row_max<float>(m);
-  m = __builtin_fmaxf(__builtin_fmaxf(readlane(m, 0), readlane(m, 16)),
-                      __builtin_fmaxf(readlane(m, 32), readlane(m, 48)));
-  return m * (float)N;
+    for (int r = 0; r < 4; ++r) m = max(m, w16::abs_bits(E.v[b][r]));
+  return w16::wave_max_abs_bits(m) * (float)N;
 }
 
 // (X v) for NV vectors in FP32 slots of 32 values (v + 32 q), X an FP32 tile:

@@ -426,16 +426,31 @@ __device__ __forceinline__ void right_solve(Mat<R>& A, Mat<R>& T, const Lane& L)
   fence_sched();
 }
 
+// the largest over the wave of |x| for the FP32 bit patterns u = bits(|x|) of
+// each lane, in every lane: the bit patterns of non-negative floats order as
+// unsigned integers as the values do (a NaN, above +inf, wins)
+__device__ __forceinline__ float wave_max_abs_bits(unsigned u) {
+  u = max(u, (unsigned)__builtin_amdgcn_update_dpp(0, (int)u, 0xB1, 0xF, 0xF, false));
+  u = max(u, (unsigned)__builtin_amdgcn_update_dpp(0, (int)u, 0x4E, 0xF, 0xF, false));
+  u = max(u, (unsigned)__builtin_amdgcn_update_dpp(0, (int)u, 0x141, 0xF, 0xF, false));
+  u = max(u, (unsigned)__builtin_amdgcn_update_dpp(0, (int)u, 0x140, 0xF, 0xF, false));
+  const unsigned a = (unsigned)__builtin_amdgcn_readlane((int)u, 0);
+  const unsigned b = (unsigned)__builtin_amdgcn_readlane((int)u, 16);
+  const unsigned c = (unsigned)__builtin_amdgcn_readlane((int)u, 32);
+  const unsigned d = (unsigned)__builtin_amdgcn_readlane((int)u, 48);
+  return __builtin_bit_cast(float, max(max(a, b), max(c, d)));
+}
+__device__ __forceinline__ unsigned abs_bits(float x) {
+  return __builtin_bit_cast(unsigned, __builtin_fabsf(x));
+}
+
 // N max |E_ij| over the wave (bounds the infinity norm of E), wave-uniform
 template <typename R>
 __device__ __forceinline__ float residual_size(const Mat<R>& E) {
-  float m = 0.0f;
+  unsigned m = 0u;
 #pragma unroll
-  for (int r = 0; r < 4; ++r) m = __builtin_fmaxf(m, __builtin_fabsf((float)E.v[r]));
-  m = row_max<float>(m);
-  m = __builtin_fmaxf(__builtin_fmaxf(readlane(m, 0), readlane(m, 16)),
-                      __builtin_fmaxf(readlane(m, 32), readlane(m, 48)));
-  return m * (float)N;
+  for (int r = 0; r < 4; ++r) m = max(m, abs_bits((float)E.v[r]));
+  return wave_max_abs_bits(m) * (float)N;
 }
 
 // Residual sizes at which the Newton-Schulz inverse stops: the iterate is

@@ -162,15 +162,12 @@ __device__ __forceinline__ void put_vec(double* slot, const Lane& L, const Vec& 
 
 // N max |E_ij| over the wave, wave-uniform
 __device__ __forceinline__ float residual_size(const Mat& E) {
-  float m = 0.0f;
+  unsigned m = 0u;
 #pragma unroll
   for (int b = 0; b < 4; ++b)
 #pragma unroll
-    for (int r = 0; r < 4; ++r) m = __builtin_fmaxf(m, __builtin_fabsf((float)E.v[b][r]));
-  m = row_max<float>(m);
-  m = __builtin_fmaxf(__builtin_fmaxf(readlane(m, 0), readlane(m, 16)),
-                      __builtin_fmaxf(readlane(m, 32), readlane(m, 48)));
-  return m * (float)N;
+    for (int r = 0; r < 4; ++r) m = max(m, w16::abs_bits((float)E.v[b][r]));
+  return w16::wave_max_abs_bits(m) * (float)N;
 }
 
 // X = (I - P)^-1 by Newton-Schulz for P in registers and in the tile tP (tX:
