@@ -206,3 +206,43 @@ def test_adjoint32_autograd_matches_adjoint_batch():
     for k, name in NAMES.items():
         np.testing.assert_allclose(x[k].grad.cpu().numpy(), g_ref[name], rtol=1e-12, atol=1e-300)
     np.testing.assert_allclose(A.grad.item(), g_ref["surface_albedo"].sum(), rtol=1e-12)
+
+
+@pytest.mark.parametrize("n,tau_big", [(16, 1e15), (32, 1e10)])
+def test_adjoint_reports_layers_past_checkpoint_depth(n, tau_big):
+    """A layer needing more doubling steps than the checkpoint slab holds (64
+    up to order 16: from tau = 2^49 at omega >= 0.1; 48 above: from tau = 2^33) gets
+    zero gradients and is reported by adrt_adjoint_status (ADRT_ERR_DOUBLING_STEPS,
+    25); the other columns' gradients are those of a batch without it."""
+    from addingdoublingrt_amd._lib import AdrtError
+    W, L = 3, 4
+    tau, ssa, mom, B = batch32(W, L, seed=50 + n, M=2 * n)
+    ssa[1, 2] = 0.5
+    tau[1, 2] = tau_big
+    cfg = BatchConfig(num_quadrature=n, surface_albedo=0.2, solar_flux=1.0, solar_mu=0.6)
+    cot = random_cot(W, L, 8)
+    args = (t(tau), t(ssa), t(mom), cfg)
+    kw = dict(planck_levels=t(B))
+    out = solve_batch(*args, **kw, adjoint_workspace=True)
+    gen = out.pop("_generation")
+    with pytest.raises(AdrtError) as e:
+        adjoint_batch(*args, out, {k: t(v) for k, v in cot.items()}, **kw,
+                      forward_generation=gen)
+    assert e.value.code == 25
+    g = adjoint_batch(*args, out, {k: t(v) for k, v in cot.items()}, **kw,
+                      forward_generation=0, check_status=False)
+    # the flag stays set until read: read (and clear) it on the cached workspace
+    from addingdoublingrt_amd import _lib
+    from addingdoublingrt_amd.solver import get_workspace
+    ws = get_workspace(W, L, n, mom.shape[-1], 0, True)
+    stream = _lib.C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    assert _lib.lib().adrt_adjoint_status(ws.handle, stream) == 25
+    g = {k: v.cpu().numpy() for k, v in g.items() if v is not None}
+    assert g["delta_tau"][1, 2] == 0.0 and g["single_scat_albedo"][1, 2] == 0.0
+    assert np.all(g["phase_moments"][1, 2] == 0.0)
+    # the same batch without the thick layer's column
+    keep = [0, 2]
+    ref, _ = gpu_grads(tau[keep], ssa[keep], mom[keep], cfg, {k: v[keep] for k, v in cot.items()},
+                       B[keep])
+    for k in ("delta_tau", "single_scat_albedo", "phase_moments", "planck_levels"):
+        np.testing.assert_allclose(g[k][keep], ref[k], rtol=1e-12, atol=1e-300, err_msg=k)
