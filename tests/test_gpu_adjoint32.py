@@ -246,3 +246,31 @@ def test_adjoint_reports_layers_past_checkpoint_depth(n, tau_big):
                        B[keep])
     for k in ("delta_tau", "single_scat_albedo", "phase_moments", "planck_levels"):
         np.testing.assert_allclose(g[k][keep], ref[k], rtol=1e-12, atol=1e-300, err_msg=k)
+
+
+@pytest.mark.parametrize("case", ["divergence_scaling", "no_surface"])
+def test_adjoint32_vs_fd_options(case):
+    """Options of tests/test_gpu_adjoint.py at N = 32: divergence cotangents with
+    spectrum scaling, and a cold black surface (no surface term)."""
+    tau, ssa, mom, B = batch32(W=3, L=4, seed=sum(map(ord, case)), M=14)
+    W, L = tau.shape
+    if case == "divergence_scaling":
+        cfg = BatchConfig(num_quadrature=32, surface_albedo=0.1, solar_flux=1.0, solar_mu=0.7,
+                          spectrum_scaling=0.6)
+        Bs = B[:, -1] * 1.1
+        keys = ("flux_up", "flux_down", "mean_intensity", "flux_direct", "flux_divergence")
+    else:
+        cfg = BatchConfig(num_quadrature=32, solar_flux=1.0, solar_mu=0.5, surface_emission=0.0)
+        Bs = None
+        keys = ("flux_up", "flux_down", "mean_intensity", "flux_direct")
+    Bt = B[:, 0] * 0.9
+    cot = random_cot(W, L, 6, keys)
+    g, _ = gpu_grads(tau, ssa, mom, cfg, cot, B, Bs, Bt)
+    which = ("tau", "ssa", "mom", "B", "Bt") + (() if Bs is None else ("Bs",))
+    fd, nz = fd_grads(tau, ssa, mom, cfg, cot, B, Bs, Bt, which=which)
+    for k, v in fd.items():
+        assert_close(g[NAMES[k]], v, f"N=32 {case} d/d{NAMES[k]}", noise=nz[k])
+    if cfg.surface_albedo > 0.0:
+        gA, nA = albedo_fd(tau, ssa, mom, cfg, cot, B, Bs, Bt)
+        got = g["surface_albedo"].sum()
+        assert abs(got - gA) <= REL * abs(gA) + nA, (got, gA, nA)
