@@ -9,7 +9,8 @@ The protocol of tests/test_gpu_adjoint.py, through the C-ABI:
   nodes); a column with a purely absorbing layer and a diffusion lower
   boundary (tau and Planck values: omega = 0 is a kink of the forward);
   near-conservative thick layers as config 3 draws them (1 - omega to 1e-3,
-  tau to 60, 64 HG moments; tau gradients);
+  tau to 60, 64 HG moments; tau gradients); divergence cotangents with
+  spectrum scaling; no surface;
 * the exact forward-mode temperature Jacobian of the oracle (u^T J against
   the adjoint gradient chained with the device dB/dT, 1e-8) at N = 32 and
   N = 20 with delta-M; on columns of the 8192 x 80 config-3 workload, whose
@@ -17,7 +18,9 @@ The protocol of tests/test_gpu_adjoint.py, through the C-ABI:
   1e-7 from the extended-precision one, within the parity tests' accuracy
   class against the extended-precision oracle;
 * the forward on an N = 32 reverse-pass workspace bit for bit equal to the
-  plain forward, and torch.autograd (adrt.solve) equal to adjoint_batch.
+  plain forward, and torch.autograd (adrt.solve) equal to adjoint_batch;
+* layers past the checkpoint depth (orders 16 and 32): zero gradients,
+  reported by adrt_adjoint_status, the other columns unchanged.
 
 Inputs stay away from the non-differentiable points of the forward (tau at
 a power of two, omega at the 0.01 / 0.1 thresholds) and from layers with an
