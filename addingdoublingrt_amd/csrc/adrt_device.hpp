@@ -168,31 +168,39 @@ __device__ __forceinline__ void eye_minus_mul(const R* X, const R (&y)[Geo<N>::R
 }
 
 // Distributed mat-vecs with the matrix in an LDS tile and NV vectors in
-// consecutive LDS slots of N values (v + q*N): s[q] = (X v_q)_i for row i (both
-// lanes of a column compute it).  Pipelined like mul() so that the loads of
-// several mat-vecs are not hoisted together.
+// consecutive LDS slots of N values (v + q*N): s[q] = (X v_q)_i for row i, in
+// both lanes of column i.  With two lanes per column each sums half of the k
+// range and the halves are added across the pair (the two lanes add the same
+// two numbers).  Pipelined like mul() so that the loads of several mat-vecs
+// are not hoisted together.
 template <int N, int NV, typename R>
 __device__ __forceinline__ void matvecs(const R* X, const R* v, int i, R (&s)[NV]) {
-  R x = X[i], vv[NV];
+  constexpr int H = Geo<N>::H, KN = N / H;
+  const int k0 = H == 1 ? 0 : (int)(threadIdx.x % H) * KN;
+  R x = X[k0 * N + i], vv[NV];
 #pragma unroll
-  for (int q = 0; q < NV; ++q) { s[q] = R(0); vv[q] = v[q * N]; }
+  for (int q = 0; q < NV; ++q) { s[q] = R(0); vv[q] = v[q * N + k0]; }
 #pragma unroll
-  for (int k = 0; k < N; ++k) {
+  for (int kk = 0; kk < KN; ++kk) {
     R xn = R(0), vn[NV];
-    if (k + 1 < N) {
-      xn = X[(k + 1) * N + i];
+    if (kk + 1 < KN) {
+      xn = X[(k0 + kk + 1) * N + i];
 #pragma unroll
-      for (int q = 0; q < NV; ++q) vn[q] = v[q * N + k + 1];
+      for (int q = 0; q < NV; ++q) vn[q] = v[q * N + k0 + kk + 1];
     }
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int q = 0; q < NV; ++q) s[q] = fma_(x, vv[q], s[q]);
     __builtin_amdgcn_sched_barrier(0);
-    if (k + 1 < N) {
+    if (kk + 1 < KN) {
       x = xn;
 #pragma unroll
       for (int q = 0; q < NV; ++q) vv[q] = vn[q];
     }
+  }
+  if constexpr (H == 2) {
+#pragma unroll
+    for (int q = 0; q < NV; ++q) s[q] = s[q] + swap_pair(s[q]);
   }
 }
 
