@@ -693,12 +693,11 @@ __device__ __forceinline__ R col_reduce(R p, const Lane& L) {
   return p;
 }
 
-// sum over all 64 lanes (fixed order, identical in every lane)
+// sum over all 64 lanes (fixed order, identical in every lane): the DPP row
+// sum of each 16-lane row, then the four rows across the column
 template <typename R>
 __device__ __forceinline__ R wave_sum(R p, const Lane& L) {
-#pragma unroll
-  for (int m = 1; m < 64; m <<= 1) p += shfl(p, L.l ^ m);
-  return p;
+  return col_reduce<R>(row_sum<R>(p), L);
 }
 
 // (M^T v)_j with M in registers and v in an LDS slot of 16 values
