@@ -4,9 +4,9 @@
 // adrt_wave32.hpp (four 16 x 16 blocks per wave, rows 16 (b & 1) + h + 4 r of
 // column 16 (b >> 1) + j; vectors as the pair of elements j, 16 + j):
 //
-//  1. adding32_kernel<true> solves the transposed interface system for the
-//     adjoint vectors D_l, U_l (every column; the forward's interface
-//     intensities come from adding32_kernel<false> or the composite kernel);
+//  1. adding32_kernel<true> and, for the columns the prep kernel flags,
+//     adding32_composite_kernel<true> solve the transposed interface system
+//     for the adjoint vectors D_l, U_l;
 //  2. layer32_adjoint_kernel, one wave per (wavenumber, layer) on a
 //     persistent grid: the cotangents of the layer record are the outer
 //     products of forward and adjoint interface vectors,

@@ -550,7 +550,7 @@ int adding_chunk(const adrt::dev::FwdArgs& a, adrt_workspace* ws, hipStream_t st
     rc = fork_join(
         ws, stream,
         [&](hipStream_t s) { w32::adding32_kernel<false><<<g, 64 * w32::WAVES, 0, s>>>(a); },
-        [&](hipStream_t s) { w32::adding32_composite_kernel<<<g, 64 * w32::WAVES, 0, s>>>(a); });
+        [&](hipStream_t s) { w32::adding32_composite_kernel<false><<<g, 64 * w32::WAVES, 0, s>>>(a); });
   } else {
     const unsigned g = (unsigned)((a.W + GPB - 1) / GPB);
     rc = fork_join(
@@ -954,16 +954,21 @@ int adding_dispatch(const adrt::dev::FwdArgs& a, adrt_workspace* ws, hipStream_t
 }
 
 // the reverse pass of one chunk at quadrature order 32 (the forward's records
-// and interface intensities are in the workspace): the transposed adding for
-// every column, the doubling reverse on a persistent grid, the start reverse,
-// the column terms and the shared-moment sums
+// and interface intensities are in the workspace): the transposed adding (base
+// sweep, and the composite form beside it for the flagged columns), the
+// doubling reverse on a persistent grid, the start reverse, the column terms
+// and the shared-moment sums
 int adjoint_chunk32(const adrt::dev::FwdArgs& a, adrt::dev::w16::AdjArgs b, adrt_workspace* ws,
                     hipStream_t stream, bool shared_moments) {
   using namespace adrt::dev;
   int e = prof_begin(ws, 3, stream);
-  w32::adding32_kernel<true><<<(unsigned)((a.W + w32::WAVES - 1) / w32::WAVES), 64 * w32::WAVES,
-                               0, stream>>>(a);
+  const unsigned g = (unsigned)((a.W + w32::WAVES - 1) / w32::WAVES);
+  int rc = fork_join(
+      ws, stream,
+      [&](hipStream_t s) { w32::adding32_kernel<true><<<g, 64 * w32::WAVES, 0, s>>>(a); },
+      [&](hipStream_t s) { w32::adding32_composite_kernel<true><<<g, 64 * w32::WAVES, 0, s>>>(a); });
   prof_end(ws, e, stream);
+  if (rc) return rc;
   const long tasks = (long)a.W * a.L;
   const long waves = std::min(tasks, (long)ws->cus * 4 * w32::REV32_WAVES_PER_SIMD);
   b.nwaves = (int)waves;

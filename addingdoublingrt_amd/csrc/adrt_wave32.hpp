@@ -803,12 +803,29 @@ __device__ __forceinline__ Vec left_inverse(const Mat& P, const double* vb, cons
   return x;
 }
 
+// cotangents of I_up(l) and I_dn(l) (elements j, 16 + j) from the output
+// cotangents, as the N = 16 reverse takes them (w16::ibar16)
+__device__ __forceinline__ void ibar32(const FwdArgs& a, long gw, int l, const Lane& L, Vec& up,
+                                       Vec& dn) {
+  w16::ibar16(a, gw, l, a.mu[L.j], a.wt[L.j], up.lo, dn.lo);
+  w16::ibar16(a, gw, l, a.mu[16 + L.j], a.wt[16 + L.j], up.hi, dn.hi);
+}
+
+// R or T of a layer record, or (ADJ) its transpose
+template <bool ADJ>
+__device__ __forceinline__ void rec_mat(const double* m, const Lane& L, Mat& M) {
+  if constexpr (ADJ) gload_t(m, L, M);
+  else gload(m, L, M);
+}
+
 // rbase step: new = add(top = stored layer, bot = C)  (src/adding.hpp:246-258)
-__device__ __forceinline__ void add32_on_top(Comp& C, const double* rec, int lscat, const Lane& L,
-                             const Ws& s) {
+// (ADJ: the layer of the transposed system -- transposed R and T, the
+// cotangent sources tsu, tsd)
+template <bool ADJ>
+__device__ __forceinline__ void add32_on_top(Comp& C, const double* rec, int lscat, const Vec& tsu,
+                                             const Vec& tsd, const Lane& L, const Ws& s) {
   const double* Rt = rec;
   const double* Tt = rec + N * N;
-  const Vec tsu = gvec(rec + 2 * N * N, L), tsd = gvec(rec + 2 * N * N + N, L);
   Mat A, B, X, D1;
   if (!lscat) {
     // NonscatTop (src/adding.hpp:117-174): t = diag(top.T_ab)
@@ -843,8 +860,8 @@ __device__ __forceinline__ void add32_on_top(Comp& C, const double* rec, int lsc
     const Vec t = diag(C.Tab, L);
     put_vec(s.v0, L, t);
     put_vec(s.v1, L, C.su);
-    gload(Tt, L, B);
-    gload(Rt, L, A);
+    rec_mat<ADJ>(Tt, L, B);
+    rec_mat<ADJ>(Rt, L, A);
     put(s.tile, L, B);
     put(s.tile2, L, A);
     wave_sync();
@@ -871,7 +888,7 @@ __device__ __forceinline__ void add32_on_top(Comp& C, const double* rec, int lsc
     return;
   }
   // General (src/adding.hpp:69-110)
-  gload(Rt, L, A);
+  rec_mat<ADJ>(Rt, L, A);
   put(s.tile, L, C.Rab);
   put(s.tile2, L, A);
   put_vec(s.v0, L, tsd);
@@ -883,9 +900,9 @@ __device__ __forceinline__ void add32_on_top(Comp& C, const double* rec, int lsc
   wave_sync();
   put(s.tile, L, X);
   wave_sync();
-  gload(Tt, L, B);
+  rec_mat<ADJ>(Tt, L, B);
   right_inverse(X, B, D1, s, L);                         // D1 = top.T_ba A1^-1
-  gload(Rt, L, A);
+  rec_mat<ADJ>(Rt, L, A);
   put(s.tile2, L, A);
   wave_sync();
   mul(s.tile2, C.Rab, L, X);                             // P2 = top.R_ba bot.R_ab
@@ -900,9 +917,9 @@ __device__ __forceinline__ void add32_on_top(Comp& C, const double* rec, int lsc
   wave_sync();
   const Vec nsd = vadd(C.sd, matvec(s.tile, s.v3, L));
   const Vec nsu = vadd(tsu, matvec(s.tile2, s.v2, L));
-  gload(Rt, L, A);
+  rec_mat<ADJ>(Rt, L, A);
   mul(s.tile, A, L, X);                                  // D2 top.R_ba
-  gload(Tt, L, A);
+  rec_mat<ADJ>(Tt, L, A);
   mul(s.tile, A, L, C.Tab);                              // T_ab = D2 top.T_ab
   mul(s.tile2, C.Rab, L, B);                             // D1 bot.R_ab
   mul(s.tile2, C.Tba, L, D1);                            // new T_ba = D1 bot.T_ba
@@ -911,7 +928,7 @@ __device__ __forceinline__ void add32_on_top(Comp& C, const double* rec, int lsc
   put(s.tile2, L, B);
   wave_sync();
   mul_add(s.tile, C.Tba, C.Rba, L, C.Rba);               // R_ba = bot.R_ba + D2 R_ba T_ba
-  gload(Rt, L, B);
+  rec_mat<ADJ>(Rt, L, B);
   mul_add(s.tile2, A, B, L, C.Rab);                      // R_ab = top.R_ab + D1 R_ab T_ab
   wave_sync();
   C.Tba = D1;
@@ -921,19 +938,20 @@ __device__ __forceinline__ void add32_on_top(Comp& C, const double* rec, int lsc
 }
 
 // rtop step: new = add(top = C, bot = stored layer)
-__device__ __forceinline__ void add32_below(Comp& C, const double* rec, int lscat, const Lane& L,
-                            const Ws& s) {
+// (ADJ: as add32_on_top)
+template <bool ADJ>
+__device__ __forceinline__ void add32_below(Comp& C, const double* rec, int lscat, const Vec& bsu,
+                                            const Vec& bsd, const Lane& L, const Ws& s) {
   const double* Rb = rec;
   const double* Tb = rec + N * N;
-  const Vec bsu = gvec(rec + 2 * N * N, L), bsd = gvec(rec + 2 * N * N + N, L);
   Mat A, B, X, D1;
   if (!C.scat) {
     // NonscatTop with top = C: t = diag(C.T_ab)
     const Vec t = diag(C.Tab, L);
     put_vec(s.v0, L, t);
     put_vec(s.v1, L, C.sd);
-    gload(Rb, L, A);
-    gload(Tb, L, B);
+    rec_mat<ADJ>(Rb, L, A);
+    rec_mat<ADJ>(Tb, L, B);
     put(s.tile, L, A);
     put(s.tile2, L, B);
     wave_sync();
@@ -988,7 +1006,7 @@ __device__ __forceinline__ void add32_below(Comp& C, const double* rec, int lsca
     return;
   }
   // General with top = C, bot = layer
-  gload(Rb, L, A);
+  rec_mat<ADJ>(Rb, L, A);
   put(s.tile, L, A);
   put(s.tile2, L, C.Rba);
   put_vec(s.v0, L, C.sd);
@@ -1002,13 +1020,13 @@ __device__ __forceinline__ void add32_below(Comp& C, const double* rec, int lsca
   wave_sync();
   right_inverse(X, C.Tba, D1, s, L);                      // D1 = top.T_ba A1^-1
   put(s.tile2, L, C.Rba);
-  gload(Rb, L, A);
+  rec_mat<ADJ>(Rb, L, A);
   wave_sync();
   mul(s.tile2, A, L, X);                                  // P2 = top.R_ba bot.R_ab
   wave_sync();
   put(s.tile, L, X);
   wave_sync();
-  gload(Tb, L, B);
+  rec_mat<ADJ>(Tb, L, B);
   right_inverse(X, B, A, s, L);                           // D2 = bot.T_ab A2^-1
   put_vec(s.v2, L, rhs_up);
   put_vec(s.v3, L, rhs_dn);
@@ -1017,9 +1035,9 @@ __device__ __forceinline__ void add32_below(Comp& C, const double* rec, int lsca
   wave_sync();
   const Vec nsu = vadd(C.su, matvec(s.tile, s.v2, L));
   const Vec nsd = vadd(bsd, matvec(s.tile2, s.v3, L));
-  gload(Rb, L, B);
+  rec_mat<ADJ>(Rb, L, B);
   mul(s.tile, B, L, X);                                   // D1 bot.R_ab
-  gload(Tb, L, B);
+  rec_mat<ADJ>(Tb, L, B);
   mul(s.tile, B, L, D1);                                  // new T_ba = D1 bot.T_ba
   mul(s.tile2, C.Rba, L, A);                              // D2 top.R_ba
   wave_sync();
@@ -1030,7 +1048,7 @@ __device__ __forceinline__ void add32_below(Comp& C, const double* rec, int lsca
   wave_sync();
   put(s.tile2, L, A);
   wave_sync();
-  gload(Rb, L, A);
+  rec_mat<ADJ>(Rb, L, A);
   mul_add(s.tile2, B, A, L, C.Rba);                       // R_ba = bot.R_ba + D2 R_ba T_ba
   wave_sync();
   C.Tab = X;
@@ -1043,7 +1061,10 @@ __device__ __forceinline__ void add32_below(Comp& C, const double* rec, int lsca
 // The reference's form, for the columns the prep kernel flags (consecutive
 // scattering layers with unstable doubling starts; the N = 16 adding's
 // composite kernel gives the reason); adding32_kernel below does the others.
-// 2 waves per SIMD measured 16 % faster than one wave with all 512 registers
+// 2 waves per SIMD measured 16 % faster than one wave with all 512 registers.
+// ADJ: the transposed system of the reverse pass for these columns (as
+// adding16_composite_kernel<true>)
+template <bool ADJ>
 __global__ void __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(2, 8)))
 adding32_composite_kernel(FwdArgs a) {
   __shared__ double smem[WAVES * SLOT];
@@ -1069,6 +1090,20 @@ adding32_composite_kernel(FwdArgs a) {
   const double wt_c[2] = {a.wt[j], a.wt[16 + j]};
   const double* Bw = a.B ? a.B + gw * (Lay + 1) : nullptr;
   auto Blev = [&](int l) -> double { return Bw ? Bw[lev(a, l)] : 0.0; };
+  // the sources of layer k, or (ADJ) the cotangents of I_dn(k) and I_up(k+1)
+  auto sources = [&](int k, Vec& su, Vec& sd) {
+    if constexpr (ADJ) {
+      Vec u, d;
+      ibar32(a, gw, k, L, u, d);
+      su = d;
+      ibar32(a, gw, k + 1, L, u, d);
+      sd = u;
+    } else {
+      const double* rec = lay + (long)k * REC;
+      su = gvec(rec + 2 * N * N, L);
+      sd = gvec(rec + 2 * N * N + N, L);
+    }
+  };
 
   // boundary conditions (src/solver.cpp:385-419, 516-551, 585-603)
   const double Bs = a.surf_w ? a.surf_w[gw] : a.surf_em;
@@ -1077,9 +1112,15 @@ adding32_composite_kernel(FwdArgs a) {
   const int has_surface = !a.diffusion && (A > 0.0 || Bs > 0.0);
   const int ltot = Lay + (has_surface ? 1 : 0);
   const double tau_tot = a.tau_cum[(long)w * (Lay + 1) + Lay];
-  const Vec Itd{Btop, Btop};
+  Vec Itd{Btop, Btop};
   Vec Ibu{0.0, 0.0};
-  if (a.diffusion) {
+  if constexpr (ADJ) {
+    Vec u, d;
+    ibar32(a, gw, 0, L, u, d);
+    Itd = u;
+    ibar32(a, gw, Lay, L, u, d);
+    if (!has_surface) Ibu = d;
+  } else if (a.diffusion) {
     const double Bb = Blev(Lay);
     const double dt = a.tau_s[(long)w * Lay + Lay - 1];
     const double dBdt = (dt > 0.0) ? (Bb - Blev(Lay - 1)) / dt : 0.0;
@@ -1096,25 +1137,34 @@ adding32_composite_kernel(FwdArgs a) {
     for (int b = 0; b < 4; ++b)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        C.Rab.v[b][r] = (bi(b) + rw(L.h, r) < a.n_real) ? rr[b >> 1] : 0.0;
+        const int i = bi(b) + rw(L.h, r), c = bj(b) + j;
+        if constexpr (ADJ)  // R_s^T(i, c) = R_s(c, i)
+          C.Rab.v[b][r] = (c < a.n_real) ? 2.0 * A * a.mu[i] * a.wt[i] * a.xfac : 0.0;
+        else
+          C.Rab.v[b][r] = (i < a.n_real) ? rr[b >> 1] : 0.0;
         C.Rba.v[b][r] = C.Rab.v[b][r];
         C.Tab.v[b][r] = 0.0;
         C.Tba.v[b][r] = 0.0;
       }
-    double ssol = 0.0;
-    if (a.has_solar && A > 0.0) ssol = (A / kPi) * a.F0 * a.mu0 * exp(-tau_tot / a.mu0);
-    const double su = (1.0 - A) * Bs + ssol;
-    C.su = Vec{su, su};
+    if constexpr (ADJ) {
+      Vec u, d;
+      ibar32(a, gw, Lay, L, u, d);
+      C.su = d;
+    } else {
+      double ssol = 0.0;
+      if (a.has_solar && A > 0.0) ssol = (A / kPi) * a.F0 * a.mu0 * exp(-tau_tot / a.mu0);
+      const double su = (1.0 - A) * Bs + ssol;
+      C.su = Vec{su, su};
+    }
     C.sd = Vec{0.0, 0.0};
     C.scat = A > 0.0;
   } else {
     const double* rec = lay + (long)(Lay - 1) * REC;
-    gload(rec, L, C.Rab);
-    gload(rec + N * N, L, C.Tab);
+    rec_mat<ADJ>(rec, L, C.Rab);
+    rec_mat<ADJ>(rec + N * N, L, C.Tab);
     C.Rba = C.Rab;
     C.Tba = C.Tab;
-    C.su = gvec(rec + 2 * N * N, L);
-    C.sd = gvec(rec + 2 * N * N + N, L);
+    sources(Lay - 1, C.su, C.sd);
     C.scat = lflag[Lay - 1];
   }
   for (int n = 1;; ++n) {  // C == rbase[n]
@@ -1129,15 +1179,20 @@ adding32_composite_kernel(FwdArgs a) {
     }
     if (n == ltot) break;
     const int k = ltot - 1 - n;
-    add32_on_top(C, lay + (long)k * REC, lflag[k], L, s);
+    Vec su, sd;
+    sources(k, su, sd);
+    add32_on_top<ADJ>(C, lay + (long)k * REC, lflag[k], su, sd, L, s);
   }
 
+  // interface intensities (the reverse pass's) or (ADJ) adjoint vectors
+  double* vec = ADJ ? a.Av : a.Iv;
   auto emit = [&](int l, const Vec& Iup, const Vec& Idn) {
-    if (a.Iv && L.h == 0) {  // interface intensities for the reverse pass
-      double* vec = a.Iv + ((long)w * (Lay + 1) + l) * 2 * N;
-      store_vec(vec, L, Iup);
-      store_vec(vec + N, L, Idn);
+    if (vec && L.h == 0) {
+      double* o = vec + ((long)w * (Lay + 1) + l) * 2 * N;
+      store_vec(o, L, Iup);
+      store_vec(o + N, L, Idn);
     }
+    if constexpr (ADJ) return;
     const double c0 = 2.0 * kPi * wt_c[0] * mu_c[0], c1 = 2.0 * kPi * wt_c[1] * mu_c[1];
     const double fu = row_sum<double>(c0 * Iup.lo + c1 * Iup.hi);
     const double fdn = row_sum<double>(c0 * Idn.lo + c1 * Idn.hi);
@@ -1177,12 +1232,11 @@ adding32_composite_kernel(FwdArgs a) {
   }
 
   // ---- top-down sweep with interfaces (src/solver.cpp:574-583, 644-678) ----
-  gload(lay, L, C.Rab);
-  gload(lay + N * N, L, C.Tab);
+  rec_mat<ADJ>(lay, L, C.Rab);
+  rec_mat<ADJ>(lay + N * N, L, C.Tab);
   C.Rba = C.Rab;
   C.Tba = C.Tab;
-  C.su = gvec(lay + 2 * N * N, L);
-  C.sd = gvec(lay + 2 * N * N + N, L);
+  sources(0, C.su, C.sd);
   C.scat = lflag[0];
   for (int l = 1; l <= Lay; ++l) {  // C == rtop[l]
     const int n_base = ltot - l;
@@ -1240,7 +1294,11 @@ adding32_composite_kernel(FwdArgs a) {
       Iup = Ibu;
     }
     emit(l, Iup, Idn);
-    if (l < Lay) add32_below(C, lay + (long)l * REC, lflag[l], L, s);
+    if (l < Lay) {
+      Vec su, sd;
+      sources(l, su, sd);
+      add32_below<ADJ>(C, lay + (long)l * REC, lflag[l], su, sd, L, s);
+    }
   }
 }
 
@@ -1281,21 +1339,6 @@ __device__ __forceinline__ void left_solve_mat_tile(double* t1, double* t2, cons
   wave_sync();
   get(t2, L, G);
   wave_sync();
-}
-
-// cotangents of I_up(l) and I_dn(l) (elements j, 16 + j) from the output
-// cotangents, as the N = 16 reverse takes them (w16::ibar16)
-__device__ __forceinline__ void ibar32(const FwdArgs& a, long gw, int l, const Lane& L, Vec& up,
-                                       Vec& dn) {
-  w16::ibar16(a, gw, l, a.mu[L.j], a.wt[L.j], up.lo, dn.lo);
-  w16::ibar16(a, gw, l, a.mu[16 + L.j], a.wt[16 + L.j], up.hi, dn.hi);
-}
-
-// R or T of a layer record, or (ADJ) its transpose
-template <bool ADJ>
-__device__ __forceinline__ void rec_mat(const double* m, const Lane& L, Mat& M) {
-  if constexpr (ADJ) gload_t(m, L, M);
-  else gload(m, L, M);
 }
 
 // layer record rec on top of the base B: stores at o what the downward sweep
@@ -1397,16 +1440,15 @@ __device__ __forceinline__ void base32_add(Base& B, const double* rec, int lscat
   B.s = vadd(su, ts);
 }
 
-// ADJ: the transposed system of the reverse pass (as adding16_kernel<true>),
-// for every column: the adjoint vectors of the columns the forward solved in
-// the composite form also come from this sweep
+// ADJ: the transposed system of the reverse pass (as adding16_kernel<true>);
+// the columns the prep kernel flags take adding32_composite_kernel<ADJ>
 template <bool ADJ>
 __global__ void __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(2, 8)))
 adding32_kernel(FwdArgs a) {
   __shared__ double smem[WAVES * SLOT];
   const int wv = wave_index();
   const int w = blockIdx.x * WAVES + wv;
-  if (w >= a.W || (!ADJ && a.ucol && a.ucol[w])) return;
+  if (w >= a.W || (a.ucol && a.ucol[w])) return;
   const Lane L;
   const int j = L.j;
   const long gw = (long)(a.w0 + w);

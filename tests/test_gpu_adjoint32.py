@@ -20,12 +20,14 @@ The protocol of tests/test_gpu_adjoint.py, through the C-ABI:
 * the forward on an N = 32 reverse-pass workspace bit for bit equal to the
   plain forward, and torch.autograd (adrt.solve) equal to adjoint_batch;
 * layers past the checkpoint depth (orders 16 and 32): zero gradients,
-  reported by adrt_adjoint_status, the other columns unchanged.
+  reported by adrt_adjoint_status, the other columns unchanged;
+* columns whose adding runs in the reference's composite form (two unstable
+  scattering layers in a row) at orders 16 and 32, beside ordinary columns.
 
 Inputs stay away from the non-differentiable points of the forward (tau at
 a power of two, omega at the 0.01 / 0.1 thresholds) and from layers with an
 unstable doubling start (omega < 0.01), which tests/test_gpu_adjoint.py
-covers at N <= 16.
+covers at N <= 16, except in the composite-column test below.
 """
 import numpy as np
 import pytest
@@ -277,3 +279,37 @@ def test_adjoint32_vs_fd_options(case):
         gA, nA = albedo_fd(tau, ssa, mom, cfg, cot, B, Bs, Bt)
         got = g["surface_albedo"].sum()
         assert abs(got - gA) <= REL * abs(gA) + nA, (got, gA, nA)
+
+
+@pytest.mark.parametrize("n", [16, 32])
+def test_adjoint_vs_fd_composite_columns(n):
+    """Columns with two scattering layers in a row whose doubling start is
+    unstable (omega < 0.01) run the adding in the reference's composite form
+    (src/solver.cpp:560-582); their reverse pass is the transposed composite sweep.  The
+    Planck, surface and top sources enter linearly, so their gradients are held
+    to the bar; tau and omega of the unstable layers to the loose bound of
+    tests/test_gpu_adjoint.py.  Ordinary columns share the batch.
+
+    The unstable layers' start thickness is 5 to 10 times the smallest
+    quadrature cosine at both orders (mu_min = 0.0053 at N = 16, 0.0014 at
+    N = 32).  At N = 32 the thicknesses of the N = 16 case (0.3, 0.2) put it
+    near 27 and the doubling amplifies the start to ~1e11, after which FP64
+    holds even the linear Planck gradients of those columns only to ~5e-5 and
+    the neighbours' tau gradients to ~1e-4..5e-3 against the
+    extended-precision differences."""
+    from test_gpu_adjoint import unstable_layers
+    W, L = 4, 6
+    tau, ssa, mom, B = batch32(W, L, seed=90 + n, M=2 * n)
+    for w in (0, 2):
+        ssa[w, 2], ssa[w, 3] = 0.005, 0.004
+        tau[w, 2], tau[w, 3] = (0.3, 0.2) if n == 16 else (0.02, 0.015)
+    cfg = BatchConfig(num_quadrature=n, surface_albedo=0.2, solar_flux=1.0, solar_mu=0.6)
+    loose = unstable_layers(tau, ssa, mom, cfg)
+    assert loose[0, 2] and loose[0, 3] and loose[2, 2] and loose[2, 3]
+    Bs, Bt = B[:, -1] * 1.1, B[:, 0] * 0.9
+    cot = random_cot(W, L, 17, ("flux_up", "flux_down", "mean_intensity", "flux_direct"))
+    g, _ = gpu_grads(tau, ssa, mom, cfg, cot, B, Bs, Bt)
+    fd, nz = fd_grads(tau, ssa, mom, cfg, cot, B, Bs, Bt, which=("tau", "ssa", "B", "Bs", "Bt"))
+    for k, v in fd.items():
+        assert_close(g[NAMES[k]], v, f"N={n} composite columns d/d{NAMES[k]}",
+                     loose if k in ("tau", "ssa") else None, noise=nz[k])
