@@ -281,14 +281,26 @@ def test_adjoint32_vs_fd_options(case):
         assert abs(got - gA) <= REL * abs(gA) + nA, (got, gA, nA)
 
 
-@pytest.mark.parametrize("n", [16, 32])
-def test_adjoint_vs_fd_composite_columns(n):
+COMPOSITE_CASES = {
+    "n16": dict(n=16, cfg=dict(surface_albedo=0.2, solar_flux=1.0, solar_mu=0.6)),
+    "n32": dict(n=32, cfg=dict(surface_albedo=0.2, solar_flux=1.0, solar_mu=0.6)),
+    "n32_reversed_diffusion": dict(n=32, cfg=dict(use_diffusion_lower_bc=True, solar_flux=0.7,
+                                                  solar_mu=0.8, index_from_bottom=True)),
+    "n32_no_surface": dict(n=32, cfg=dict(solar_flux=1.0, solar_mu=0.5, surface_emission=0.0),
+                           cold_surface=True),
+}
+
+
+@pytest.mark.parametrize("case", list(COMPOSITE_CASES))
+def test_adjoint_vs_fd_composite_columns(case):
     """Columns with two scattering layers in a row whose doubling start is
     unstable (omega < 0.01) run the adding in the reference's composite form
     (src/solver.cpp:560-582); their reverse pass is the transposed composite sweep.  The
     Planck, surface and top sources enter linearly, so their gradients are held
     to the bar; tau and omega of the unstable layers to the loose bound of
-    tests/test_gpu_adjoint.py.  Ordinary columns share the batch.
+    tests/test_gpu_adjoint.py.  Ordinary columns share the batch; the boundary
+    options (surface, diffusion lower boundary, cold black surface, bottom-first
+    indexing) each set the composite sweep's boundary values differently.
 
     The unstable layers' start thickness is 5 to 10 times the smallest
     quadrature cosine at both orders (mu_min = 0.0053 at N = 16, 0.0014 at
@@ -298,18 +310,27 @@ def test_adjoint_vs_fd_composite_columns(n):
     the neighbours' tau gradients to ~1e-4..5e-3 against the
     extended-precision differences."""
     from test_gpu_adjoint import unstable_layers
+    c = COMPOSITE_CASES[case]
+    n = c["n"]
     W, L = 4, 6
-    tau, ssa, mom, B = batch32(W, L, seed=90 + n, M=2 * n)
+    tau, ssa, mom, B = batch32(W, L, seed=90 + n + len(case), M=2 * n)
     for w in (0, 2):
         ssa[w, 2], ssa[w, 3] = 0.005, 0.004
         tau[w, 2], tau[w, 3] = (0.3, 0.2) if n == 16 else (0.02, 0.015)
-    cfg = BatchConfig(num_quadrature=n, surface_albedo=0.2, solar_flux=1.0, solar_mu=0.6)
+    cfg = BatchConfig(num_quadrature=n, **c["cfg"])
     loose = unstable_layers(tau, ssa, mom, cfg)
+    # the unstable pair is adjacent in either indexing
     assert loose[0, 2] and loose[0, 3] and loose[2, 2] and loose[2, 3]
-    Bs, Bt = B[:, -1] * 1.1, B[:, 0] * 0.9
+    Bs = None if c.get("cold_surface") else B[:, -1] * 1.1
+    Bt = B[:, 0] * 0.9
     cot = random_cot(W, L, 17, ("flux_up", "flux_down", "mean_intensity", "flux_direct"))
     g, _ = gpu_grads(tau, ssa, mom, cfg, cot, B, Bs, Bt)
-    fd, nz = fd_grads(tau, ssa, mom, cfg, cot, B, Bs, Bt, which=("tau", "ssa", "B", "Bs", "Bt"))
+    which = ("tau", "ssa", "B", "Bt") + (() if Bs is None else ("Bs",))
+    fd, nz = fd_grads(tau, ssa, mom, cfg, cot, B, Bs, Bt, which=which)
     for k, v in fd.items():
-        assert_close(g[NAMES[k]], v, f"N={n} composite columns d/d{NAMES[k]}",
+        assert_close(g[NAMES[k]], v, f"{case} composite columns d/d{NAMES[k]}",
                      loose if k in ("tau", "ssa") else None, noise=nz[k])
+    if not cfg.use_diffusion_lower_bc and cfg.surface_albedo > 0.0:
+        gA, nA = albedo_fd(tau, ssa, mom, cfg, cot, B, Bs, Bt)
+        got = g["surface_albedo"].sum()
+        assert abs(got - gA) <= REL * abs(gA) + nA, (got, gA, nA)
