@@ -349,6 +349,7 @@ struct adrt_workspace {
   double *tau_s = nullptr, *omega_s = nullptr, *tau_cum = nullptr, *fdm = nullptr;
   double *lay = nullptr, *rb = nullptr;
   int *mode = nullptr, *lflag = nullptr, *flag = nullptr, *redo = nullptr, *ucol = nullptr;
+  int *ulist = nullptr, *ucount = nullptr;  // the flagged columns (FwdArgs::ulist)
   // reverse pass (ADRT_WORKSPACE_ADJOINT)
   int flags = 0;
   int *pord = nullptr, *pflag = nullptr;  // pair mode (orders <= 8 padded to 16)
@@ -389,7 +390,7 @@ size_t per_wavenumber_bytes(int L, int Np, int M, bool adjoint) {
   const size_t d = sizeof(double);
   size_t b = 0;
   b += 4 * (size_t)L * d + (size_t)(L + 1) * d;      // tau_s, omega_s, fdm, spare, tau_cum
-  b += 3 * (size_t)L * sizeof(int) + sizeof(int);     // mode, lflag, redo; ucol
+  b += 3 * (size_t)L * sizeof(int) + 2 * sizeof(int); // mode, lflag, redo; ucol, ulist
   b += (size_t)L * (2 * Np * Np + 2 * Np) * d;        // layer store
   b += (size_t)L * (2 * Np * Np + 4 * Np) * d;        // adding records
   b += (size_t)(3 * L + 3) * d;                       // raw-coefficient optics
@@ -462,6 +463,7 @@ int layers_chunk(const adrt::dev::FwdArgs& a, adrt_workspace* ws, hipStream_t st
   using namespace adrt::dev;
   constexpr int GPB = Geo<N>::GPB;  // lane groups per 256-thread block
   int e = prof_begin(ws, 0, stream);
+  if (a.ulist) w16::fill_int_kernel<<<1, 1, 0, stream>>>(a.ucount, 0);  // a kernel: capturable
   prep_kernel<<<(a.W + 255) / 256, 256, 0, stream>>>(a);
   prof_end(ws, e, stream);
   const long tasks = (long)a.W * a.L;
@@ -705,7 +707,7 @@ int adrt_workspace_create_ex(const adrt_batch_config* cfg, size_t max_bytes, int
   const size_t phase_bytes =
       ws->Np == adrt::dev::w16::N ? (size_t)ws->L * adrt::dev::w16::PH16 * 8 : 0;
   const size_t fixed =
-      align256((size_t)(2 * ws->Np + ws->Lmax * ws->Np + ws->Lmax) * 8) + 512 + slab_bytes +
+      align256((size_t)(2 * ws->Np + ws->Lmax * ws->Np + ws->Lmax) * 8) + 768 + slab_bytes +
       align256(phase_bytes) +
       (adjoint ? align256((size_t)ws->L * adrt::dev::w16::MOM_SLICES * ws->M * 8) : 0);
   long chunk = max_bytes > fixed ? (long)((max_bytes - fixed) / (per_w + 4096)) : 1;
@@ -723,7 +725,8 @@ int adrt_workspace_create_ex(const adrt_batch_config* cfg, size_t max_bytes, int
   const size_t o_tau_s = take(C * L * 8), o_om = take(C * L * 8), o_fdm = take(C * L * 8);
   const size_t o_cum = take(C * (L + 1) * 8);
   const size_t o_mode = take(C * L * 4), o_lflag = take(C * L * 4), o_flag = take(4);
-  const size_t o_redo = take(C * L * 4), o_ucol = take(C * 4);
+  const size_t o_redo = take(C * L * 4), o_ucol = take(C * 4), o_ulist = take(C * 4);
+  const size_t o_ucount = take(4);
   const size_t o_lay = take(C * L * (2 * Np * Np + 2 * Np) * 8);
   const size_t o_rb = take(C * L * (2 * Np * Np + 4 * Np) * 8);
   const size_t o_phase = take(phase_bytes);
@@ -778,6 +781,8 @@ int adrt_workspace_create_ex(const adrt_batch_config* cfg, size_t max_bytes, int
   ws->lflag = (int*)(ws->base + o_lflag);
   ws->redo = (int*)(ws->base + o_redo);
   ws->ucol = (int*)(ws->base + o_ucol);
+  ws->ulist = (int*)(ws->base + o_ulist);
+  ws->ucount = (int*)(ws->base + o_ucount);
   ws->flag = (int*)(ws->base + o_flag);
   ws->lay = (double*)(ws->base + o_lay);
   ws->rb = (double*)(ws->base + o_rb);
@@ -897,6 +902,8 @@ adrt::dev::FwdArgs make_args(const adrt_batch_config* cfg, const adrt_device_inp
   a.lflag = ws->lflag;
   a.redo = ws->redo;
   a.ucol = ws->ucol;
+  a.ulist = ws->ulist;
+  a.ucount = ws->ucount;
   a.rb = ws->rb;
   a.pord = ws->pord;  // two layers per wave (orders <= 8 on reverse-pass workspaces)
   a.pflag = ws->pflag;
@@ -1097,6 +1104,7 @@ int adrt_source_jacobian_batch(const adrt_batch_config* cfg, const adrt_device_i
     p.rb = ws->jrb;
     p.lflag = ws->jlflag;
     p.ucol = ws->jucol;
+    p.ulist = p.ucount = nullptr;  // the pseudo-columns' flags: no list
     p.tau_s = ws->jtau_s;
     p.tau_cum = ws->jtau_cum;
     p.basis = nullptr;

@@ -67,6 +67,10 @@ struct FwdArgs {
   int* pflag;       // [W][L]   pair mode: the layer runs in a pair (both layers of it scatter)
   int* ucol;        // [W]      N = 16 adding in the reference's composite form (consecutive
                     //          scattering layers with unstable doubling starts)
+  int* ulist;       // [W]      the flagged columns, in no set order, so that the composite
+                    //          kernels' first waves take them all (null: each wave checks
+                    //          ucol of its own column)
+  int* ucount;      // [1]      entries of ulist (zeroed before the prep kernel)
   double* rb;       // [W][L][2N^2 + 4N]   adding records (base composites; N = 16: the
                     //                     downward-sweep operators of each layer)
   double* basis;    // [W][L][2N] thermal source basis or null: y, z of the layer
@@ -175,6 +179,21 @@ __global__ void __launch_bounds__(256) prep_kernel(FwdArgs a) {
   }
   a.tau_cum[(long)w * (L + 1) + L] = cum;
   if (a.ucol) a.ucol[w] = ucol;
+  if (ucol && a.ulist) a.ulist[atomicAdd(a.ucount, 1)] = w;
+}
+
+// The column a wave (or lane group) of a composite-form adding kernel takes:
+// entry w of the prep kernel's list of flagged columns, so that the first
+// waves of the grid -- dispatched first, beside the base sweep -- hold them
+// all; without a list, column w itself if it is flagged.  Returns false for a
+// wave with no column; w is wave-uniform either way.
+__device__ __forceinline__ bool composite_column(const FwdArgs& a, int& w) {
+  if (a.ulist) {
+    if (w >= *a.ucount) return false;
+    w = a.ulist[w];
+    return true;
+  }
+  return w < a.W && a.ucol && a.ucol[w];
 }
 
 template <int N>
@@ -766,8 +785,8 @@ adding_composite_kernel(FwdArgs a) {
   const int g = threadIdx.x / G;
   const int j = (threadIdx.x % G) / H;
   const int h = threadIdx.x % H;
-  const int w = blockIdx.x * GPB + g;
-  if (w >= a.W || !a.ucol || !a.ucol[w]) return;
+  int w = blockIdx.x * GPB + g;
+  if (!composite_column(a, w)) return;
   const long gw = (long)(a.w0 + w);
   const int L = a.L;
   R* sm = smem + g * S;

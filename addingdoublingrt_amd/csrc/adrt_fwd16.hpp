@@ -1104,8 +1104,8 @@ adding16_composite_kernel(FwdArgs a) {
   using R = double;
   __shared__ R smem[4 * Lds16::size];
   const int wv = wave_index();
-  const int w = blockIdx.x * 4 + wv;
-  if (w >= a.W || !a.ucol || !a.ucol[w]) return;
+  int w = blockIdx.x * 4 + wv;
+  if (!composite_column(a, w)) return;
   const Lane L;
   const int j = L.j;
   const long gw = (long)(a.w0 + w);

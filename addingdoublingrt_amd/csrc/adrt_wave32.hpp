@@ -1069,8 +1069,8 @@ __global__ void __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu
 adding32_composite_kernel(FwdArgs a) {
   __shared__ double smem[WAVES * SLOT];
   const int wv = wave_index();
-  const int w = blockIdx.x * WAVES + wv;
-  if (w >= a.W || !a.ucol || !a.ucol[w]) return;
+  int w = blockIdx.x * WAVES + wv;
+  if (!composite_column(a, w)) return;
   const Lane L;
   const int j = L.j;
   const long gw = (long)(a.w0 + w);
