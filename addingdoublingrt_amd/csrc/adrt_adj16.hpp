@@ -34,6 +34,7 @@ struct AdjArgs {
   int nwaves;     // waves of the persistent layer grid
   int nn_cap;     // doubling steps a slab holds
   int* status;    // bit 0: a layer needed more doubling steps than nn_cap
+  int* queue;     // task counter of the persistent grid, zeroed before each launch
   // per-task results, chunk-local [W][L]
   double* g_taus;  // d/d scaled tau (layer terms; column terms added later)
   double* g_omgs;  // d/d scaled omega
@@ -52,6 +53,27 @@ struct AdjArgs {
   double* d_top;     // [W]
   double* d_albedo;  // [W] per-wavenumber contributions
 };
+
+// The persistent grids take their tasks CHUNK at a time from a counter that
+// the host zeroes before each launch: layers differ in doubling count (1 to
+// ~20 steps), so a wave that drew short layers draws more instead of the grid
+// waiting for the wave with the longest fixed share.  f(task) for every task
+// below ntask, each once over the grid; the loop is wave-uniform and every
+// wave leaves it once the counter has passed ntask.
+template <int CHUNK, class F>
+__device__ __forceinline__ void for_tasks(int* queue, long ntask, F f) {
+  for (;;) {
+    int t0 = 0;
+    if ((threadIdx.x & 63) == 0) t0 = atomicAdd(queue, CHUNK);
+    t0 = __builtin_amdgcn_readfirstlane(t0);
+    if ((long)t0 >= ntask) break;
+    const long t1 = (long)t0 + CHUNK < ntask ? (long)t0 + CHUNK : ntask;
+    for (long task = t0; task < t1; ++task) f(task);
+  }
+}
+// chunks: tasks of every layer; tasks of which most are skipped (the routed
+// layers of the mixed form, the layers pair mode leaves)
+constexpr int TASK_CHUNK = 4, SPARSE_TASK_CHUNK = 32;
 
 // waves per SIMD of the persistent doubling-reverse grid (its register budget:
 // 512 / 3 -> 168 registers per lane)
@@ -765,15 +787,15 @@ layer16_adjoint_kernel(FwdArgs a, AdjArgs b) {
   double* sm = smem + wv * LdsA::size;
   double* slab = b.slab + (long)gwave * slab_stride(b.nn_cap);
   const long ntask = (long)a.W * a.L;
-  for (long task = gwave; task < ntask; task += b.nwaves) {
+  for_tasks<(REDO || UNPAIRED) ? SPARSE_TASK_CHUNK : TASK_CHUNK>(b.queue, ntask, [&](long task) {
     if constexpr (REDO) {
-      if (!a.redo[task]) continue;
+      if (!a.redo[task]) return;
     }
     if constexpr (UNPAIRED) {
-      if (a.pflag[task]) continue;
+      if (a.pflag[task]) return;
     }
     layer16_adjoint_task(a, b, task, L, sm, slab);
-  }
+  });
 }
 
 // the column terms of the reverse pass for interface vectors of NV elements;

@@ -498,8 +498,8 @@ layer32_adjoint_kernel(FwdArgs a, AdjArgs b) {
   const Lane L;
   double* slab = b.slab + (long)gwave * slab_stride32(b.nn_cap);
   const long ntask = (long)a.W * a.L;
-  for (long task = gwave; task < ntask; task += b.nwaves)
-    layer32_adjoint_task(a, b, task, L, smem, slab);
+  w16::for_tasks<2>(b.queue, ntask,
+                     [&](long task) { layer32_adjoint_task(a, b, task, L, smem, slab); });
 }
 
 // ---------------------------------------------------------------------------

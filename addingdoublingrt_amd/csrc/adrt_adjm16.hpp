@@ -449,10 +449,10 @@ layer16m_adjoint_kernel(FwdArgs a, AdjArgs b) {
   double* sm = smem + wv * LdsAM::size;
   double* slab = b.slab + (long)gwave * slab_stride(b.nn_cap);
   const long ntask = (long)a.W * a.L;
-  for (long task = gwave; task < ntask; task += b.nwaves) {
-    if (a.redo[task]) continue;  // FP64 layer (layer16_adjoint_kernel<true>)
+  for_tasks<TASK_CHUNK>(b.queue, ntask, [&](long task) {
+    if (a.redo[task]) return;  // FP64 layer (layer16_adjoint_kernel<true>)
     layer16m_adjoint_task(a, b, task, L, sm, slab);
-  }
+  });
 }
 
 }  // namespace w16

@@ -357,6 +357,7 @@ struct adrt_workspace {
          *g_Bb = nullptr, *g_Ftop = nullptr, *g_f = nullptr, *g_chi = nullptr, *scot = nullptr, *slab = nullptr,
          *mom_part = nullptr, *phase = nullptr, *raw = nullptr;
   int* status = nullptr;
+  int* queue = nullptr;  // task counter of the persistent reverse grids (after status)
   // source Jacobians (ADRT_WORKSPACE_JACOBIAN): the thermal basis of the
   // chunk's columns and (L + 3) pseudo-columns per column
   double *basis = nullptr, *jlay = nullptr, *jrb = nullptr, *jtau_s = nullptr,
@@ -745,7 +746,7 @@ int adrt_workspace_create_ex(const adrt_batch_config* cfg, size_t max_bytes, int
                   : Np == adrt::dev::w32::N ? C * L * adrt::dev::w32::SCOT32 * 8
                                             : 0);
     o_slab = take(slab_bytes);
-    o_status = take(4);
+    o_status = take(8);  // status, queue
   }
   size_t o_basis = 0, o_jlay = 0, o_jrb = 0, o_jlflag = 0, o_jtau_s = 0, o_jtau_cum = 0,
          o_jB = 0, o_jsurf = 0, o_jtop = 0, o_jssa = 0, o_jfdir = 0, o_jucol = 0;
@@ -821,6 +822,7 @@ int adrt_workspace_create_ex(const adrt_batch_config* cfg, size_t max_bytes, int
     ws->mom_part = (double*)(ws->base + o_part);
     ws->slab = (double*)(ws->base + o_slab);
     ws->status = (int*)(ws->base + o_status);
+    ws->queue = ws->status + 1;
     if (hipMemset(ws->status, 0, sizeof(int)) != hipSuccess) {
       (void)hipFree(ws->base);  // already failing
       delete ws;
@@ -980,6 +982,7 @@ int adjoint_chunk32(const adrt::dev::FwdArgs& a, adrt::dev::w16::AdjArgs b, adrt
   const long waves = std::min(tasks, (long)ws->cus * 4 * w32::REV32_WAVES_PER_SIMD);
   b.nwaves = (int)waves;
   e = prof_begin(ws, 4, stream);
+  w16::fill_int_kernel<<<1, 1, 0, stream>>>(b.queue, 0);
   w32::layer32_adjoint_kernel<<<(unsigned)waves, 64, 0, stream>>>(a, b);
   prof_end(ws, e, stream);
   e = prof_begin(ws, 6, stream);
@@ -1255,6 +1258,7 @@ int adrt_adjoint_batch(const adrt_batch_config* cfg, const adrt_device_inputs* i
   b.nwaves = ws->nwaves;
   b.nn_cap = ws->nn_cap;
   b.status = ws->status;
+  b.queue = ws->queue;
   b.g_taus = ws->g_taus;
   b.g_omgs = ws->g_omgs;
   b.g_Bt = ws->g_Bt;
@@ -1320,7 +1324,9 @@ int adrt_adjoint_batch(const adrt_batch_config* cfg, const adrt_device_inputs* i
       w16::AdjArgs bm = b;
       const long mblocks = std::min((tasks + 3) / 4, (long)ws->cus * w16::MREV_WAVES_PER_SIMD);
       bm.nwaves = (int)(mblocks * 4);
+      w16::fill_int_kernel<<<1, 1, 0, stream>>>(b.queue, 0);
       w16::layer16m_adjoint_kernel<<<(unsigned)mblocks, 256, 0, stream>>>(a, bm);
+      w16::fill_int_kernel<<<1, 1, 0, stream>>>(b.queue, 0);
       w16::layer16_adjoint_kernel<true><<<(unsigned)blocks, 256, 0, stream>>>(a, b);
       prof_end(ws, e, stream);
       e = prof_begin(ws, 6, stream);
@@ -1332,12 +1338,17 @@ int adrt_adjoint_batch(const adrt_batch_config* cfg, const adrt_device_inputs* i
       const long pairs = (long)a.W * (a.L / 2);
       const long pblocks = std::min((pairs + 3) / 4, (long)ws->cus * w16::PAIR_REV_WAVES_PER_SIMD);
       bp.nwaves = (int)(pblocks * 4);
-      if (pblocks > 0) w16::layer16p_adjoint_kernel<<<(unsigned)pblocks, 256, 0, stream>>>(a, bp);
+      if (pblocks > 0) {
+        w16::fill_int_kernel<<<1, 1, 0, stream>>>(b.queue, 0);
+        w16::layer16p_adjoint_kernel<<<(unsigned)pblocks, 256, 0, stream>>>(a, bp);
+      }
+      w16::fill_int_kernel<<<1, 1, 0, stream>>>(b.queue, 0);
       w16::layer16_adjoint_kernel<false, true><<<(unsigned)blocks, 256, 0, stream>>>(a, b);
       prof_end(ws, e, stream);
       e = prof_begin(ws, 6, stream);
       w16::start16_adjoint_kernel<false><<<task_blocks, 256, 0, stream>>>(a, b);
     } else {
+      w16::fill_int_kernel<<<1, 1, 0, stream>>>(b.queue, 0);
       w16::layer16_adjoint_kernel<false><<<(unsigned)blocks, 256, 0, stream>>>(a, b);
       prof_end(ws, e, stream);
       e = prof_begin(ws, 6, stream);

@@ -741,9 +741,9 @@ layer16p_adjoint_kernel(FwdArgs a, AdjArgs b) {
   double* sm = smem + wv * LdsA::size;
   double* slab = b.slab + (long)gwave * slab_stride(b.nn_cap);
   const long npair = (long)a.W * (a.L / 2);
-  for (long p = gwave; p < npair; p += b.nwaves) {
+  for_tasks<TASK_CHUNK>(b.queue, npair, [&](long p) {
     long tA, tB;
-    if (!pair_tasks(a, p, tA, tB)) continue;  // left to layer16_adjoint_kernel<false, true>
+    if (!pair_tasks(a, p, tA, tB)) return;  // left to layer16_adjoint_kernel<false, true>
     if (doubling_count(a.tau_s[tA], a.omega_s[tA]) > b.nn_cap ||
         doubling_count(a.tau_s[tB], a.omega_s[tB]) > b.nn_cap) {
       // beyond the slab: flagged, no gradients (as layer16_adjoint_task)
@@ -761,10 +761,10 @@ layer16p_adjoint_kernel(FwdArgs a, AdjArgs b) {
         }
         if (t == tB) break;
       }
-      continue;
+      return;
     }
     layer16p_adjoint_pair(a, b, tA, tB, L, sm, slab);
-  }
+  });
 }
 
 }  // namespace w16
