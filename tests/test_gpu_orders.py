@@ -1,0 +1,72 @@
+"""Every quadrature order from 2 to 32, through the C-ABI.
+
+The library runs orders 2, 4, 8, 16 and 32 on their own layouts and every
+other order padded to the next of them with zero-weight nodes (mu = 1,
+w = 0), the reverse pass at 16 (orders up to 16) or 32.  Per order, on a small
+batch with the options rotated over the orders (solar beam with a surface,
+delta-M with per-column moments, diffusion lower boundary, bottom-first
+indexing, cold black surface):
+
+* the forward against the FP64 oracle and the extended-precision oracle, in
+  the accuracy class of tests/test_gpu_parity.py (1e-10 of the FP64 oracle on
+  well-conditioned columns);
+* the reverse pass's gradients of the thermal sources (Planck levels, surface
+  and top emission) against the contraction of the cotangents with the source
+  Jacobians (planck_source_vjp), a separate route through the adding: the
+  outputs are affine in the sources, so both are exact up to rounding (1e-10).
+"""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+from addingdoublingrt_amd import BatchConfig  # noqa: E402
+from addingdoublingrt_amd.solver import planck_source_vjp  # noqa: E402
+from test_gpu_adjoint import gpu_grads, random_cot, t  # noqa: E402
+from test_gpu_parity import run_both  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+OPTIONS = [
+    dict(surface_albedo=0.2, solar_flux=1.0, solar_mu=0.6),
+    dict(use_delta_m=True, surface_albedo=0.1, solar_flux=0.5, solar_mu=0.4),
+    dict(use_diffusion_lower_bc=True, solar_flux=0.8, solar_mu=0.9),
+    dict(index_from_bottom=True, surface_albedo=0.3, solar_flux=1.0, solar_mu=0.8),
+    dict(surface_emission=0.0, solar_flux=1.0, solar_mu=0.5),
+]
+
+
+def order_batch(n, W=5, L=7):
+    rng = np.random.default_rng(700 + n)
+    tau = 10.0 ** rng.uniform(-2.0, 1.0, (W, L))
+    frac = np.log2(tau) % 1.0
+    tau = np.where((frac < 1e-3) | (frac > 1 - 1e-3), tau * 1.01, tau)
+    ssa = rng.uniform(0.1, 0.98, (W, L))
+    g = rng.uniform(0.2, 0.85, (W, L))
+    mom = g[..., None] ** np.arange(2 * n + 3)
+    B = rng.uniform(0.5, 2.0, (W, L + 1))
+    return tau, ssa, mom, B
+
+
+@pytest.mark.parametrize("n", list(range(2, 33)))
+def test_every_order(n):
+    opt = OPTIONS[n % len(OPTIONS)]
+    tau, ssa, mom, B = order_batch(n)
+    W, L = tau.shape
+    cfg = BatchConfig(num_quadrature=n, **opt)
+    cold = opt.get("surface_emission") == 0.0
+    Bs = None if cold or cfg.use_diffusion_lower_bc else B[:, -1] * 1.05
+    Bt = B[:, 0] * 0.9
+    run_both(tau, ssa, mom, cfg, B, Bs, Bt, label=f"order {n} {sorted(opt)}")
+    cot = random_cot(W, L, n, ("flux_up", "flux_down", "mean_intensity", "flux_divergence"))
+    g, _ = gpu_grads(tau, ssa, mom, cfg, cot, B, Bs, Bt)
+    vB, vS, vT = planck_source_vjp(t(tau), t(ssa), t(mom), cfg,
+                                   {k: t(v) for k, v in cot.items()})
+    vB = vB.cpu().numpy()
+    np.testing.assert_allclose(g["planck_levels"], vB, rtol=1e-10, atol=1e-12 * np.abs(vB).max(),
+                               err_msg=f"order {n}: d/dplanck_levels")
+    np.testing.assert_allclose(g["top_emission"], vT.cpu().numpy(), rtol=1e-10, atol=1e-12,
+                               err_msg=f"order {n}: d/dtop_emission")
+    if Bs is not None:
+        np.testing.assert_allclose(g["surface_emission"], vS.cpu().numpy(), rtol=1e-10,
+                                   atol=1e-12, err_msg=f"order {n}: d/dsurface_emission")
