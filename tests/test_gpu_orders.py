@@ -13,7 +13,10 @@ indexing, cold black surface):
 * the reverse pass's gradients of the thermal sources (Planck levels, surface
   and top emission) against the contraction of the cotangents with the source
   Jacobians (planck_source_vjp), a separate route through the adding: the
-  outputs are affine in the sources, so both are exact up to rounding (1e-10).
+  outputs are affine in the sources, so both are exact up to rounding (1e-10);
+* the reverse pass's gradients of tau, omega and the moments against central
+  differences of the extended-precision oracle (the bar of
+  tests/test_gpu_adjoint.py), on a smaller batch.
 """
 import numpy as np
 import pytest
@@ -36,13 +39,13 @@ OPTIONS = [
 ]
 
 
-def order_batch(n, W=5, L=7):
+def order_batch(n, W=5, L=7, g_range=(0.2, 0.85)):
     rng = np.random.default_rng(700 + n)
     tau = 10.0 ** rng.uniform(-2.0, 1.0, (W, L))
     frac = np.log2(tau) % 1.0
     tau = np.where((frac < 1e-3) | (frac > 1 - 1e-3), tau * 1.01, tau)
     ssa = rng.uniform(0.1, 0.98, (W, L))
-    g = rng.uniform(0.2, 0.85, (W, L))
+    g = rng.uniform(*g_range, (W, L))
     mom = g[..., None] ** np.arange(2 * n + 3)
     B = rng.uniform(0.5, 2.0, (W, L + 1))
     return tau, ssa, mom, B
@@ -70,3 +73,28 @@ def test_every_order(n):
     if Bs is not None:
         np.testing.assert_allclose(g["surface_emission"], vS.cpu().numpy(), rtol=1e-10,
                                    atol=1e-12, err_msg=f"order {n}: d/dsurface_emission")
+
+
+@pytest.mark.parametrize("n", list(range(2, 33)))
+def test_every_order_reverse_vs_differences(n):
+    """The reverse pass at every order against central differences of the
+    extended-precision oracle (the protocol and bar of tests/test_gpu_adjoint.py)
+    for tau, omega and the per-column moments, on a small batch."""
+    from test_gpu_adjoint import NAMES, assert_close, fd_grads, unstable_layers
+    opt = OPTIONS[(n + 2) % len(OPTIONS)]
+    # delta-M: the fraction f = g^(2n) stays above 1e-4, far from the 1e-12
+    # switch of the reference's delta-M (a difference step must not cross it)
+    g_range = (0.87, 0.93) if opt.get("use_delta_m") else (0.2, 0.85)
+    tau, ssa, mom, B = order_batch(n, W=2, L=4, g_range=g_range)
+    W, L = tau.shape
+    cfg = BatchConfig(num_quadrature=n, **opt)
+    cold = opt.get("surface_emission") == 0.0
+    Bs = None if cold or cfg.use_diffusion_lower_bc else B[:, -1] * 1.05
+    Bt = B[:, 0] * 0.9
+    cot = random_cot(W, L, 100 + n, ("flux_up", "flux_down", "mean_intensity", "flux_direct"))
+    g, _ = gpu_grads(tau, ssa, mom, cfg, cot, B, Bs, Bt)
+    fd, nz = fd_grads(tau, ssa, mom, cfg, cot, B, Bs, Bt, which=("tau", "ssa", "mom"))
+    loose = unstable_layers(tau, ssa, mom, cfg)
+    for k, v in fd.items():
+        assert_close(g[NAMES[k]], v, f"order {n} d/d{NAMES[k]}", loose, skip_loose=(k == "mom"),
+                     noise=nz[k])
