@@ -16,7 +16,9 @@ indexing, cold black surface):
   outputs are affine in the sources, so both are exact up to rounding (1e-10);
 * the reverse pass's gradients of tau, omega and the moments against central
   differences of the extended-precision oracle (the bar of
-  tests/test_gpu_adjoint.py), on a smaller batch.
+  tests/test_gpu_adjoint.py), on a smaller batch;
+* at orders 2..16, the mixed_reverse form: outputs bit-identical to FP64,
+  gradients within 1e-4 of the FP64 reverse pass per column.
 """
 import numpy as np
 import pytest
@@ -98,3 +100,35 @@ def test_every_order_reverse_vs_differences(n):
     for k, v in fd.items():
         assert_close(g[NAMES[k]], v, f"order {n} d/d{NAMES[k]}", loose, skip_loose=(k == "mom"),
                      noise=nz[k])
+
+
+@pytest.mark.parametrize("n", list(range(2, 17)))
+def test_every_order_mixed_reverse(n):
+    """The mixed_reverse form (FP32 reverse doubling) at every order it takes:
+    outputs bit-identical to the FP64 form's, gradients within the 1e-4 bar of
+    the FP64 reverse pass, per column and norm-wise."""
+    from addingdoublingrt_amd import adjoint_batch, solve_batch
+    from test_gpu_mixed import grad_errors
+    opt = OPTIONS[n % len(OPTIONS)]
+    tau, ssa, mom, B = order_batch(n, W=16, L=12)
+    W, L = tau.shape
+    cfg = BatchConfig(num_quadrature=n, **opt)
+    args = (t(tau), t(ssa), t(mom), cfg)
+    kw = dict(planck_levels=t(B), top_emission=t(B[:, 0] * 0.9))
+    cot = {k: t(v) for k, v in random_cot(W, L, 200 + n, ("flux_up", "flux_down",
+                                                          "mean_intensity")).items()}
+    res = {}
+    for prec in ("f64", "mixed_reverse"):
+        out = solve_batch(*args, **kw, adjoint_workspace=True, precision=prec)
+        gen = out.pop("_generation")
+        g = adjoint_batch(*args, out, cot, **kw, forward_generation=gen, precision=prec)
+        torch.cuda.synchronize()
+        res[prec] = ({k: v.cpu().numpy() for k, v in out.items()},
+                     {k: v.cpu().numpy() for k, v in g.items() if v is not None})
+    for k, v in res["f64"][0].items():
+        assert np.array_equal(v, res["mixed_reverse"][0][k]), f"order {n}: {k}"
+    errs = grad_errors(res["mixed_reverse"][1], res["f64"][1], W)
+    worst = {k: float(v.max()) for k, v in errs.items()}
+    print(f"order {n} mixed_reverse gradients, worst column: {worst}")
+    for k, v in worst.items():
+        assert v <= 1e-4, (n, k, v)
