@@ -18,8 +18,10 @@ relative norm.
   start, FP64-refined solves, FP64 adding and fluxes; the same layers in
   FP64).  On the 65536 x 100, N = 16 workload it leaves 23 % of the columns
   beyond 1e-6 in the fluxes (largest 3.6e-4, median 4.8e-7), so its tests at
-  the bar are expected to fail (xfail, with the measured figures printed); a
-  regression bound at its measured ceiling guards it.  Its gradients of
+  the bar are expected to fail there and on config 2 (strict xfail, with the
+  measured figures printed) and pass on the N = 8 workloads (configs 1 and 4,
+  and config 2's gradients); a regression bound at its measured ceiling
+  guards it.  Its gradients of
   unstable-start layers (which both forms compute in FP64, but from interface
   vectors that differ by the FP32 rounding of the other layers' records) are
   reported apart from the others.  At quadrature order 32 (forward only) it
@@ -136,9 +138,17 @@ def test_mixed_forward_regression(k, W):
     assert e.max() <= 4e-4, (float(e.max()), int(e.argmax()))
 
 
-@pytest.mark.xfail(reason="the FP32-forward form misses the 1e-6 flux bar on about a quarter of "
-                          "the headline workload's columns (largest 3.6e-4)", strict=False)
-@pytest.mark.parametrize("k,W", WORKLOADS)
+def _at_bar_cases(misses, reason):
+    """WORKLOADS with the measured misses of the bar marked xfail (strict: a
+    case that starts to pass is reported), the others held to it"""
+    return [pytest.param(k, W, marks=pytest.mark.xfail(reason=reason, strict=True))
+            if (k, W) in misses else (k, W) for k, W in WORKLOADS]
+
+
+@pytest.mark.parametrize("k,W", _at_bar_cases(
+    {(2, 16), (5, 24), (5, 4096)},
+    "the FP32-forward form misses the 1e-6 flux bar on N = 16 workloads with thick or "
+    "unstable-start layers (a quarter of the headline workload's columns, largest 3.6e-4)"))
 def test_mixed_forward_vs_oracle_at_bar(k, W):
     cols, d, cfg = _inputs(k, W)
     out = solve_batch(d["delta_tau"], d["single_scat_albedo"], d["phase_moments"], cfg,
@@ -160,10 +170,10 @@ def test_mixed_forward_vs_oracle_at_bar(k, W):
     assert e.max() <= FLUX_BAR, (float(e.max()), int(e.argmax()))
 
 
-@pytest.mark.xfail(reason="the FP32-forward form's gradients miss 1e-4 where its interface "
-                          "vectors differ from FP64 (unstable-start layers, thick layers)",
-                   strict=False)
-@pytest.mark.parametrize("k,W", WORKLOADS)
+@pytest.mark.parametrize("k,W", _at_bar_cases(
+    {(5, 24), (5, 4096)},
+    "the FP32-forward form's gradients miss 1e-4 on the headline workload, where its "
+    "interface vectors differ from FP64 (unstable-start layers, thick layers)"))
 def test_mixed_gradients_at_bar(k, W):
     cols, d, cfg = _inputs(k, W)
     _, g64 = _run(cols, d, cfg, "f64", 11 + k)
@@ -185,7 +195,7 @@ def test_mixed_gradients_at_bar(k, W):
 
 
 # quadrature order 32 (workload config 3: 1 - omega down to 1e-6, 64 moments):
-# the mixed forward alone (the reverse pass runs up to order 16).  Measured on
+# the mixed forward alone (the reduced-precision reverse forms run up to order 16).  Measured on
 # 8192 columns: per column largest 1.6e-4, median 7.7e-6, 5.5 % within 1e-6 --
 # the floor that rounding the thin-layer start to FP32 already sets
 # (tests/test_mixed32_ceiling.py) -- and no faster than FP64.
