@@ -720,30 +720,18 @@ __device__ void layer16_adjoint_task(const FwdArgs& a, const AdjArgs& b, long ta
 // Kept apart from the doubling reverse so that kernel keeps to 128 registers.
 // 4 waves per SIMD (measured: 24.0 ms on the 65536 x 100 workload against
 // 27.6 ms at 3 and 32.4 ms at 5).
-// REDO: only the layers the mixed-precision forward left to the FP64 kernels;
-// RowT = float: the other layers of the mixed form (start cotangents in the
-// FP32 register order of layer16m_adjoint_kernel).
-template <bool REDO, typename RowT = double>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8)))
-start16_adjoint_kernel(FwdArgs a, AdjArgs b) {
-  __shared__ double smem[4 * LDS_START16];
-  const int wv = wave_index();
-  const long task = (long)blockIdx.x * 4 + wv;
+// RowT = float: a layer of the mixed form's FP32 reverse (start cotangents in
+// the FP32 register order of layer16m_adjoint_kernel).
+template <typename RowT>
+__device__ __forceinline__ void start16_adjoint_task(const FwdArgs& a, const AdjArgs& b, long task,
+                                                     const Lane& L, double* dtile) {
   const int Lay = a.L;
-  if (task >= (long)a.W * Lay) return;
-  if constexpr (REDO) {
-    if (!a.redo[task]) return;
-  } else if constexpr (sizeof(RowT) == 4) {
-    if (a.redo[task]) return;
-  }
   const double tau = a.tau_s[task];
   const double omega_in = a.omega_s[task];
   if (tau <= 0.0 || omega_in <= 0.0) return;  // finished by the doubling reverse
-  const Lane L;
   const int j = L.j;
   const int w = (int)(task / Lay), l = (int)(task % Lay);
   const long gw = (long)(a.w0 + w);
-  double* dtile = smem + wv * LDS_START16;
   double* csl = dtile + TILE;
   double* tiles = csl + N;  // two tiles for the Legendre sums
   Phase16<RowT> p;
@@ -772,6 +760,28 @@ start16_adjoint_kernel(FwdArgs a, AdjArgs b) {
     b.g_omgs[task] = g_om;
     b.g_Ftop[task] = g_F;
     b.g_f[task] = g_fr;
+  }
+}
+
+// MIXED: the layers of the mixed_reverse form, in one launch: those the
+// forward routed to the FP64 kernels (redo) in the FP64 register order of
+// layer16_adjoint_kernel<true>, the others in the FP32 order.
+template <bool MIXED = false>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8)))
+start16_adjoint_kernel(FwdArgs a, AdjArgs b) {
+  __shared__ double smem[4 * LDS_START16];
+  const int wv = wave_index();
+  const long task = (long)blockIdx.x * 4 + wv;
+  if (task >= (long)a.W * a.L) return;
+  const Lane L;
+  double* dtile = smem + wv * LDS_START16;
+  if constexpr (MIXED) {
+    if (a.redo[task])
+      start16_adjoint_task<double>(a, b, task, L, dtile);
+    else
+      start16_adjoint_task<float>(a, b, task, L, dtile);
+  } else {
+    start16_adjoint_task<double>(a, b, task, L, dtile);
   }
 }
 

@@ -101,7 +101,7 @@ __device__ void layer16m_adjoint_task(const FwdArgs& a, const AdjArgs& b, long t
   }
 
   // the thin-layer start as the forward computed it; its reverse is
-  // start16_adjoint_kernel<false, float>'s
+  // start16_adjoint_kernel<true>'s (in the FP32 register order)
   Mat<R> Rm, Sm;
   double y, z = 0.0, sus, sds, gam, gk, tau0;
   int nn;

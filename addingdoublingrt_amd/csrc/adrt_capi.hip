@@ -1331,7 +1331,6 @@ int adrt_adjoint_batch(const adrt_batch_config* cfg, const adrt_device_inputs* i
       prof_end(ws, e, stream);
       e = prof_begin(ws, 6, stream);
       w16::start16_adjoint_kernel<true><<<task_blocks, 256, 0, stream>>>(a, b);
-      w16::start16_adjoint_kernel<false, float><<<task_blocks, 256, 0, stream>>>(a, b);
     } else if (a.pord) {
       // two layers per wave (orders up to 8 padded to 16)
       w16::AdjArgs bp = b;
