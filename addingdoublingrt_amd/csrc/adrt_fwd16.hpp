@@ -403,9 +403,11 @@ __device__ __forceinline__ void layer16_task(const FwdArgs& a, long task, const 
 }
 
 // UNPAIRED: only the layers pair mode leaves (orders up to 8 on reverse-pass
-// workspaces: a.pflag, adrt_pair16.hpp)
+// workspaces: a.pflag, adrt_pair16.hpp).  4 waves per SIMD (122 registers, no
+// spills): 120.9 ms on the 65536 x 100 workload against 121.4 - 122.5 ms at 5
+// (96 registers, 22 spilled).
 template <bool REDO, bool UNPAIRED = false>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) layer16_kernel(FwdArgs a) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) layer16_kernel(FwdArgs a) {
   __shared__ double smem[4 * Lds16::size];
   const int wv = wave_index();
   const long task = (long)blockIdx.x * 4 + wv;
