@@ -278,9 +278,11 @@ __device__ __forceinline__ void pair_start(const FwdArgs& a, const Lane& L, cons
 
 
 // ---------------------------------------------------------------------------
-// forward: phase setup + doubling of the two layers of pair p (FP64 records)
+// forward: phase setup + doubling of the two layers of pair p (FP64 records);
+// 4 waves per SIMD: 2.78 ms on the 4096 x 60, N = 8 workload against 3.30 ms
+// at 5 (20 registers spilled there)
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8)))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8)))
 layer16p_kernel(FwdArgs a) {
   using R = double;
   __shared__ R smem[4 * Lds16::size];
